@@ -1,0 +1,150 @@
+/*
+ * subby_amd — C ABI of the MI355X (gfx950) per-column sufficient-statistics engine.
+ *
+ * Every entry point returns an int status (SB_OK == 0, negative on error) and
+ * leaves a human-readable message retrievable with sb_last_error().  Buffers
+ * named d_* are device pointers owned by the caller; small tree arrays
+ * (parent_index, distances) are host pointers.  All work is stream-ordered on
+ * the hipStream_t passed as `stream` (NULL = default stream) and returns
+ * without synchronising.
+ *
+ * Layouts (row-major, the reference's own):
+ *   alignment  int8  [R][C]   tokens 0..A-1 observed, A ungapped-unobserved,
+ *                             A+1 or -1 gap (any other value reads as a gap)
+ *   loglike    f64   [C]
+ *   root       T     [A][C]   posterior root distribution
+ *   counts     T     [A][A][C] diagonal = dwell times, off-diagonal = counts
+ *   mask       u8    [R][C]   explicit branch mask (row 0 ignored)
+ * T is float for SB_F32 and double for SB_F64.
+ *
+ * Trees are rooted binary trees in preorder: parent_index[0] == -1 and
+ * 0 <= parent_index[n] < n; every node has 0 or 2 children.
+ */
+#ifndef SUBBY_AMD_H
+#define SUBBY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SB_OK = 0,
+  SB_ERR_ARG = -1,         /* bad argument (null pointer, size, unknown flag) */
+  SB_ERR_TREE = -2,        /* parent_index not preorder or not binary */
+  SB_ERR_ALPHABET = -3,    /* A outside 2..64 */
+  SB_ERR_OOM = -4,         /* device or pinned allocation failed */
+  SB_ERR_HIP = -5,         /* a HIP runtime call failed */
+  SB_ERR_MODEL = -6,       /* unknown model id or invalid eigensystem */
+  SB_ERR_UNSUPPORTED = -7  /* combination not supported (e.g. F81 on non-F81 model) */
+};
+
+enum { SB_F32 = 0, SB_F64 = 1 };
+enum { SB_MASK_NONE = 0, SB_MASK_AUTO = 1, SB_MASK_EXPLICIT = 2 };
+
+typedef struct sb_context sb_context;
+typedef struct sb_batch sb_batch;
+
+/* Message of the last failing call on this host thread. */
+const char* sb_last_error(void);
+int sb_version(void);
+
+/* Context: one per device.  Holds the resident model table.
+ * Replaces the module-level state of subby/jax/__init__.py (none: JAX is
+ * functional); the context exists so eigensystems stay resident on device. */
+int sb_create(sb_context** out, int device);
+int sb_destroy(sb_context* ctx);
+
+/* Upload an eigensystem (DiagModel, subby/jax/types.py:8-11): eigenvalues[A],
+ * eigenvectors[A*A] row-major with V[a*A+k] = component a of eigenvector k,
+ * pi[A].  The model stays resident until sb_model_drop / sb_destroy. */
+int sb_model_put(sb_context* ctx, int A, const double* eigenvalues,
+                 const double* eigenvectors, const double* pi, int* model_id);
+int sb_model_drop(sb_context* ctx, int model_id);
+
+/* Single-block calls (drop-in for subby.jax LogLike / RootProb / Counts). */
+
+/* subby/jax/__init__.py:207-291 (LogLike, single DiagModel branch :285-291). */
+int sb_loglike(sb_context* ctx, int model_id, const int8_t* d_alignment,
+               const int32_t* parent_index, const double* distances, int R, int C,
+               int dtype, double* d_loglike, void* stream);
+
+/* subby/jax/__init__.py:543-650 (RootProb, single model :635-650).
+ * d_loglike may be NULL. */
+int sb_root_prob(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                 const int32_t* parent_index, const double* distances, int R, int C,
+                 int dtype, void* d_root, double* d_loglike, void* stream);
+
+/* subby/jax/__init__.py:338-447 (Counts, single model :413-447).
+ * f81_fast selects the closed-form F81/JC path (f81_fast.py:7-141);
+ * mask_mode: SB_MASK_NONE, SB_MASK_AUTO (Steiner mask, oracle.py:1372-1435),
+ * SB_MASK_EXPLICIT (d_mask).  d_root and d_loglike may be NULL. */
+int sb_counts(sb_context* ctx, int model_id, const int8_t* d_alignment,
+              const int32_t* parent_index, const double* distances, int R, int C,
+              int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
+              void* d_counts, void* d_root, double* d_loglike, void* stream);
+
+/* Gradient of sum_c g_c logL_c (subby/jax/vjp.py:62-133, method 'eigen').
+ * d_g: [C] f64 cotangent, or NULL for all ones.
+ * d_dist_grad: [R] f64 output, entry 0 (root) is 0.
+ * n_rate: number of rate parameters; rate_weights: host [n_rate][A*A] f64 with
+ *   W_ij = dQ_ij/dtheta / Q_ij (i != j) and W_ii = dQ_ii/dtheta, so that
+ *   dlogL/dtheta = sum_ij W_ij Counts_ij (unmasked) — HR2002 eqs. 3-4.
+ *   (The reference has no rate-parameter gradient; this extends vjp.py.)
+ * d_rate_grad: [n_rate] f64 output (may be NULL when n_rate == 0).
+ * d_loglike may be NULL. */
+int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                    const int32_t* parent_index, const double* distances, int R, int C,
+                    int dtype, const double* d_g, double* d_dist_grad,
+                    int n_rate, const double* rate_weights, double* d_rate_grad,
+                    double* d_loglike, void* stream);
+
+/* Batched calls: many (tree, alignment-block) pairs in one persistent launch.
+ * Mirrors jax.vmap(Counts) of scripts/batch_eigensub_maf.py:528-532 without
+ * requiring equal tree sizes. */
+typedef struct sb_block {
+  int model_id;
+  int R, C;
+  const int8_t* d_alignment;    /* [R][C] */
+  const int32_t* parent_index;  /* host [R] */
+  const double* distances;      /* host [R] */
+  int want;                     /* SB_WANT_* bits */
+  int f81_fast;
+  int mask_mode;
+  const uint8_t* d_mask;        /* [R][C] or NULL */
+  void* d_counts;               /* [A][A][C] or NULL */
+  void* d_root;                 /* [A][C] or NULL */
+  double* d_loglike;            /* [C] or NULL */
+} sb_block;
+
+enum { SB_WANT_LOGLIKE = 1, SB_WANT_ROOT = 2, SB_WANT_COUNTS = 4 };
+
+/* A batch launcher bound to one stream; owns its staging, plan and scratch
+ * buffers so several launchers can run concurrently on different streams. */
+int sb_batch_create(sb_context* ctx, void* stream, sb_batch** out);
+int sb_batch_destroy(sb_batch* b);
+/* Enqueue every block; all blocks must use one A and one dtype.
+ * use_graph != 0 replays a captured hipGraph of the kernel sequence. */
+int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype,
+                    int use_graph);
+/* Tile shape: waves per workgroup (1,2,4,8) and resident waves per CU used to
+ * size the persistent grid; 0 keeps the current value. */
+int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
+/* Block until the launcher's last submission has finished. */
+int sb_batch_sync(sb_batch* b);
+
+/* Plain path: the same statistics through one launch per tree level per pass
+ * with inside/outside partials materialised in HBM (the data flow of the
+ * reference's WebGPU engine, subby/webgpu/phylo_gpu.js:611-724).  Kept as the
+ * one-block-per-launch baseline. */
+int sb_counts_plain(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                    const int32_t* parent_index, const double* distances, int R, int C,
+                    int mask_mode, float* d_counts, float* d_root, double* d_loglike,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,94 @@
+"""ctypes binding of libsubby_amd.so (the C ABI in include/subby_amd.h).
+
+The library is built in-tree by ``subby_amd.build``.  Nothing here falls back
+to a CPU path: a missing library or a missing GPU raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libsubby_amd.so')
+
+SB_OK = 0
+SB_F32, SB_F64 = 0, 1
+SB_MASK_NONE, SB_MASK_AUTO, SB_MASK_EXPLICIT = 0, 1, 2
+SB_WANT_LOGLIKE, SB_WANT_ROOT, SB_WANT_COUNTS = 1, 2, 4
+
+# Every exported symbol of include/subby_amd.h.
+EXPORTS = [
+    'sb_last_error', 'sb_version', 'sb_create', 'sb_destroy', 'sb_model_put',
+    'sb_model_drop', 'sb_loglike', 'sb_root_prob', 'sb_counts', 'sb_loglike_grad',
+    'sb_batch_create', 'sb_batch_destroy', 'sb_batch_submit', 'sb_batch_set_tiling',
+    'sb_batch_sync', 'sb_counts_plain',
+]
+
+
+class SbBlock(ctypes.Structure):
+    """sb_block of include/subby_amd.h."""
+    _fields_ = [
+        ('model_id', ctypes.c_int),
+        ('R', ctypes.c_int),
+        ('C', ctypes.c_int),
+        ('d_alignment', ctypes.c_void_p),
+        ('parent_index', ctypes.c_void_p),
+        ('distances', ctypes.c_void_p),
+        ('want', ctypes.c_int),
+        ('f81_fast', ctypes.c_int),
+        ('mask_mode', ctypes.c_int),
+        ('d_mask', ctypes.c_void_p),
+        ('d_counts', ctypes.c_void_p),
+        ('d_root', ctypes.c_void_p),
+        ('d_loglike', ctypes.c_void_p),
+    ]
+
+
+class SubbyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'subby_amd error {code}: {msg}')
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the shared library and declare signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f'{path} is missing: build it with `python -m subby_amd.build` '
+            '(hipcc --offload-arch=gfx950); there is no CPU fallback')
+    lib = ctypes.CDLL(path)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    lib.sb_last_error.restype = ctypes.c_char_p
+    lib.sb_last_error.argtypes = []
+    lib.sb_version.restype = i
+    lib.sb_create.argtypes = [P(vp), i]
+    lib.sb_destroy.argtypes = [vp]
+    lib.sb_model_put.argtypes = [vp, i, vp, vp, vp, P(i)]
+    lib.sb_model_drop.argtypes = [vp, i]
+    lib.sb_loglike.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp]
+    lib.sb_root_prob.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp]
+    lib.sb_counts.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]
+    lib.sb_loglike_grad.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, i, vp, vp, vp, vp]
+    lib.sb_batch_create.argtypes = [vp, vp, P(vp)]
+    lib.sb_batch_destroy.argtypes = [vp]
+    lib.sb_batch_submit.argtypes = [vp, i, vp, i, i]
+    lib.sb_batch_set_tiling.argtypes = [vp, i, i]
+    lib.sb_batch_sync.argtypes = [vp]
+    lib.sb_counts_plain.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp, vp]
+    for name in EXPORTS:
+        if name not in ('sb_last_error',):
+            getattr(lib, name).restype = i
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != SB_OK:
+        msg = _lib.sb_last_error().decode(errors='replace') if _lib else '?'
+        raise SubbyError(rc, msg)
+    return rc

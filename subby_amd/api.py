@@ -1,0 +1,230 @@
+"""Drop-in host API: LogLike / Counts / RootProb / LogLikeCustomGrad.
+
+Same call shape as the reference's subby.jax wrappers
+(subby/jax/__init__.py:207-291 LogLike, :338-447 Counts, :543-650 RootProb,
+:294-335 LogLikeCustomGrad) for a single DiagModel or RateModel:
+
+    alignment : (R, C) int tokens (numpy, torch CPU or torch CUDA)
+    tree      : Tree(parentIndex (R,), distanceToParent (R,))
+    model     : DiagModel(eigenvalues (A,), eigenvectors (A,A) V[a,k], pi (A,))
+
+Results are torch tensors on the engine's GPU: LogLike (C,) float64,
+Counts (A, A, C) and RootProb (A, C) in the compute precision ('fp32' by
+default, 'fp64' on request).  ``maxChunkSize`` is accepted for signature
+compatibility; results do not depend on it.  Errors the reference raises as
+Python assertions (non-binary tree, F81 with an unsupported model) raise
+``SubbyError`` / ``ValueError`` here.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib
+from .models import Tree, DiagModel, RateModel, ensure_diag
+
+_PREC = {'fp32': (_lib.SB_F32, torch.float32), 'fp64': (_lib.SB_F64, torch.float64)}
+
+
+def _stream_ptr(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Engine:
+    """One sb_context on one GPU plus the resident-model cache."""
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError('subby_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
+        self.lib = _lib.load()
+        self.device = torch.device('cuda', device)
+        self.ctx = ctypes.c_void_p()
+        _lib.check(self.lib.sb_create(ctypes.byref(self.ctx), device))
+        self._models = {}
+        self._lock = threading.Lock()
+
+    def close(self):
+        if self.ctx:
+            self.lib.sb_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- resident models
+    def model_id(self, model) -> int:
+        m = ensure_diag(model)
+        ev = np.ascontiguousarray(m.eigenvalues, dtype=np.float64)
+        V = np.ascontiguousarray(m.eigenvectors, dtype=np.float64)
+        pi = np.ascontiguousarray(m.pi, dtype=np.float64)
+        A = len(pi)
+        if ev.shape != (A,) or V.shape != (A, A):
+            raise ValueError(f'model shapes {ev.shape}, {V.shape}, {pi.shape} are inconsistent')
+        key = (A, ev.tobytes(), V.tobytes(), pi.tobytes())
+        with self._lock:
+            mid = self._models.get(key)
+            if mid is None:
+                out = ctypes.c_int()
+                _lib.check(self.lib.sb_model_put(self.ctx, A, ev.ctypes.data, V.ctypes.data,
+                                                 pi.ctypes.data, ctypes.byref(out)))
+                mid = out.value
+                self._models[key] = mid
+        return mid
+
+    def drop_models(self):
+        with self._lock:
+            for mid in self._models.values():
+                self.lib.sb_model_drop(self.ctx, mid)
+            self._models.clear()
+
+    # ---- inputs
+    def alignment(self, alignment, A) -> torch.Tensor:
+        """(R, C) int8 tokens on the device; tokens outside -1..A+1 read as gaps."""
+        if isinstance(alignment, torch.Tensor) and alignment.device == self.device \
+                and alignment.dtype == torch.int8 and alignment.is_contiguous():
+            return alignment
+        a = alignment.detach().cpu().numpy() if isinstance(alignment, torch.Tensor) else np.asarray(alignment)
+        if a.ndim != 2:
+            raise ValueError(f'alignment must be (R, C), got shape {a.shape}')
+        a = np.where((a >= -1) & (a <= A + 1), a, -1).astype(np.int8)
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+
+    @staticmethod
+    def tree_arrays(tree, R):
+        parent = np.ascontiguousarray(_to_np(tree.parentIndex), dtype=np.int32)
+        dist = np.ascontiguousarray(_to_np(tree.distanceToParent), dtype=np.float64)
+        if parent.shape != (R,) or dist.shape != (R,):
+            raise ValueError(f'tree arrays must have shape ({R},), got {parent.shape} and {dist.shape}')
+        return parent, dist
+
+
+def _to_np(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+_default = None
+_default_lock = threading.Lock()
+
+
+def default_engine() -> Engine:
+    global _default
+    with _default_lock:
+        if _default is None:
+            _default = Engine(torch.cuda.current_device() if torch.cuda.is_available() else 0)
+        return _default
+
+
+def _prep(alignment, tree, model, engine):
+    eng = engine or default_engine()
+    m = ensure_diag(model)
+    A = len(m.pi)
+    aln = eng.alignment(alignment, A)
+    R, C = aln.shape
+    parent, dist = eng.tree_arrays(tree, R)
+    return eng, m, A, aln, R, C, parent, dist
+
+
+def LogLike(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
+            engine: Engine | None = None) -> torch.Tensor:
+    """Per-column log-likelihoods, (C,) float64 (subby/jax/__init__.py:207-291)."""
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    dt, _ = _PREC[precision]
+    out = torch.empty(C, dtype=torch.float64, device=eng.device)
+    _lib.check(eng.lib.sb_loglike(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
+                                  dist.ctypes.data, R, C, dt, out.data_ptr(), _stream_ptr(eng.device)))
+    return out
+
+
+def RootProb(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
+             engine: Engine | None = None) -> torch.Tensor:
+    """Posterior root distribution, (A, C) (subby/jax/__init__.py:543-650)."""
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    dt, tdt = _PREC[precision]
+    out = torch.empty((A, C), dtype=tdt, device=eng.device)
+    _lib.check(eng.lib.sb_root_prob(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
+                                    dist.ctypes.data, R, C, dt, out.data_ptr(), None,
+                                    _stream_ptr(eng.device)))
+    return out
+
+
+def _mask_args(branch_mask, eng, R, C):
+    if branch_mask is None:
+        return _lib.SB_MASK_NONE, None, None
+    if isinstance(branch_mask, str):
+        if branch_mask != 'auto':
+            raise ValueError(f'branch_mask must be "auto", None or an (R, C) bool array, got {branch_mask!r}')
+        return _lib.SB_MASK_AUTO, None, None
+    mk = branch_mask
+    if isinstance(mk, torch.Tensor):
+        mk = mk.to(eng.device)
+    else:
+        mk = torch.from_numpy(np.asarray(mk)).to(eng.device)
+    if tuple(mk.shape) != (R, C):
+        raise ValueError(f'branch_mask must have shape ({R}, {C}), got {tuple(mk.shape)}')
+    mk = (mk != 0).to(torch.uint8).contiguous()
+    return _lib.SB_MASK_EXPLICIT, mk, mk.data_ptr()
+
+
+def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool = False,
+           branch_mask='auto', precision: str = 'fp32', engine: Engine | None = None,
+           return_loglike: bool = False, return_root: bool = False):
+    """Expected substitution counts (off-diagonal) and dwell times (diagonal), (A, A, C)
+    (subby/jax/__init__.py:338-447).  ``branch_mask``: "auto" (Steiner mask of
+    the ungapped leaves), None, or an (R, C) bool array."""
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    dt, tdt = _PREC[precision]
+    mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
+    counts = torch.empty((A, A, C), dtype=tdt, device=eng.device)
+    ll = torch.empty(C, dtype=torch.float64, device=eng.device) if return_loglike else None
+    root = torch.empty((A, C), dtype=tdt, device=eng.device) if return_root else None
+    _lib.check(eng.lib.sb_counts(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
+                                 dist.ctypes.data, R, C, dt, 1 if f81_fast_flag else 0, mode, mptr,
+                                 counts.data_ptr(), root.data_ptr() if root is not None else None,
+                                 ll.data_ptr() if ll is not None else None, _stream_ptr(eng.device)))
+    if return_loglike or return_root:
+        res = [counts]
+        if return_loglike:
+            res.append(ll)
+        if return_root:
+            res.append(root)
+        return tuple(res)
+    return counts
+
+
+def loglike_grad(alignment, tree, model, g=None, rate_weights=None, precision: str = 'fp32',
+                 engine: Engine | None = None):
+    """d(sum_c g_c logL_c)/d(distanceToParent) (R,) and, for each Fisher weight
+    matrix W in ``rate_weights``, sum_ij W_ij Counts_ij (unmasked): the
+    derivative w.r.t. that rate parameter (subby/jax/vjp.py:62-133 + HR2002).
+
+    Returns (dist_grad (R,) f64, rate_grad (n_rate,) f64, logLike (C,) f64)."""
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    dt, _ = _PREC[precision]
+    dev = eng.device
+    gt = None
+    if g is not None:
+        gt = torch.as_tensor(g, dtype=torch.float64, device=dev).contiguous()
+        if gt.shape != (C,):
+            raise ValueError(f'g must have shape ({C},)')
+    W = np.zeros((0, A, A)) if rate_weights is None else np.asarray(rate_weights, dtype=np.float64)
+    if W.ndim == 2:
+        W = W[None]
+    W = np.ascontiguousarray(W)
+    nr = W.shape[0]
+    dgrad = torch.empty(R, dtype=torch.float64, device=dev)
+    rgrad = torch.empty(max(nr, 1), dtype=torch.float64, device=dev)
+    ll = torch.empty(C, dtype=torch.float64, device=dev)
+    _lib.check(eng.lib.sb_loglike_grad(
+        eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data, dist.ctypes.data, R, C, dt,
+        gt.data_ptr() if gt is not None else None, dgrad.data_ptr(), nr,
+        W.ctypes.data if nr else None, rgrad.data_ptr() if nr else None, ll.data_ptr(),
+        _stream_ptr(dev)))
+    return dgrad, rgrad[:nr], ll

@@ -1,0 +1,571 @@
+// Fused inside / outside / count-accumulation kernel for small alphabets
+// (padded alphabet AP = 4 or 8), gfx950.
+//
+// One thread owns one alignment column; one wave64 owns a tile of 64
+// consecutive columns of one block and walks the whole tree for it:
+//
+//   inside pass  (subby/jax/pruning.py:94-118, oracle.py:784-838)
+//     internal nodes in postorder; U[p] = obs(p) * (M_l U_l) * (M_r U_r)
+//   outside pass (subby/jax/outside.py:94-126, oracle.py:845-924) fused with
+//     count accumulation (eigensub.py:87-135 or f81_fast.py:7-141):
+//     internal nodes p in preorder; x = obs(p) * (pi or D[p] M_p);
+//     D[l] = (M_r U_r) * x, D[r] = (M_l U_l) * x, and both branches are
+//     accumulated right away, so leaf vectors never touch memory.
+//   epilogue: back-transform V C V^T (eigensub.py:138-172) or the F81
+//     combination, root posterior (subby/jax/__init__.py:635-650) and the
+//     log-likelihood.
+//
+// Only internal-node vectors are staged, in a per-wave scratch slot that the
+// same wave reads back (stays in L2 / MALL).  All per-branch matrices and the
+// tree records are wave-uniform and arrive through scalar loads.
+//
+// Rescaling: every node vector is scaled by an exact power of two 2^-e with
+// e = frexp-exponent of its max (max lands in [0.5, 1)), and the per-node
+// log-normaliser is kept as the integer e summed over the subtree.  This is
+// the reference's per-node max-rescaling (clamp 1e-300) with an exact
+// log-scale: ln-normaliser = e * ln 2.  The log-likelihood is formed in FP64
+// and scale(n,c) = exp(lnD + lnU - ll) = ldexp(1/s0, eD + eU - e0) exactly.
+#include <hip/hip_runtime.h>
+#include "../../include/subby_amd.h"
+#include "sb_plan.h"
+
+// Read-only, wave-uniform data (tree records, branch matrices, model) is read
+// through the constant address space so it arrives by scalar loads; per-lane
+// data goes through the global address space (no flat instructions).
+#define SB_CONST __attribute__((address_space(4)))
+#define SB_GLOBAL __attribute__((address_space(1)))
+
+namespace {
+
+template <typename T>
+__device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
+template <typename T>
+__device__ inline SB_GLOBAL T* glb(void* p) { return (SB_GLOBAL T*)(p); }
+template <typename T>
+__device__ inline const SB_GLOBAL T* glb(const void* p) { return (const SB_GLOBAL T*)(p); }
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Chunk;
+template <> struct Chunk<float> { typedef f32x4 type; static constexpr int n = 4; };
+template <> struct Chunk<double> { typedef f64x2 type; static constexpr int n = 2; };
+
+__device__ inline int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ inline int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ inline float ldexp_(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+__device__ inline double ldexp_(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+
+template <typename T, int AP>
+struct Tile {
+  static constexpr int EPC = Chunk<T>::n;        // elements per 16-byte chunk
+  static constexpr int NCH = AP / EPC;           // chunks per vector
+  typedef typename Chunk<T>::type CT;
+
+  // Scratch slot of one wave: U vectors, D vectors, U words, D words.
+  static __device__ inline long long vec_bytes(int nint) {
+    return (long long)nint * NCH * SB_WAVE * 16;
+  }
+  static __device__ inline void store_vec(SB_GLOBAL char* region, int slot, int lane, const T* v) {
+    SB_GLOBAL CT* p = (SB_GLOBAL CT*)(region);
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      CT c;
+      if constexpr (EPC == 4) { c.x = v[q * 4]; c.y = v[q * 4 + 1]; c.z = v[q * 4 + 2]; c.w = v[q * 4 + 3]; }
+      else { c.x = v[q * 2]; c.y = v[q * 2 + 1]; }
+      p[((long long)slot * NCH + q) * SB_WAVE + lane] = c;
+    }
+  }
+  static __device__ inline void load_vec(const SB_GLOBAL char* region, int slot, int lane, T* v) {
+    const SB_GLOBAL CT* p = (const SB_GLOBAL CT*)(region);
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      CT c = p[((long long)slot * NCH + q) * SB_WAVE + lane];
+      if constexpr (EPC == 4) { v[q * 4] = c.x; v[q * 4 + 1] = c.y; v[q * 4 + 2] = c.z; v[q * 4 + 3] = c.w; }
+      else { v[q * 2] = c.x; v[q * 2 + 1] = c.y; }
+    }
+  }
+};
+
+// Likelihood vector of a token: one-hot for 0..A-1, ones on real states otherwise.
+template <typename T, int AP>
+__device__ inline void token_vec(int tok, int A, const SB_CONST T* ones, T* u) {
+  const bool obs = (tok >= 0) && (tok < A);
+#pragma unroll
+  for (int a = 0; a < AP; ++a) u[a] = obs ? (a == tok ? T(1) : T(0)) : ones[a];
+}
+
+// w = M u  (w_a = sum_j M[a][j] u_j)
+template <typename T, int AP>
+__device__ inline void matvec(const SB_CONST T* M, const T* u, T* w) {
+#pragma unroll
+  for (int a = 0; a < AP; ++a) {
+    T s = M[a * AP] * u[0];
+#pragma unroll
+    for (int j = 1; j < AP; ++j) s = fma(M[a * AP + j], u[j], s);
+    w[a] = s;
+  }
+}
+
+// x = d^T M  (x_a = sum_i d_i M[i][a])
+template <typename T, int AP>
+__device__ inline void vecmat(const T* d, const SB_CONST T* M, T* x) {
+#pragma unroll
+  for (int a = 0; a < AP; ++a) {
+    T s = d[0] * M[a];
+#pragma unroll
+    for (int i = 1; i < AP; ++i) s = fma(d[i], M[i * AP + a], s);
+    x[a] = s;
+  }
+}
+
+// Scale v by 2^-e, e = frexp exponent of max(v); returns e (0 for a zero vector).
+template <typename T, int AP>
+__device__ inline int rescale(T* v) {
+  T m = v[0];
+#pragma unroll
+  for (int a = 1; a < AP; ++a) m = fmax(m, v[a]);
+  const int e = frexp_exp(m);
+#pragma unroll
+  for (int a = 0; a < AP; ++a) v[a] = ldexp_(v[a], -e);
+  return e;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, SB_WAVE);
+  return v;
+}
+
+// Child of an internal node: its inside vector, cumulative exponent and flags.
+template <typename T, int AP>
+struct Child {
+  T u[AP];
+  int e;
+  bool has;     // subtree contains an ungapped leaf (token 0..A)
+  bool branch;  // internal node with an ungapped leaf under both children
+  bool leaf;
+};
+
+template <typename T, int AP>
+__device__ inline void load_child(Child<T, AP>& ch, int n, int slot, const SB_GLOBAL int8_t* aln,
+                                  long long C, int A, const SB_CONST T* ones,
+                                  const SB_GLOBAL char* Ureg, const SB_GLOBAL int* Uw, int lane) {
+  if (slot < 0) {
+    const int tok = aln[(long long)n * C];
+    token_vec<T, AP>(tok, A, ones, ch.u);
+    ch.e = 0;
+    ch.has = (tok >= 0) && (tok <= A);
+    ch.branch = false;
+    ch.leaf = true;
+  } else {
+    Tile<T, AP>::load_vec(Ureg, slot, lane, ch.u);
+    const int w = Uw[slot * SB_WAVE + lane];
+    ch.e = w >> 2;
+    ch.has = (w & 1) != 0;
+    ch.branch = (w & 2) != 0;
+    ch.leaf = false;
+  }
+}
+
+template <typename T, int AP, bool F81>
+struct Acc {
+  // eigen: c[k][l];  F81: c = term1[i][j], t2a[i], t2b[j], t3
+  T c[AP * AP];
+  T t2a[F81 ? AP : 1], t2b[F81 ? AP : 1], t3;
+  __device__ inline void zero() {
+#pragma unroll
+    for (int i = 0; i < AP * AP; ++i) c[i] = T(0);
+    if constexpr (F81) {
+#pragma unroll
+      for (int i = 0; i < AP; ++i) { t2a[i] = T(0); t2b[i] = T(0); }
+    }
+    t3 = T(0);
+  }
+};
+
+}  // namespace
+
+// Branch parameters in FP64 (subby/jax/diagonalize.py:141-172, eigensub.py:7-50,
+// f81_fast.py:56-66), one workgroup of 64 threads per (block, node) record.
+template <typename T>
+__global__ void __launch_bounds__(64) sb_branch_params_kernel(const SbLaunch* __restrict__ L) {
+  const int AP = L->AP;
+  const int njobs = L->njobs;
+  SB_GLOBAL T* out = glb<T>(L->bt);
+  __shared__ double mu[64], E[64], sq[64];
+  for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+  const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
+  const SB_CONST double* m = cst<double>(job->model64);
+  const double t = job->t;
+  const int k = threadIdx.x;
+  __syncthreads();
+  if (k < AP) {
+    mu[k] = m[sb_m64_mu(AP) + k];
+    E[k] = exp(mu[k] * t);
+    sq[k] = sqrt(m[sb_m64_pi(AP) + k]);
+  }
+  __syncthreads();
+  SB_GLOBAL T* o = out + (long long)jb * sb_bt_stride(AP);
+  const SB_CONST double* V = m + sb_m64_V(AP);
+  for (int idx = threadIdx.x; idx < AP * AP; idx += 64) {
+    const int i = idx / AP, j = idx - (idx / AP) * AP;
+    double s = 0.0;
+    for (int q = 0; q < AP; ++q) s = fma(V[i * AP + q] * E[q], V[j * AP + q], s);
+    o[sb_bt_M(AP) + idx] = T(s * sq[j] / sq[i]);
+    const double d = mu[i] - mu[j];
+    double J;
+    if (fabs(d) < 1e-8) J = t * E[i];
+    else J = E[j] * expm1(d * t) / d;
+    o[sb_bt_J(AP) + idx] = T(J);
+  }
+  if (k < AP) {
+    o[sb_bt_E(AP) + k] = T(E[k]);
+    o[sb_bt_dE(AP) + k] = T(mu[k] * E[k]);
+  }
+  if (k == 0) {
+    const double fm = m[sb_m64_misc(AP)];
+    const double e = exp(-fm * t);
+    const double p = -expm1(-fm * t);
+    o[sb_bt_coef(AP) + 0] = T(t * e);
+    o[sb_bt_coef(AP) + 1] = T(p / fm - t * e);
+    o[sb_bt_coef(AP) + 2] = T(t * (1.0 + e) - 2.0 * p / fm);
+    o[sb_bt_coef(AP) + 3] = T(t);
+  }
+  }
+}
+
+__global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *L->counter = 0;
+}
+
+template <typename T, int AP, bool F81, bool GRAD>
+__global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
+  typedef Tile<T, AP> TL;
+  const SbLaunch& args = *Lp;
+  const int lane = threadIdx.x & (SB_WAVE - 1);
+  const int wpb = blockDim.x / SB_WAVE;
+  const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / SB_WAVE);
+  SB_GLOBAL char* wscr = glb<char>(args.scratch) + (long long)gw * args.wave_stride;
+  const int NI = args.nint_max;
+  SB_GLOBAL char* Ureg = wscr;
+  SB_GLOBAL char* Dreg = Ureg + TL::vec_bytes(NI);
+  SB_GLOBAL int* Uw = (SB_GLOBAL int*)(Dreg + TL::vec_bytes(NI));
+  SB_GLOBAL int* Dw = Uw + NI * SB_WAVE;
+
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(args.counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if (t >= args.ntiles) break;
+    const SB_CONST int* tbp = cst<int>(args.tiles) + 2 * t;
+    const int2 tb = make_int2(tbp[0], tbp[1]);
+    const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(args.blocks) + tb.x;
+    const long long C = B->C;
+    const int A = B->A;
+    const int flags = B->flags;
+    const int nint = B->nint;
+    const int c = tb.y * SB_WAVE + lane;
+    const bool valid = c < C;
+    const long long cc = valid ? c : C - 1;
+    const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln) + cc;
+    const SB_CONST T* bt = cst<T>(B->bt);
+    const SB_CONST T* md = cst<T>(B->model);
+    const SB_CONST int* sched = cst<int>(B->sched);
+    const SB_CONST T* ones = md + sb_mr_ones(AP);
+    const SB_CONST T* pi = md + sb_mr_pi(AP);
+    const int BTS = sb_bt_stride(AP);
+
+    // ---------------- inside pass ----------------
+    T u0[AP];
+    int e0 = 0;
+    bool root_branch = false;
+    if (nint == 0) token_vec<T, AP>(aln[0], A, ones, u0);  // single-node tree
+    for (int i = nint - 1; i >= 0; --i) {
+      const SB_CONST int* rec = sched + i * 8;
+      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
+      Child<T, AP> cl, cr;
+      load_child<T, AP>(cl, l, sl, aln, C, A, ones, Ureg, Uw, lane);
+      load_child<T, AP>(cr, r, sr, aln, C, A, ones, Ureg, Uw, lane);
+      T wl[AP], wr[AP], up[AP];
+      matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), cl.u, wl);
+      matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), cr.u, wr);
+      token_vec<T, AP>(aln[(long long)p * C], A, ones, up);
+#pragma unroll
+      for (int a = 0; a < AP; ++a) up[a] = up[a] * wl[a] * wr[a];
+      const int e = rescale<T, AP>(up);
+      const int ep = cl.e + cr.e + e;
+      const bool has = cl.has || cr.has;
+      const bool br = cl.has && cr.has;
+      if (i > 0) {
+        TL::store_vec(Ureg, i, lane, up);
+        Uw[i * SB_WAVE + lane] = (ep << 2) | (br ? 2 : 0) | (has ? 1 : 0);
+      } else {
+#pragma unroll
+        for (int a = 0; a < AP; ++a) u0[a] = up[a];
+        e0 = ep;
+        root_branch = br;
+      }
+    }
+    // log-likelihood and root posterior (pruning.py:84-87; __init__.py:642-650)
+    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    double s0d = 0.0;
+#pragma unroll
+    for (int a = 0; a < AP; ++a)
+      if (a < A) s0d = fma(pi64[a], double(u0[a]), s0d);
+    const double ll = double(e0) * 0.69314718055994530942 + log(s0d);
+    const T inv_s0 = T(1.0 / s0d);
+    if (valid && (flags & SB_F_LL) && B->loglike) glb<double>(B->loglike)[c] = ll;
+    if (valid && (flags & SB_F_ROOT) && B->root) {
+      SB_GLOBAL T* root = glb<T>(B->root);
+#pragma unroll
+      for (int a = 0; a < AP; ++a)
+        if (a < A) root[(long long)a * C + c] = pi[a] * u0[a] * inv_s0;
+    }
+    if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
+
+    // ---------------- outside pass + accumulation ----------------
+    const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
+    const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) + cc : nullptr;
+    const SB_CONST T* Vs = md + sb_mr_Vs(AP);
+    const SB_CONST T* Vi = md + sb_mr_Vi(AP);
+    const double gc = GRAD ? ((B->g && valid) ? glb<double>(B->g)[cc] : (valid ? 1.0 : 0.0)) : 0.0;
+    SB_GLOBAL double* gpart =
+        GRAD ? glb<double>(args.grad_part) + (long long)(B->tile_base + tb.y) * args.grad_stride : nullptr;
+
+    Acc<T, AP, F81> acc;
+    acc.zero();
+
+    for (int i = 0; i < nint; ++i) {
+      const SB_CONST int* rec = sched + i * 8;
+      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
+      T x[AP];
+      int ex;
+      bool stp;
+      token_vec<T, AP>(aln[(long long)p * C], A, ones, x);
+      if (i == 0) {
+#pragma unroll
+        for (int a = 0; a < AP; ++a) x[a] *= pi[a];
+        ex = 0;
+        stp = root_branch;
+      } else {
+        T dp[AP], xm[AP];
+        TL::load_vec(Dreg, i, lane, dp);
+        const int w = Dw[i * SB_WAVE + lane];
+        ex = w >> 1;
+        stp = (w & 1) != 0;
+        vecmat<T, AP>(dp, bt + (long long)p * BTS + sb_bt_M(AP), xm);
+#pragma unroll
+        for (int a = 0; a < AP; ++a) x[a] *= xm[a];
+      }
+      Child<T, AP> ch[2];
+      load_child<T, AP>(ch[0], l, sl, aln, C, A, ones, Ureg, Uw, lane);
+      load_child<T, AP>(ch[1], r, sr, aln, C, A, ones, Ureg, Uw, lane);
+      T w[2][AP];
+      matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
+      matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int n = s == 0 ? l : r;
+        const int sn = s == 0 ? sl : sr;
+        T d[AP];
+#pragma unroll
+        for (int a = 0; a < AP; ++a) d[a] = w[1 - s][a] * x[a];
+        const int ed = ch[1 - s].e + ex + rescale<T, AP>(d);
+        // Steiner membership (oracle.py:1398-1431)
+        const bool st = (ch[s].leaf ? ch[s].has : ch[s].branch) || (stp && ch[s].has);
+        if (sn >= 0) {
+          TL::store_vec(Dreg, sn, lane, d);
+          Dw[sn * SB_WAVE + lane] = (ed << 1) | (st ? 1 : 0);
+        }
+        bool active;
+        if (GRAD || mask_mode == SB_MASK_NONE) active = true;
+        else if (mask_mode == SB_MASK_AUTO) active = st && stp;
+        else active = emask[(long long)n * C] != 0;
+        const T sc_raw = ldexp_(inv_s0, ed + ch[s].e - e0);
+        const T sc = active ? sc_raw : T(0);
+        const SB_CONST T* bn = bt + (long long)n * BTS;
+        const T* u = ch[s].u;
+        if constexpr (F81) {
+          const T al = bn[sb_bt_coef(AP) + 0], be = bn[sb_bt_coef(AP) + 1], ga = bn[sb_bt_coef(AP) + 2];
+          T Ds[AP];
+          T piU = T(0), Dsum = T(0);
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            Ds[a] = d[a] * sc;
+            piU = fma(pi[a], u[a], piU);
+            Dsum += Ds[a];
+          }
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            const T ad = al * Ds[a];
+#pragma unroll
+            for (int b = 0; b < AP; ++b) acc.c[a * AP + b] = fma(ad, u[b], acc.c[a * AP + b]);
+            acc.t2a[a] = fma(be * Ds[a], piU, acc.t2a[a]);
+            acc.t2b[a] = fma(be * Dsum, u[a], acc.t2b[a]);
+          }
+          acc.t3 = fma(ga * Dsum, piU, acc.t3);
+        } else {
+          T Ut[AP], Dt[AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            T su = T(0), sd = T(0);
+#pragma unroll
+            for (int b = 0; b < AP; ++b) {
+              su = fma(u[b], Vs[b * AP + k], su);
+              sd = fma(d[b], Vi[b * AP + k], sd);
+            }
+            Ut[k] = su;
+            Dt[k] = sd;
+          }
+          const SB_CONST T* J = bn + sb_bt_J(AP);
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            const T dk = Dt[k] * sc;
+#pragma unroll
+            for (int q = 0; q < AP; ++q) acc.c[k * AP + q] = fma(dk * J[k * AP + q], Ut[q], acc.c[k * AP + q]);
+          }
+          if constexpr (GRAD) {
+            // d logL / d t_n = scale * sum_k D~_k mu_k e^{mu_k t} U~_k  (vjp.py:92-117)
+            const SB_CONST T* dE = bn + sb_bt_dE(AP);
+            T s = T(0);
+#pragma unroll
+            for (int k = 0; k < AP; ++k) s = fma(Dt[k] * dE[k], Ut[k], s);
+            const double v = wave_sum(gc * double(s) * double(sc_raw));
+            if (lane == 0) gpart[n] = v;
+          }
+        }
+      }
+    }
+
+    // ---------------- epilogue: natural-basis counts ----------------
+    T cnt[AP * AP];
+    if constexpr (F81) {
+      const T fmu = md[sb_mr_misc(AP)];
+#pragma unroll
+      for (int i = 0; i < AP; ++i)
+#pragma unroll
+        for (int j = 0; j < AP; ++j) {
+          const T I = acc.c[i * AP + j] + acc.t2a[i] + pi[i] * (acc.t2b[j] + acc.t3);
+          cnt[i * AP + j] = (i == j) ? I : fmu * pi[j] * I;
+        }
+    } else {
+      const SB_CONST T* V = md + sb_mr_V(AP);
+      const SB_CONST T* S = md + sb_mr_S(AP);
+      T tmp[AP * AP];
+#pragma unroll
+      for (int i = 0; i < AP; ++i)
+#pragma unroll
+        for (int q = 0; q < AP; ++q) {
+          T s = T(0);
+#pragma unroll
+          for (int k = 0; k < AP; ++k) s = fma(V[i * AP + k], acc.c[k * AP + q], s);
+          tmp[i * AP + q] = s;
+        }
+#pragma unroll
+      for (int i = 0; i < AP; ++i)
+#pragma unroll
+        for (int j = 0; j < AP; ++j) {
+          T s = T(0);
+#pragma unroll
+          for (int q = 0; q < AP; ++q) s = fma(tmp[i * AP + q], V[j * AP + q], s);
+          cnt[i * AP + j] = (i == j) ? s : S[i * AP + j] * s;
+        }
+    }
+    if (valid && (flags & SB_F_COUNTS) && B->counts) {
+      SB_GLOBAL T* out = glb<T>(B->counts);
+#pragma unroll
+      for (int i = 0; i < AP; ++i)
+#pragma unroll
+        for (int j = 0; j < AP; ++j)
+          if (i < A && j < A) out[((long long)i * A + j) * C + c] = cnt[i * AP + j];
+    }
+    if constexpr (GRAD) {
+      // Rate-parameter gradients by the Fisher identity: sum_ij W_ij N_ij.
+      SB_GLOBAL double* rpart = glb<double>(args.rate_part) + (long long)(B->tile_base + tb.y) * SB_MAX_RATE;
+      for (int q = 0; q < B->n_rate; ++q) {
+        const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < AP * AP; ++i) s = fma(W[i], double(cnt[i]), s);
+        const double v = wave_sum(gc * s);
+        if (lane == 0) rpart[q] = v;
+      }
+    }
+  }
+}
+
+// Deterministic reduction of per-tile gradient partials, one thread per
+// (block, branch) / (block, rate parameter).
+__global__ void sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
+  const SB_GLOBAL double* grad_part = glb<double>(L->grad_part);
+  const SB_GLOBAL double* rate_part = glb<double>(L->rate_part);
+  const int grad_stride = L->grad_stride;
+  for (int b = blockIdx.x; b < L->nblocks; b += gridDim.x) {
+  const SbBlockDesc* B = L->blocks + b;
+  if (!(B->flags & SB_F_GRAD)) continue;
+  for (int n = threadIdx.x; n < B->R; n += blockDim.x) {
+    double s = 0.0;
+    if (n > 0)
+      for (int t = 0; t < B->ntiles; ++t) s += grad_part[(long long)(B->tile_base + t) * grad_stride + n];
+    if (B->dist_grad) B->dist_grad[n] = s;
+  }
+  for (int q = threadIdx.x; q < B->n_rate; q += blockDim.x) {
+    double s = 0.0;
+    for (int t = 0; t < B->ntiles; ++t) s += rate_part[(long long)(B->tile_base + t) * SB_MAX_RATE + q];
+    if (B->rate_grad) B->rate_grad[q] = s;
+  }
+  }
+}
+
+// Explicit instantiations and host-visible launch helpers.
+#define SB_INST(T, AP, F81, GRAD) \
+  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD>(const SbLaunch*);
+SB_INST(float, 4, false, false)
+SB_INST(float, 4, true, false)
+SB_INST(float, 4, false, true)
+SB_INST(float, 8, false, false)
+SB_INST(float, 8, true, false)
+SB_INST(float, 8, false, true)
+SB_INST(double, 4, false, false)
+SB_INST(double, 4, true, false)
+SB_INST(double, 4, false, true)
+SB_INST(double, 8, false, false)
+SB_INST(double, 8, true, false)
+SB_INST(double, 8, false, true)
+template __global__ void sb_branch_params_kernel<float>(const SbLaunch*);
+template __global__ void sb_branch_params_kernel<double>(const SbLaunch*);
+
+extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
+                                            dim3 block, const SbLaunch* L, hipStream_t s) {
+  void (*k)(const SbLaunch*) = nullptr;
+#define SB_PICK(T, APV)                                                          \
+  if (AP == APV) {                                                               \
+    if (grad) k = sb_fused_small_kernel<T, APV, false, true>;                    \
+    else if (f81) k = sb_fused_small_kernel<T, APV, true, false>;                \
+    else k = sb_fused_small_kernel<T, APV, false, false>;                        \
+  }
+  if (dtype == 0) { SB_PICK(float, 4) SB_PICK(float, 8) }
+  else { SB_PICK(double, 4) SB_PICK(double, 8) }
+#undef SB_PICK
+  if (!k) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, grid, block, 0, s, L);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s) {
+  if (dtype == 0)
+    hipLaunchKernelGGL(sb_branch_params_kernel<float>, dim3(nwg), dim3(64), 0, s, L);
+  else
+    hipLaunchKernelGGL(sb_branch_params_kernel<double>, dim3(nwg), dim3(64), 0, s, L);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s) {
+  hipLaunchKernelGGL(sb_reset_kernel, dim3(1), dim3(64), 0, s, L);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s) {
+  hipLaunchKernelGGL(sb_grad_reduce_kernel, dim3(nwg), dim3(256), 0, s, L);
+  return hipGetLastError();
+}

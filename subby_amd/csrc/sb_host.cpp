@@ -1,0 +1,824 @@
+// Host side of the C ABI (include/subby_amd.h): context, resident model
+// table, tree planner and the batch launcher.
+//
+// The planner replaces the per-call Python/JAX preprocessing of the reference
+// (children_of, subby/jax/_utils.py:46-75; compute_sub_matrices,
+// diagonalize.py:141-172; compute_J, eigensub.py:7-50): tree tables are built
+// here in C++, and M(t), J(t) are evaluated in FP64 on the device by
+// sb_branch_params_kernel from the resident eigensystem.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/subby_amd.h"
+#include "sb_plan.h"
+
+extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
+                                            dim3 block, const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
+                                      int nwaves, const SbLaunch* L, hipStream_t s);
+extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max);
+extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
+struct SbPlainBuffers {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+extern "C" hipError_t sb_plain_run(const int8_t* aln, const int32_t* parent, int R, int C,
+                                   const float* d_bt, const float* d_model, int mask_mode,
+                                   float* counts, float* root, double* ll, SbPlainBuffers* buf,
+                                   hipStream_t s);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define SB_HIP(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(SB_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+int pad_alphabet(int A) {
+  static const int sizes[] = {4, 8, 16, 20, 32, 64};
+  for (int s : sizes)
+    if (A <= s) return s;
+  return -1;
+}
+
+bool is_small(int AP) { return AP <= 8; }
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct Model {
+  bool alive = false;
+  int A = 0, AP = 0;
+  std::vector<double> mu, V, pi;   // host copies (plain path)
+  double* d64 = nullptr;  // FP64 master
+  void* rec[2] = {nullptr, nullptr};  // float / double records
+};
+
+}  // namespace
+
+struct sb_context {
+  int device = 0;
+  int num_cus = 256;
+  std::vector<Model> models;
+  std::mutex mu;
+  sb_batch* internal = nullptr;
+  SbPlainBuffers plain;
+  DevBuf plain_bt;
+  std::vector<float> plain_bt_host;
+  std::vector<double> plain_ll_scratch;
+};
+
+struct GraphKey {
+  int dtype, AP, f81, grad, large;
+  bool operator==(const GraphKey& o) const {
+    return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large;
+  }
+};
+
+struct sb_batch {
+  sb_context* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  SbLaunch* header = nullptr;       // fixed device address
+  DevBuf plan, bt, scratch, part, stage;
+  int* counter = nullptr;
+  char* host = nullptr;             // pinned staging: header + plan
+  size_t host_cap = 0;
+  hipEvent_t staged = nullptr;      // host staging may be rewritten after this
+  hipEvent_t done = nullptr;        // last submission finished
+  bool staged_pending = false;
+  int waves_per_group = 4;
+  int waves_per_cu = 8;
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+};
+
+namespace {
+
+// Grow a device buffer; waits for the launcher's stream before freeing the old one.
+int ensure(sb_batch* b, DevBuf& buf, size_t bytes) {
+  if (bytes <= buf.cap) return SB_OK;
+  SB_HIP(hipStreamSynchronize(b->stream));
+  if (buf.p) SB_HIP(hipFree(buf.p));
+  buf.p = nullptr;
+  buf.cap = 0;
+  size_t want = std::max(bytes, buf.cap * 3 / 2);
+  want = (want + 4095) & ~size_t(4095);
+  if (hipMalloc(&buf.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    buf.p = nullptr;
+    return fail(SB_ERR_OOM, "device allocation of %zu bytes failed", want);
+  }
+  buf.cap = want;
+  return SB_OK;
+}
+
+int ensure_host(sb_batch* b, size_t bytes) {
+  if (b->staged_pending) {
+    SB_HIP(hipEventSynchronize(b->staged));
+    b->staged_pending = false;
+  }
+  if (bytes <= b->host_cap) return SB_OK;
+  if (b->host) SB_HIP(hipHostFree(b->host));
+  b->host = nullptr;
+  size_t want = std::max(bytes, b->host_cap * 3 / 2);
+  want = (want + 4095) & ~size_t(4095);
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->host), want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    b->host = nullptr;
+    b->host_cap = 0;
+    return fail(SB_ERR_OOM, "pinned allocation of %zu bytes failed", want);
+  }
+  b->host_cap = want;
+  return SB_OK;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct TreePlan {
+  int nint = 0;
+  std::vector<int> recs;  // nint * 8
+};
+
+int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
+  if (R < 1) return fail(SB_ERR_TREE, "tree has %d nodes", R);
+  if (parent[0] != -1) return fail(SB_ERR_TREE, "parentIndex[0] must be -1, got %d", parent[0]);
+  std::vector<int> first(R, -1), second(R, -1);
+  for (int n = 1; n < R; ++n) {
+    const int p = parent[n];
+    if (p < 0 || p >= n)
+      return fail(SB_ERR_TREE, "parentIndex[%d]=%d is not a preorder parent (need 0 <= p < n)", n, p);
+    if (first[p] < 0) first[p] = n;
+    else if (second[p] < 0) second[p] = n;
+    else return fail(SB_ERR_TREE, "Node %d has more than 2 children; tree must be binary", p);
+  }
+  std::vector<int> slot(R, -1);
+  int nint = 0;
+  for (int n = 0; n < R; ++n) {
+    if (first[n] >= 0 && second[n] < 0)
+      return fail(SB_ERR_TREE, "Node %d has 1 child; expected 0 or 2", n);
+    if (first[n] >= 0) slot[n] = nint++;
+  }
+  tp.nint = nint;
+  tp.recs.assign(size_t(nint) * 8, 0);
+  for (int n = 0; n < R; ++n) {
+    if (slot[n] < 0) continue;
+    int* r = &tp.recs[size_t(slot[n]) * 8];
+    r[0] = n;
+    r[1] = first[n];
+    r[2] = second[n];
+    r[4] = slot[first[n]];
+    r[5] = slot[second[n]];
+  }
+  return SB_OK;
+}
+
+Model* get_model(sb_context* ctx, int id) {
+  if (id < 0 || id >= int(ctx->models.size()) || !ctx->models[id].alive) return nullptr;
+  return &ctx->models[id];
+}
+
+struct BlockIn {
+  sb_block blk;
+  int flags;  // SB_F_*
+  const double* g;
+  double* dist_grad;
+  double* rate_grad;
+  int n_rate;
+  const double* rate_w_host;  // [n_rate][A*A]
+};
+
+int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
+              dim3 block, int nwaves, int nwg_reduce);
+
+int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves, int nwg_params,
+            int nwg_reduce) {
+  hipStream_t s = b->stream;
+  SB_HIP(sb_launch_branch_params(key.dtype, nwg_params, b->header, s));
+  SB_HIP(sb_launch_reset(b->header, s));
+  if (key.large)
+    SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
+                           b->header, s));
+  else
+    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, grid, block, b->header, s));
+  if (key.grad) SB_HIP(sb_launch_grad_reduce(nwg_reduce, b->header, s));
+  return SB_OK;
+}
+
+int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
+              dim3 block, int nwaves, int nwg_reduce) {
+  for (auto& g : b->graphs)
+    if (g.first == key) {
+      *out = g.second;
+      return SB_OK;
+    }
+  hipStream_t cap = nullptr;
+  SB_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipStream_t saved = b->stream;
+  b->stream = cap;
+  SB_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue(b, key, grid, block, nwaves, nwg_params, nwg_reduce);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(cap, &graph);
+  b->stream = saved;
+  (void)hipStreamDestroy(cap);
+  if (rc != SB_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SB_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(SB_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  b->graphs.push_back({key, exec});
+  *out = exec;
+  return SB_OK;
+}
+
+// Plan, upload and launch a list of blocks on the launcher's stream.
+int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph) {
+  sb_context* ctx = b->ctx;
+  if (dtype != SB_F32 && dtype != SB_F64) return fail(SB_ERR_ARG, "dtype must be SB_F32 or SB_F64");
+  const size_t esz = dtype == SB_F32 ? 4 : 8;
+  const int nb = int(in.size());
+  if (nb == 0) return SB_OK;
+
+  // ---- validate and plan every block
+  int AP = -1;
+  bool f81 = false, grad = false;
+  std::vector<TreePlan> plans(nb);
+  std::vector<const Model*> mods(nb);
+  int nint_max = 0, R_max = 1;
+  size_t nsched = 0, njobs = 0, ntiles = 0, nrate = 0;
+  for (int i = 0; i < nb; ++i) {
+    const sb_block& k = in[i].blk;
+    const Model* m = get_model(ctx, k.model_id);
+    if (!m) return fail(SB_ERR_MODEL, "block %d: unknown model id %d", i, k.model_id);
+    if (AP < 0) AP = m->AP;
+    if (m->AP != AP)
+      return fail(SB_ERR_UNSUPPORTED, "block %d: padded alphabet %d differs from %d in one batch", i,
+                  m->AP, AP);
+    if (k.R < 1 || k.C < 0) return fail(SB_ERR_ARG, "block %d: bad sizes R=%d C=%d", i, k.R, k.C);
+    if (!k.parent_index || !k.distances) return fail(SB_ERR_ARG, "block %d: null tree arrays", i);
+    if (k.C > 0 && !k.d_alignment) return fail(SB_ERR_ARG, "block %d: null alignment", i);
+    for (int n = 0; n < k.R; ++n)
+      if (!std::isfinite(k.distances[n]))
+        return fail(SB_ERR_ARG, "block %d: distanceToParent[%d] is not finite", i, n);
+    const int mm = k.mask_mode;
+    if (mm < SB_MASK_NONE || mm > SB_MASK_EXPLICIT) return fail(SB_ERR_ARG, "block %d: bad mask_mode", i);
+    if (mm == SB_MASK_EXPLICIT && !k.d_mask && k.C > 0)
+      return fail(SB_ERR_ARG, "block %d: explicit mask without mask buffer", i);
+    if (in[i].flags & SB_F_GRAD) grad = true;
+    if (k.f81_fast) f81 = true;
+    int rc = plan_tree(k.parent_index, k.R, plans[i]);
+    if (rc != SB_OK) {
+      std::string msg = g_err;
+      return fail(rc, "block %d: %s", i, msg.c_str());
+    }
+    mods[i] = m;
+    nint_max = std::max(nint_max, plans[i].nint);
+    R_max = std::max(R_max, k.R);
+    nsched += plans[i].recs.size();
+    njobs += k.R;
+    ntiles += (size_t(k.C) + SB_WAVE - 1) / SB_WAVE;
+    nrate += size_t(in[i].n_rate) * AP * AP;
+  }
+  for (int i = 0; i < nb; ++i) {
+    if ((in[i].flags & SB_F_GRAD) == 0 && grad)
+      return fail(SB_ERR_UNSUPPORTED, "gradient and non-gradient blocks cannot share a batch");
+    if (bool(in[i].blk.f81_fast) != f81)
+      return fail(SB_ERR_UNSUPPORTED, "F81 and eigensub blocks cannot share a batch");
+  }
+  if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
+  const bool large = !is_small(AP);
+
+  // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights
+  size_t off = 0;
+  const size_t o_hdr = off;
+  off = align_up(off + sizeof(SbLaunch), 256);
+  const size_t o_plan = off;
+  const size_t o_blocks = off;
+  off = align_up(off + sizeof(SbBlockDesc) * nb, 256);
+  const size_t o_sched = off;
+  off = align_up(off + sizeof(int) * nsched, 256);
+  const size_t o_tiles = off;
+  off = align_up(off + sizeof(int2) * ntiles, 256);
+  const size_t o_jobs = off;
+  off = align_up(off + sizeof(SbBranchJob) * njobs, 256);
+  const size_t o_rate = off;
+  off = align_up(off + sizeof(double) * nrate, 256);
+  const size_t total = off;
+  const size_t plan_bytes = total - o_plan;
+
+  int rc = ensure_host(b, total);
+  if (rc) return rc;
+  rc = ensure(b, b->plan, plan_bytes);
+  if (rc) return rc;
+  const int bts = sb_bt_stride(AP);
+  rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
+  if (rc) return rc;
+
+  // persistent grid
+  const int wpg = b->waves_per_group;
+  int nwaves = ctx->num_cus * b->waves_per_cu;
+  nwaves = (nwaves + wpg - 1) / wpg * wpg;
+  long long wave_stride;
+  if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max);
+  else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
+  wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
+  rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);
+  if (rc) return rc;
+  const int grad_stride = R_max;
+  if (grad) {
+    rc = ensure(b, b->part, std::max<size_t>(1, ntiles) * (grad_stride + SB_MAX_RATE) * sizeof(double));
+    if (rc) return rc;
+  }
+
+  char* h = b->host;
+  char* dplan = static_cast<char*>(b->plan.p);
+  auto dev = [&](size_t o) { return dplan + (o - o_plan); };
+  SbBlockDesc* hb = reinterpret_cast<SbBlockDesc*>(h + o_blocks);
+  int* hs = reinterpret_cast<int*>(h + o_sched);
+  int2* ht = reinterpret_cast<int2*>(h + o_tiles);
+  SbBranchJob* hj = reinterpret_cast<SbBranchJob*>(h + o_jobs);
+  double* hr = reinterpret_cast<double*>(h + o_rate);
+
+  size_t ps = 0, pj = 0, pr = 0, tb = 0;
+  std::vector<std::pair<int, int>> order;  // (nint, block) for tile ordering
+  for (int i = 0; i < nb; ++i) {
+    const sb_block& k = in[i].blk;
+    const Model* m = mods[i];
+    SbBlockDesc& d = hb[i];
+    std::memset(&d, 0, sizeof d);
+    d.aln = k.d_alignment;
+    d.mask = k.d_mask;
+    d.counts = k.d_counts;
+    d.root = k.d_root;
+    d.loglike = k.d_loglike;
+    d.sched = reinterpret_cast<const int*>(dev(o_sched) + ps * sizeof(int));
+    d.bt = static_cast<char*>(b->bt.p) + pj * bts * esz;
+    d.model = m->rec[dtype];
+    d.model64 = m->d64;
+    d.g = in[i].g;
+    d.dist_grad = in[i].dist_grad;
+    d.rate_grad = in[i].rate_grad;
+    d.n_rate = in[i].n_rate;
+    d.rate_w = reinterpret_cast<const double*>(dev(o_rate) + pr * sizeof(double));
+    d.R = k.R;
+    d.C = k.C;
+    d.nint = plans[i].nint;
+    int fl = in[i].flags;
+    if (k.f81_fast) fl |= SB_F_F81;
+    fl |= (k.mask_mode & 3) << SB_F_MASK_SHIFT;
+    d.flags = fl;
+    d.A = m->A;
+    d.tile_base = int(tb);
+    d.ntiles = int((size_t(k.C) + SB_WAVE - 1) / SB_WAVE);
+    std::memcpy(hs + ps, plans[i].recs.data(), plans[i].recs.size() * sizeof(int));
+    ps += plans[i].recs.size();
+    for (int n = 0; n < k.R; ++n) {
+      hj[pj + n].model64 = m->d64;
+      hj[pj + n].t = k.distances[n];
+    }
+    pj += k.R;
+    for (int q = 0; q < in[i].n_rate; ++q)
+      for (int a = 0; a < AP; ++a)
+        for (int c = 0; c < AP; ++c)
+          hr[pr + size_t(q) * AP * AP + a * AP + c] =
+              (a < m->A && c < m->A) ? in[i].rate_w_host[size_t(q) * m->A * m->A + a * m->A + c] : 0.0;
+    pr += size_t(in[i].n_rate) * AP * AP;
+    tb += d.ntiles;
+    order.push_back({plans[i].nint, i});
+  }
+  // tiles: most expensive trees first (longest-processing-time order)
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first > y.first; });
+  size_t pt = 0;
+  for (auto& o : order) {
+    const int i = o.second;
+    for (int t = 0; t < hb[i].ntiles; ++t) ht[pt++] = make_int2(i, t);
+  }
+
+  SbLaunch* L = reinterpret_cast<SbLaunch*>(h + o_hdr);
+  std::memset(L, 0, sizeof *L);
+  L->blocks = reinterpret_cast<const SbBlockDesc*>(dev(o_blocks));
+  L->tiles = reinterpret_cast<const int2*>(dev(o_tiles));
+  L->jobs = reinterpret_cast<const SbBranchJob*>(dev(o_jobs));
+  L->bt = b->bt.p;
+  L->counter = b->counter;
+  L->scratch = static_cast<char*>(b->scratch.p);
+  L->grad_part = grad ? static_cast<double*>(b->part.p) : nullptr;
+  L->rate_part = grad ? static_cast<double*>(b->part.p) + ntiles * grad_stride : nullptr;
+  L->stage = b->stage.p;
+  L->wave_stride = wave_stride;
+  L->nblocks = nb;
+  L->ntiles = int(ntiles);
+  L->njobs = int(njobs);
+  L->nint_max = nint_max;
+  L->grad_stride = grad_stride;
+  L->AP = AP;
+
+  hipStream_t s = b->stream;
+  SB_HIP(hipMemcpyAsync(b->header, h + o_hdr, sizeof(SbLaunch), hipMemcpyHostToDevice, s));
+  SB_HIP(hipMemcpyAsync(b->plan.p, h + o_plan, plan_bytes, hipMemcpyHostToDevice, s));
+  SB_HIP(hipEventRecord(b->staged, s));
+  b->staged_pending = true;
+
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, large ? 1 : 0};
+  const dim3 block(64 * wpg);
+  const dim3 grid(nwaves / wpg);
+  const int nwg_params = 1024;
+  const int nwg_reduce = 64;
+  if (use_graph) {
+    hipGraphExec_t exec = nullptr;
+    rc = get_graph(b, key, &exec, nwg_params, grid, block, nwaves, nwg_reduce);
+    if (rc) return rc;
+    SB_HIP(hipGraphLaunch(exec, s));
+  } else {
+    rc = enqueue(b, key, grid, block, nwaves, nwg_params, nwg_reduce);
+    if (rc) return rc;
+  }
+  SB_HIP(hipEventRecord(b->done, s));
+  return SB_OK;
+}
+
+int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
+  sb_batch* b = new sb_batch();
+  b->ctx = ctx;
+  b->stream = stream;
+  if (hipMalloc(reinterpret_cast<void**>(&b->header), sizeof(SbLaunch)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&b->counter), 256) != hipSuccess) {
+    (void)hipGetLastError();
+    delete b;
+    return fail(SB_ERR_OOM, "launcher allocation failed");
+  }
+  SB_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+  SB_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  *out = b;
+  return SB_OK;
+}
+
+void batch_free(sb_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage})
+    if (d->p) (void)hipFree(d->p);
+  if (b->header) (void)hipFree(b->header);
+  if (b->counter) (void)hipFree(b->counter);
+  if (b->host) (void)hipHostFree(b->host);
+  if (b->staged) (void)hipEventDestroy(b->staged);
+  if (b->done) (void)hipEventDestroy(b->done);
+  delete b;
+}
+
+// Single-block calls go through the context's own launcher; if the caller
+// switches streams, the new stream first waits for the launcher's last work.
+int single(sb_context* ctx, const BlockIn& bi, int dtype, void* stream) {
+  if (!ctx) return fail(SB_ERR_ARG, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!ctx->internal) {
+    int rc = batch_create(ctx, s, &ctx->internal);
+    if (rc) return rc;
+  }
+  sb_batch* b = ctx->internal;
+  if (b->stream != s) {
+    SB_HIP(hipStreamWaitEvent(s, b->done, 0));
+    b->stream = s;
+  }
+  std::vector<BlockIn> v(1, bi);
+  return submit(b, v, dtype, 0);
+}
+
+BlockIn make_block(int model_id, const int8_t* aln, const int32_t* parent, const double* dist, int R,
+                   int C) {
+  BlockIn bi;
+  std::memset(&bi, 0, sizeof bi);
+  bi.blk.model_id = model_id;
+  bi.blk.R = R;
+  bi.blk.C = C;
+  bi.blk.d_alignment = aln;
+  bi.blk.parent_index = parent;
+  bi.blk.distances = dist;
+  return bi;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sb_last_error(void) { return g_err.c_str(); }
+int sb_version(void) { return 1; }
+
+int sb_create(sb_context** out, int device) {
+  if (!out) return fail(SB_ERR_ARG, "null output pointer");
+  int n = 0;
+  SB_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SB_ERR_ARG, "device %d not present (%d devices)", device, n);
+  SB_HIP(hipSetDevice(device));
+  sb_context* ctx = new sb_context();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  *out = ctx;
+  return SB_OK;
+}
+
+int sb_destroy(sb_context* ctx) {
+  if (!ctx) return SB_OK;
+  (void)hipSetDevice(ctx->device);
+  batch_free(ctx->internal);
+  if (ctx->plain.p) (void)hipFree(ctx->plain.p);
+  if (ctx->plain_bt.p) (void)hipFree(ctx->plain_bt.p);
+  for (auto& m : ctx->models) {
+    if (m.d64) (void)hipFree(m.d64);
+    for (void* r : m.rec)
+      if (r) (void)hipFree(r);
+  }
+  delete ctx;
+  return SB_OK;
+}
+
+int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double* eigenvectors,
+                 const double* pi, int* model_id) {
+  if (!ctx || !eigenvalues || !eigenvectors || !pi || !model_id) return fail(SB_ERR_ARG, "null argument");
+  const int AP = pad_alphabet(A);
+  if (A < 2 || AP < 0) return fail(SB_ERR_ALPHABET, "alphabet size %d outside 2..64", A);
+  double psum = 0.0;
+  for (int a = 0; a < A; ++a) {
+    if (!(pi[a] > 0.0) || !std::isfinite(pi[a])) return fail(SB_ERR_MODEL, "pi[%d]=%g must be > 0", a, pi[a]);
+    if (!std::isfinite(eigenvalues[a])) return fail(SB_ERR_MODEL, "eigenvalue %d not finite", a);
+    psum += pi[a];
+  }
+  for (int i = 0; i < A * A; ++i)
+    if (!std::isfinite(eigenvectors[i])) return fail(SB_ERR_MODEL, "eigenvector entry %d not finite", i);
+  if (!(psum < 1.0 + 1e-6 && psum > 1.0 - 1e-6)) return fail(SB_ERR_MODEL, "pi sums to %g, not 1", psum);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+
+  // FP64 master with inert pads: mu=0, V=identity block, pi=1 (pads only enter
+  // M through sqrt(pi_j/pi_i) on the identity block).
+  std::vector<double> m64(sb_m64_size(AP), 0.0);
+  double f81 = 0.0;
+  for (int a = 0; a < A; ++a) f81 += pi[a] * pi[a];
+  f81 = 1.0 / (1.0 - f81);
+  for (int k = 0; k < AP; ++k) m64[sb_m64_mu(AP) + k] = k < A ? eigenvalues[k] : 0.0;
+  for (int a = 0; a < AP; ++a)
+    for (int k = 0; k < AP; ++k)
+      m64[sb_m64_V(AP) + a * AP + k] =
+          (a < A && k < A) ? eigenvectors[a * A + k] : (a == k ? 1.0 : 0.0);
+  for (int a = 0; a < AP; ++a) m64[sb_m64_pi(AP) + a] = a < A ? pi[a] : 1.0;
+  m64[sb_m64_misc(AP)] = f81;
+  m64[sb_m64_misc(AP) + 1] = A;
+
+  std::vector<double> rec(sb_mr_size(AP), 0.0);
+  const double* V = &m64[sb_m64_V(AP)];
+  for (int a = 0; a < AP; ++a) {
+    const double sp = a < A ? std::sqrt(pi[a]) : 1.0;
+    for (int k = 0; k < AP; ++k) {
+      rec[sb_mr_V(AP) + a * AP + k] = V[a * AP + k];
+      rec[sb_mr_Vs(AP) + a * AP + k] = sp * V[a * AP + k];
+      rec[sb_mr_Vi(AP) + a * AP + k] = V[a * AP + k] / sp;
+    }
+  }
+  for (int i = 0; i < AP; ++i)
+    for (int j = 0; j < AP; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < AP; ++k) s += V[i * AP + k] * m64[sb_m64_mu(AP) + k] * V[j * AP + k];
+      rec[sb_mr_S(AP) + i * AP + j] = (i < A && j < A) ? s : 0.0;
+    }
+  for (int a = 0; a < AP; ++a) {
+    rec[sb_mr_pi(AP) + a] = a < A ? pi[a] : 0.0;
+    rec[sb_mr_ones(AP) + a] = a < A ? 1.0 : 0.0;
+  }
+  rec[sb_mr_misc(AP)] = f81;
+  std::vector<float> recf(rec.begin(), rec.end());
+
+  Model m;
+  m.alive = true;
+  m.A = A;
+  m.AP = AP;
+  m.mu.assign(eigenvalues, eigenvalues + A);
+  m.V.assign(eigenvectors, eigenvectors + A * A);
+  m.pi.assign(pi, pi + A);
+  if (hipMalloc(reinterpret_cast<void**>(&m.d64), m64.size() * 8) != hipSuccess ||
+      hipMalloc(&m.rec[0], recf.size() * 4) != hipSuccess || hipMalloc(&m.rec[1], rec.size() * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SB_ERR_OOM, "model allocation failed");
+  }
+  SB_HIP(hipMemcpy(m.d64, m64.data(), m64.size() * 8, hipMemcpyHostToDevice));
+  SB_HIP(hipMemcpy(m.rec[0], recf.data(), recf.size() * 4, hipMemcpyHostToDevice));
+  SB_HIP(hipMemcpy(m.rec[1], rec.data(), rec.size() * 8, hipMemcpyHostToDevice));
+  int id = -1;
+  for (size_t i = 0; i < ctx->models.size(); ++i)
+    if (!ctx->models[i].alive) { id = int(i); break; }
+  if (id < 0) {
+    id = int(ctx->models.size());
+    ctx->models.push_back(m);
+  } else {
+    ctx->models[id] = m;
+  }
+  *model_id = id;
+  return SB_OK;
+}
+
+int sb_model_drop(sb_context* ctx, int model_id) {
+  if (!ctx) return fail(SB_ERR_ARG, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  Model* m = get_model(ctx, model_id);
+  if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
+  SB_HIP(hipDeviceSynchronize());
+  (void)hipFree(m->d64);
+  for (void*& r : m->rec) {
+    (void)hipFree(r);
+    r = nullptr;
+  }
+  m->d64 = nullptr;
+  m->alive = false;
+  return SB_OK;
+}
+
+int sb_loglike(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+               const double* distances, int R, int C, int dtype, double* d_loglike, void* stream) {
+  if (!d_loglike && C > 0) return fail(SB_ERR_ARG, "null loglike output");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = SB_F_LL;
+  bi.blk.d_loglike = d_loglike;
+  return single(ctx, bi, dtype, stream);
+}
+
+int sb_root_prob(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+                 const double* distances, int R, int C, int dtype, void* d_root, double* d_loglike,
+                 void* stream) {
+  if (!d_root && C > 0) return fail(SB_ERR_ARG, "null root output");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = SB_F_ROOT | (d_loglike ? SB_F_LL : 0);
+  bi.blk.d_root = d_root;
+  bi.blk.d_loglike = d_loglike;
+  return single(ctx, bi, dtype, stream);
+}
+
+int sb_counts(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+              const double* distances, int R, int C, int dtype, int f81_fast, int mask_mode,
+              const uint8_t* d_mask, void* d_counts, void* d_root, double* d_loglike, void* stream) {
+  if (!d_counts && C > 0) return fail(SB_ERR_ARG, "null counts output");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = SB_F_COUNTS | (d_root ? SB_F_ROOT : 0) | (d_loglike ? SB_F_LL : 0);
+  bi.blk.f81_fast = f81_fast ? 1 : 0;
+  bi.blk.mask_mode = mask_mode;
+  bi.blk.d_mask = d_mask;
+  bi.blk.d_counts = d_counts;
+  bi.blk.d_root = d_root;
+  bi.blk.d_loglike = d_loglike;
+  return single(ctx, bi, dtype, stream);
+}
+
+int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+                    const double* distances, int R, int C, int dtype, const double* d_g,
+                    double* d_dist_grad, int n_rate, const double* rate_weights, double* d_rate_grad,
+                    double* d_loglike, void* stream) {
+  if (!d_dist_grad) return fail(SB_ERR_ARG, "null dist_grad output");
+  if (n_rate < 0 || n_rate > SB_MAX_RATE) return fail(SB_ERR_ARG, "n_rate must be 0..%d", SB_MAX_RATE);
+  if (n_rate > 0 && (!rate_weights || !d_rate_grad)) return fail(SB_ERR_ARG, "null rate arrays");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = SB_F_GRAD | (d_loglike ? SB_F_LL : 0);
+  bi.blk.d_loglike = d_loglike;
+  bi.g = d_g;
+  bi.dist_grad = d_dist_grad;
+  bi.rate_grad = d_rate_grad;
+  bi.n_rate = n_rate;
+  bi.rate_w_host = rate_weights;
+  return single(ctx, bi, dtype, stream);
+}
+
+int sb_batch_create(sb_context* ctx, void* stream, sb_batch** out) {
+  if (!ctx || !out) return fail(SB_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  return batch_create(ctx, static_cast<hipStream_t>(stream), out);
+}
+
+int sb_batch_destroy(sb_batch* b) {
+  if (!b) return SB_OK;
+  std::lock_guard<std::mutex> lock(b->ctx->mu);
+  batch_free(b);
+  return SB_OK;
+}
+
+int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu) {
+  if (!b) return fail(SB_ERR_ARG, "null launcher");
+  if (waves_per_group) {
+    if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4)
+      return fail(SB_ERR_ARG, "waves_per_group must be 1, 2 or 4");
+    b->waves_per_group = waves_per_group;
+  }
+  if (waves_per_cu) {
+    if (waves_per_cu < 1 || waves_per_cu > 32) return fail(SB_ERR_ARG, "waves_per_cu must be 1..32");
+    b->waves_per_cu = waves_per_cu;
+  }
+  return SB_OK;
+}
+
+int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype, int use_graph) {
+  if (!b || (n_blocks > 0 && !blocks) || n_blocks < 0) return fail(SB_ERR_ARG, "bad batch arguments");
+  std::lock_guard<std::mutex> lock(b->ctx->mu);
+  SB_HIP(hipSetDevice(b->ctx->device));
+  std::vector<BlockIn> v(n_blocks);
+  for (int i = 0; i < n_blocks; ++i) {
+    std::memset(&v[i], 0, sizeof v[i]);
+    v[i].blk = blocks[i];
+    int w = blocks[i].want;
+    if (w & ~(SB_WANT_LOGLIKE | SB_WANT_ROOT | SB_WANT_COUNTS)) return fail(SB_ERR_ARG, "block %d: bad want bits", i);
+    v[i].flags = ((w & SB_WANT_LOGLIKE) ? SB_F_LL : 0) | ((w & SB_WANT_ROOT) ? SB_F_ROOT : 0) |
+                 ((w & SB_WANT_COUNTS) ? SB_F_COUNTS : 0);
+  }
+  return submit(b, v, dtype, use_graph);
+}
+
+int sb_counts_plain(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+                    const double* distances, int R, int C, int mask_mode, float* d_counts, float* d_root,
+                    double* d_loglike, void* stream) {
+  if (!ctx || !parent_index || !distances) return fail(SB_ERR_ARG, "null argument");
+  if (!d_loglike) return fail(SB_ERR_ARG, "plain path needs a loglike buffer");
+  if (mask_mode != SB_MASK_NONE && mask_mode != SB_MASK_AUTO)
+    return fail(SB_ERR_UNSUPPORTED, "plain path supports mask_mode none or auto");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  const Model* m = get_model(ctx, model_id);
+  if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
+  if (m->A != 4) return fail(SB_ERR_UNSUPPORTED, "plain path is implemented for A = 4 only");
+  TreePlan tp;
+  int rc = plan_tree(parent_index, R, tp);
+  if (rc) return rc;
+  if (C <= 0) return SB_OK;
+  // M(t), J(t) in FP64 on the host (diagonalize.py:141-172, eigensub.py:7-50)
+  const int A = 4, bts = sb_bt_stride(4);
+  ctx->plain_bt_host.assign(size_t(R) * bts, 0.f);
+  for (int n = 0; n < R; ++n) {
+    const double t = distances[n];
+    double E[4];
+    for (int k = 0; k < A; ++k) E[k] = std::exp(m->mu[k] * t);
+    float* o = &ctx->plain_bt_host[size_t(n) * bts];
+    for (int i = 0; i < A; ++i)
+      for (int j = 0; j < A; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < A; ++k) s += m->V[i * A + k] * E[k] * m->V[j * A + k];
+        o[sb_bt_M(4) + i * A + j] = float(s * std::sqrt(m->pi[j] / m->pi[i]));
+        const double d = m->mu[i] - m->mu[j];
+        o[sb_bt_J(4) + i * A + j] = float(std::fabs(d) < 1e-8 ? t * E[i] : E[j] * std::expm1(d * t) / d);
+      }
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = ctx->plain_bt_host.size() * sizeof(float);
+  if (bytes > ctx->plain_bt.cap) {
+    SB_HIP(hipStreamSynchronize(s));
+    if (ctx->plain_bt.p) SB_HIP(hipFree(ctx->plain_bt.p));
+    ctx->plain_bt.p = nullptr;
+    ctx->plain_bt.cap = 0;
+    if (hipMalloc(&ctx->plain_bt.p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SB_ERR_OOM, "plain branch table allocation failed");
+    }
+    ctx->plain_bt.cap = bytes;
+  }
+  SB_HIP(hipMemcpyAsync(ctx->plain_bt.p, ctx->plain_bt_host.data(), bytes, hipMemcpyHostToDevice, s));
+  SB_HIP(sb_plain_run(d_alignment, parent_index, R, C, static_cast<const float*>(ctx->plain_bt.p),
+                      static_cast<const float*>(m->rec[0]), mask_mode, d_counts, d_root, d_loglike,
+                      &ctx->plain, s));
+  return SB_OK;
+}
+
+int sb_batch_sync(sb_batch* b) {
+  if (!b) return fail(SB_ERR_ARG, "null launcher");
+  SB_HIP(hipEventSynchronize(b->done));
+  return SB_OK;
+}
+
+}  // extern "C"

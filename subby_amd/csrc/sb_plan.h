@@ -1,0 +1,102 @@
+// Plan records shared by the host planner and the gfx950 kernels.
+//
+// A launch processes a list of blocks (tree + alignment block + model).  The
+// host planner (sb_host.cpp) validates each tree, numbers its internal nodes
+// in increasing node index ("slots"), and writes one 8-int record per
+// internal node:
+//     {p, l, r, 0, slot(l) or -1, slot(r) or -1, 0, 0}
+// where l is the first child of p in index order (the reference's left child,
+// subby/oracle/oracle.py:19-49).  Walking the records backwards is a
+// postorder (children have larger indices than parents); forwards a preorder.
+#pragma once
+#include <stdint.h>
+
+#define SB_WAVE 64
+#define SB_MAX_RATE 8
+
+// Per-branch parameter record, element type T (float or double), padded
+// alphabet AP:  M[AP*AP] | J[AP*AP] | coef[4] = {alpha, beta, gamma, t} |
+//               E[AP] = exp(mu_k t) | dE[AP] = mu_k exp(mu_k t)
+__host__ __device__ inline int sb_bt_stride(int AP) { return 2 * AP * AP + 4 + 2 * AP; }
+__host__ __device__ inline int sb_bt_M(int) { return 0; }
+__host__ __device__ inline int sb_bt_J(int AP) { return AP * AP; }
+__host__ __device__ inline int sb_bt_coef(int AP) { return 2 * AP * AP; }
+__host__ __device__ inline int sb_bt_E(int AP) { return 2 * AP * AP + 4; }
+__host__ __device__ inline int sb_bt_dE(int AP) { return 2 * AP * AP + 4 + AP; }
+
+// Model record, element type T, padded alphabet AP (pads are inert states
+// that stay exactly zero in every vector):
+//   V[AP*AP]   eigenvectors, V[a*AP+k]
+//   Vs[AP*AP]  sqrt(pi_a) V[a,k]      (inside projection,  eigensub.py:53-84)
+//   Vi[AP*AP]  V[a,k] / sqrt(pi_a)    (outside projection)
+//   S[AP*AP]   V diag(mu) V^T          (back-transform,     eigensub.py:138-172)
+//   pi[AP]     (pads 0)
+//   ones[AP]   1 for real states, 0 for pads (unobserved-token likelihood)
+//   misc[4]    {f81_mu, 0, 0, 0}
+__host__ __device__ inline int sb_mr_V(int) { return 0; }
+__host__ __device__ inline int sb_mr_Vs(int AP) { return AP * AP; }
+__host__ __device__ inline int sb_mr_Vi(int AP) { return 2 * AP * AP; }
+__host__ __device__ inline int sb_mr_S(int AP) { return 3 * AP * AP; }
+__host__ __device__ inline int sb_mr_pi(int AP) { return 4 * AP * AP; }
+__host__ __device__ inline int sb_mr_ones(int AP) { return 4 * AP * AP + AP; }
+__host__ __device__ inline int sb_mr_misc(int AP) { return 4 * AP * AP + 2 * AP; }
+__host__ __device__ inline int sb_mr_size(int AP) { return 4 * AP * AP + 2 * AP + 4; }
+
+// FP64 master copy of a model, used by the branch-parameter kernel:
+//   mu[AP] | V[AP*AP] | pi[AP] | misc[4] = {f81_mu, A, 0, 0}
+__host__ __device__ inline int sb_m64_mu(int) { return 0; }
+__host__ __device__ inline int sb_m64_V(int AP) { return AP; }
+__host__ __device__ inline int sb_m64_pi(int AP) { return AP + AP * AP; }
+__host__ __device__ inline int sb_m64_misc(int AP) { return 2 * AP + AP * AP; }
+__host__ __device__ inline int sb_m64_size(int AP) { return 2 * AP + AP * AP + 4; }
+
+enum {
+  SB_F_F81 = 1,
+  SB_F_MASK_SHIFT = 1,     // 2 bits: 0 none, 1 auto, 2 explicit
+  SB_F_COUNTS = 8,
+  SB_F_ROOT = 16,
+  SB_F_LL = 32,
+  SB_F_GRAD = 64,
+};
+
+struct SbBlockDesc {
+  const int8_t* aln;       // [R][C]
+  const uint8_t* mask;     // [R][C] or null
+  void* counts;            // [A][A][C] of T or null
+  void* root;              // [A][C] of T or null
+  double* loglike;         // [C] or null
+  const int* sched;        // nint records of 8 ints
+  const void* bt;          // R branch records of T
+  const void* model;       // model record of T
+  const double* model64;   // FP64 master (pi used for the log-likelihood)
+  const double* g;         // [C] cotangent of logLike (gradient) or null = ones
+  const double* rate_w;    // [n_rate][AP*AP] Fisher weights (gradient)
+  double* dist_grad;       // [R] output (gradient)
+  double* rate_grad;       // [n_rate] output (gradient)
+  int R, C, nint, flags;
+  int A, tile_base, ntiles, n_rate;
+};
+
+struct SbBranchJob {       // one per (block, node): M(t), J(t) ... in FP64
+  const double* model64;
+  double t;
+};
+
+// Launch header: lives at a fixed device address per launcher, so the kernel
+// sequence of one variant can be captured in a hipGraph once and replayed for
+// any batch (the header and the plan are re-uploaded before each replay).
+struct SbLaunch {
+  const SbBlockDesc* blocks;
+  const int2* tiles;       // {block, tile-in-block}, longest trees first
+  const SbBranchJob* jobs; // one per (block, node)
+  void* bt;                // branch records, filled by the params kernel
+  int* counter;            // dynamic tile counter (zeroed by sb_reset_kernel)
+  char* scratch;           // nwaves * wave_stride bytes
+  double* grad_part;       // [tile][grad_stride]: per-tile branch-length partials
+  double* rate_part;       // [tile][SB_MAX_RATE]
+  void* stage;             // per-branch staging for large alphabets
+  long long wave_stride;
+  long long stage_stride;  // bytes per staged block column-branch vector set
+  int nblocks, ntiles, njobs, nint_max;
+  int grad_stride, AP, pad0, pad1;
+};

@@ -1,0 +1,129 @@
+"""Host-side model construction in FP64 (the eigendecomposition stays on the host).
+
+Mirrors subby/jax/models.py (hky85_diag :7-73, jukes_cantor_model :76-97,
+f81_model :100-123) and subby/jax/diagonalize.py (diagonalize_rate_matrix
+:6-26, reconstruct_rate_matrix :29-47) with NumPy float64.
+"""
+from __future__ import annotations
+
+from typing import NamedTuple
+
+import numpy as np
+
+
+class Tree(NamedTuple):
+    """subby/formats.py:15-17: parentIndex (R,) preorder, distanceToParent (R,)."""
+    parentIndex: object
+    distanceToParent: object
+
+
+class DiagModel(NamedTuple):
+    """subby/jax/types.py:8-11: eigenvalues (A,), eigenvectors V[a,k], pi (A,)."""
+    eigenvalues: object
+    eigenvectors: object
+    pi: object
+
+
+class RateModel(NamedTuple):
+    """subby/jax/types.py:21-23: reversible rate matrix and root distribution."""
+    subRate: object
+    rootProb: object
+
+
+_TS = np.array([[0, 0, 1, 0], [0, 0, 0, 1], [1, 0, 0, 0], [0, 1, 0, 0]], dtype=bool)
+
+
+def hky85_diag(kappa, pi) -> DiagModel:
+    """Closed-form HKY85 eigensystem of S = Pi^1/2 Q Pi^-1/2 (models.py:7-73)."""
+    pi = np.asarray(pi, dtype=np.float64)
+    pA, pC, pG, pT = pi
+    pR, pY = pA + pG, pC + pT
+    beta = 1.0 / (2.0 * pR * pY + 2.0 * kappa * (pA * pG + pC * pT))
+    ev = np.array([0.0, -beta, -beta * (pR + kappa * pY), -beta * (pY + kappa * pR)])
+    s = np.sqrt(pi)
+    w1 = np.array([s[0] * pY, -s[1] * pR, s[2] * pY, -s[3] * pR]) / np.sqrt(pR * pY)
+    w2 = np.array([0.0, s[1] * pT, 0.0, -s[3] * pC]) / np.sqrt(pC * pT * pY)
+    w3 = np.array([s[0] * pG, 0.0, -s[2] * pA, 0.0]) / np.sqrt(pA * pG * pR)
+    return DiagModel(ev, np.column_stack([s, w1, w2, w3]), pi)
+
+
+def hky85_rate_matrix(kappa, pi) -> np.ndarray:
+    """Q_ij = beta kappa^{ts(i,j)} pi_j (i != j), rows sum to 0, mean rate 1."""
+    pi = np.asarray(pi, dtype=np.float64)
+    Q = np.where(_TS, kappa, 1.0) * pi[None, :]
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    return Q / -(pi @ np.diag(Q))
+
+
+def hky85_log_kappa_weights(kappa, pi) -> np.ndarray:
+    """Fisher weights W for theta = log(kappa): W_ij = dlog Q_ij/dtheta (i != j),
+    W_ii = dQ_ii/dtheta, so dlogL/dtheta = sum_ij W_ij Counts_ij (unmasked)
+    (Holmes & Rubin 2002, eqs. 3-4)."""
+    pi = np.asarray(pi, dtype=np.float64)
+    pA, pC, pG, pT = pi
+    pR, pY = pA + pG, pC + pT
+    den = 2.0 * pR * pY + 2.0 * kappa * (pA * pG + pC * pT)
+    dlog_beta = -2.0 * kappa * (pA * pG + pC * pT) / den
+    Q = hky85_rate_matrix(kappa, pi)
+    dlogQ = dlog_beta + _TS.astype(np.float64)
+    W = dlogQ.copy()
+    off = ~np.eye(4, dtype=bool)
+    dQ = np.where(off, Q * dlogQ, 0.0)
+    np.fill_diagonal(W, -dQ.sum(1))
+    return W
+
+
+def jukes_cantor_model(A) -> DiagModel:
+    """JC: pi uniform, eigenvalues (0, -A/(A-1) x (A-1)), QR basis (models.py:76-97)."""
+    pi = np.ones(A) / A
+    ev = np.zeros(A)
+    ev[1:] = -float(A) / (A - 1.0)
+    Q, _ = np.linalg.qr(np.column_stack([np.ones(A) / np.sqrt(A), np.eye(A)]))
+    return DiagModel(ev, Q[:, :A], pi)
+
+
+def f81_model(pi) -> DiagModel:
+    """F81: eigenvalues (0, -mu, ...), mu = 1/(1 - sum pi^2) (models.py:100-123)."""
+    pi = np.asarray(pi, dtype=np.float64)
+    A = len(pi)
+    ev = np.zeros(A)
+    ev[1:] = -1.0 / (1.0 - np.sum(pi ** 2))
+    Q, _ = np.linalg.qr(np.column_stack([np.sqrt(pi), np.eye(A)]))
+    return DiagModel(ev, Q[:, :A], pi)
+
+
+def diagonalize_rate_matrix(subRate, rootProb) -> DiagModel:
+    """eigh of the symmetrised S_ij = Q_ij sqrt(pi_i/pi_j) (diagonalize.py:6-26)."""
+    Q = np.asarray(subRate, dtype=np.float64)
+    pi = np.asarray(rootProb, dtype=np.float64)
+    s = np.sqrt(pi)
+    S = Q * s[:, None] / s[None, :]
+    ev, V = np.linalg.eigh(0.5 * (S + S.T))
+    return DiagModel(ev, V, pi)
+
+
+def reversible_rate_matrix(exchangeabilities, pi) -> np.ndarray:
+    """Q_ij = s_ij pi_j (i != j), normalised so that -sum_i pi_i Q_ii = 1."""
+    s = np.asarray(exchangeabilities, dtype=np.float64)
+    pi = np.asarray(pi, dtype=np.float64)
+    s = 0.5 * (s + s.T)
+    Q = s * pi[None, :]
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    return Q / -(pi @ np.diag(Q))
+
+
+def ensure_diag(model) -> DiagModel:
+    """RateModel -> DiagModel (subby/jax/__init__.py:44-50)."""
+    if isinstance(model, RateModel):
+        return diagonalize_rate_matrix(model.subRate, model.rootProb)
+    if isinstance(model, DiagModel):
+        return DiagModel(np.asarray(model.eigenvalues, dtype=np.float64),
+                         np.asarray(model.eigenvectors, dtype=np.float64),
+                         np.asarray(model.pi, dtype=np.float64))
+    if isinstance(model, dict):
+        return DiagModel(np.asarray(model['eigenvalues'], dtype=np.float64),
+                         np.asarray(model['eigenvectors'], dtype=np.float64),
+                         np.asarray(model['pi'], dtype=np.float64))
+    raise TypeError(f'unsupported model type {type(model).__name__}')

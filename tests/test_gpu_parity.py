@@ -1,0 +1,323 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances (stated per output, FP32 path vs FP64 oracle):
+  logLike       max_c |d| / max(|ref|, 1)                  <= 1e-5
+  counts        max_c max_ij |d| / max_ij |ref|   (normwise) <= 1e-4
+  root          max |d|                                       <= 1e-5
+FP64 path: the same measures <= 1e-12.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import subby_oracle as O
+from tests.golden_cases import CASES, load
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = dict(ll=1e-5, counts=1e-4, root=1e-5)
+TOL64 = dict(ll=1e-12, counts=1e-12, root=1e-12)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope='module')
+def sb():
+    import subby_amd
+    from subby_amd import _lib
+    assert torch.cuda.is_available(), 'GPU tests need a ROCm GPU'
+    _lib.load()
+    return subby_amd
+
+
+@pytest.fixture(scope='module')
+def eng(sb):
+    return sb.Engine(0)
+
+
+def otree(t):
+    return O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent))
+
+
+def omodel(m):
+    return O.DiagModel(*(np.asarray(x, dtype=np.float64) for x in m))
+
+
+def err_ll(got, ref):
+    got = got.detach().cpu().numpy()
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0))) if len(ref) else 0.0
+
+
+def err_counts(got, ref):
+    got = got.detach().cpu().double().numpy()
+    if ref.shape[-1] == 0:
+        return 0.0
+    num = np.abs(got - ref).max(axis=(0, 1))
+    den = np.maximum(np.abs(ref).max(axis=(0, 1)), 1e-300)
+    return float(np.max(num / den))
+
+
+def err_root(got, ref):
+    got = got.detach().cpu().double().numpy()
+    return float(np.max(np.abs(got - ref))) if ref.size else 0.0
+
+
+def check_all(sb, eng, aln, tree, model, precision='fp32', mask='auto', f81=False):
+    tol = TOL32 if precision == 'fp32' else TOL64
+    ot, om = otree(tree), omodel(model)
+    ref_cnt, ref_ll, ref_root = O.counts_loglike_root(np.asarray(aln), ot, om, f81_fast=f81, branch_mask=mask)
+    cnt, ll, root = sb.Counts(aln, tree, model, f81_fast_flag=f81, branch_mask=mask, precision=precision,
+                              engine=eng, return_loglike=True, return_root=True)
+    ll2 = sb.LogLike(aln, tree, model, precision=precision, engine=eng)
+    root2 = sb.RootProb(aln, tree, model, precision=precision, engine=eng)
+    e = dict(ll=err_ll(ll, ref_ll), counts=err_counts(cnt, ref_cnt), root=err_root(root, ref_root),
+             ll2=err_ll(ll2, ref_ll), root2=err_root(root2, ref_root))
+    say('   errors', precision, {k: f'{v:.2e}' for k, v in e.items()})
+    assert e['ll'] <= tol['ll'] and e['ll2'] <= tol['ll']
+    assert e['counts'] <= tol['counts']
+    assert e['root'] <= tol['root'] and e['root2'] <= tol['root']
+    return cnt, ll, root
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('case', [c for c in CASES if c != '7node_jc64'])
+def test_goldens(sb, eng, case, precision):
+    say('golden', case, precision)
+    g = load(case)
+    m = sb.DiagModel(g['inputs.model.eigenvalues'], g['inputs.model.eigenvectors'], g['inputs.model.pi'])
+    t = sb.Tree(g['inputs.parentIndex'], g['inputs.distanceToParent'])
+    aln = g['inputs.alignment']
+    tol = TOL32 if precision == 'fp32' else TOL64
+    cnt, ll, root = sb.Counts(aln, t, m, branch_mask=None, precision=precision, engine=eng,
+                              return_loglike=True, return_root=True)
+    assert err_ll(ll, g['outputs.logLike']) <= tol['ll']
+    assert err_counts(cnt, g['outputs.counts']) <= tol['counts']
+    assert err_root(root, g['outputs.root_prob']) <= tol['root']
+    check_all(sb, eng, aln, t, m, precision, mask='auto')
+    check_all(sb, eng, aln, t, m, precision, mask='auto', f81=True)
+
+
+def _c1(sb, seed=1):
+    rng = np.random.default_rng(seed)
+    t = sb.synthetic.random_tree(8, rng, 0.1)
+    m = sb.jukes_cantor_model(4)
+    aln = sb.synthetic.simulate(t, m, 1000, rng, gap_frac=0.05)
+    return t, m, aln
+
+
+def test_config_small_jc_fp64(sb, eng):
+    """JC A=4, 8 leaves, C=1000, 5% gaps: FP64 end to end at 1e-12."""
+    say('config small JC fp64')
+    t, m, aln = _c1(sb)
+    for mask in ('auto', None):
+        check_all(sb, eng, aln, t, m, 'fp64', mask=mask)
+        check_all(sb, eng, aln, t, m, 'fp64', mask=mask, f81=True)
+        check_all(sb, eng, aln, t, m, 'fp32', mask=mask)
+
+
+def _hky(sb, C, seed=2, leaves=64, mean=0.05):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(np.full(4, 5.0))
+    m = sb.hky85_diag(2.0, pi)
+    t = sb.synthetic.random_tree(leaves, rng, mean)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+    return t, m, aln
+
+
+def test_hky85_127_nodes_fp32(sb, eng):
+    say('HKY85 127 nodes C=20000 fp32')
+    t, m, aln = _hky(sb, 20000)
+    check_all(sb, eng, aln, t, m, 'fp32', mask='auto')
+    check_all(sb, eng, aln, t, m, 'fp32', mask=None)
+    check_all(sb, eng, aln, t, m, 'fp64', mask='auto')
+
+
+@pytest.mark.parametrize('A', [2, 3, 5, 8])
+def test_padded_alphabets(sb, eng, A):
+    say('padded alphabet', A)
+    rng = np.random.default_rng(10 + A)
+    pi = rng.dirichlet(np.full(A, 3.0))
+    Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi)
+    m = sb.diagonalize_rate_matrix(Q, pi)
+    t = sb.synthetic.random_tree(20, rng, 0.1)
+    aln = sb.synthetic.simulate(t, m, 777, rng, gap_frac=0.05, ambig_frac=0.02)
+    check_all(sb, eng, aln, t, m, 'fp32')
+    check_all(sb, eng, aln, t, m, 'fp64')
+    f = sb.f81_model(pi)
+    check_all(sb, eng, aln, t, f, 'fp32', f81=True)
+    check_all(sb, eng, aln, t, f, 'fp64', f81=True)
+
+
+def test_edge_shapes(sb, eng):
+    say('edge shapes')
+    rng = np.random.default_rng(5)
+    m = sb.hky85_diag(3.0, rng.dirichlet(np.full(4, 5.0)))
+    # single-node tree, 3-node tree, C = 1, 63, 65, all-gap and observed internal nodes
+    for R, C in [(1, 5), (3, 1), (3, 65), (9, 63), (31, 129)]:
+        if R == 1:
+            t = sb.Tree(np.array([-1]), np.array([0.0]))
+        else:
+            t = sb.synthetic.random_tree((R + 1) // 2, rng, 0.2)
+        aln = rng.integers(-1, 6, size=(R, C)).astype(np.int8)
+        aln[:, 0] = -1
+        for prec in ('fp32', 'fp64'):
+            check_all(sb, eng, aln, t, m, prec, mask='auto')
+            check_all(sb, eng, aln, t, m, prec, mask=None)
+
+
+def test_deep_caterpillar_mask(sb, eng):
+    """Deep unbalanced tree: the exact (oracle) Steiner mask, not a depth-bounded one."""
+    say('caterpillar')
+    L = 40
+    parent = [-1]
+    spine = 0
+    for _ in range(L - 1):
+        parent += [spine, spine]
+        spine = len(parent) - 1
+    parent = np.array(parent, dtype=np.int32)
+    R = len(parent)
+    rng = np.random.default_rng(6)
+    t = sb.Tree(parent, np.r_[0.0, rng.uniform(0.01, 0.3, R - 1)])
+    aln = np.full((R, 64), -1, dtype=np.int8)
+    aln[R - 1] = rng.integers(0, 4, 64)
+    aln[1] = rng.integers(0, 4, 64)
+    aln[5, ::3] = 2
+    m = sb.jukes_cantor_model(4)
+    check_all(sb, eng, aln, t, m, 'fp64', mask='auto')
+    check_all(sb, eng, aln, t, m, 'fp32', mask='auto')
+
+
+def test_explicit_mask(sb, eng):
+    say('explicit mask')
+    t, m, aln = _hky(sb, 300, seed=8, leaves=10)
+    mk = np.random.default_rng(9).random((len(t.parentIndex), 300)) < 0.6
+    ref = O.Counts(aln, otree(t), omodel(m), branch_mask=mk)
+    got = sb.Counts(aln, t, m, branch_mask=mk, precision='fp64', engine=eng)
+    assert err_counts(got, ref) <= 1e-12
+    got = sb.Counts(aln, t, m, branch_mask=torch.as_tensor(mk), precision='fp32', engine=eng)
+    assert err_counts(got, ref) <= 1e-4
+
+
+def test_errors(sb, eng):
+    say('errors')
+    m = sb.jukes_cantor_model(4)
+    aln = np.zeros((4, 3), dtype=np.int8)
+    with pytest.raises(sb.SubbyError, match='children'):
+        sb.LogLike(aln, sb.Tree(np.array([-1, 0, 0, 0]), np.zeros(4)), m, engine=eng)
+    with pytest.raises(sb.SubbyError, match='preorder'):
+        sb.LogLike(aln[:3], sb.Tree(np.array([-1, 2, 0]), np.zeros(3)), m, engine=eng)
+    with pytest.raises(sb.SubbyError, match='1 child'):
+        sb.LogLike(aln[:2], sb.Tree(np.array([-1, 0]), np.zeros(2)), m, engine=eng)
+    with pytest.raises(sb.SubbyError, match='alphabet'):
+        eng.model_id(sb.jukes_cantor_model(65))
+    with pytest.raises(ValueError):
+        sb.Counts(aln[:3], sb.Tree(np.array([-1, 0, 0]), np.zeros(3)), m, branch_mask='bogus', engine=eng)
+    # the engine still works after errors
+    ll = sb.LogLike(aln[:3], sb.Tree(np.array([-1, 0, 0]), np.array([0, .1, .2])), m, engine=eng)
+    assert torch.isfinite(ll).all()
+
+
+def test_batch_equals_single_bitwise(sb, eng):
+    """Uneven trees, column counts and models in one launch == one block per call, bitwise."""
+    say('batch vs single')
+    from subby_amd.batch import Launcher, Block
+    rng = np.random.default_rng(11)
+    blocks = []
+    for i in range(12):
+        leaves = int(rng.integers(2, 70))
+        C = int(rng.integers(1, 900))
+        m = sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+        t = sb.synthetic.random_tree(leaves, rng, 0.08)
+        aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.05)
+        blocks.append(Block(aln, t, m))
+    L = Launcher(eng)
+    L.submit(blocks)
+    L.sync()
+    for b in blocks:
+        cnt, ll, root = sb.Counts(b.alignment, b.tree, b.model, engine=eng, return_loglike=True,
+                                  return_root=True)
+        torch.cuda.synchronize()
+        assert torch.equal(cnt, b.counts) and torch.equal(ll, b.loglike) and torch.equal(root, b.root)
+    # and against the oracle
+    for b in blocks[:4]:
+        ref_cnt, ref_ll, ref_root = O.counts_loglike_root(b.alignment, otree(b.tree), omodel(b.model))
+        assert err_counts(b.counts, ref_cnt) <= 1e-4 and err_ll(b.loglike, ref_ll) <= 1e-5
+
+
+def test_graph_replays_match_eager(sb, eng):
+    """hipGraph replays, with eager work and syncs in between, equal the eager launch."""
+    say('graph replay')
+    from subby_amd.batch import Launcher, Block
+    rng = np.random.default_rng(12)
+    L = Launcher(eng)
+    for rep in range(4):
+        blocks = []
+        for i in range(int(rng.integers(1, 6))):
+            m = sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+            t = sb.synthetic.random_tree(int(rng.integers(2, 50)), rng, 0.08)
+            blocks.append(Block(sb.synthetic.simulate(t, m, int(rng.integers(1, 500)), rng), t, m))
+        L.submit(blocks, use_graph=True)
+        L.sync()
+        for b in blocks:
+            cnt, ll, root = sb.Counts(b.alignment, b.tree, b.model, engine=eng, return_loglike=True,
+                                      return_root=True)
+            torch.cuda.synchronize()
+            assert torch.equal(cnt, b.counts) and torch.equal(ll, b.loglike) and torch.equal(root, b.root)
+
+
+def test_gradient_vs_oracle_and_fd(sb, eng):
+    say('gradient')
+    t, m, aln = _hky(sb, 3000, seed=13, leaves=20, mean=0.1)
+    rng = np.random.default_rng(14)
+    g = rng.normal(size=3000)
+    ref, _ = O.loglike_branch_grad(aln, otree(t), omodel(m), g)
+    for prec, tol in (('fp64', 1e-10), ('fp32', 1e-4)):
+        d, _, ll = sb.loglike_grad(aln, t, m, g=g, precision=prec, engine=eng)
+        d = d.cpu().numpy()
+        err = np.abs(d - ref).max() / np.abs(ref).max()
+        say('   grad err', prec, err)
+        assert err <= tol
+    # log kappa through the Fisher identity
+    pi = np.asarray(m.pi)
+    W = sb.hky85_log_kappa_weights(2.0, pi)
+    _, r, _ = sb.loglike_grad(aln, t, m, g=g, rate_weights=W, precision='fp64', engine=eng)
+    refk = O.hky85_loglike_kappa_grad(aln, otree(t), 2.0, pi, g)
+    assert abs(float(r[0]) - refk) <= 1e-9 * max(1.0, abs(refk))
+
+
+def test_plain_path(sb, eng):
+    say('plain path')
+    from subby_amd.batch import counts_plain
+    t, m, aln = _hky(sb, 5000, seed=15)
+    for mask in ('auto', None):
+        ref_cnt, ref_ll, ref_root = O.counts_loglike_root(aln, otree(t), omodel(m), branch_mask=mask)
+        cnt, ll, root = counts_plain(aln, t, m, branch_mask=mask, engine=eng)
+        assert err_ll(ll, ref_ll) <= 1e-5
+        assert err_counts(cnt, ref_cnt) <= 1e-4
+        assert err_root(root, ref_root) <= 1e-5
+
+
+def test_full_size_hky85_properties(sb, eng):
+    """HKY85, 127 nodes, C = 200,000: size-independent checks plus a column sample vs the oracle."""
+    say('full size HKY85 C=200000')
+    t, m, aln = _hky(sb, 200000, seed=16)
+    cnt, ll, root = sb.Counts(aln, t, m, engine=eng, return_loglike=True, return_root=True)
+    torch.cuda.synchronize()
+    mask = O.compute_branch_mask(aln, np.asarray(t.parentIndex), 4)
+    mask[0] = False
+    dwell = torch.einsum('iic->c', cnt.double()).cpu().numpy()
+    expect = (np.asarray(t.distanceToParent)[:, None] * mask).sum(0)
+    assert np.max(np.abs(dwell - expect) / np.maximum(expect, 1e-3)) <= 1e-4
+    assert np.abs(root.double().sum(0).cpu().numpy() - 1).max() <= 1e-5
+    assert cnt.min().item() > -1e-4
+    cols = np.random.default_rng(17).choice(200000, 3000, replace=False)
+    sub = aln[:, cols]
+    rc, rl, rr = O.counts_loglike_root(sub, otree(t), omodel(m))
+    assert err_counts(cnt[:, :, cols], rc) <= 1e-4
+    assert err_ll(ll[cols], rl) <= 1e-5
+    assert err_root(root[:, cols], rr) <= 1e-5
