@@ -1,0 +1,222 @@
+"""Benchmark: FP32 Counts column*branches/s on the HKY85 127-node workload.
+
+Workload (one GPU): HKY85, kappa = 2, pi ~ Dirichlet(5,5,5,5); random binary
+tree with 64 leaves (127 nodes), branch lengths ~ Exp(mean 0.05); C = 200,000
+columns simulated under the model, 3% gaps, 1% ambiguous leaf cells; int8
+alignment resident in HBM; FP32 partials, Steiner branch mask "auto"; each
+step returns Counts (4,4,C), logLike (C,) and root posteriors (4,C).
+
+A step is one Counts call through the persistent fused path, launched as a
+captured hipGraph.  value = C*(R-1) / (seconds per step).
+
+Extra keys on the JSON line:
+  roofline      HBM roofline of the fused kernel: algorithmic bytes per launch
+                (every inside/outside partial written once and read once,
+                int8 tokens, FP32 partials, FP64 log-scales) / kernel time
+                measured with HIP events on the launch stream.
+  cpu_baseline  the FP64 NumPy oracle (subby.jax data flow, 128-column inside
+                chunks) on a bounded column sample, one thread.
+  plain         the plain one-block path (per-level launches, partials in HBM)
+                on the same block.
+  batched_vs_plain  a stream of HKY85 127-node blocks (lengths LogNormal,
+                mean 500, clipped to [20, 10000]): one persistent launch for all
+                blocks vs the plain path one block per call.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def b_alg_counts(R, A, C):
+    """Algorithmic bytes of one Counts call: tokens (1 B), U written+read for R
+    nodes, D written+read for R-1 nodes (A FP32 + one FP64 log-scale each),
+    counts + root + FP64 logLike out."""
+    return C * (R * 1 + 2 * R * (4 * A + 8) + 2 * (R - 1) * (4 * A + 8) + 4 * A * A + 4 * A + 8)
+
+
+def make_c2(sb, C=200000, seed=20260):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(np.full(4, 5.0))
+    m = sb.hky85_diag(2.0, pi)
+    t = sb.synthetic.random_tree(64, rng, 0.05)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+    return t, m, aln
+
+
+def make_stream(sb, n_blocks, seed=777):
+    rng = np.random.default_rng(seed)
+    blocks = []
+    sigma = 1.0
+    mu = np.log(500.0) - 0.5 * sigma ** 2
+    for _ in range(n_blocks):
+        C = int(np.clip(round(rng.lognormal(mu, sigma)), 20, 10000))
+        m = sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        blocks.append((t, m, aln))
+    return blocks
+
+
+def cpu_baseline(t, m, aln, budget_s=12.0):
+    """FP64 oracle on a bounded sample of the same workload, one thread."""
+    from threadpoolctl import threadpool_limits
+    from oracle import subby_oracle as O
+    sample = 4000
+    sub = np.ascontiguousarray(aln[:, :sample])
+    ot, om = O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent)), O.DiagModel(*m)
+    R = len(t.parentIndex)
+    with threadpool_limits(limits=1):
+        O.counts_chunked(sub[:, :256], ot, om)           # warm-up
+        times = []
+        t0 = time.perf_counter()
+        while True:
+            s = time.perf_counter()
+            O.counts_chunked(sub, ot, om, chunk=128)
+            times.append(time.perf_counter() - s)
+            if time.perf_counter() - t0 > budget_s or len(times) >= 5:
+                break
+    med = float(np.median(times))
+    return {"value": sample * (R - 1) / med, "unit": "column*branches/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} columns of the same HKY85 127-node block, Counts(mask auto) in FP64 "
+                      f"NumPy (subby.jax data flow, 128-column inside chunks), median of {len(times)} reps, "
+                      f"{med:.2f} s/rep, 1 BLAS thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--stream-blocks', type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build '
+              '(columns shard across GPUs with no exchange; see the design notes)', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import subby_amd as sb
+    from subby_amd.batch import Launcher, Block, counts_plain
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    log('bench: building the HKY85 127-node workload (C = 200,000)')
+    t, m, aln_np = make_c2(sb)
+    R, C, A = len(t.parentIndex), aln_np.shape[1], 4
+    eng = sb.Engine(0)
+    dev = eng.device
+    aln = torch.from_numpy(aln_np).to(dev)
+    L = Launcher(eng)
+    blk = Block(aln, t, m)
+    prep = L.prepare([blk])
+    torch.cuda.synchronize()
+
+    log(f'bench: warmup {args.warmup}, timed {args.steps} steps')
+    for _ in range(args.warmup):
+        L.launch(prep, use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        L.launch(prep, use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    units = C * (R - 1)
+    value = units / (wall / args.steps)
+
+    # kernel duration with HIP events on the launcher's stream (eager launches)
+    L.set_timing(True)
+    for _ in range(max(5, args.steps // 2)):
+        L.launch(prep, use_graph=False)
+    kms, nl = L.kernel_ms()
+    L.set_timing(False)
+    kavg = kms / nl
+    balg = b_alg_counts(R, A, C)
+    achieved = balg / (kavg * 1e-3) / 1e9
+    log(f'bench: {ms_step:.3f} ms/step, kernel {kavg:.3f} ms, {value:.3e} col*br/s')
+
+    # parity spot check of the timed outputs against the oracle on a column sample
+    from oracle import subby_oracle as O
+    cols = np.arange(0, C, 97)[:1500]
+    rc, rl, rr = O.counts_loglike_root(aln_np[:, cols], O.Tree(*t), O.DiagModel(*m))
+    gc = blk.counts[:, :, cols].double().cpu().numpy()
+    err_c = float(np.max(np.abs(gc - rc).max((0, 1)) / np.abs(rc).max((0, 1))))
+    gl = blk.loglike[cols].cpu().numpy()
+    err_l = float(np.max(np.abs(gl - rl) / np.maximum(np.abs(rl), 1)))
+
+    out = {
+        "metric": "Counts column*branches/sec (FP32, HKY85 127-node tree, batched persistent path)",
+        "value": value, "unit": "column*branches/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded simulation under the model)",
+        "config": {"workload": "HKY85 kappa=2 pi~Dir(5,5,5,5); 64-leaf random tree (R=127), "
+                               "t~Exp(0.05); C=200000; 3% gaps, 1% ambiguous; int8 alignment; "
+                               "Counts+logLike+root, mask auto", "R": R, "C": C, "A": A,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
+                     "alg_bytes_per_launch": balg},
+        "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
+    }
+
+    if not args.no_plain:
+        log('bench: plain path on the same block')
+        counts_plain(aln, t, m, engine=eng)
+        torch.cuda.synchronize()
+        reps = 5
+        s = time.perf_counter()
+        for _ in range(reps):
+            counts_plain(aln, t, m, engine=eng)
+        torch.cuda.synchronize()
+        pt = (time.perf_counter() - s) / reps
+        out["plain"] = {"value": units / pt, "ms_per_call": pt * 1e3, "unit": "column*branches/s",
+                        "speedup_fused_vs_plain": value / (units / pt)}
+
+        log(f'bench: block stream ({args.stream_blocks} blocks, 127-node trees)')
+        stream = make_stream(sb, args.stream_blocks)
+        dblocks = [Block(torch.from_numpy(a).to(dev), tt, mm) for tt, mm, a in stream]
+        sunits = sum(a.shape[1] * (len(tt.parentIndex) - 1) for tt, mm, a in stream)
+        L2 = Launcher(eng)
+        sprep = L2.prepare(dblocks)
+        L2.launch(sprep, use_graph=True)
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        for _ in range(5):
+            L2.launch(sprep, use_graph=True)
+        torch.cuda.synchronize()
+        bt = (time.perf_counter() - s) / 5
+        for tt, mm, a in stream[:3]:
+            counts_plain(a, tt, mm, engine=eng)
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        for b in dblocks:
+            counts_plain(b.alignment, b.tree, b.model, engine=eng)
+        torch.cuda.synchronize()
+        ptot = time.perf_counter() - s
+        out["batched_vs_plain"] = {
+            "blocks": len(stream), "columns": int(sum(a.shape[1] for _, _, a in stream)),
+            "batched": sunits / bt, "plain": sunits / ptot, "ratio": (sunits / bt) / (sunits / ptot),
+            "unit": "column*branches/s"}
+
+    if not args.no_cpu:
+        log('bench: CPU baseline (FP64 oracle, bounded sample)')
+        out["cpu_baseline"] = cpu_baseline(t, m, aln_np)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

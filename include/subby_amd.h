@@ -129,11 +129,18 @@ int sb_batch_destroy(sb_batch* b);
  * use_graph != 0 replays a captured hipGraph of the kernel sequence. */
 int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype,
                     int use_graph);
-/* Tile shape: waves per workgroup (1,2,4,8) and resident waves per CU used to
+/* Tile shape: waves per workgroup (1, 2 or 4) and resident waves per CU used to
  * size the persistent grid; 0 keeps the current value. */
 int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
 /* Block until the launcher's last submission has finished. */
 int sb_batch_sync(sb_batch* b);
+/* Kernel timing: when enabled, eager (non-graph) submissions bracket the
+ * main kernel with HIP events on the launcher's stream; sb_batch_kernel_ms
+ * synchronises, returns the summed main-kernel time and the number of
+ * launches since the last call, and resets both.  (No reference counterpart:
+ * measurement hook for the benchmark.) */
+int sb_batch_set_timing(sb_batch* b, int enable);
+int sb_batch_kernel_ms(sb_batch* b, double* total_ms, int* launches);
 
 /* Plain path: the same statistics through one launch per tree level per pass
  * with inside/outside partials materialised in HBM (the data flow of the

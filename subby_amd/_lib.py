@@ -19,7 +19,7 @@ EXPORTS = [
     'sb_last_error', 'sb_version', 'sb_create', 'sb_destroy', 'sb_model_put',
     'sb_model_drop', 'sb_loglike', 'sb_root_prob', 'sb_counts', 'sb_loglike_grad',
     'sb_batch_create', 'sb_batch_destroy', 'sb_batch_submit', 'sb_batch_set_tiling',
-    'sb_batch_sync', 'sb_counts_plain',
+    'sb_batch_sync', 'sb_batch_set_timing', 'sb_batch_kernel_ms', 'sb_counts_plain',
 ]
 
 
@@ -79,6 +79,8 @@ def load(path=LIB_PATH):
     lib.sb_batch_submit.argtypes = [vp, i, vp, i, i]
     lib.sb_batch_set_tiling.argtypes = [vp, i, i]
     lib.sb_batch_sync.argtypes = [vp]
+    lib.sb_batch_set_timing.argtypes = [vp, i]
+    lib.sb_batch_kernel_ms.argtypes = [vp, P(d), P(i)]
     lib.sb_counts_plain.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp, vp]
     for name in EXPORTS:
         if name not in ('sb_last_error',):

@@ -73,6 +73,14 @@ class Launcher:
 
         Output tensors are allocated (on this stream) unless the blocks already
         carry them.  Returns the blocks."""
+        arr = self.submit_arrays(blocks, precision, allocate)
+        dt, _ = _PREC[precision]
+        _lib.check(self.eng.lib.sb_batch_submit(self.h, len(blocks), arr, dt, 1 if use_graph else 0))
+        return blocks
+
+    def submit_arrays(self, blocks, precision: str = 'fp32', allocate: bool = True):
+        """Build the sb_block array for ``blocks`` (outputs allocated here).  Host
+        tree arrays stay referenced by the blocks for replays."""
         dt, tdt = _PREC[precision]
         dev = self.eng.device
         arr = (_lib.SbBlock * len(blocks))()
@@ -115,14 +123,29 @@ class Launcher:
                 s.d_counts = b.counts.data_ptr() if b.counts is not None else None
                 s.d_root = b.root.data_ptr() if b.root is not None else None
                 s.d_loglike = b.loglike.data_ptr() if b.loglike is not None else None
-        _lib.check(self.eng.lib.sb_batch_submit(self.h, len(blocks), arr, dt, 1 if use_graph else 0))
-        for b in blocks:
-            # host tree arrays were copied into the staging buffer by the planner
-            b._keep = [k for k in b._keep if isinstance(k, torch.Tensor)]
-        return blocks
+        return arr
 
     def sync(self):
         _lib.check(self.eng.lib.sb_batch_sync(self.h))
+
+    def set_timing(self, enable=True):
+        _lib.check(self.eng.lib.sb_batch_set_timing(self.h, 1 if enable else 0))
+
+    def kernel_ms(self):
+        """(summed main-kernel milliseconds, launches) since the last call."""
+        tot, n = ctypes.c_double(), ctypes.c_int()
+        _lib.check(self.eng.lib.sb_batch_kernel_ms(self.h, ctypes.byref(tot), ctypes.byref(n)))
+        return tot.value, n.value
+
+    def prepare(self, blocks, precision='fp32'):
+        """Allocate outputs and build the sb_block array once; replay with launch()."""
+        self._prepared = (self.submit_arrays(blocks, precision), precision)
+        return self._prepared
+
+    def launch(self, prepared=None, use_graph=True):
+        arr, precision = prepared or self._prepared
+        dt, _ = _PREC[precision]
+        _lib.check(self.eng.lib.sb_batch_submit(self.h, len(arr), arr, dt, 1 if use_graph else 0))
 
 
 def counts_plain(alignment, tree, model, branch_mask='auto', engine: Engine | None = None):

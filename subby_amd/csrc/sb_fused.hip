@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(64) sb_branch_params_kernel(const SbLaunch* __
     double s = 0.0;
     for (int q = 0; q < AP; ++q) s = fma(V[i * AP + q] * E[q], V[j * AP + q], s);
     o[sb_bt_M(AP) + idx] = T(s * sq[j] / sq[i]);
+    if (AP > 8) o[sb_bt_Mt(AP) + j * AP + i] = T(s * sq[j] / sq[i]);
     const double d = mu[i] - mu[j];
     double J;
     if (fabs(d) < 1e-8) J = t * E[i];

@@ -114,6 +114,10 @@ struct sb_batch {
   int waves_per_group = 4;
   int waves_per_cu = 8;
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+  // kernel timing (eager submissions only)
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;  // recorded pairs
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tfree;
 };
 
 namespace {
@@ -219,11 +223,26 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
   hipStream_t s = b->stream;
   SB_HIP(sb_launch_branch_params(key.dtype, nwg_params, b->header, s));
   SB_HIP(sb_launch_reset(b->header, s));
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if (b->timing) {
+    if (!b->tfree.empty()) {
+      ev = b->tfree.back();
+      b->tfree.pop_back();
+    } else {
+      SB_HIP(hipEventCreate(&ev.first));
+      SB_HIP(hipEventCreate(&ev.second));
+    }
+    SB_HIP(hipEventRecord(ev.first, s));
+  }
   if (key.large)
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
                            b->header, s));
   else
     SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, grid, block, b->header, s));
+  if (b->timing) {
+    SB_HIP(hipEventRecord(ev.second, s));
+    b->tev.push_back(ev);
+  }
   if (key.grad) SB_HIP(sb_launch_grad_reduce(nwg_reduce, b->header, s));
   return SB_OK;
 }
@@ -238,9 +257,12 @@ int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_par
   hipStream_t cap = nullptr;
   SB_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipStream_t saved = b->stream;
+  const bool saved_timing = b->timing;
   b->stream = cap;
+  b->timing = false;
   SB_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
   int rc = enqueue(b, key, grid, block, nwaves, nwg_params, nwg_reduce);
+  b->timing = saved_timing;
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(cap, &graph);
   b->stream = saved;
@@ -304,7 +326,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     R_max = std::max(R_max, k.R);
     nsched += plans[i].recs.size();
     njobs += k.R;
-    ntiles += (size_t(k.C) + SB_WAVE - 1) / SB_WAVE;
+    (void)0;
     nrate += size_t(in[i].n_rate) * AP * AP;
   }
   for (int i = 0; i < nb; ++i) {
@@ -315,6 +337,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
+  const int tile_cols = large ? SB_LARGE_TC : SB_WAVE;
+  for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
   // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights
   size_t off = 0;
@@ -397,7 +421,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     d.flags = fl;
     d.A = m->A;
     d.tile_base = int(tb);
-    d.ntiles = int((size_t(k.C) + SB_WAVE - 1) / SB_WAVE);
+    d.ntiles = int((size_t(k.C) + tile_cols - 1) / tile_cols);
     std::memcpy(hs + ps, plans[i].recs.data(), plans[i].recs.size() * sizeof(int));
     ps += plans[i].recs.size();
     for (int n = 0; n < k.R; ++n) {
@@ -441,6 +465,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   L->nint_max = nint_max;
   L->grad_stride = grad_stride;
   L->AP = AP;
+  L->tile_cols = tile_cols;
 
   hipStream_t s = b->stream;
   SB_HIP(hipMemcpyAsync(b->header, h + o_hdr, sizeof(SbLaunch), hipMemcpyHostToDevice, s));
@@ -486,6 +511,11 @@ void batch_free(sb_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
   for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto* v : {&b->tev, &b->tfree})
+    for (auto& e : *v) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
   for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage})
     if (d->p) (void)hipFree(d->p);
   if (b->header) (void)hipFree(b->header);
@@ -619,6 +649,35 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
     rec[sb_mr_ones(AP) + a] = a < A ? 1.0 : 0.0;
   }
   rec[sb_mr_misc(AP)] = f81;
+  // F81 structure: one zero eigenvalue with eigenvector +-sqrt(pi), all others
+  // equal to lambda.  Then M(t) = e^{lambda t} I + (1 - e^{lambda t}) 1 pi^T,
+  // which the large-alphabet kernel applies in O(A).
+  {
+    int k0 = -1, ks = -1;
+    double lam = 0.0, scale = 0.0;
+    for (int k = 0; k < A; ++k) scale = std::max(scale, std::fabs(eigenvalues[k]));
+    bool ok = scale > 0.0;
+    for (int k = 0; k < A && ok; ++k) {
+      if (std::fabs(eigenvalues[k]) <= 1e-12 * scale) {
+        if (k0 >= 0) ok = false;
+        k0 = k;
+      } else if (ks < 0) {
+        ks = k;
+        lam = eigenvalues[k];
+      } else if (std::fabs(eigenvalues[k] - lam) > 1e-12 * scale) {
+        ok = false;
+      }
+    }
+    if (ok && k0 >= 0 && ks >= 0) {
+      double dot = 0.0;
+      for (int a = 0; a < A; ++a) dot += eigenvectors[a * A + k0] * std::sqrt(pi[a]);
+      ok = std::fabs(std::fabs(dot) - 1.0) < 1e-10;
+    } else {
+      ok = false;
+    }
+    rec[sb_mr_misc(AP) + 1] = ok ? 1.0 : 0.0;
+    rec[sb_mr_misc(AP) + 2] = ok ? double(ks) : 0.0;
+  }
   std::vector<float> recf(rec.begin(), rec.end());
 
   Model m;
@@ -812,6 +871,28 @@ int sb_counts_plain(sb_context* ctx, int model_id, const int8_t* d_alignment, co
   SB_HIP(sb_plain_run(d_alignment, parent_index, R, C, static_cast<const float*>(ctx->plain_bt.p),
                       static_cast<const float*>(m->rec[0]), mask_mode, d_counts, d_root, d_loglike,
                       &ctx->plain, s));
+  return SB_OK;
+}
+
+int sb_batch_set_timing(sb_batch* b, int enable) {
+  if (!b) return fail(SB_ERR_ARG, "null launcher");
+  b->timing = enable != 0;
+  return SB_OK;
+}
+
+int sb_batch_kernel_ms(sb_batch* b, double* total_ms, int* launches) {
+  if (!b || !total_ms || !launches) return fail(SB_ERR_ARG, "null argument");
+  double tot = 0.0;
+  for (auto& e : b->tev) {
+    SB_HIP(hipEventSynchronize(e.second));
+    float ms = 0.f;
+    SB_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    tot += ms;
+    b->tfree.push_back(e);
+  }
+  *launches = int(b->tev.size());
+  b->tev.clear();
+  *total_ms = tot;
   return SB_OK;
 }
 

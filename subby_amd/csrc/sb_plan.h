@@ -13,11 +13,16 @@
 
 #define SB_WAVE 64
 #define SB_MAX_RATE 8
+#define SB_LARGE_TC 4   // columns per wave tile on the large-alphabet path
 
 // Per-branch parameter record, element type T (float or double), padded
 // alphabet AP:  M[AP*AP] | J[AP*AP] | coef[4] = {alpha, beta, gamma, t} |
-//               E[AP] = exp(mu_k t) | dE[AP] = mu_k exp(mu_k t)
-__host__ __device__ inline int sb_bt_stride(int AP) { return 2 * AP * AP + 4 + 2 * AP; }
+//               E[AP] = exp(mu_k t) | dE[AP] = mu_k exp(mu_k t) |
+//               (AP > 8 only) Mt[AP*AP] = M transposed
+__host__ __device__ inline int sb_bt_stride(int AP) {
+  return (AP > 8 ? 3 : 2) * AP * AP + 4 + 2 * AP;
+}
+__host__ __device__ inline int sb_bt_Mt(int AP) { return 2 * AP * AP + 4 + 2 * AP; }
 __host__ __device__ inline int sb_bt_M(int) { return 0; }
 __host__ __device__ inline int sb_bt_J(int AP) { return AP * AP; }
 __host__ __device__ inline int sb_bt_coef(int AP) { return 2 * AP * AP; }
@@ -32,7 +37,7 @@ __host__ __device__ inline int sb_bt_dE(int AP) { return 2 * AP * AP + 4 + AP; }
 //   S[AP*AP]   V diag(mu) V^T          (back-transform,     eigensub.py:138-172)
 //   pi[AP]     (pads 0)
 //   ones[AP]   1 for real states, 0 for pads (unobserved-token likelihood)
-//   misc[4]    {f81_mu, 0, 0, 0}
+//   misc[4]    {f81_mu, F81-structured (0/1), index of a nonzero eigenvalue, 0}
 __host__ __device__ inline int sb_mr_V(int) { return 0; }
 __host__ __device__ inline int sb_mr_Vs(int AP) { return AP * AP; }
 __host__ __device__ inline int sb_mr_Vi(int AP) { return 2 * AP * AP; }
@@ -98,5 +103,5 @@ struct SbLaunch {
   long long wave_stride;
   long long stage_stride;  // bytes per staged block column-branch vector set
   int nblocks, ntiles, njobs, nint_max;
-  int grad_stride, AP, pad0, pad1;
+  int grad_stride, AP, tile_cols, pad1;
 };

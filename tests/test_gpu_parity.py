@@ -321,3 +321,39 @@ def test_full_size_hky85_properties(sb, eng):
     assert err_counts(cnt[:, :, cols], rc) <= 1e-4
     assert err_ll(ll[cols], rl) <= 1e-5
     assert err_root(root[:, cols], rr) <= 1e-5
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_golden_jc64(sb, eng, precision):
+    say('golden 7node_jc64', precision)
+    g = load('7node_jc64')
+    m = sb.DiagModel(g['inputs.model.eigenvalues'], g['inputs.model.eigenvectors'], g['inputs.model.pi'])
+    t = sb.Tree(g['inputs.parentIndex'], g['inputs.distanceToParent'])
+    aln = g['inputs.alignment']
+    tol = TOL32 if precision == 'fp32' else TOL64
+    for f81 in (False, True):
+        cnt, ll, root = sb.Counts(aln, t, m, branch_mask=None, f81_fast_flag=f81, precision=precision,
+                                  engine=eng, return_loglike=True, return_root=True)
+        assert err_ll(ll, g['outputs.logLike']) <= tol['ll']
+        assert err_counts(cnt, g['outputs.counts']) <= tol['counts']
+        assert err_root(root, g['outputs.root_prob']) <= tol['root']
+
+
+@pytest.mark.parametrize('A', [12, 20, 61, 64])
+def test_large_alphabets(sb, eng, A):
+    say('large alphabet', A)
+    rng = np.random.default_rng(40 + A)
+    pi = rng.dirichlet(np.full(A, 2.0))
+    Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi)
+    m = sb.diagonalize_rate_matrix(Q, pi)
+    t = sb.synthetic.random_tree(15, rng, 0.1)
+    aln = sb.synthetic.simulate(t, m, 203, rng, gap_frac=0.05, ambig_frac=0.02)
+    check_all(sb, eng, aln, t, m, 'fp32')
+    check_all(sb, eng, aln, t, m, 'fp64')
+    f = sb.f81_model(pi)
+    check_all(sb, eng, aln, t, f, 'fp32', f81=True)
+    check_all(sb, eng, aln, t, f, 'fp64', f81=True)
+    check_all(sb, eng, aln, t, f, 'fp32', mask=None)
+    d, _, _ = sb.loglike_grad(aln, t, m, precision='fp64', engine=eng)
+    ref, _ = O.loglike_branch_grad(aln, otree(t), omodel(m))
+    assert np.abs(d.cpu().numpy() - ref).max() <= 1e-9 * np.abs(ref).max()
