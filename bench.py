@@ -99,6 +99,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--stream-blocks', type=int, default=300)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build '
@@ -111,6 +113,16 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log('bench: copy calibration done (3 x 1 GiB read + 1 GiB write)')
 
     log('bench: building the HKY85 127-node workload (C = 200,000)')
     t, m, aln_np = make_c2(sb)

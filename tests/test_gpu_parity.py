@@ -67,8 +67,8 @@ def err_root(got, ref):
     return float(np.max(np.abs(got - ref))) if ref.size else 0.0
 
 
-def check_all(sb, eng, aln, tree, model, precision='fp32', mask='auto', f81=False):
-    tol = TOL32 if precision == 'fp32' else TOL64
+def check_all(sb, eng, aln, tree, model, precision='fp32', mask='auto', f81=False, tol=None):
+    tol = tol or (TOL32 if precision == 'fp32' else TOL64)
     ot, om = otree(tree), omodel(model)
     ref_cnt, ref_ll, ref_root = O.counts_loglike_root(np.asarray(aln), ot, om, f81_fast=f81, branch_mask=mask)
     cnt, ll, root = sb.Counts(aln, tree, model, f81_fast_flag=f81, branch_mask=mask, precision=precision,
@@ -349,7 +349,11 @@ def test_large_alphabets(sb, eng, A):
     t = sb.synthetic.random_tree(15, rng, 0.1)
     aln = sb.synthetic.simulate(t, m, 203, rng, gap_frac=0.05, ambig_frac=0.02)
     check_all(sb, eng, aln, t, m, 'fp32')
-    check_all(sb, eng, aln, t, m, 'fp64')
+    # FP64 bound 1e-10 for a general 64-state model: its closest eigenvalues are
+    # 0.002 apart, and the oracle's (reference's) J = (e^{mu_k t} - e^{mu_l t})/(mu_k - mu_l)
+    # loses digits there; the kernel's e^{mu_l t} expm1((mu_k - mu_l) t)/(mu_k - mu_l)
+    # does not (tests/test_oracle_invariants.py::test_J_expm1_form_is_the_accurate_one).
+    check_all(sb, eng, aln, t, m, 'fp64', tol=dict(ll=1e-12, counts=1e-10, root=1e-12))
     f = sb.f81_model(pi)
     check_all(sb, eng, aln, t, f, 'fp32', f81=True)
     check_all(sb, eng, aln, t, f, 'fp64', f81=True)

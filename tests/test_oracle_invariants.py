@@ -322,3 +322,29 @@ def test_scaling_identity_links_gradient_and_counts():
     off = ~np.eye(4, dtype=bool)
     rhs = (np.where(off[..., None], N, 0).sum((0, 1)) + np.einsum('i,iic->c', np.diag(Q), N)).sum()
     np.testing.assert_allclose(tree.distanceToParent @ grad, rhs, rtol=1e-10)
+
+
+def test_J_expm1_form_is_the_accurate_one():
+    """For close eigenvalues the reference's J = (e^{mu_k t} - e^{mu_l t})/(mu_k - mu_l)
+    cancels; e^{mu_l t} expm1((mu_k - mu_l) t)/(mu_k - mu_l) (the form the kernels
+    evaluate in FP64) agrees with an 80-bit extended-precision evaluation to a few ulp."""
+    mu = np.array([-1.4474406912345678, -1.4474406912345678 + 3e-6, -1.4266377, -0.7])
+    t = np.array([0.0, 0.05, 0.3])
+    ref = O.compute_J(mu, t)
+    mul = mu.astype(np.longdouble)
+    tl = t.astype(np.longdouble)[:, None, None]
+    ek = np.exp(mul[None, :, None] * tl)
+    el = np.exp(mul[None, None, :] * tl)
+    d = (mul[:, None] - mul[None, :])[None]
+    deg = np.abs(d) < 1e-8
+    with np.errstate(all='ignore'):
+        exact = np.where(deg, tl * ek, el * np.expm1(d * tl) / np.where(deg, 1, d))
+    with np.errstate(all='ignore'):
+        d64 = (mu[:, None] - mu[None, :])[None]
+        alt = np.where(np.abs(d64) < 1e-8, t[:, None, None] * np.exp(mu[None, :, None] * t[:, None, None]),
+                       np.exp(mu[None, None, :] * t[:, None, None]) * np.expm1(d64 * t[:, None, None])
+                       / np.where(np.abs(d64) < 1e-8, 1, d64))
+    err_alt = float(np.max(np.abs(alt - exact) / np.maximum(np.abs(exact), 1e-300)))
+    err_ref = float(np.max(np.abs(ref - exact) / np.maximum(np.abs(exact), 1e-300)))
+    assert err_alt < 1e-14
+    assert err_ref > 10 * err_alt
