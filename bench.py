@@ -132,6 +132,9 @@ def main():
     aln = torch.from_numpy(aln_np).to(dev)
     L = Launcher(eng)
     blk = Block(aln, t, m)
+    from subby_amd.autotune import tune
+    tuned = tune(L, [blk])
+    log(f"bench: autotuned launch shape {tuned['best']} ({tuned['ms']:.3f} ms)")
     prep = L.prepare([blk])
     torch.cuda.synchronize()
 
@@ -182,6 +185,8 @@ def main():
                      "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
         "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
+        "launch_shape": {"waves_per_group": tuned['best'][0], "waves_per_cu": tuned['best'][1],
+                         "autotune_table": tuned['table']},
     }
 
     if not args.no_plain:
@@ -202,6 +207,7 @@ def main():
         dblocks = [Block(torch.from_numpy(a).to(dev), tt, mm) for tt, mm, a in stream]
         sunits = sum(a.shape[1] * (len(tt.parentIndex) - 1) for tt, mm, a in stream)
         L2 = Launcher(eng)
+        tune(L2, dblocks)
         sprep = L2.prepare(dblocks)
         L2.launch(sprep, use_graph=True)
         torch.cuda.synchronize()

@@ -129,8 +129,9 @@ int sb_batch_destroy(sb_batch* b);
  * use_graph != 0 replays a captured hipGraph of the kernel sequence. */
 int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype,
                     int use_graph);
-/* Tile shape: waves per workgroup (1, 2 or 4) and resident waves per CU used to
- * size the persistent grid; 0 keeps the current value. */
+/* Tile shape of the persistent grid: waves per workgroup (1, 2 or 4; 0 keeps
+ * the current value) and waves per CU (1..32; 0 = as many as the kernel's
+ * occupancy allows, the default; negative keeps the current value). */
 int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
 /* Block until the launcher's last submission has finished. */
 int sb_batch_sync(sb_batch* b);

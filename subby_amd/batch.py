@@ -45,16 +45,16 @@ class Launcher:
     """A sb_batch bound to one HIP stream (several may run concurrently)."""
 
     def __init__(self, engine: Engine | None = None, stream: torch.cuda.Stream | None = None,
-                 waves_per_group: int = 0, waves_per_cu: int = 0):
+                 waves_per_group: int = 0, waves_per_cu: int = -1):
         self.eng = engine or default_engine()
         self.stream = stream or torch.cuda.current_stream(self.eng.device)
         self.h = ctypes.c_void_p()
         _lib.check(self.eng.lib.sb_batch_create(self.eng.ctx, ctypes.c_void_p(self.stream.cuda_stream),
                                                 ctypes.byref(self.h)))
-        if waves_per_group or waves_per_cu:
+        if waves_per_group or waves_per_cu >= 0:
             self.set_tiling(waves_per_group, waves_per_cu)
 
-    def set_tiling(self, waves_per_group=0, waves_per_cu=0):
+    def set_tiling(self, waves_per_group=0, waves_per_cu=-1):
         _lib.check(self.eng.lib.sb_batch_set_tiling(self.h, waves_per_group, waves_per_cu))
 
     def close(self):

@@ -536,8 +536,7 @@ SB_INST(double, 8, false, true)
 template __global__ void sb_branch_params_kernel<float>(const SbLaunch*);
 template __global__ void sb_branch_params_kernel<double>(const SbLaunch*);
 
-extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
-                                            dim3 block, const SbLaunch* L, hipStream_t s) {
+static void (*pick_small(int dtype, int AP, bool f81, bool grad))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
 #define SB_PICK(T, APV)                                                          \
   if (AP == APV) {                                                               \
@@ -548,6 +547,21 @@ extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool gr
   if (dtype == 0) { SB_PICK(float, 4) SB_PICK(float, 8) }
   else { SB_PICK(double, 4) SB_PICK(double, 8) }
 #undef SB_PICK
+  return k;
+}
+
+// Resident workgroups per CU for a block of `threads` threads (0 on error).
+extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad);
+  if (!k) return 0;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
+                                            dim3 block, const SbLaunch* L, hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad);
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, grid, block, 0, s, L);
   return hipGetLastError();

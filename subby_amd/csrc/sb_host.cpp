@@ -25,6 +25,8 @@ extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool gr
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max);
+extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
+extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
@@ -94,9 +96,10 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large;
+  int dtype, AP, f81, grad, large, nwaves, wpg;
   bool operator==(const GraphKey& o) const {
-    return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large;
+    return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
+           nwaves == o.nwaves && wpg == o.wpg;
   }
 };
 
@@ -112,7 +115,7 @@ struct sb_batch {
   hipEvent_t done = nullptr;        // last submission finished
   bool staged_pending = false;
   int waves_per_group = 4;
-  int waves_per_cu = 8;
+  int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
   bool timing = false;
@@ -366,9 +369,23 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
   if (rc) return rc;
 
-  // persistent grid
-  const int wpg = b->waves_per_group;
-  int nwaves = ctx->num_cus * b->waves_per_cu;
+  // persistent grid: resident waves per CU from the kernel's occupancy unless
+  // the launcher was given an explicit value; never more waves than tiles
+  const int wpg = large ? 1 : b->waves_per_group;
+  int wpc = b->waves_per_cu;
+  if (wpc <= 0) {
+    const int occ = large ? sb_large_occupancy(dtype, AP, f81, grad)
+                          : sb_fused_small_occupancy(dtype, AP, f81, grad, 64 * wpg);
+    wpc = occ > 0 ? occ * wpg : 8;
+  }
+  int nwaves = ctx->num_cus * wpc;
+  {
+    // small batches launch fewer waves (a power of two >= the tile count, so
+    // few distinct grids and hence few captured graphs)
+    long long want = 64;
+    while (want < (long long)ntiles) want *= 2;
+    if (want < nwaves) nwaves = int(want);
+  }
   nwaves = (nwaves + wpg - 1) / wpg * wpg;
   long long wave_stride;
   if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max);
@@ -473,7 +490,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
-  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, large ? 1 : 0};
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, large ? 1 : 0, nwaves, wpg};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -798,8 +815,8 @@ int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu) {
       return fail(SB_ERR_ARG, "waves_per_group must be 1, 2 or 4");
     b->waves_per_group = waves_per_group;
   }
-  if (waves_per_cu) {
-    if (waves_per_cu < 1 || waves_per_cu > 32) return fail(SB_ERR_ARG, "waves_per_cu must be 1..32");
+  if (waves_per_cu >= 0) {
+    if (waves_per_cu > 32) return fail(SB_ERR_ARG, "waves_per_cu must be 0..32 (0: from occupancy)");
     b->waves_per_cu = waves_per_cu;
   }
   return SB_OK;

@@ -483,8 +483,7 @@ extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max) {
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
 
-extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int, int nwaves,
-                                      const SbLaunch* L, hipStream_t s) {
+static void (*pick_large(int dtype, int AP, bool f81, bool grad))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
 #define SB_LPICK(T, AWV)                                             \
   {                                                                  \
@@ -498,7 +497,21 @@ extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, in
     if (AP <= 32) SB_LPICK(double, 32) else SB_LPICK(double, 64)
   }
 #undef SB_LPICK
+  return k;
+}
+
+extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad) {
+  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad);
+  int n = 0;
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int, int nwaves,
+                                      const SbLaunch* L, hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad);
   if (!k || AP > 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
   return hipGetLastError();
 }
+
