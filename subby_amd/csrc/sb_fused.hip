@@ -94,6 +94,32 @@ __device__ inline void token_vec(int tok, int A, const SB_CONST T* ones, T* u) {
   for (int a = 0; a < AP; ++a) u[a] = obs ? (a == tok ? T(1) : T(0)) : ones[a];
 }
 
+// Branch-free variant with the real-state mask already in registers.
+template <typename T, int AP>
+__device__ inline void token_vec_r(int tok, int A, const T* ones, T* u) {
+  const T fill = (tok >= 0 && tok < A) ? T(0) : T(1);
+#pragma unroll
+  for (int a = 0; a < AP; ++a) u[a] = (a == tok) ? T(1) : fill * ones[a];
+}
+
+// Child data as loaded (decoded when used): internal -> vector + word, leaf -> token in w.
+template <typename T, int AP>
+struct Raw {
+  T u[AP];
+  int w;
+};
+
+template <typename T, int AP>
+__device__ inline void load_raw(Raw<T, AP>& r, int n, int slot, const SB_GLOBAL int8_t* aln, long long C,
+                                const SB_GLOBAL char* reg, const SB_GLOBAL int* words, int lane) {
+  if (slot < 0) {
+    r.w = aln[(long long)n * C];
+  } else {
+    Tile<T, AP>::load_vec(reg, slot, lane, r.u);
+    r.w = words[slot * SB_WAVE + lane];
+  }
+}
+
 // w = M u  (w_a = sum_j M[a][j] u_j)
 template <typename T, int AP>
 __device__ inline void matvec(const SB_CONST T* M, const T* u, T* w) {
@@ -163,6 +189,24 @@ __device__ inline void load_child(Child<T, AP>& ch, int n, int slot, const SB_GL
     ch.e = w >> 2;
     ch.has = (w & 1) != 0;
     ch.branch = (w & 2) != 0;
+    ch.leaf = false;
+  }
+}
+
+template <typename T, int AP>
+__device__ inline void decode_child(const Raw<T, AP>& r, int slot, int A, const T* ones, Child<T, AP>& ch) {
+  if (slot < 0) {
+    token_vec_r<T, AP>(r.w, A, ones, ch.u);
+    ch.e = 0;
+    ch.has = (r.w >= 0) && (r.w <= A);
+    ch.branch = false;
+    ch.leaf = true;
+  } else {
+#pragma unroll
+    for (int a = 0; a < AP; ++a) ch.u[a] = r.u[a];
+    ch.e = r.w >> 2;
+    ch.has = (r.w & 1) != 0;
+    ch.branch = (r.w & 2) != 0;
     ch.leaf = false;
   }
 }
@@ -272,34 +316,70 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     const SB_CONST T* bt = cst<T>(B->bt);
     const SB_CONST T* md = cst<T>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
-    const SB_CONST T* ones = md + sb_mr_ones(AP);
-    const SB_CONST T* pi = md + sb_mr_pi(AP);
+    T ones[AP], pi[AP];
+#pragma unroll
+    for (int a = 0; a < AP; ++a) {
+      ones[a] = md[sb_mr_ones(AP) + a];
+      pi[a] = md[sb_mr_pi(AP) + a];
+    }
     const int BTS = sb_bt_stride(AP);
 
     // ---------------- inside pass ----------------
+    // Internal nodes in postorder.  The next node's children are fetched one
+    // iteration ahead; a child that is the node being computed now (always the
+    // case for an internal left child) is forwarded from registers.
     T u0[AP];
     int e0 = 0;
     bool root_branch = false;
-    if (nint == 0) token_vec<T, AP>(aln[0], A, ones, u0);  // single-node tree
+    if (nint == 0) token_vec_r<T, AP>(aln[0], A, ones, u0);  // single-node tree
+    Raw<T, AP> nl, nr;
+    int np_ = 0, nl_ = 0, nr_ = 0, nsl = -1, nsr = -1;
+    if (nint > 0) {
+      const SB_CONST int* rec = sched + (nint - 1) * 8;
+      np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+      load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane);
+      load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane);
+    }
     for (int i = nint - 1; i >= 0; --i) {
-      const SB_CONST int* rec = sched + i * 8;
-      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
+      const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
+      const Raw<T, AP> rl = nl, rr = nr;
+      const int tokp = aln[(long long)p * C];
+      bool fwd_l = false, fwd_r = false;
+      if (i > 0) {
+        const SB_CONST int* rec = sched + (i - 1) * 8;
+        np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+        fwd_l = nsl == i;
+        fwd_r = nsr == i;
+        if (!fwd_l) load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane);
+        if (!fwd_r) load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane);
+      }
       Child<T, AP> cl, cr;
-      load_child<T, AP>(cl, l, sl, aln, C, A, ones, Ureg, Uw, lane);
-      load_child<T, AP>(cr, r, sr, aln, C, A, ones, Ureg, Uw, lane);
+      decode_child<T, AP>(rl, sl, A, ones, cl);
+      decode_child<T, AP>(rr, sr, A, ones, cr);
       T wl[AP], wr[AP], up[AP];
       matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), cl.u, wl);
       matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), cr.u, wr);
-      token_vec<T, AP>(aln[(long long)p * C], A, ones, up);
+      token_vec_r<T, AP>(tokp, A, ones, up);
 #pragma unroll
       for (int a = 0; a < AP; ++a) up[a] = up[a] * wl[a] * wr[a];
       const int e = rescale<T, AP>(up);
       const int ep = cl.e + cr.e + e;
       const bool has = cl.has || cr.has;
       const bool br = cl.has && cr.has;
+      const int word = (ep << 2) | (br ? 2 : 0) | (has ? 1 : 0);
       if (i > 0) {
         TL::store_vec(Ureg, i, lane, up);
-        Uw[i * SB_WAVE + lane] = (ep << 2) | (br ? 2 : 0) | (has ? 1 : 0);
+        Uw[i * SB_WAVE + lane] = word;
+        if (fwd_l) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) nl.u[a] = up[a];
+          nl.w = word;
+        }
+        if (fwd_r) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) nr.u[a] = up[a];
+          nr.w = word;
+        }
       } else {
 #pragma unroll
         for (int a = 0; a < AP; ++a) u0[a] = up[a];
@@ -336,31 +416,51 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     Acc<T, AP, F81> acc;
     acc.zero();
 
+    // Internal nodes in preorder.  The next node's children (read-only U
+    // region) and its own outside vector are fetched one iteration ahead; an
+    // outside vector computed in this iteration is forwarded from registers.
+    Raw<T, AP> ql, qr, qd;
+    int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
+    if (nint > 0) {
+      const SB_CONST int* rec = sched;
+      qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
+      load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane);
+      load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane);
+    }
     for (int i = 0; i < nint; ++i) {
-      const SB_CONST int* rec = sched + i * 8;
-      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
+      const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
+      const Raw<T, AP> rl = ql, rr = qr, rd = qd;
+      const int tokp = aln[(long long)p * C];
+      bool fwd0 = false, fwd1 = false;
+      if (i + 1 < nint) {
+        const SB_CONST int* rec = sched + (i + 1) * 8;
+        qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
+        fwd0 = qp == l;
+        fwd1 = qp == r;
+        if (!fwd0 && !fwd1) load_raw<T, AP>(qd, 0, i + 1, aln, C, Dreg, Dw, lane);
+        load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane);
+        load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane);
+      }
       T x[AP];
       int ex;
       bool stp;
-      token_vec<T, AP>(aln[(long long)p * C], A, ones, x);
+      token_vec_r<T, AP>(tokp, A, ones, x);
       if (i == 0) {
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= pi[a];
         ex = 0;
         stp = root_branch;
       } else {
-        T dp[AP], xm[AP];
-        TL::load_vec(Dreg, i, lane, dp);
-        const int w = Dw[i * SB_WAVE + lane];
-        ex = w >> 1;
-        stp = (w & 1) != 0;
-        vecmat<T, AP>(dp, bt + (long long)p * BTS + sb_bt_M(AP), xm);
+        T xm[AP];
+        ex = rd.w >> 1;
+        stp = (rd.w & 1) != 0;
+        vecmat<T, AP>(rd.u, bt + (long long)p * BTS + sb_bt_M(AP), xm);
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= xm[a];
       }
       Child<T, AP> ch[2];
-      load_child<T, AP>(ch[0], l, sl, aln, C, A, ones, Ureg, Uw, lane);
-      load_child<T, AP>(ch[1], r, sr, aln, C, A, ones, Ureg, Uw, lane);
+      decode_child<T, AP>(rl, sl, A, ones, ch[0]);
+      decode_child<T, AP>(rr, sr, A, ones, ch[1]);
       T w[2][AP];
       matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
       matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
@@ -374,9 +474,15 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         const int ed = ch[1 - s].e + ex + rescale<T, AP>(d);
         // Steiner membership (oracle.py:1398-1431)
         const bool st = (ch[s].leaf ? ch[s].has : ch[s].branch) || (stp && ch[s].has);
+        const int dword = (ed << 1) | (st ? 1 : 0);
         if (sn >= 0) {
           TL::store_vec(Dreg, sn, lane, d);
-          Dw[sn * SB_WAVE + lane] = (ed << 1) | (st ? 1 : 0);
+          Dw[sn * SB_WAVE + lane] = dword;
+        }
+        if ((s == 0 && fwd0) || (s == 1 && fwd1)) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) qd.u[a] = d[a];
+          qd.w = dword;
         }
         bool active;
         if (GRAD || mask_mode == SB_MASK_NONE) active = true;
