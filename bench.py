@@ -99,6 +99,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--stream-blocks', type=int, default=300)
+    ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()
@@ -133,8 +134,11 @@ def main():
     L = Launcher(eng)
     blk = Block(aln, t, m)
     from subby_amd.autotune import tune
-    tuned = tune(L, [blk])
-    log(f"bench: autotuned launch shape {tuned['best']} ({tuned['ms']:.3f} ms)")
+    if args.no_tune:
+        tuned = {'best': (4, 0), 'ms': None, 'table': []}
+    else:
+        tuned = tune(L, [blk])
+        log(f"bench: autotuned launch shape {tuned['best']} ({tuned['ms']:.3f} ms)")
     prep = L.prepare([blk])
     torch.cuda.synchronize()
 
@@ -207,7 +211,8 @@ def main():
         dblocks = [Block(torch.from_numpy(a).to(dev), tt, mm) for tt, mm, a in stream]
         sunits = sum(a.shape[1] * (len(tt.parentIndex) - 1) for tt, mm, a in stream)
         L2 = Launcher(eng)
-        tune(L2, dblocks)
+        if not args.no_tune:
+            tune(L2, dblocks)
         sprep = L2.prepare(dblocks)
         L2.launch(sprep, use_graph=True)
         torch.cuda.synchronize()

@@ -237,6 +237,7 @@ __global__ void __launch_bounds__(64) sb_branch_params_kernel(const SbLaunch* __
   const int njobs = L->njobs;
   SB_GLOBAL T* out = glb<T>(L->bt);
   __shared__ double mu[64], E[64], sq[64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *L->counter = 0;  // tile counter of the launch that follows
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
   const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
   const SB_CONST double* m = cst<double>(job->model64);

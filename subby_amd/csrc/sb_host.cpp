@@ -106,7 +106,7 @@ struct GraphKey {
 struct sb_batch {
   sb_context* ctx = nullptr;
   hipStream_t stream = nullptr;
-  SbLaunch* header = nullptr;       // fixed device address
+  SbLaunch* header = nullptr;       // start of the plan buffer (fixed until it grows)
   DevBuf plan, bt, scratch, part, stage;
   int* counter = nullptr;
   char* host = nullptr;             // pinned staging: header + plan
@@ -129,10 +129,10 @@ namespace {
 int ensure(sb_batch* b, DevBuf& buf, size_t bytes) {
   if (bytes <= buf.cap) return SB_OK;
   SB_HIP(hipStreamSynchronize(b->stream));
+  size_t want = std::max(bytes, buf.cap * 3 / 2);
   if (buf.p) SB_HIP(hipFree(buf.p));
   buf.p = nullptr;
   buf.cap = 0;
-  size_t want = std::max(bytes, buf.cap * 3 / 2);
   want = (want + 4095) & ~size_t(4095);
   if (hipMalloc(&buf.p, want) != hipSuccess) {
     (void)hipGetLastError();
@@ -149,9 +149,9 @@ int ensure_host(sb_batch* b, size_t bytes) {
     b->staged_pending = false;
   }
   if (bytes <= b->host_cap) return SB_OK;
+  size_t want = std::max(bytes, b->host_cap * 3 / 2);
   if (b->host) SB_HIP(hipHostFree(b->host));
   b->host = nullptr;
-  size_t want = std::max(bytes, b->host_cap * 3 / 2);
   want = (want + 4095) & ~size_t(4095);
   if (hipHostMalloc(reinterpret_cast<void**>(&b->host), want, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
@@ -224,8 +224,7 @@ int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_par
 int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves, int nwg_params,
             int nwg_reduce) {
   hipStream_t s = b->stream;
-  SB_HIP(sb_launch_branch_params(key.dtype, nwg_params, b->header, s));
-  SB_HIP(sb_launch_reset(b->header, s));
+  SB_HIP(sb_launch_branch_params(key.dtype, nwg_params, b->header, s));  // also zeroes the tile counter
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if (b->timing) {
     if (!b->tfree.empty()) {
@@ -347,7 +346,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   size_t off = 0;
   const size_t o_hdr = off;
   off = align_up(off + sizeof(SbLaunch), 256);
-  const size_t o_plan = off;
   const size_t o_blocks = off;
   off = align_up(off + sizeof(SbBlockDesc) * nb, 256);
   const size_t o_sched = off;
@@ -359,12 +357,20 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   const size_t o_rate = off;
   off = align_up(off + sizeof(double) * nrate, 256);
   const size_t total = off;
-  const size_t plan_bytes = total - o_plan;
 
   int rc = ensure_host(b, total);
   if (rc) return rc;
-  rc = ensure(b, b->plan, plan_bytes);
-  if (rc) return rc;
+  {
+    void* before = b->plan.p;
+    rc = ensure(b, b->plan, total);
+    if (rc) return rc;
+    if (b->plan.p != before) {
+      // captured graphs embed the header address: drop them
+      for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
+      b->graphs.clear();
+      b->header = static_cast<SbLaunch*>(b->plan.p);
+    }
+  }
   const int bts = sb_bt_stride(AP);
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
   if (rc) return rc;
@@ -401,7 +407,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
 
   char* h = b->host;
   char* dplan = static_cast<char*>(b->plan.p);
-  auto dev = [&](size_t o) { return dplan + (o - o_plan); };
+  auto dev = [&](size_t o) { return dplan + o; };
   SbBlockDesc* hb = reinterpret_cast<SbBlockDesc*>(h + o_blocks);
   int* hs = reinterpret_cast<int*>(h + o_sched);
   int2* ht = reinterpret_cast<int2*>(h + o_tiles);
@@ -485,8 +491,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   L->tile_cols = tile_cols;
 
   hipStream_t s = b->stream;
-  SB_HIP(hipMemcpyAsync(b->header, h + o_hdr, sizeof(SbLaunch), hipMemcpyHostToDevice, s));
-  SB_HIP(hipMemcpyAsync(b->plan.p, h + o_plan, plan_bytes, hipMemcpyHostToDevice, s));
+  SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
@@ -512,8 +517,7 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
   sb_batch* b = new sb_batch();
   b->ctx = ctx;
   b->stream = stream;
-  if (hipMalloc(reinterpret_cast<void**>(&b->header), sizeof(SbLaunch)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&b->counter), 256) != hipSuccess) {
+  if (hipMalloc(reinterpret_cast<void**>(&b->counter), 256) != hipSuccess) {
     (void)hipGetLastError();
     delete b;
     return fail(SB_ERR_OOM, "launcher allocation failed");
@@ -535,7 +539,6 @@ void batch_free(sb_batch* b) {
     }
   for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage})
     if (d->p) (void)hipFree(d->p);
-  if (b->header) (void)hipFree(b->header);
   if (b->counter) (void)hipFree(b->counter);
   if (b->host) (void)hipHostFree(b->host);
   if (b->staged) (void)hipEventDestroy(b->staged);
