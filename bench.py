@@ -34,6 +34,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, 'profiles', 'fused_small_hky85_c2.md')
+
+
+def profiled_traffic(path=PROFILE):
+    """HBM bytes per fused launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch:'):
+                    return int(float(line.split(':', 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def b_alg_counts(R, A, C):
@@ -185,7 +199,7 @@ def main():
                                "Counts+logLike+root, mask auto", "R": R, "C": C, "A": A,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
         "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
