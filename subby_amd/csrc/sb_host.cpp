@@ -83,11 +83,32 @@ struct Model {
 
 }  // namespace
 
+struct ModelRun {  // one pending host->device copy of model records
+  char* dst;
+  size_t src, len;
+};
+
 struct sb_context {
   int device = 0;
   int num_cus = 256;
   std::vector<Model> models;
   std::mutex mu;
+  // Model arena: records of every model live in large device chunks that never
+  // move.  sb_model_put stages the records in pinned memory; the next
+  // submission on any stream flushes them (merged copies) before its kernels,
+  // and every submission waits for the last flush.
+  std::vector<void*> arena_chunks;
+  char* arena_cur = nullptr;
+  size_t arena_off = 0, arena_cap = 0;
+  char* pend[2] = {nullptr, nullptr};
+  size_t pend_cap[2] = {0, 0};
+  size_t pend_len = 0;
+  int pend_k = 0;
+  bool pend_busy[2] = {false, false};
+  hipEvent_t pend_ev[2] = {nullptr, nullptr};
+  std::vector<ModelRun> runs;
+  hipEvent_t model_ev = nullptr;
+  bool model_ev_valid = false;
   sb_batch* internal = nullptr;
   SbPlainBuffers plain;
   DevBuf plain_bt;
@@ -206,6 +227,76 @@ int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
 Model* get_model(sb_context* ctx, int id) {
   if (id < 0 || id >= int(ctx->models.size()) || !ctx->models[id].alive) return nullptr;
   return &ctx->models[id];
+}
+
+// Reserve `bytes` (multiple of 256) in the model arena.
+int arena_alloc(sb_context* ctx, size_t bytes, char** out) {
+  if (!ctx->arena_cur || ctx->arena_off + bytes > ctx->arena_cap) {
+    const size_t chunk = std::max<size_t>(64u << 20, bytes);
+    void* p = nullptr;
+    if (hipMalloc(&p, chunk) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SB_ERR_OOM, "model arena allocation of %zu bytes failed", chunk);
+    }
+    ctx->arena_chunks.push_back(p);
+    ctx->arena_cur = static_cast<char*>(p);
+    ctx->arena_off = 0;
+    ctx->arena_cap = chunk;
+  }
+  *out = ctx->arena_cur + ctx->arena_off;
+  ctx->arena_off += bytes;
+  return SB_OK;
+}
+
+// Append bytes destined for device address dst to the pending model staging.
+int stage_model_bytes(sb_context* ctx, char* dst, const void* src, size_t len) {
+  int k = ctx->pend_k;
+  if (ctx->pend_busy[k]) {
+    SB_HIP(hipEventSynchronize(ctx->pend_ev[k]));
+    ctx->pend_busy[k] = false;
+  }
+  if (ctx->pend_len + len > ctx->pend_cap[k]) {
+    size_t want = std::max<size_t>({ctx->pend_len + len, ctx->pend_cap[k] * 2, size_t(1) << 20});
+    char* nb = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&nb), want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SB_ERR_OOM, "pinned model staging allocation failed");
+    }
+    if (ctx->pend[k]) {
+      std::memcpy(nb, ctx->pend[k], ctx->pend_len);
+      SB_HIP(hipHostFree(ctx->pend[k]));
+    }
+    ctx->pend[k] = nb;
+    ctx->pend_cap[k] = want;
+  }
+  std::memcpy(ctx->pend[k] + ctx->pend_len, src, len);
+  if (!ctx->runs.empty() && ctx->runs.back().dst + ctx->runs.back().len == dst &&
+      ctx->runs.back().src + ctx->runs.back().len == ctx->pend_len)
+    ctx->runs.back().len += len;
+  else
+    ctx->runs.push_back({dst, ctx->pend_len, len});
+  ctx->pend_len += len;
+  return SB_OK;
+}
+
+// Enqueue pending model uploads on stream s and make s wait for all uploads.
+int flush_models(sb_context* ctx, hipStream_t s) {
+  if (ctx->model_ev_valid) SB_HIP(hipStreamWaitEvent(s, ctx->model_ev, 0));
+  if (!ctx->runs.empty()) {
+    const int k = ctx->pend_k;
+    for (const ModelRun& r : ctx->runs)
+      SB_HIP(hipMemcpyAsync(r.dst, ctx->pend[k] + r.src, r.len, hipMemcpyHostToDevice, s));
+    if (!ctx->pend_ev[k]) SB_HIP(hipEventCreateWithFlags(&ctx->pend_ev[k], hipEventDisableTiming));
+    if (!ctx->model_ev) SB_HIP(hipEventCreateWithFlags(&ctx->model_ev, hipEventDisableTiming));
+    SB_HIP(hipEventRecord(ctx->pend_ev[k], s));
+    SB_HIP(hipEventRecord(ctx->model_ev, s));
+    ctx->model_ev_valid = true;
+    ctx->pend_busy[k] = true;
+    ctx->runs.clear();
+    ctx->pend_len = 0;
+    ctx->pend_k = k ^ 1;
+  }
+  return SB_OK;
 }
 
 struct BlockIn {
@@ -491,6 +582,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   L->tile_cols = tile_cols;
 
   hipStream_t s = b->stream;
+  rc = flush_models(ctx, s);
+  if (rc) return rc;
   SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
@@ -607,11 +700,13 @@ int sb_destroy(sb_context* ctx) {
   batch_free(ctx->internal);
   if (ctx->plain.p) (void)hipFree(ctx->plain.p);
   if (ctx->plain_bt.p) (void)hipFree(ctx->plain_bt.p);
-  for (auto& m : ctx->models) {
-    if (m.d64) (void)hipFree(m.d64);
-    for (void* r : m.rec)
-      if (r) (void)hipFree(r);
+  (void)hipDeviceSynchronize();
+  for (void* c : ctx->arena_chunks) (void)hipFree(c);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->pend[k]) (void)hipHostFree(ctx->pend[k]);
+    if (ctx->pend_ev[k]) (void)hipEventDestroy(ctx->pend_ev[k]);
   }
+  if (ctx->model_ev) (void)hipEventDestroy(ctx->model_ev);
   delete ctx;
   return SB_OK;
 }
@@ -707,14 +802,20 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.mu.assign(eigenvalues, eigenvalues + A);
   m.V.assign(eigenvectors, eigenvectors + A * A);
   m.pi.assign(pi, pi + A);
-  if (hipMalloc(reinterpret_cast<void**>(&m.d64), m64.size() * 8) != hipSuccess ||
-      hipMalloc(&m.rec[0], recf.size() * 4) != hipSuccess || hipMalloc(&m.rec[1], rec.size() * 8) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(SB_ERR_OOM, "model allocation failed");
-  }
-  SB_HIP(hipMemcpy(m.d64, m64.data(), m64.size() * 8, hipMemcpyHostToDevice));
-  SB_HIP(hipMemcpy(m.rec[0], recf.data(), recf.size() * 4, hipMemcpyHostToDevice));
-  SB_HIP(hipMemcpy(m.rec[1], rec.data(), rec.size() * 8, hipMemcpyHostToDevice));
+  const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
+               bd = align_up(rec.size() * 8, 256);
+  char* base = nullptr;
+  int rc = arena_alloc(ctx, b64 + bf + bd, &base);
+  if (rc) return rc;
+  std::vector<char> blob(b64 + bf + bd, 0);
+  std::memcpy(blob.data(), m64.data(), m64.size() * 8);
+  std::memcpy(blob.data() + b64, recf.data(), recf.size() * 4);
+  std::memcpy(blob.data() + b64 + bf, rec.data(), rec.size() * 8);
+  rc = stage_model_bytes(ctx, base, blob.data(), blob.size());
+  if (rc) return rc;
+  m.d64 = reinterpret_cast<double*>(base);
+  m.rec[0] = base + b64;
+  m.rec[1] = base + b64 + bf;
   int id = -1;
   for (size_t i = 0; i < ctx->models.size(); ++i)
     if (!ctx->models[i].alive) { id = int(i); break; }
@@ -733,12 +834,8 @@ int sb_model_drop(sb_context* ctx, int model_id) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   Model* m = get_model(ctx, model_id);
   if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
-  SB_HIP(hipDeviceSynchronize());
-  (void)hipFree(m->d64);
-  for (void*& r : m->rec) {
-    (void)hipFree(r);
-    r = nullptr;
-  }
+  // arena space is reclaimed with the context (records may still be in use
+  // by queued launches)
   m->d64 = nullptr;
   m->alive = false;
   return SB_OK;
@@ -875,6 +972,8 @@ int sb_counts_plain(sb_context* ctx, int model_id, const int8_t* d_alignment, co
       }
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
+  rc = flush_models(ctx, s);
+  if (rc) return rc;
   const size_t bytes = ctx->plain_bt_host.size() * sizeof(float);
   if (bytes > ctx->plain_bt.cap) {
     SB_HIP(hipStreamSynchronize(s));

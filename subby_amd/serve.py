@@ -1,0 +1,259 @@
+"""Continuous-batching server for (tree, alignment-block) requests.
+
+Requests arrive over time (Poisson in the benchmark).  The loop keeps several
+launcher lanes, each a persistent-kernel launcher on its own HIP stream with
+its own pinned staging and device buffers.  Whenever a lane is free, the
+requests that have arrived are admitted into it, first come first served,
+while the in-flight bytes (alignments + outputs of every batch still on the
+GPU) stay within the HBM budget; the batch is uploaded with one copy, run as
+one persistent launch (replayed hipGraph), and its outputs come back with one
+copy.  Batches on different lanes overlap on the device, and new requests are
+admitted as soon as a lane and memory free up; nothing waits for a full batch.
+
+Models stay resident: each distinct eigensystem is uploaded once into the
+engine's model arena.  Every column is computed independently of what else is
+in the batch, so a request's result is bitwise the same served alone or in
+any mix.
+
+The reference's batched driver (scripts/batch_eigensub_maf.py:423-698) buckets
+blocks by tree size, pads columns to 128 and sizes jax.vmap batches by a 4 GiB
+budget; here no padding or bucketing is needed.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .api import Engine, default_engine, _PREC
+from .models import ensure_diag
+
+_BLOCK_DTYPE = np.dtype({
+    'names': ['model_id', 'R', 'C', 'aln', 'parent', 'dist', 'want', 'f81', 'mask_mode',
+              'mask', 'counts', 'root', 'loglike'],
+    'formats': ['<i4', '<i4', '<i4', '<u8', '<u8', '<u8', '<i4', '<i4', '<i4', '<u8', '<u8', '<u8', '<u8'],
+    'offsets': [0, 4, 8, 16, 24, 32, 40, 44, 48, 56, 64, 72, 80],
+    'itemsize': 88})
+assert _BLOCK_DTYPE.itemsize == ctypes.sizeof(_lib.SbBlock)
+
+
+def _align(x, a=256):
+    return (x + a - 1) // a * a
+
+
+@dataclass
+class Request:
+    alignment: np.ndarray          # (R, C) int8 tokens
+    tree: object                   # Tree(parentIndex, distanceToParent)
+    model: object                  # DiagModel / RateModel
+    arrival: float = 0.0           # seconds after the start of the run
+    rid: int = 0
+    branch_mask: str | None = 'auto'
+
+
+@dataclass
+class Result:
+    rid: int
+    counts: np.ndarray
+    loglike: np.ndarray
+    root: np.ndarray
+    arrival: float
+    done: float
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclass
+class _Lane:
+    launcher: object
+    stream: torch.cuda.Stream
+    host_in: torch.Tensor = None
+    dev_in: torch.Tensor = None
+    dev_out: torch.Tensor = None
+    host_out: torch.Tensor = None
+    event: torch.cuda.Event = None
+    batch: list = field(default_factory=list)
+    bytes: int = 0
+    busy: bool = False
+
+
+class Server:
+    def __init__(self, engine: Engine | None = None, hbm_budget: int = 8 << 30, lanes: int = 3,
+                 max_blocks: int = 2048, max_cols: int = 1 << 19, precision: str = 'fp32',
+                 use_graph: bool = True):
+        from .batch import Launcher
+        self.eng = engine or default_engine()
+        self.dev = self.eng.device
+        self.budget = int(hbm_budget)
+        self.max_blocks = max_blocks
+        self.max_cols = max_cols
+        self.precision = precision
+        self.dt, self.tdt = _PREC[precision]
+        self.esz = 4 if precision == 'fp32' else 8
+        self.use_graph = use_graph
+        self.lanes = []
+        for _ in range(lanes):
+            st = torch.cuda.Stream(self.dev)
+            self.lanes.append(_Lane(launcher=Launcher(self.eng, stream=st), stream=st))
+        self.in_flight = 0
+
+    # ---- per-request footprint
+    def _bytes(self, r: Request):
+        R, C = r.alignment.shape
+        A = len(r.model.pi)
+        return _align(R * C), _align(A * A * C * self.esz) + _align(C * 8) + _align(A * C * self.esz)
+
+    def _ensure(self, lane, nin, nout):
+        if lane.host_in is None or lane.host_in.numel() < nin:
+            n = max(nin, 1 << 22)
+            lane.host_in = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+            lane.dev_in = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        if lane.dev_out is None or lane.dev_out.numel() < nout:
+            n = max(nout, 1 << 22)
+            lane.dev_out = torch.empty(n, dtype=torch.uint8, device=self.dev)
+            lane.host_out = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+
+    def _launch(self, lane, reqs):
+        sizes = [self._bytes(r) for r in reqs]
+        nin = sum(s[0] for s in sizes)
+        nout = sum(s[1] for s in sizes)
+        self._ensure(lane, nin, nout)
+        hin = lane.host_in.numpy()
+        blocks = np.zeros(len(reqs), dtype=_BLOCK_DTYPE)
+        keep = []
+        din = lane.dev_in.data_ptr()
+        dout = lane.dev_out.data_ptr()
+        oi = oo = 0
+        layout = []
+        for k, (r, (bi, bo)) in enumerate(zip(reqs, sizes)):
+            R, C = r.alignment.shape
+            m = ensure_diag(r.model)
+            A = len(m.pi)
+            hin[oi:oi + R * C] = r.alignment.reshape(-1).view(np.uint8)
+            parent = np.ascontiguousarray(r.tree.parentIndex, dtype=np.int32)
+            dist = np.ascontiguousarray(r.tree.distanceToParent, dtype=np.float64)
+            keep += [parent, dist]
+            oc = oo
+            ol = oc + _align(A * A * C * self.esz)
+            orr = ol + _align(C * 8)
+            b = blocks[k]
+            b['model_id'] = self.eng.model_id(m)
+            b['R'], b['C'] = R, C
+            b['aln'] = din + oi
+            b['parent'] = parent.ctypes.data
+            b['dist'] = dist.ctypes.data
+            b['want'] = _lib.SB_WANT_LOGLIKE | _lib.SB_WANT_ROOT | _lib.SB_WANT_COUNTS
+            b['mask_mode'] = _lib.SB_MASK_AUTO if r.branch_mask == 'auto' else _lib.SB_MASK_NONE
+            b['counts'] = dout + oc
+            b['loglike'] = dout + ol
+            b['root'] = dout + orr
+            layout.append((r, A, C, oc, ol, orr))
+            oi += bi
+            oo += bo
+        with torch.cuda.stream(lane.stream):
+            lane.dev_in[:oi].copy_(lane.host_in[:oi], non_blocking=True)
+        _lib.check(self.eng.lib.sb_batch_submit(lane.launcher.h, len(reqs), blocks.ctypes.data, self.dt,
+                                                1 if self.use_graph else 0))
+        with torch.cuda.stream(lane.stream):
+            lane.host_out[:oo].copy_(lane.dev_out[:oo], non_blocking=True)
+            lane.event = torch.cuda.Event()
+            lane.event.record(lane.stream)
+        lane.batch = layout
+        lane.bytes = nin + nout
+        lane.busy = True
+        self.in_flight += lane.bytes
+        del keep  # the planner copied the tree arrays during submit
+
+    def _complete(self, lane, now, results):
+        hout = lane.host_out.numpy()
+        np_t = np.float32 if self.esz == 4 else np.float64
+        for r, A, C, oc, ol, orr in lane.batch:
+            cnt = hout[oc:oc + A * A * C * self.esz].view(np_t).reshape(A, A, C).copy()
+            ll = hout[ol:ol + C * 8].view(np.float64).copy()
+            root = hout[orr:orr + A * C * self.esz].view(np_t).reshape(A, C).copy()
+            results.append(Result(r.rid, cnt, ll, root, r.arrival, now))
+        self.in_flight -= lane.bytes
+        lane.busy = False
+        lane.batch = []
+
+    def run(self, requests, realtime: bool = True):
+        """Serve ``requests`` (sorted by arrival).  Returns (results, stats)."""
+        pending = list(requests)
+        results = []
+        head = 0
+        t0 = time.perf_counter()
+        batches = 0
+        while head < len(pending) or any(l.busy for l in self.lanes):
+            now = time.perf_counter() - t0
+            progressed = False
+            for lane in self.lanes:
+                if lane.busy and lane.event.query():
+                    self._complete(lane, time.perf_counter() - t0, results)
+                    progressed = True
+            for lane in self.lanes:
+                if lane.busy or head >= len(pending):
+                    continue
+                if realtime and pending[head].arrival > now:
+                    break
+                take, cols, nbytes = [], 0, 0
+                while head < len(pending) and len(take) < self.max_blocks:
+                    r = pending[head]
+                    if realtime and r.arrival > now:
+                        break
+                    bi, bo = self._bytes(r)
+                    if take and (cols + r.alignment.shape[1] > self.max_cols or
+                                 self.in_flight + nbytes + bi + bo > self.budget):
+                        break
+                    if not take and self.in_flight > 0 and self.in_flight + bi + bo > self.budget:
+                        break
+                    take.append(r)
+                    cols += r.alignment.shape[1]
+                    nbytes += bi + bo
+                    head += 1
+                if take:
+                    self._launch(lane, take)
+                    batches += 1
+                    progressed = True
+            if not progressed:
+                if head < len(pending) and realtime:
+                    wait = pending[head].arrival - (time.perf_counter() - t0)
+                    if wait > 2e-4 and not any(l.busy for l in self.lanes):
+                        time.sleep(min(wait, 1e-3))
+                        continue
+                time.sleep(2e-5)
+        makespan = time.perf_counter() - t0
+        lat = np.array([r.latency for r in results]) if results else np.zeros(1)
+        units = sum(int(r.alignment.shape[1]) * (len(r.tree.parentIndex) - 1) for r in requests)
+        stats = {'requests': len(results), 'batches': batches, 'makespan_s': makespan,
+                 'throughput': units / makespan, 'p50_ms': float(np.percentile(lat, 50) * 1e3),
+                 'p99_ms': float(np.percentile(lat, 99) * 1e3), 'mean_ms': float(lat.mean() * 1e3)}
+        results.sort(key=lambda r: r.rid)
+        return results, stats
+
+
+def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000)):
+    """Synthetic request stream: Poisson arrivals at ``qps``; each request has its
+    own random tree (leaves uniform in ``leaves``), LogNormal block length with
+    arithmetic mean ``mean_cols`` (log-sd ``sd``) clipped to ``clip``, and an
+    HKY85 model with kappa ~ U(1,5), pi ~ Dirichlet(5,5,5,5)."""
+    from . import synthetic
+    from .models import hky85_diag
+    rng = np.random.default_rng(seed)
+    mu = np.log(mean_cols) - 0.5 * sd * sd
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / qps)
+        nl = int(rng.integers(leaves[0], leaves[1] + 1))
+        C = int(np.clip(round(rng.lognormal(mu, sd)), clip[0], clip[1]))
+        m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+        tr = synthetic.random_tree(nl, rng, 0.1)
+        aln = synthetic.simulate(tr, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        out.append(Request(aln, tr, m, arrival=t, rid=i))
+    return out
