@@ -114,6 +114,9 @@ def main():
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--stream-blocks', type=int, default=300)
     ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
+    ap.add_argument('--serve-requests', type=int, default=20000)
+    ap.add_argument('--serve-pool', type=int, default=500,
+                    help='distinct simulated (tree, alignment) pairs the request stream draws from')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()
@@ -246,6 +249,31 @@ def main():
         out["batched_vs_plain"] = {
             "blocks": len(stream), "columns": int(sum(a.shape[1] for _, _, a in stream)),
             "batched": sunits / bt, "plain": sunits / ptot, "ratio": (sunits / bt) / (sunits / ptot),
+            "unit": "column*branches/s"}
+
+    if args.serve_requests > 0:
+        from subby_amd.serve import Server, make_stream
+        log(f'bench: serving {args.serve_requests} requests (continuous batching)')
+        reqs = make_stream(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool)
+        srv = Server(eng, hbm_budget=16 << 30, lanes=3)
+        srv.run(reqs[:200], realtime=False)                     # warm-up (graphs, buffers)
+        _, sat = srv.run(reqs, realtime=False)
+        rate = sat['requests'] / sat['makespan_s']
+        qps = 0.5 * rate
+        arr = np.cumsum(np.random.default_rng(5).exponential(1.0 / qps, size=len(reqs)))
+        for r, a in zip(reqs, arr):
+            r.arrival = float(a)
+        _, lat = srv.run(reqs, realtime=True)
+        out["served"] = {
+            "workload": "stream of HKY85 blocks: random trees with 4-100 leaves, block length LogNormal "
+                        "(mean 500, sd 1.0) clipped to [20, 10000], kappa~U(1,5), pi~Dir(5,5,5,5) per request; "
+                        f"{args.serve_pool} simulated (tree, alignment) pairs drawn with replacement; "
+                        "FP32 Counts+logLike+root, results copied back to host",
+            "requests": len(reqs),
+            "saturated": {"throughput": sat['throughput'], "requests_per_s": rate,
+                          "p50_ms": sat['p50_ms'], "p99_ms": sat['p99_ms'], "batches": sat['batches']},
+            "poisson": {"qps": qps, "throughput": lat['throughput'], "p50_ms": lat['p50_ms'],
+                        "p99_ms": lat['p99_ms'], "batches": lat['batches']},
             "unit": "column*branches/s"}
 
     if not args.no_cpu:

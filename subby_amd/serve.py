@@ -237,23 +237,36 @@ class Server:
         return results, stats
 
 
-def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000)):
-    """Synthetic request stream: Poisson arrivals at ``qps``; each request has its
-    own random tree (leaves uniform in ``leaves``), LogNormal block length with
-    arithmetic mean ``mean_cols`` (log-sd ``sd``) clipped to ``clip``, and an
-    HKY85 model with kappa ~ U(1,5), pi ~ Dirichlet(5,5,5,5)."""
+def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000),
+                pool=None):
+    """Synthetic request stream: Poisson arrivals at ``qps``; each request has a
+    random tree (leaves uniform in ``leaves``), LogNormal block length with
+    arithmetic mean ``mean_cols`` (log-sd ``sd``) clipped to ``clip``, and its own
+    HKY85 model with kappa ~ U(1,5), pi ~ Dirichlet(5,5,5,5).  With ``pool`` set,
+    only that many (tree, alignment) pairs are simulated and requests draw from
+    them (each request still gets its own model), to keep generation fast."""
     from . import synthetic
     from .models import hky85_diag
     rng = np.random.default_rng(seed)
     mu = np.log(mean_cols) - 0.5 * sd * sd
-    t = 0.0
-    out = []
-    for i in range(n):
-        t += rng.exponential(1.0 / qps)
+
+    def one():
         nl = int(rng.integers(leaves[0], leaves[1] + 1))
         C = int(np.clip(round(rng.lognormal(mu, sd)), clip[0], clip[1]))
         m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
         tr = synthetic.random_tree(nl, rng, 0.1)
         aln = synthetic.simulate(tr, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        return tr, m, aln
+
+    shapes = [one() for _ in range(pool)] if pool else None
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / qps)
+        if shapes:
+            tr, _, aln = shapes[int(rng.integers(len(shapes)))]
+            m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+        else:
+            tr, m, aln = one()
         out.append(Request(aln, tr, m, arrival=t, rid=i))
     return out

@@ -62,37 +62,39 @@ def simulate(tree: Tree, model: DiagModel, C: int, rng, gap_frac: float = 0.0,
     """(R, C) tokens simulated down the tree; internal nodes get token A."""
     parent = np.asarray(tree.parentIndex)
     R = len(parent)
-    pi = np.asarray(model.pi, dtype=np.float64)
+    mu, V, pi = (np.asarray(x, dtype=np.float64) for x in model)
     A = len(pi)
+    s = np.sqrt(pi)
+    t = np.asarray(tree.distanceToParent, dtype=np.float64)
+    M = np.einsum('ak,rk,bk->rab', V, np.exp(mu[None, :] * t[:, None]), V) / s[None, :, None] * s[None, None, :]
+    M = np.clip(M, 0.0, None)
+    cum = np.cumsum(M / M.sum(2, keepdims=True), axis=2)
+    cum[..., -1] = 1.0
     states = np.empty((R, C), dtype=np.int16)
+    u = rng.random((R, C))
     cpi = np.cumsum(pi)
     cpi[-1] = 1.0
-    states[0] = np.searchsorted(cpi, rng.random(C), side='right')
+    states[0] = np.searchsorted(cpi, u[0], side='right')
     for n in range(1, R):
-        M = sub_matrix(model, float(tree.distanceToParent[n]))
-        cum = np.cumsum(M / M.sum(1, keepdims=True), axis=1)
-        cum[:, -1] = 1.0
         ps = states[parent[n]]
-        u = rng.random(C)
-        out = np.empty(C, dtype=np.int16)
         if A <= 8:
-            out[:] = (u[:, None] >= cum[ps]).sum(1)
+            states[n] = (u[n][:, None] >= cum[n][ps]).sum(1)
         else:
+            out = np.empty(C, dtype=np.int16)
             for a in np.unique(ps):
                 idx = np.nonzero(ps == a)[0]
-                out[idx] = np.searchsorted(cum[a], u[idx], side='right')
-        states[n] = np.minimum(out, A - 1)
+                out[idx] = np.searchsorted(cum[n, a], u[n, idx], side='right')
+            states[n] = out
+    np.minimum(states, A - 1, out=states)
     is_internal = np.zeros(R, dtype=bool)
     is_internal[parent[1:]] = True
-    aln = states.astype(np.int16)
+    aln = states
     aln[is_internal] = A
-    leaf = ~is_internal
     if gap_frac > 0 or ambig_frac > 0:
-        u = rng.random((R, C))
-        gap = leaf[:, None] & (u < gap_frac)
-        amb = leaf[:, None] & (u >= gap_frac) & (u < gap_frac + ambig_frac)
-        aln[gap] = -1
-        aln[amb] = A
+        g = rng.random((R, C))
+        leaf = ~is_internal[:, None]
+        aln[leaf & (g < gap_frac)] = -1
+        aln[leaf & (g >= gap_frac) & (g < gap_frac + ambig_frac)] = A
     return aln.astype(dtype)
 
 
