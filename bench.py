@@ -252,9 +252,9 @@ def main():
             "unit": "column*branches/s"}
 
     if args.serve_requests > 0:
-        from subby_amd.serve import Server, make_stream
+        from subby_amd.serve import Server, make_stream as make_requests
         log(f'bench: serving {args.serve_requests} requests (continuous batching)')
-        reqs = make_stream(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool)
+        reqs = make_requests(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool)
         srv = Server(eng, hbm_budget=16 << 30, lanes=3)
         srv.run(reqs[:200], realtime=False)                     # warm-up (graphs, buffers)
         _, sat = srv.run(reqs, realtime=False)
