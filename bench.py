@@ -84,7 +84,7 @@ def cpu_baseline(t, m, aln, budget_s=12.0):
     """FP64 oracle on a bounded sample of the same workload, one thread."""
     from threadpoolctl import threadpool_limits
     from oracle import subby_oracle as O
-    sample = 4000
+    sample = 40000
     sub = np.ascontiguousarray(aln[:, :sample])
     ot, om = O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent)), O.DiagModel(*m)
     R = len(t.parentIndex)
@@ -96,7 +96,7 @@ def cpu_baseline(t, m, aln, budget_s=12.0):
             s = time.perf_counter()
             O.counts_chunked(sub, ot, om, chunk=128)
             times.append(time.perf_counter() - s)
-            if time.perf_counter() - t0 > budget_s or len(times) >= 5:
+            if time.perf_counter() - t0 > budget_s or len(times) >= 10:
                 break
     med = float(np.median(times))
     return {"value": sample * (R - 1) / med, "unit": "column*branches/s", "cores": 1, "kind": "port",
