@@ -105,6 +105,129 @@ def cpu_baseline(t, m, aln, budget_s=12.0):
                       f"{med:.2f} s/rep, 1 BLAS thread"}
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+
+def flops_counts_per_column(R, A):
+    """Separable algorithmic flops of one column: inside 2A^2(R-1), outside 4A^2(R-1),
+    projections 4A^2(R-1), accumulation 3A^2(R-1), back-transform 4A^3."""
+    return 13 * A * A * (R - 1) + 4 * A ** 3
+
+
+def other_workload(args, log):
+    """Measurement lines for the other configurations (not the headline metric)."""
+    import torch
+    import subby_amd as sb
+    from subby_amd.batch import Launcher, Block
+    from oracle import subby_oracle as O
+    eng = sb.Engine(0)
+    dev = eng.device
+    w = args.workload
+    if w in ('c3a', 'c3b'):
+        A = 64
+        rng = np.random.default_rng(3000 if w == 'c3a' else 3001)
+        t = sb.synthetic.random_tree(100, rng, 0.1)
+        pi = rng.dirichlet(np.full(A, 2.0))
+        if w == 'c3a':
+            m, C, f81 = sb.f81_model(pi), 1_000_000, True
+        else:
+            Q = sb.reversible_rate_matrix(rng.lognormal(0.0, 1.0, (A, A)), pi)
+            m, C, f81 = sb.diagonalize_rate_matrix(Q, pi), 20_000, False
+        log(f'bench {w}: simulating A=64, R={len(t.parentIndex)}, C={C}')
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        R = len(t.parentIndex)
+        aln = torch.from_numpy(aln_np).to(dev)
+        L = Launcher(eng)
+        blk = Block(aln, t, m, f81_fast=f81)
+        prep = L.prepare([blk])
+        L.launch(prep, use_graph=False)
+        torch.cuda.synchronize()
+        steps = max(3, args.steps // 10)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            L.launch(prep, use_graph=True)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        L.set_timing(True)
+        for _ in range(3):
+            L.launch(prep, use_graph=False)
+        kms, nl = L.kernel_ms()
+        kavg = kms / nl
+        cols = np.arange(0, C, max(1, C // 400))[:400]
+        rc, rl, rr = O.counts_loglike_root(aln_np[:, cols], O.Tree(*t), O.DiagModel(*m), f81_fast=f81)
+        gc = blk.counts[:, :, cols].double().cpu().numpy()
+        err_c = float(np.max(np.abs(gc - rc).max((0, 1)) / np.abs(rc).max((0, 1))))
+        err_l = float(np.max(np.abs(blk.loglike[cols].cpu().numpy() - rl) / np.maximum(np.abs(rl), 1)))
+        units = C * (R - 1)
+        if w == 'c3a':
+            balg = b_alg_counts(R, A, C)
+            roof = {"bound": "hbm", "achieved": balg / (kavg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": balg / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                    "alg_bytes_per_launch": balg}
+        else:
+            fl = flops_counts_per_column(R, A) * C
+            ach = fl / (kavg * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "alg_flops_per_launch": fl,
+                    "note": "VALU kernel (no MFMA yet); peak is the FP32 dense figure"}
+        roof.update({"kernel": f"sb_large_kernel<float,64,{'true' if f81 else 'false'},false>", "kernel_ms": kavg})
+        out = {"metric": f"Counts column*branches/sec ({'F81 fast path' if f81 else 'general reversible'}, A=64, FP32)",
+               "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
+               "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": w, "A": A, "R": R, "C": C}, "roofline": roof,
+               "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": len(cols)}}
+    else:  # c5 gradient
+        rng = np.random.default_rng(5000)
+        pi = rng.dirichlet(np.full(4, 5.0))
+        kappa = 2.0
+        m = sb.hky85_diag(kappa, pi)
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        C = 50_000
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        aln = torch.from_numpy(aln_np).to(dev)
+        R = len(t.parentIndex)
+        W = sb.hky85_log_kappa_weights(kappa, pi)
+        for _ in range(3):
+            sb.loglike_grad(aln, t, m, rate_weights=W, engine=eng)
+        torch.cuda.synchronize()
+        steps = max(5, args.steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dg, rg, _ = sb.loglike_grad(aln, t, m, rate_weights=W, engine=eng)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        dg = dg.cpu().numpy()
+        rg = float(rg.cpu().numpy()[0])
+        # central finite differences (h = 1e-4) of the FP64 GPU log-likelihood on 12 branches + log kappa
+        h = 1e-4
+        idx = np.linspace(1, R - 1, 12).astype(int)
+        fd = []
+        for n in idx:
+            tp = np.array(t.distanceToParent, dtype=np.float64)
+            tm = tp.copy()
+            tp[n] += h
+            tm[n] -= h
+            fp = sb.LogLike(aln, sb.Tree(t.parentIndex, tp), m, precision='fp64', engine=eng).sum().item()
+            fmn = sb.LogLike(aln, sb.Tree(t.parentIndex, tm), m, precision='fp64', engine=eng).sum().item()
+            fd.append((fp - fmn) / (2 * h))
+        fd = np.array(fd)
+        lk = np.log(kappa)
+        fkp = sb.LogLike(aln, t, sb.hky85_diag(np.exp(lk + h), pi), precision='fp64', engine=eng).sum().item()
+        fkm = sb.LogLike(aln, t, sb.hky85_diag(np.exp(lk - h), pi), precision='fp64', engine=eng).sum().item()
+        fdk = (fkp - fkm) / (2 * h)
+        err_t = float(np.max(np.abs(dg[idx] - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())))
+        err_k = abs(rg - fdk) / abs(fdk)
+        units = C * (R - 1)
+        out = {"metric": "gradient column*branches/sec (FP32 forward + adjoint, HKY85 R=127, "
+                         "d/d branch lengths and d/d log kappa)",
+               "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
+               "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": "c5", "A": 4, "R": R, "C": C},
+               "parity": {"branch_grad_rel_vs_fd_fp64": err_t, "log_kappa_rel_vs_fd_fp64": err_k,
+                          "fd_step": h, "branches_checked": len(idx)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -117,6 +240,10 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5'],
+                    help='c2 (default, the headline line) or one of the other configurations: '
+                         'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
+                         'c5 HKY85 gradient R=127 C=5e4')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()
@@ -131,6 +258,9 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.workload != 'c2':
+        return other_workload(args, log)
 
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device='cuda')

@@ -361,3 +361,53 @@ def test_large_alphabets(sb, eng, A):
     d, _, _ = sb.loglike_grad(aln, t, m, precision='fp64', engine=eng)
     ref, _ = O.loglike_branch_grad(aln, otree(t), omodel(m))
     assert np.abs(d.cpu().numpy() - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_autograd_functions(sb, eng):
+    """torch.autograd.Function wrappers: branch lengths and log kappa."""
+    say('autograd functions')
+    t, m, aln = _hky(sb, 4000, seed=21, leaves=30, mean=0.1)
+    d = torch.tensor(np.asarray(t.distanceToParent), dtype=torch.float64, requires_grad=True)
+    ll = sb.LogLikeCustomGrad(aln, sb.Tree(t.parentIndex, d), m, engine=eng)
+    g = torch.as_tensor(np.random.default_rng(22).normal(size=4000), device=ll.device)
+    (ll * g).sum().backward()
+    ref, _ = O.loglike_branch_grad(aln, otree(t), omodel(m), g.cpu().numpy())
+    err = np.abs(d.grad.numpy() - ref).max() / np.abs(ref).max()
+    say('   branch grad err', err)
+    assert err <= 1e-4
+    pi = np.asarray(m.pi)
+    lk = torch.tensor(np.log(2.0), dtype=torch.float64, requires_grad=True)
+    d2 = torch.tensor(np.asarray(t.distanceToParent), dtype=torch.float64, requires_grad=True)
+    ll = sb.hky85_loglike(aln, sb.Tree(t.parentIndex, d2), lk, pi, engine=eng)
+    ll.sum().backward()
+    refk = O.hky85_loglike_kappa_grad(aln, otree(t), 2.0, pi)
+    say('   log kappa grad', float(lk.grad), refk)
+    assert abs(float(lk.grad) - refk) <= 1e-4 * abs(refk)
+    ref2, _ = O.loglike_branch_grad(aln, otree(t), omodel(m))
+    assert np.abs(d2.grad.numpy() - ref2).max() <= 1e-4 * np.abs(ref2).max()
+
+
+def test_gradient_config_fd(sb, eng):
+    """HKY85, 127 nodes, C = 50,000: adjoint vs central FD (h = 1e-4, FP64), rel <= 1e-4."""
+    say('gradient config C=50000')
+    t, m, aln = _hky(sb, 50000, seed=23)
+    R = len(t.parentIndex)
+    dev = eng.device
+    aln_d = torch.from_numpy(aln).to(dev)
+    pi = np.asarray(m.pi)
+    W = sb.hky85_log_kappa_weights(2.0, pi)
+    dg, rg, _ = sb.loglike_grad(aln_d, t, m, rate_weights=W, engine=eng)
+    dg = dg.cpu().numpy()
+    h = 1e-4
+    for n in np.linspace(1, R - 1, 8).astype(int):
+        tp = np.array(t.distanceToParent, dtype=np.float64)
+        tm = tp.copy()
+        tp[n] += h
+        tm[n] -= h
+        fd = (sb.LogLike(aln_d, sb.Tree(t.parentIndex, tp), m, precision='fp64', engine=eng).sum().item()
+              - sb.LogLike(aln_d, sb.Tree(t.parentIndex, tm), m, precision='fp64', engine=eng).sum().item()) / (2 * h)
+        assert abs(dg[n] - fd) <= 1e-4 * max(abs(fd), 1.0), (n, dg[n], fd)
+    fkp = sb.LogLike(aln_d, t, sb.hky85_diag(2.0 * np.exp(h), pi), precision='fp64', engine=eng).sum().item()
+    fkm = sb.LogLike(aln_d, t, sb.hky85_diag(2.0 * np.exp(-h), pi), precision='fp64', engine=eng).sum().item()
+    fdk = (fkp - fkm) / (2 * h)
+    assert abs(float(rg[0]) - fdk) <= 1e-4 * abs(fdk)
