@@ -388,7 +388,12 @@ def test_autograd_functions(sb, eng):
 
 
 def test_gradient_config_fd(sb, eng):
-    """HKY85, 127 nodes, C = 50,000: adjoint vs central FD (h = 1e-4, FP64), rel <= 1e-4."""
+    """HKY85, 127 nodes, C = 50,000, gradient of sum_c logL_c.
+
+    FP64-vector adjoint vs central FD (h = 1e-4, FP64): per-branch relative error <= 1e-4.
+    FP32 adjoint vs the FP64 adjoint: normwise (max_n |d| / max_n |ref|) <= 1e-4.  Per-branch
+    relative error of the FP32 adjoint is limited by cancellation across columns (the summed
+    derivative is up to ~5000x smaller than the sum of the per-column magnitudes)."""
     say('gradient config C=50000')
     t, m, aln = _hky(sb, 50000, seed=23)
     R = len(t.parentIndex)
@@ -396,8 +401,9 @@ def test_gradient_config_fd(sb, eng):
     aln_d = torch.from_numpy(aln).to(dev)
     pi = np.asarray(m.pi)
     W = sb.hky85_log_kappa_weights(2.0, pi)
-    dg, rg, _ = sb.loglike_grad(aln_d, t, m, rate_weights=W, engine=eng)
-    dg = dg.cpu().numpy()
+    d64, r64, _ = sb.loglike_grad(aln_d, t, m, rate_weights=W, precision='fp64', engine=eng)
+    d32, r32, _ = sb.loglike_grad(aln_d, t, m, rate_weights=W, precision='fp32', engine=eng)
+    d64, d32 = d64.cpu().numpy(), d32.cpu().numpy()
     h = 1e-4
     for n in np.linspace(1, R - 1, 8).astype(int):
         tp = np.array(t.distanceToParent, dtype=np.float64)
@@ -406,8 +412,12 @@ def test_gradient_config_fd(sb, eng):
         tm[n] -= h
         fd = (sb.LogLike(aln_d, sb.Tree(t.parentIndex, tp), m, precision='fp64', engine=eng).sum().item()
               - sb.LogLike(aln_d, sb.Tree(t.parentIndex, tm), m, precision='fp64', engine=eng).sum().item()) / (2 * h)
-        assert abs(dg[n] - fd) <= 1e-4 * max(abs(fd), 1.0), (n, dg[n], fd)
+        assert abs(d64[n] - fd) <= 1e-4 * max(abs(fd), 1.0), (n, d64[n], fd)
     fkp = sb.LogLike(aln_d, t, sb.hky85_diag(2.0 * np.exp(h), pi), precision='fp64', engine=eng).sum().item()
     fkm = sb.LogLike(aln_d, t, sb.hky85_diag(2.0 * np.exp(-h), pi), precision='fp64', engine=eng).sum().item()
     fdk = (fkp - fkm) / (2 * h)
-    assert abs(float(rg[0]) - fdk) <= 1e-4 * abs(fdk)
+    assert abs(float(r64[0]) - fdk) <= 1e-4 * abs(fdk)
+    norm = np.abs(d32 - d64).max() / np.abs(d64).max()
+    say('   fp32 adjoint normwise vs fp64', norm, 'log kappa', float(r32[0]), float(r64[0]))
+    assert norm <= 1e-4
+    assert abs(float(r32[0]) - float(r64[0])) <= 1e-3 * abs(float(r64[0]))
