@@ -201,28 +201,36 @@ def other_workload(args, log):
             timing[prec] = (time.perf_counter() - t0) / steps
             grads[prec] = (dg.cpu().numpy(), float(rg.cpu().numpy()[0]))
         wall = timing['fp32']
-        # central finite differences (h = 1e-4) of the FP64 GPU log-likelihood on 12 branches + log kappa
+        # central finite differences of the FP64 GPU log-likelihood (h = 1e-4) on 12 branches and
+        # log kappa, plain and Richardson-extrapolated with h/2 (truncation O(h^4))
         h = 1e-4
         idx = np.linspace(1, R - 1, 12).astype(int)
-        fd = []
-        for n in idx:
+
+        def total(tree, model):
+            return sb.LogLike(aln, tree, model, precision='fp64', engine=eng).sum().item()
+
+        def fd_b(n, hh):
             tp = np.array(t.distanceToParent, dtype=np.float64)
             tm = tp.copy()
-            tp[n] += h
-            tm[n] -= h
-            fp = sb.LogLike(aln, sb.Tree(t.parentIndex, tp), m, precision='fp64', engine=eng).sum().item()
-            fmn = sb.LogLike(aln, sb.Tree(t.parentIndex, tm), m, precision='fp64', engine=eng).sum().item()
-            fd.append((fp - fmn) / (2 * h))
-        fd = np.array(fd)
-        lk = np.log(kappa)
-        fkp = sb.LogLike(aln, t, sb.hky85_diag(np.exp(lk + h), pi), precision='fp64', engine=eng).sum().item()
-        fkm = sb.LogLike(aln, t, sb.hky85_diag(np.exp(lk - h), pi), precision='fp64', engine=eng).sum().item()
-        fdk = (fkp - fkm) / (2 * h)
+            tp[n] += hh
+            tm[n] -= hh
+            return (total(sb.Tree(t.parentIndex, tp), m) - total(sb.Tree(t.parentIndex, tm), m)) / (2 * hh)
+
+        def fd_k(hh):
+            return (total(t, sb.hky85_diag(kappa * np.exp(hh), pi))
+                    - total(t, sb.hky85_diag(kappa * np.exp(-hh), pi))) / (2 * hh)
+
+        fd = np.array([fd_b(n, h) for n in idx])
+        rich = np.array([(4 * fd_b(n, h / 2) - fd_b(n, h)) / 3 for n in idx])
+        fdk = fd_k(h)
+        richk = (4 * fd_k(h / 2) - fdk) / 3
         par = {"fd_step": h, "branches_checked": len(idx)}
         for prec in ('fp32', 'fp64'):
             dg, rg = grads[prec]
             par[f"{prec}_branch_rel_vs_fd"] = float(np.max(np.abs(dg[idx] - fd) / np.abs(fd)))
+            par[f"{prec}_branch_rel_vs_richardson_fd"] = float(np.max(np.abs(dg[idx] - rich) / np.abs(rich)))
             par[f"{prec}_log_kappa_rel_vs_fd"] = abs(rg - fdk) / abs(fdk)
+            par[f"{prec}_log_kappa_rel_vs_richardson_fd"] = abs(rg - richk) / abs(richk)
         par["fp32_normwise_vs_fp64"] = float(np.abs(grads['fp32'][0] - grads['fp64'][0]).max()
                                              / np.abs(grads['fp64'][0]).max())
         par["fp64_ms_per_step"] = timing['fp64'] * 1e3
