@@ -25,6 +25,7 @@ extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool gr
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max);
+extern "C" int sb_large_tile_cols(int dtype);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
@@ -430,7 +431,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
-  const int tile_cols = large ? SB_LARGE_TC : SB_WAVE;
+  const int tile_cols = large ? sb_large_tile_cols(dtype) : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
   // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights

@@ -23,7 +23,9 @@
 
 namespace {
 
-constexpr int TC = SB_LARGE_TC;
+// Columns per wave tile: 4 in FP32, 2 in FP64 (the accumulator rows are 2x wider).
+template <typename T> struct TCof { static constexpr int v = SB_LARGE_TC; };
+template <> struct TCof<double> { static constexpr int v = SB_LARGE_TC / 2; };
 
 template <typename T>
 __device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
@@ -37,40 +39,68 @@ __device__ inline int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x)
 __device__ inline float ldexp_(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
 __device__ inline double ldexp_(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
 
+// Wavefront reductions with DPP: quad swaps, half-row and row mirrors give every
+// lane its 16-lane row total; the four row totals are combined via readlane.
+template <int CTRL>
+__device__ inline int dpp_i(int x) { return __builtin_amdgcn_mov_dpp(x, CTRL, 0xf, 0xf, true); }
+template <int CTRL>
+__device__ inline float dpp(float x) { return __int_as_float(dpp_i<CTRL>(__float_as_int(x))); }
+template <int CTRL>
+__device__ inline double dpp(double x) {
+  const int lo = dpp_i<CTRL>(__double2loint(x)), hi = dpp_i<CTRL>(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float rl(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ inline double rl(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
 template <typename T>
 __device__ inline T wsum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, SB_WAVE);
-  return v;
+  v += dpp<0xb1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4e>(v);   // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x140>(v);  // row_mirror
+  return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
 }
 template <typename T>
 __device__ inline T wmax(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, SB_WAVE));
-  return v;
+  v = fmax(v, dpp<0xb1>(v));
+  v = fmax(v, dpp<0x4e>(v));
+  v = fmax(v, dpp<0x141>(v));
+  v = fmax(v, dpp<0x140>(v));
+  return fmax(fmax(rl(v, 0), rl(v, 16)), fmax(rl(v, 32), rl(v, 48)));
 }
 
 // Per-lane view of TC column vectors: v[c] = component `lane` of column c.
 template <typename T>
 struct Vec {
+  static constexpr int TC = TCof<T>::v;
   T v[TC];
 };
 
-// w_c[lane] = sum_j P[j*AP + lane] * u_c[j]; lanes >= AP give 0.
+// w_c[lane] = sum_j P[j*AP + lane] * u_c[j]; lanes >= AP give 0.  P is first
+// staged in LDS (sm[j*64 + lane], one coalesced row per step, all loads in
+// flight together), u is published to LDS and read back as broadcasts.
 template <typename T>
-__device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T>& u, Vec<T>& w, T* bc, int lane) {
+__device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T>& u, Vec<T>& w, T* bc, T* sm,
+                                int lane) {
+  constexpr int TC = TCof<T>::v;
   __syncthreads();
+  const bool on = lane < AP;
+#pragma unroll 8
+  for (int j = 0; j < AP; ++j) sm[j * SB_WAVE + lane] = on ? P[j * AP + lane] : T(0);
 #pragma unroll
   for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < TC; ++c) w.v[c] = T(0);
-  const bool on = lane < AP;
   for (int j = 0; j < AP; j += 4) {
-    const T m0 = on ? P[(j + 0) * AP + lane] : T(0);
-    const T m1 = on ? P[(j + 1) * AP + lane] : T(0);
-    const T m2 = on ? P[(j + 2) * AP + lane] : T(0);
-    const T m3 = on ? P[(j + 3) * AP + lane] : T(0);
+    const T m0 = sm[(j + 0) * SB_WAVE + lane];
+    const T m1 = sm[(j + 1) * SB_WAVE + lane];
+    const T m2 = sm[(j + 2) * SB_WAVE + lane];
+    const T m3 = sm[(j + 3) * SB_WAVE + lane];
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
       const T* b = bc + c * SB_WAVE + j;
@@ -85,6 +115,7 @@ __device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T>& u, V
 // w = M u for an F81-structured M: w_a = e u_a + (1 - e) (pi . u) on real states.
 template <typename T>
 __device__ inline void f81_matvec(T e, T pia, T onea, const Vec<T>& u, Vec<T>& w) {
+  constexpr int TC = TCof<T>::v;
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
     const T pu = wsum(pia * u.v[c]);
@@ -95,6 +126,7 @@ __device__ inline void f81_matvec(T e, T pia, T onea, const Vec<T>& u, Vec<T>& w
 // x = d^T M for an F81-structured M: x_a = e d_a + (1 - e) pi_a sum_i d_i.
 template <typename T>
 __device__ inline void f81_vecmat(T e, T pia, const Vec<T>& d, Vec<T>& x) {
+  constexpr int TC = TCof<T>::v;
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
     const T sd = wsum(d.v[c]);
@@ -105,6 +137,7 @@ __device__ inline void f81_vecmat(T e, T pia, const Vec<T>& d, Vec<T>& x) {
 template <typename T>
 __device__ inline void token_vec(const SB_GLOBAL int8_t* aln, long long row_off, const long long* cc, int A,
                                  T onea, int lane, Vec<T>& u, int* tok) {
+  constexpr int TC = TCof<T>::v;
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
     const int t = aln[row_off + cc[c]];
@@ -116,6 +149,7 @@ __device__ inline void token_vec(const SB_GLOBAL int8_t* aln, long long row_off,
 
 template <typename T>
 __device__ inline void rescale(Vec<T>& w, int* e) {
+  constexpr int TC = TCof<T>::v;
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
     const int x = frexp_exp(wmax(w.v[c]));
@@ -124,6 +158,7 @@ __device__ inline void rescale(Vec<T>& w, int* e) {
   }
 }
 
+template <int TC>
 struct Flags {
   int e[TC];
   bool has[TC], branch[TC];
@@ -134,9 +169,12 @@ struct Flags {
 
 template <typename T, int AW, bool F81, bool GRAD>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
+  constexpr int TC = TCof<T>::v;
+  typedef Flags<TC> Flags;
   const SbLaunch& L = *Lp;
   __shared__ T bc[TC * SB_WAVE];
-  __shared__ T accb[64 * 64];
+  __shared__ T sm[64 * 64];      // staged AP x AP matrix; also the epilogue accumulator
+  T* accb = sm;
   const int lane = threadIdx.x;
   const int gw = blockIdx.x;
   const int AP = L.AP;
@@ -183,7 +221,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     auto apply_M = [&](int n, const Vec<T>& u, Vec<T>& w) {
       const SB_GLOBAL T* bn = bt + (long long)n * BTS;
       if (f81m) f81_matvec<T>(bn[sb_bt_E(AP) + kstar], pia, onea, u, w);
-      else contract<T>(bn + sb_bt_Mt(AP), AP, u, w, bc, lane);
+      else contract<T>(bn + sb_bt_Mt(AP), AP, u, w, bc, sm, lane);
     };
     auto load_child = [&](int n, int slot, Vec<T>& u, Flags& f) {
       if (slot < 0) {
@@ -310,7 +348,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
           const SB_GLOBAL T* bp = bt + (long long)p * BTS;
           if (f81m) f81_vecmat<T>(bp[sb_bt_E(AP) + kstar], pia, dp, x);
-          else contract<T>(bp + sb_bt_M(AP), AP, dp, x, bc, lane);
+          else contract<T>(bp + sb_bt_M(AP), AP, dp, x, bc, sm, lane);
 #pragma unroll
           for (int c = 0; c < TC; ++c) x.v[c] *= o.v[c];
         }
@@ -366,8 +404,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
         } else {
           Vec<T> Ut, Dt;
-          contract<T>(Vs, AP, u, Ut, bc, lane);
-          contract<T>(Vi, AP, d, Dt, bc, lane);
+          contract<T>(Vs, AP, u, Ut, bc, sm, lane);
+          contract<T>(Vi, AP, d, Dt, bc, sm, lane);
           __syncthreads();
 #pragma unroll
           for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = Ut.v[c];
@@ -402,13 +440,11 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const SB_GLOBAL T* S = md + sb_mr_S(AP);
     const T fmu = mdc[sb_mr_misc(AP)];
     double rsum[SB_MAX_RATE];
-    if constexpr (GRAD) {
 #pragma unroll
-      for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
-    }
+    for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
-      T cnt[AW];
+      // natural-basis counts of column c into accb[j*64 + i] (i = lane = row)
       if constexpr (F81) {
         __syncthreads();
         bc[lane] = t2b[c];
@@ -417,7 +453,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         for (int j = 0; j < AW; ++j) {
           const T I = acc[c][j] + t2a[c] + pia * (bc[j] + t3[c]);
           const T pj = md[sb_mr_pi(AP) + (j < AP ? j : 0)];
-          cnt[j] = (j == lane) ? I : fmu * pj * I;
+          accb[j * SB_WAVE + lane] = (j == lane) ? I : fmu * pj * I;
         }
       } else {
         __syncthreads();
@@ -432,38 +468,37 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
 #pragma unroll
           for (int q = 0; q < AW; ++q) tmp[q] = fma(vik, accb[k * 64 + q], tmp[q]);
         }
-#pragma unroll
-        for (int j = 0; j < AW; ++j) {
+        __syncthreads();
+        for (int jj = 0; jj < AP; ++jj) {
           T sacc = T(0);
-          if (j < AP) {
 #pragma unroll
-            for (int q = 0; q < AW; ++q)
-              if (q < AP) sacc = fma(tmp[q], Vc[j * AP + q], sacc);
-          }
-          cnt[j] = (j == lane) ? sacc : (on && j < AP ? S[lane * AP + j] : T(0)) * sacc;
+          for (int q = 0; q < AW; ++q)
+            if (q < AP) sacc = fma(tmp[q], Vc[jj * AP + q], sacc);
+          accb[jj * SB_WAVE + lane] = (jj == lane) ? sacc : (on ? S[lane * AP + jj] : T(0)) * sacc;
         }
       }
       if (valid[c] && (flags & SB_F_COUNTS) && B->counts && lane < A) {
         SB_GLOBAL T* out = glb<T>(B->counts);
-#pragma unroll
-        for (int j = 0; j < AW; ++j)
-          if (j < A) out[((long long)lane * A + j) * C + cc[c]] = cnt[j];
+        for (int jj = 0; jj < A; ++jj) out[((long long)lane * A + jj) * C + cc[c]] = accb[jj * SB_WAVE + lane];
       }
       if constexpr (GRAD) {
         for (int q = 0; q < B->n_rate; ++q) {
           const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
-          double s = 0.0;
+          double sq = 0.0;
+          if (on)
+            for (int jj = 0; jj < AP; ++jj) sq = fma(W[lane * AP + jj], double(accb[jj * SB_WAVE + lane]), sq);
+          const double v = gcol[c] * wsum(sq);
 #pragma unroll
-          for (int j = 0; j < AW; ++j)
-            if (j < AP && on) s = fma(W[lane * AP + j], double(cnt[j]), s);
-          rsum[q] += gcol[c] * wsum(s);
+          for (int qq = 0; qq < SB_MAX_RATE; ++qq)
+            if (qq == q) rsum[qq] += v;
         }
       }
     }
     if constexpr (GRAD) {
       SB_GLOBAL double* rpart = glb<double>(L.rate_part) + (long long)(B->tile_base + tidx) * SB_MAX_RATE;
-      for (int q = 0; q < B->n_rate; ++q)
-        if (lane == 0) rpart[q] = rsum[q];
+#pragma unroll
+      for (int q = 0; q < SB_MAX_RATE; ++q)
+        if (lane == 0 && q < B->n_rate) rpart[q] = rsum[q];
     }
   }
 }
@@ -477,8 +512,11 @@ SB_LINST(float, 64)
 SB_LINST(double, 32)
 SB_LINST(double, 64)
 
+extern "C" int sb_large_tile_cols(int dtype) { return dtype == 0 ? TCof<float>::v : TCof<double>::v; }
+
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max) {
   const long long esz = dtype == 0 ? 4 : 8;
+  const int TC = sb_large_tile_cols(dtype);
   (void)AP;
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
