@@ -24,8 +24,8 @@ extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool gr
                                             dim3 block, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
-extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max);
-extern "C" int sb_large_tile_cols(int dtype);
+extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81);
+extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
@@ -431,7 +431,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
-  const int tile_cols = large ? sb_large_tile_cols(dtype) : SB_WAVE;
+  const int tile_cols = large ? sb_large_tile_cols(dtype, AP, f81) : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
   // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights
@@ -486,7 +486,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   nwaves = (nwaves + wpg - 1) / wpg * wpg;
   long long wave_stride;
-  if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max);
+  if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
   rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);

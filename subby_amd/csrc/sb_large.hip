@@ -1,13 +1,17 @@
 // Large-alphabet path (padded alphabet AP = 16, 20, 32 or 64), gfx950.
 //
-// One wave64 owns a tile of SB_LARGE_TC columns of one block; lane a holds
-// component a of every state vector of those columns, so the inside pass
+// One wave64 owns a tile of TC columns of one block; lane a holds component a
+// of every state vector of those columns, so the inside pass
 // (pruning.py:94-118), the outside pass (outside.py:94-126), the eigenbasis
 // projection (eigensub.py:53-84) and the accumulation (eigensub.py:87-135 or
 // f81_fast.py:7-141) run lane-parallel over states and loop over the tree.
-// Products with an AP x AP matrix read one matrix row per lane-step from
-// L1/L2 (coalesced) and broadcast the vector being contracted from LDS.
-// Lane k keeps row k of the per-column A x A accumulator in registers.
+// Products with an AP x AP matrix stage the matrix in LDS (one coalesced row
+// per step, loads issued in batches) and broadcast the vector being contracted
+// from LDS.  Lane k keeps row k of the per-column A x A accumulator in
+// registers.  Per-column rescale maxima and dot products are DPP wavefront
+// reductions.  The next tree node's child / parent data is fetched one
+// iteration ahead, and a vector computed for the next node is forwarded in
+// registers.
 // Rescaling, Steiner flags and scales follow the small-alphabet kernel:
 // exact power-of-two rescaling with integer exponents, and
 // scale = ldexp(1/s0, eD + eU - e0).
@@ -22,10 +26,6 @@
 #define SB_GLOBAL __attribute__((address_space(1)))
 
 namespace {
-
-// Columns per wave tile: 4 in FP32, 2 in FP64 (the accumulator rows are 2x wider).
-template <typename T> struct TCof { static constexpr int v = SB_LARGE_TC; };
-template <> struct TCof<double> { static constexpr int v = SB_LARGE_TC / 2; };
 
 template <typename T>
 __device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
@@ -73,24 +73,53 @@ __device__ inline T wmax(T v) {
   return fmax(fmax(rl(v, 0), rl(v, 16)), fmax(rl(v, 32), rl(v, 48)));
 }
 
+// Columns per wave tile: 4, except where the A x A accumulator is widest
+// (FP64, or the general 64-state FP32 model) where 2 keeps it in registers.
+template <typename T, int AW, bool F81>
+struct TCof {
+  static constexpr int v = (sizeof(T) == 8 || (AW == 64 && !F81)) ? 2 : 4;
+};
+
 // Per-lane view of TC column vectors: v[c] = component `lane` of column c.
-template <typename T>
+template <typename T, int TC>
 struct Vec {
-  static constexpr int TC = TCof<T>::v;
   T v[TC];
 };
 
-// w_c[lane] = sum_j P[j*AP + lane] * u_c[j]; lanes >= AP give 0.  P is first
-// staged in LDS (sm[j*64 + lane], one coalesced row per step, all loads in
-// flight together), u is published to LDS and read back as broadcasts.
+// Child data as loaded: internal node -> vector + word per column, leaf -> token per column.
+template <typename T, int TC>
+struct Raw {
+  T u[TC];
+  int w[TC];
+};
+
+template <int TC>
+struct Flags {
+  int e[TC];
+  bool has[TC], branch[TC];
+  bool leaf;
+};
+
+// Stage P (AP x AP, read as P[j*AP + lane]) in LDS as sm[j*64 + lane].
 template <typename T>
-__device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T>& u, Vec<T>& w, T* bc, T* sm,
-                                int lane) {
-  constexpr int TC = TCof<T>::v;
-  __syncthreads();
+__device__ inline void stage(const SB_GLOBAL T* P, int AP, T* sm, int lane) {
   const bool on = lane < AP;
-#pragma unroll 8
-  for (int j = 0; j < AP; ++j) sm[j * SB_WAVE + lane] = on ? P[j * AP + lane] : T(0);
+  for (int j0 = 0; j0 < AP; j0 += 16) {
+    T r[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) r[q] = (on && j0 + q < AP) ? P[(j0 + q) * AP + lane] : T(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (j0 + q < AP) sm[(j0 + q) * SB_WAVE + lane] = r[q];
+  }
+}
+
+// w_c[lane] = sum_j P[j*AP + lane] * u_c[j]; lanes >= AP give 0.
+template <typename T, int TC>
+__device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T, TC>& u, Vec<T, TC>& w, T* bc, T* sm,
+                                int lane) {
+  __syncthreads();
+  stage<T>(P, AP, sm, lane);
 #pragma unroll
   for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
   __syncthreads();
@@ -113,43 +142,27 @@ __device__ inline void contract(const SB_GLOBAL T* P, int AP, const Vec<T>& u, V
 }
 
 // w = M u for an F81-structured M: w_a = e u_a + (1 - e) (pi . u) on real states.
-template <typename T>
-__device__ inline void f81_matvec(T e, T pia, T onea, const Vec<T>& u, Vec<T>& w) {
-  constexpr int TC = TCof<T>::v;
+template <typename T, int TC>
+__device__ inline void f81_matvec(T e, T pia, T onea, const Vec<T, TC>& u, Vec<T, TC>& w) {
+  T pu[TC];
 #pragma unroll
-  for (int c = 0; c < TC; ++c) {
-    const T pu = wsum(pia * u.v[c]);
-    w.v[c] = onea * fma(T(1) - e, pu, e * u.v[c]);
-  }
+  for (int c = 0; c < TC; ++c) pu[c] = wsum(pia * u.v[c]);
+#pragma unroll
+  for (int c = 0; c < TC; ++c) w.v[c] = onea * fma(T(1) - e, pu[c], e * u.v[c]);
 }
 
 // x = d^T M for an F81-structured M: x_a = e d_a + (1 - e) pi_a sum_i d_i.
-template <typename T>
-__device__ inline void f81_vecmat(T e, T pia, const Vec<T>& d, Vec<T>& x) {
-  constexpr int TC = TCof<T>::v;
+template <typename T, int TC>
+__device__ inline void f81_vecmat(T e, T pia, const Vec<T, TC>& d, Vec<T, TC>& x) {
+  T sd[TC];
 #pragma unroll
-  for (int c = 0; c < TC; ++c) {
-    const T sd = wsum(d.v[c]);
-    x.v[c] = fma(T(1) - e, pia * sd, e * d.v[c]);
-  }
+  for (int c = 0; c < TC; ++c) sd[c] = wsum(d.v[c]);
+#pragma unroll
+  for (int c = 0; c < TC; ++c) x.v[c] = fma(T(1) - e, pia * sd[c], e * d.v[c]);
 }
 
-template <typename T>
-__device__ inline void token_vec(const SB_GLOBAL int8_t* aln, long long row_off, const long long* cc, int A,
-                                 T onea, int lane, Vec<T>& u, int* tok) {
-  constexpr int TC = TCof<T>::v;
-#pragma unroll
-  for (int c = 0; c < TC; ++c) {
-    const int t = aln[row_off + cc[c]];
-    tok[c] = t;
-    const bool obs = t >= 0 && t < A;
-    u.v[c] = obs ? (lane == t ? T(1) : T(0)) : onea;
-  }
-}
-
-template <typename T>
-__device__ inline void rescale(Vec<T>& w, int* e) {
-  constexpr int TC = TCof<T>::v;
+template <typename T, int TC>
+__device__ inline void rescale(Vec<T, TC>& w, int* e) {
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
     const int x = frexp_exp(wmax(w.v[c]));
@@ -158,22 +171,18 @@ __device__ inline void rescale(Vec<T>& w, int* e) {
   }
 }
 
-template <int TC>
-struct Flags {
-  int e[TC];
-  bool has[TC], branch[TC];
-  bool leaf;
-};
-
 }  // namespace
 
 template <typename T, int AW, bool F81, bool GRAD>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
-  constexpr int TC = TCof<T>::v;
-  typedef Flags<TC> Flags;
+  constexpr int TC = TCof<T, AW, F81>::v;
+  typedef Vec<T, TC> V_;
+  typedef Raw<T, TC> R_;
+  typedef Flags<TC> F_;
   const SbLaunch& L = *Lp;
   __shared__ T bc[TC * SB_WAVE];
   __shared__ T sm[64 * 64];      // staged AP x AP matrix; also the epilogue accumulator
+  __shared__ T sv[F81 ? 1 : 64 * 64];   // V^T for the back-transform
   T* accb = sm;
   const int lane = threadIdx.x;
   const int gw = blockIdx.x;
@@ -218,67 +227,118 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const bool f81m = mdc[sb_mr_misc(AP) + 1] != T(0);    // F81-structured M
     const int kstar = int(mdc[sb_mr_misc(AP) + 2]);
 
-    auto apply_M = [&](int n, const Vec<T>& u, Vec<T>& w) {
+    auto apply_M = [&](int n, const V_& u, V_& w) {
       const SB_GLOBAL T* bn = bt + (long long)n * BTS;
-      if (f81m) f81_matvec<T>(bn[sb_bt_E(AP) + kstar], pia, onea, u, w);
-      else contract<T>(bn + sb_bt_Mt(AP), AP, u, w, bc, sm, lane);
+      if (f81m) f81_matvec<T, TC>(bn[sb_bt_E(AP) + kstar], pia, onea, u, w);
+      else contract<T, TC>(bn + sb_bt_Mt(AP), AP, u, w, bc, sm, lane);
     };
-    auto load_child = [&](int n, int slot, Vec<T>& u, Flags& f) {
+    auto tokens = [&](int n, int* tok) {
+#pragma unroll
+      for (int c = 0; c < TC; ++c) tok[c] = aln[(long long)n * C + cc[c]];
+    };
+    auto token_vec = [&](const int* tok, V_& u) {
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        const bool obs = tok[c] >= 0 && tok[c] < A;
+        u.v[c] = obs ? (lane == tok[c] ? T(1) : T(0)) : onea;
+      }
+    };
+    auto load_raw = [&](R_& r, int n, int slot, const SB_GLOBAL T* vreg, const SB_GLOBAL int* wreg) {
       if (slot < 0) {
-        int tok[TC];
-        token_vec<T>(aln, (long long)n * C, cc, A, onea, lane, u, tok);
+        tokens(n, r.w);
+      } else {
+#pragma unroll
+        for (int c = 0; c < TC; ++c) {
+          r.u[c] = vreg[((long long)slot * TC + c) * SB_WAVE + lane];
+          r.w[c] = wreg[slot * TC + c];
+        }
+      }
+    };
+    auto decode = [&](const R_& r, int slot, V_& u, F_& f) {
+      if (slot < 0) {
+        token_vec(r.w, u);
 #pragma unroll
         for (int c = 0; c < TC; ++c) {
           f.e[c] = 0;
-          f.has[c] = tok[c] >= 0 && tok[c] <= A;
+          f.has[c] = r.w[c] >= 0 && r.w[c] <= A;
           f.branch[c] = false;
         }
         f.leaf = true;
       } else {
 #pragma unroll
         for (int c = 0; c < TC; ++c) {
-          u.v[c] = Ureg[((long long)slot * TC + c) * SB_WAVE + lane];
-          const int w = Uw[slot * TC + c];
-          f.e[c] = w >> 2;
-          f.has[c] = (w & 1) != 0;
-          f.branch[c] = (w & 2) != 0;
+          u.v[c] = r.u[c];
+          f.e[c] = r.w[c] >> 2;
+          f.has[c] = (r.w[c] & 1) != 0;
+          f.branch[c] = (r.w[c] & 2) != 0;
         }
         f.leaf = false;
       }
     };
 
-    // ---------------- inside pass ----------------
-    Vec<T> u0;
+    // ---------------- inside pass (postorder, one node ahead) ----------------
+    V_ u0;
     int e0[TC];
     bool rbr[TC];
 #pragma unroll
     for (int c = 0; c < TC; ++c) { e0[c] = 0; rbr[c] = false; }
     if (nint == 0) {
       int tok[TC];
-      token_vec<T>(aln, 0, cc, A, onea, lane, u0, tok);
+      tokens(0, tok);
+      token_vec(tok, u0);
+    }
+    R_ nl, nr;
+    int np_ = 0, nl_ = 0, nr_ = 0, nsl = -1, nsr = -1;
+    if (nint > 0) {
+      const SB_CONST int* rec = sched + (nint - 1) * 8;
+      np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+      load_raw(nl, nl_, nsl, Ureg, Uw);
+      load_raw(nr, nr_, nsr, Ureg, Uw);
     }
     for (int i = nint - 1; i >= 0; --i) {
-      const SB_CONST int* rec = sched + i * 8;
-      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
-      Vec<T> u, wl, up;
-      Flags fl, fr;
-      load_child(l, sl, u, fl);
+      const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
+      const R_ rl_ = nl, rr_ = nr;
+      int tokp[TC];
+      tokens(p, tokp);
+      bool fwd_l = false, fwd_r = false;
+      if (i > 0) {
+        const SB_CONST int* rec = sched + (i - 1) * 8;
+        np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+        fwd_l = nsl == i;
+        fwd_r = nsr == i;
+        if (!fwd_l) load_raw(nl, nl_, nsl, Ureg, Uw);
+        if (!fwd_r) load_raw(nr, nr_, nsr, Ureg, Uw);
+      }
+      V_ u, wl, up;
+      F_ fl, fr;
+      decode(rl_, sl, u, fl);
       apply_M(l, u, wl);
-      load_child(r, sr, u, fr);
+      decode(rr_, sr, u, fr);
       apply_M(r, u, up);
-      int tok[TC];
-      token_vec<T>(aln, (long long)p * C, cc, A, onea, lane, u, tok);
+      token_vec(tokp, u);
 #pragma unroll
       for (int c = 0; c < TC; ++c) up.v[c] = u.v[c] * wl.v[c] * up.v[c];
       int e[TC];
-      rescale<T>(up, e);
+      rescale<T, TC>(up, e);
+      int word[TC];
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        const int ep = fl.e[c] + fr.e[c] + e[c];
+        word[c] = (ep << 2) | ((fl.has[c] && fr.has[c]) ? 2 : 0) | ((fl.has[c] || fr.has[c]) ? 1 : 0);
+      }
       if (i > 0) {
 #pragma unroll
         for (int c = 0; c < TC; ++c) {
           Ureg[((long long)i * TC + c) * SB_WAVE + lane] = up.v[c];
-          const int ep = fl.e[c] + fr.e[c] + e[c];
-          const int w = (ep << 2) | ((fl.has[c] && fr.has[c]) ? 2 : 0) | ((fl.has[c] || fr.has[c]) ? 1 : 0);
-          if (lane == 0) Uw[i * TC + c] = w;
+          if (lane == 0) Uw[i * TC + c] = word[c];
+        }
+        if (fwd_l) {
+#pragma unroll
+          for (int c = 0; c < TC; ++c) { nl.u[c] = up.v[c]; nl.w[c] = word[c]; }
+        }
+        if (fwd_r) {
+#pragma unroll
+          for (int c = 0; c < TC; ++c) { nr.u[c] = up.v[c]; nr.w[c] = word[c]; }
         }
       } else {
 #pragma unroll
@@ -292,18 +352,20 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     // log-likelihood, root posterior
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
     T inv_s0[TC];
+    double s0s[TC];
+#pragma unroll
+    for (int c = 0; c < TC; ++c) s0s[c] = wsum((lane < A) ? pi64[lane < A ? lane : 0] * double(u0.v[c]) : 0.0);
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
-      const double s0 = wsum((lane < A) ? pi64[lane < A ? lane : 0] * double(u0.v[c]) : 0.0);
-      const double ll = double(e0[c]) * 0.69314718055994530942 + log(s0);
-      inv_s0[c] = T(1.0 / s0);
+      const double ll = double(e0[c]) * 0.69314718055994530942 + log(s0s[c]);
+      inv_s0[c] = T(1.0 / s0s[c]);
       if (valid[c] && lane == 0 && (flags & SB_F_LL) && B->loglike) glb<double>(B->loglike)[cc[c]] = ll;
       if (valid[c] && lane < A && (flags & SB_F_ROOT) && B->root)
         glb<T>(B->root)[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
     }
     if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
 
-    // ---------------- outside pass + accumulation ----------------
+    // ---------------- outside pass + accumulation (preorder, one node ahead) ----------------
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
     const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) : nullptr;
     double gcol[TC];
@@ -324,31 +386,48 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const SB_GLOBAL T* Vs = md + sb_mr_Vs(AP);
     const SB_GLOBAL T* Vi = md + sb_mr_Vi(AP);
 
+    R_ ql, qr, qd;
+    int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
+    if (nint > 0) {
+      qp = sched[0]; qls = sched[1]; qrs = sched[2]; qsl = sched[4]; qsr = sched[5];
+      load_raw(ql, qls, qsl, Ureg, Uw);
+      load_raw(qr, qrs, qsr, Ureg, Uw);
+    }
     for (int i = 0; i < nint; ++i) {
-      const SB_CONST int* rec = sched + i * 8;
-      const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
-      Vec<T> x;
+      const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
+      const R_ rl_ = ql, rr_ = qr, rd_ = qd;
+      int tokp[TC];
+      tokens(p, tokp);
+      bool fwd0 = false, fwd1 = false;
+      if (i + 1 < nint) {
+        const SB_CONST int* rec = sched + (i + 1) * 8;
+        qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
+        fwd0 = qp == l;
+        fwd1 = qp == r;
+        if (!fwd0 && !fwd1) load_raw(qd, 0, i + 1, Dreg, Dw);
+        load_raw(ql, qls, qsl, Ureg, Uw);
+        load_raw(qr, qrs, qsr, Ureg, Uw);
+      }
+      V_ x;
       int ex[TC];
       bool stp[TC];
       {
-        Vec<T> o;
-        int tok[TC];
-        token_vec<T>(aln, (long long)p * C, cc, A, onea, lane, o, tok);
+        V_ o;
+        token_vec(tokp, o);
         if (i == 0) {
 #pragma unroll
           for (int c = 0; c < TC; ++c) { x.v[c] = o.v[c] * pia; ex[c] = 0; stp[c] = rbr[c]; }
         } else {
-          Vec<T> dp;
+          V_ dp;
 #pragma unroll
           for (int c = 0; c < TC; ++c) {
-            dp.v[c] = Dreg[((long long)i * TC + c) * SB_WAVE + lane];
-            const int w = Dw[i * TC + c];
-            ex[c] = w >> 1;
-            stp[c] = (w & 1) != 0;
+            dp.v[c] = rd_.u[c];
+            ex[c] = rd_.w[c] >> 1;
+            stp[c] = (rd_.w[c] & 1) != 0;
           }
           const SB_GLOBAL T* bp = bt + (long long)p * BTS;
-          if (f81m) f81_vecmat<T>(bp[sb_bt_E(AP) + kstar], pia, dp, x);
-          else contract<T>(bp + sb_bt_M(AP), AP, dp, x, bc, sm, lane);
+          if (f81m) f81_vecmat<T, TC>(bp[sb_bt_E(AP) + kstar], pia, dp, x);
+          else contract<T, TC>(bp + sb_bt_M(AP), AP, dp, x, bc, sm, lane);
 #pragma unroll
           for (int c = 0; c < TC; ++c) x.v[c] *= o.v[c];
         }
@@ -357,24 +436,30 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       for (int s = 0; s < 2; ++s) {
         const int n = s == 0 ? l : r, sn = s == 0 ? sl : sr;
         const int sib = s == 0 ? r : l, ssib = s == 0 ? sr : sl;
-        Vec<T> u, d;
-        Flags fs, fn;
-        load_child(sib, ssib, u, fs);
+        V_ u, d;
+        F_ fs, fn;
+        decode(s == 0 ? rr_ : rl_, ssib, u, fs);
         apply_M(sib, u, d);
         int e[TC], ed[TC];
 #pragma unroll
         for (int c = 0; c < TC; ++c) d.v[c] *= x.v[c];
-        rescale<T>(d, e);
-        load_child(n, sn, u, fn);
+        rescale<T, TC>(d, e);
+        decode(s == 0 ? rl_ : rr_, sn, u, fn);
         bool act[TC];
         T sc[TC], scr[TC];
+        const bool fwd = s == 0 ? fwd0 : fwd1;
 #pragma unroll
         for (int c = 0; c < TC; ++c) {
           ed[c] = fs.e[c] + ex[c] + e[c];
           const bool st = (fn.leaf ? fn.has[c] : fn.branch[c]) || (stp[c] && fn.has[c]);
+          const int dword = (ed[c] << 1) | (st ? 1 : 0);
           if (sn >= 0) {
             Dreg[((long long)sn * TC + c) * SB_WAVE + lane] = d.v[c];
-            if (lane == 0) Dw[sn * TC + c] = (ed[c] << 1) | (st ? 1 : 0);
+            if (lane == 0) Dw[sn * TC + c] = dword;
+          }
+          if (fwd) {
+            qd.u[c] = d.v[c];
+            qd.w[c] = dword;
           }
           if (GRAD || mask_mode == SB_MASK_NONE) act[c] = true;
           else if (mask_mode == SB_MASK_AUTO) act[c] = st && stp[c];
@@ -389,35 +474,39 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
 #pragma unroll
           for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
           __syncthreads();
+          T Ds[TC], piU[TC], Dsum[TC];
 #pragma unroll
           for (int c = 0; c < TC; ++c) {
-            const T Ds = d.v[c] * sc[c];
-            const T piU = wsum(pia * u.v[c]);
-            const T Dsum = wsum(Ds);
-            const T ad = al * Ds;
+            Ds[c] = d.v[c] * sc[c];
+            piU[c] = wsum(pia * u.v[c]);
+            Dsum[c] = wsum(Ds[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < TC; ++c) {
+            const T ad = al * Ds[c];
 #pragma unroll
             for (int q = 0; q < AW; ++q)
               if (q < AP) acc[c][q] = fma(ad, bc[c * SB_WAVE + q], acc[c][q]);
-            t2a[c] = fma(be * Ds, piU, t2a[c]);
-            t2b[c] = fma(be * Dsum, u.v[c], t2b[c]);
-            t3[c] = fma(ga * Dsum, piU, t3[c]);
+            t2a[c] = fma(be * Ds[c], piU[c], t2a[c]);
+            t2b[c] = fma(be * Dsum[c], u.v[c], t2b[c]);
+            t3[c] = fma(ga * Dsum[c], piU[c], t3[c]);
           }
         } else {
-          Vec<T> Ut, Dt;
-          contract<T>(Vs, AP, u, Ut, bc, sm, lane);
-          contract<T>(Vi, AP, d, Dt, bc, sm, lane);
+          V_ Ut, Dt;
+          contract<T, TC>(Vs, AP, u, Ut, bc, sm, lane);
+          contract<T, TC>(Vi, AP, d, Dt, bc, sm, lane);
           __syncthreads();
+          stage<T>(bn + sb_bt_J(AP), AP, sm, lane);   // J is symmetric: row q across lanes
 #pragma unroll
           for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = Ut.v[c];
           __syncthreads();
-          const SB_GLOBAL T* J = bn + sb_bt_J(AP);   // symmetric: row q read across lanes
           T dk[TC];
 #pragma unroll
           for (int c = 0; c < TC; ++c) dk[c] = Dt.v[c] * sc[c];
 #pragma unroll
           for (int q = 0; q < AW; ++q) {
             if (q < AP) {
-              const T jq = on ? J[q * AP + lane] : T(0);
+              const T jq = sm[q * SB_WAVE + lane];
 #pragma unroll
               for (int c = 0; c < TC; ++c) acc[c][q] = fma(dk[c] * jq, bc[c * SB_WAVE + q], acc[c][q]);
             }
@@ -435,10 +524,15 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     }
 
     // ---------------- epilogue: natural-basis counts per column ----------------
-    const SB_GLOBAL T* V = md + sb_mr_V(AP);
     const SB_CONST T* Vc = mdc + sb_mr_V(AP);
     const SB_GLOBAL T* S = md + sb_mr_S(AP);
     const T fmu = mdc[sb_mr_misc(AP)];
+    if constexpr (!F81) {
+      __syncthreads();
+      // sv[k*64 + lane] = V[lane][k]
+      const SB_GLOBAL T* Vg = md + sb_mr_V(AP);
+      for (int k = 0; k < AP; ++k) sv[k * SB_WAVE + lane] = on ? Vg[lane * AP + k] : T(0);
+    }
     double rsum[SB_MAX_RATE];
 #pragma unroll
     for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
@@ -464,7 +558,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
 #pragma unroll
         for (int q = 0; q < AW; ++q) tmp[q] = T(0);
         for (int k = 0; k < AP; ++k) {
-          const T vik = on ? V[lane * AP + k] : T(0);
+          const T vik = sv[k * SB_WAVE + lane];
 #pragma unroll
           for (int q = 0; q < AW; ++q) tmp[q] = fma(vik, accb[k * 64 + q], tmp[q]);
         }
@@ -512,12 +606,19 @@ SB_LINST(float, 64)
 SB_LINST(double, 32)
 SB_LINST(double, 64)
 
-extern "C" int sb_large_tile_cols(int dtype) { return dtype == 0 ? TCof<float>::v : TCof<double>::v; }
+static int large_tc(int dtype, int AP, bool f81) {
+  if (dtype == 0) {
+    if (AP <= 32) return TCof<float, 32, false>::v;
+    return f81 ? TCof<float, 64, true>::v : TCof<float, 64, false>::v;
+  }
+  return TCof<double, 64, false>::v;
+}
 
-extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max) {
+extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81) { return large_tc(dtype, AP, f81); }
+
+extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81) {
   const long long esz = dtype == 0 ? 4 : 8;
-  const int TC = sb_large_tile_cols(dtype);
-  (void)AP;
+  const int TC = large_tc(dtype, AP, f81);
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
 
@@ -552,4 +653,3 @@ extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, in
   hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
   return hipGetLastError();
 }
-
