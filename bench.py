@@ -170,7 +170,7 @@ def other_workload(args, log):
             roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "alg_flops_per_launch": fl,
                     "note": "VALU kernel (no MFMA yet); peak is the FP32 dense figure"}
-        roof.update({"kernel": f"sb_large_kernel<float,64,{'true' if f81 else 'false'},false>", "kernel_ms": kavg})
+        roof.update({"kernel": "sb_f81w_kernel<64>" if f81 else "sb_large_kernel<float,64,false,false>", "kernel_ms": kavg})
         out = {"metric": f"Counts column*branches/sec ({'F81 fast path' if f81 else 'general reversible'}, A=64, FP32)",
                "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",

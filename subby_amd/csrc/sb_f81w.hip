@@ -1,0 +1,517 @@
+// F81 fast path for padded alphabets 16..64 in FP32 on gfx950 (MFMA accumulation).
+//
+// Applies to F81-structured models (one zero eigenvalue with eigenvector
+// sqrt(pi), all other eigenvalues equal), for which
+//     M(t) = e I + (1 - e) 1 pi^T,   e = exp(-mu t),
+// so the inside pass (pruning.py:94-118) and the outside pass
+// (outside.py:94-126) cost O(A) per column and branch.  The counts follow
+// f81_fast.py:7-141:
+//     I_ij = sum_n alpha_n Ds_i U_j + beta_n (Ds_i piU + pi_i sumDs U_j)
+//            + gamma_n pi_i sumDs piU,
+//     counts_ii = I_ii,  counts_ij = mu pi_j I_ij,
+// with Ds = D * scale * mask.  The first term is, per column, a GEMM over the
+// branches (K = 2 per internal node: its two child branches); it runs on the
+// matrix cores (v_mfma_f32_32x32x2f32) with the AW x AW accumulator of each
+// column in 16 * (AW/32)^2 accumulation registers.
+//
+// Layout: one wave owns a tile of 4 columns.  Lane l holds, for column l/16,
+// states (l%16) + 16 j, j < AW/16, of every state vector, so per-column sums and
+// maxima are four DPP steps inside a 16-lane row and every lane of the row
+// ends with the identical total.  Scaled child vectors go through LDS into the
+// MFMA operand layout (lane = state within a 32-state half x branch of the pair).
+//
+// Rescaling, Steiner flags and scales are those of the other kernels: exact
+// power-of-two rescaling with integer exponents, scale = ldexp(1/s0, eD+eU-e0).
+#include <hip/hip_runtime.h>
+
+#include "../../include/subby_amd.h"
+#include "sb_plan.h"
+
+#define SB_CONST __attribute__((address_space(4)))
+#define SB_GLOBAL __attribute__((address_space(1)))
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kTC = 4;   // columns per wave tile
+
+template <typename T>
+__device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
+template <typename T>
+__device__ inline SB_GLOBAL T* glb(void* p) { return (SB_GLOBAL T*)(p); }
+template <typename T>
+__device__ inline const SB_GLOBAL T* glb(const void* p) { return (const SB_GLOBAL T*)(p); }
+
+template <int CTRL>
+__device__ inline float dppf(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ inline double dppd(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+// Butterfly over the 16 lanes of a row (quad swaps, half-row mirror, row
+// mirror): every lane ends with the same total, bit for bit.
+__device__ inline float row_sum(float v) {
+  v += dppf<0xb1>(v);
+  v += dppf<0x4e>(v);
+  v += dppf<0x141>(v);
+  v += dppf<0x140>(v);
+  return v;
+}
+__device__ inline double row_sum(double v) {
+  v += dppd<0xb1>(v);
+  v += dppd<0x4e>(v);
+  v += dppd<0x141>(v);
+  v += dppd<0x140>(v);
+  return v;
+}
+__device__ inline float row_max(float v) {
+  v = fmaxf(v, dppf<0xb1>(v));
+  v = fmaxf(v, dppf<0x4e>(v));
+  v = fmaxf(v, dppf<0x141>(v));
+  v = fmaxf(v, dppf<0x140>(v));
+  return v;
+}
+
+// Child data as loaded: internal node -> vector + per-column word and pi.u;
+// leaf -> token.
+template <int NS>
+struct Raw {
+  float u[NS];
+  int w;
+  float pu;
+};
+
+template <int NS>
+struct Child {
+  float u[NS];
+  float pu;   // pi . u
+  int e;      // log2 scale
+  bool has, branch, leaf;
+};
+
+}  // namespace
+
+// Scratch per wave: U and D vectors of internal nodes ([slot][NS][64] floats)
+// followed by per-column words ([slot][4] int pairs: {word, pi.u or sum d}).
+template <int AW>
+__global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
+  constexpr int NS = AW / 16;   // states per lane
+  constexpr int NT = AW / 32;   // MFMA tiles per dimension
+  typedef Raw<NS> R_;
+  typedef Child<NS> C_;
+  const SbLaunch& L = *Lp;
+  __shared__ float opnd[2][2][kTC][AW];   // [a|u][branch of the pair][column][state]
+  __shared__ float piT[AW + 1];
+  const int lane = threadIdx.x;
+  const int cl = lane >> 4;      // column of this lane
+  const int rl = lane & 15;
+  const int AP = L.AP;
+  const int NI = L.nint_max;
+  const long long vbytes = (long long)NI * NS * SB_WAVE * 4;
+  SB_GLOBAL char* wscr = glb<char>(L.scratch) + (long long)blockIdx.x * L.wave_stride;
+  SB_GLOBAL float* Ureg = (SB_GLOBAL float*)wscr;
+  SB_GLOBAL float* Dreg = (SB_GLOBAL float*)(wscr + vbytes);
+  SB_GLOBAL i32x2* Uw = (SB_GLOBAL i32x2*)(wscr + 2 * vbytes);
+  SB_GLOBAL i32x2* Dw = Uw + (long long)NI * kTC;
+  const int BTS = sb_bt_stride(AP);
+
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(L.counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if (t >= L.ntiles) break;
+    const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * t;
+    const int bidx = tbp[0], tidx = tbp[1];
+    const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(L.blocks) + bidx;
+    const long long C = B->C;
+    const int A = B->A;
+    const int flags = B->flags;
+    const int nint = B->nint;
+    const long long col0 = (long long)tidx * kTC;
+    const long long mycol = col0 + cl;
+    const bool valid = mycol < C;
+    const long long cc = valid ? mycol : C - 1;
+    const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln);
+    const SB_CONST float* bt = cst<float>(B->bt);
+    const SB_CONST float* mdc = cst<float>(B->model);
+    const SB_CONST int* sched = cst<int>(B->sched);
+    const int kstar = int(mdc[sb_mr_misc(AP) + 2]);
+    float pis[NS], ones[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int s = rl + 16 * j;
+      pis[j] = s < A ? mdc[sb_mr_pi(AP) + s] : 0.f;
+      ones[j] = s < A ? 1.f : 0.f;
+    }
+    __syncthreads();
+    if (lane < AW) piT[lane] = lane < A ? mdc[sb_mr_pi(AP) + (lane < A ? lane : 0)] : 0.f;
+    if (lane == 0) piT[AW] = 1.f;   // pi . (all ones)
+    __syncthreads();
+
+    auto token = [&](int n) -> int { return aln[(long long)n * C + cc]; };
+    auto emul = [&](int n) -> float { return bt[(long long)n * BTS + sb_bt_E(AP) + kstar]; };
+    auto load_raw = [&](R_& r, int n, int slot, const SB_GLOBAL float* vreg, const SB_GLOBAL i32x2* wreg) {
+      if (slot < 0) {
+        r.w = token(n);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) r.u[j] = vreg[((long long)slot * NS + j) * SB_WAVE + lane];
+        const i32x2 x = wreg[slot * kTC + cl];
+        r.w = x.x;
+        r.pu = __int_as_float(x.y);
+      }
+    };
+    // one-hot / all-ones vector of a token, and pi . u of it
+    auto token_vec = [&](int tok, float* u) -> float {
+      const bool obs = tok >= 0 && tok < A;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) u[j] = obs ? ((rl + 16 * j) == tok ? 1.f : 0.f) : ones[j];
+      return piT[obs ? tok : AW];
+    };
+    auto decode = [&](const R_& r, int slot, C_& ch) {
+      if (slot < 0) {
+        ch.pu = token_vec(r.w, ch.u);
+        ch.e = 0;
+        ch.has = r.w >= 0 && r.w <= A;
+        ch.branch = false;
+        ch.leaf = true;
+      } else {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) ch.u[j] = r.u[j];
+        ch.pu = r.pu;
+        ch.e = r.w >> 2;
+        ch.has = (r.w & 1) != 0;
+        ch.branch = (r.w & 2) != 0;
+        ch.leaf = false;
+      }
+    };
+    // w = M u on real states
+    auto apply_M = [&](float e, const C_& ch, float* w) {
+      const float b = (1.f - e) * ch.pu;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) w[j] = ones[j] * fmaf(e, ch.u[j], b);
+    };
+    auto rescale = [&](float* v) -> int {
+      float m = v[0];
+#pragma unroll
+      for (int j = 1; j < NS; ++j) m = fmaxf(m, v[j]);
+      const int x = __builtin_amdgcn_frexp_expf(row_max(m));
+#pragma unroll
+      for (int j = 0; j < NS; ++j) v[j] = __builtin_amdgcn_ldexpf(v[j], -x);
+      return x;
+    };
+    auto dot_pi = [&](const float* v) -> float {
+      float s = 0.f;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) s = fmaf(pis[j], v[j], s);
+      return row_sum(s);
+    };
+
+    // ---------------- inside pass (postorder, one node ahead) ----------------
+    float u0[NS];
+    int e0 = 0;
+    bool rbr = false;
+    if (nint == 0) (void)token_vec(token(0), u0);
+    R_ nl, nr;
+    int np_ = 0, nl_ = 0, nr_ = 0, nsl = -1, nsr = -1;
+    if (nint > 0) {
+      const SB_CONST int* rec = sched + (nint - 1) * 8;
+      np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+      load_raw(nl, nl_, nsl, Ureg, Uw);
+      load_raw(nr, nr_, nsr, Ureg, Uw);
+    }
+    for (int i = nint - 1; i >= 0; --i) {
+      const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
+      const R_ rl_ = nl, rr_ = nr;
+      const int tokp = token(p);
+      bool fwd_l = false, fwd_r = false;
+      if (i > 0) {
+        const SB_CONST int* rec = sched + (i - 1) * 8;
+        np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
+        fwd_l = nsl == i;
+        fwd_r = nsr == i;
+        if (!fwd_l) load_raw(nl, nl_, nsl, Ureg, Uw);
+        if (!fwd_r) load_raw(nr, nr_, nsr, Ureg, Uw);
+      }
+      C_ fl, fr;
+      decode(rl_, sl, fl);
+      decode(rr_, sr, fr);
+      float wl[NS], up[NS], o[NS];
+      apply_M(emul(l), fl, wl);
+      apply_M(emul(r), fr, up);
+      (void)token_vec(tokp, o);
+#pragma unroll
+      for (int j = 0; j < NS; ++j) up[j] = o[j] * wl[j] * up[j];
+      const int e = rescale(up);
+      const int ep = fl.e + fr.e + e;
+      const int word = (ep << 2) | ((fl.has && fr.has) ? 2 : 0) | ((fl.has || fr.has) ? 1 : 0);
+      if (i > 0) {
+        const float pu = dot_pi(up);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) Ureg[((long long)i * NS + j) * SB_WAVE + lane] = up[j];
+        if (rl == 0) Uw[i * kTC + cl] = i32x2{word, __float_as_int(pu)};
+        if (fwd_l) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) nl.u[j] = up[j];
+          nl.w = word;
+          nl.pu = pu;
+        }
+        if (fwd_r) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) nr.u[j] = up[j];
+          nr.w = word;
+          nr.pu = pu;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) u0[j] = up[j];
+        e0 = ep;
+        rbr = fl.has && fr.has;
+      }
+    }
+    // log-likelihood (FP64 pi . u), root posterior
+    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    double s0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const int s = rl + 16 * j;
+      if (s < A) s0 = fma(pi64[s], double(u0[j]), s0);
+    }
+    s0 = row_sum(s0);
+    const float inv_s0 = float(1.0 / s0);
+    if (valid && rl == 0 && (flags & SB_F_LL) && B->loglike)
+      glb<double>(B->loglike)[cc] = double(e0) * 0.69314718055994530942 + log(s0);
+    if (valid && (flags & SB_F_ROOT) && B->root) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const int s = rl + 16 * j;
+        if (s < A) glb<float>(B->root)[(long long)s * C + cc] = pis[j] * u0[j] * inv_s0;
+      }
+    }
+    if (!(flags & SB_F_COUNTS)) continue;
+
+    // ---------------- outside pass + accumulation (preorder, one node ahead) ----------------
+    const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
+    const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) : nullptr;
+    f32x16 acc[kTC][NT][NT];
+#pragma unroll
+    for (int c = 0; c < kTC; ++c)
+#pragma unroll
+      for (int a = 0; a < NT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b) acc[c][a][b] = f32x16{};
+    float t2a[NS], t2b[NS], t3 = 0.f;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) { t2a[j] = 0.f; t2b[j] = 0.f; }
+
+    R_ ql, qr, qd;
+    int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
+    if (nint > 0) {
+      qp = sched[0]; qls = sched[1]; qrs = sched[2]; qsl = sched[4]; qsr = sched[5];
+      load_raw(ql, qls, qsl, Ureg, Uw);
+      load_raw(qr, qrs, qsr, Ureg, Uw);
+    }
+    for (int i = 0; i < nint; ++i) {
+      const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
+      const R_ rl_ = ql, rr_ = qr, rd_ = qd;
+      const int tokp = token(p);
+      bool fwd0 = false, fwd1 = false;
+      if (i + 1 < nint) {
+        const SB_CONST int* rec = sched + (i + 1) * 8;
+        qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
+        fwd0 = qp == l;
+        fwd1 = qp == r;
+        if (!fwd0 && !fwd1) load_raw(qd, 0, i + 1, Dreg, Dw);
+        load_raw(ql, qls, qsl, Ureg, Uw);
+        load_raw(qr, qrs, qsr, Ureg, Uw);
+      }
+      // x = obs(p) * (pi at the root, else M_p^T d_p)
+      float x[NS];
+      int ex;
+      bool stp;
+      {
+        float o[NS];
+        (void)token_vec(tokp, o);
+        if (i == 0) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) x[j] = o[j] * pis[j];
+          ex = 0;
+          stp = rbr;
+        } else {
+          ex = rd_.w >> 1;
+          stp = (rd_.w & 1) != 0;
+          const float e = emul(p);
+          const float b = (1.f - e) * rd_.pu;   // pu slot holds sum d for D vectors
+#pragma unroll
+          for (int j = 0; j < NS; ++j) x[j] = o[j] * fmaf(e, rd_.u[j], b * pis[j]);
+        }
+      }
+      C_ ch[2];
+      decode(rl_, sl, ch[0]);
+      decode(rr_, sr, ch[1]);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int n = s == 0 ? l : r, sn = s == 0 ? sl : sr;
+        const int sib = s == 0 ? r : l;
+        const C_& fs = ch[1 - s];
+        const C_& fn = ch[s];
+        float d[NS];
+        apply_M(emul(sib), fs, d);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) d[j] *= x[j];
+        const int e = rescale(d);
+        float sd = 0.f;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) sd += d[j];
+        sd = row_sum(sd);
+        const int ed = fs.e + ex + e;
+        const bool st = (fn.leaf ? fn.has : fn.branch) || (stp && fn.has);
+        const int dword = (ed << 1) | (st ? 1 : 0);
+        if (sn >= 0) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) Dreg[((long long)sn * NS + j) * SB_WAVE + lane] = d[j];
+          if (rl == 0) Dw[sn * kTC + cl] = i32x2{dword, __float_as_int(sd)};
+        }
+        if (s == 0 ? fwd0 : fwd1) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) qd.u[j] = d[j];
+          qd.w = dword;
+          qd.pu = sd;
+        }
+        bool act;
+        if (mask_mode == SB_MASK_NONE) act = true;
+        else if (mask_mode == SB_MASK_AUTO) act = st && stp;
+        else act = emask[(long long)n * C + cc] != 0;
+        const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + fn.e - e0) : 0.f;
+        const SB_CONST float* cf = bt + (long long)n * BTS + sb_bt_coef(AP);
+        const float al = cf[0], be = cf[1], ga = cf[2];
+        const float dsum = sd * sc;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          const float ds = d[j] * sc;
+          t2a[j] = fmaf(be * ds, fn.pu, t2a[j]);
+          t2b[j] = fmaf(be * dsum, fn.u[j], t2b[j]);
+          opnd[0][s][cl][rl + 16 * j] = al * ds;
+          opnd[1][s][cl][rl + 16 * j] = fn.u[j];
+        }
+        t3 = fmaf(ga * dsum, fn.pu, t3);
+      }
+      __syncthreads();
+      // acc_c += [a_l a_r] [u_l u_r]^T  (K = 2), per column and 32 x 32 tile
+      const int kb = lane >> 5, ii = lane & 31;
+#pragma unroll
+      for (int c = 0; c < kTC; ++c) {
+        float av[NT], bv[NT];
+#pragma unroll
+        for (int a = 0; a < NT; ++a) {
+          av[a] = opnd[0][kb][c][32 * a + ii];
+          bv[a] = opnd[1][kb][c][32 * a + ii];
+        }
+#pragma unroll
+        for (int a = 0; a < NT; ++a)
+#pragma unroll
+          for (int b = 0; b < NT; ++b)
+            acc[c][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[c][a][b], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+
+    // ---------------- epilogue: natural-basis counts ----------------
+    // The rank-two correction t2a_i + pi_i (t2b_j + t3) is one more K = 2 step:
+    // [t2a pi] [1 (t2b + t3)]^T.
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      opnd[0][0][cl][rl + 16 * j] = t2a[j];
+      opnd[0][1][cl][rl + 16 * j] = pis[j];
+      opnd[1][0][cl][rl + 16 * j] = 1.f;
+      opnd[1][1][cl][rl + 16 * j] = t2b[j] + t3;
+    }
+    __syncthreads();
+    const int kb = lane >> 5, ii = lane & 31;
+#pragma unroll
+    for (int c = 0; c < kTC; ++c) {
+      float av[NT], bv[NT];
+#pragma unroll
+      for (int a = 0; a < NT; ++a) {
+        av[a] = opnd[0][kb][c][32 * a + ii];
+        bv[a] = opnd[1][kb][c][32 * a + ii];
+      }
+#pragma unroll
+      for (int a = 0; a < NT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+          acc[c][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[c][a][b], 0, 0, 0);
+    }
+    // counts_ii = I_ii, counts_ij = mu pi_j I_ij; accumulator element q of tile
+    // (a, b) in this lane is I[32a + 8(q/4) + 4(lane/32) + q%4][32b + lane%32]
+    const float fmu = mdc[sb_mr_misc(AP)];
+    SB_GLOBAL float* out = glb<float>(B->counts);
+    const bool vec4 = (C % 4 == 0) && (col0 + kTC <= C);
+    if (out) {
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        const int j = 32 * b + ii;
+        const float fj = fmu * piT[j];
+#pragma unroll
+        for (int a = 0; a < NT; ++a) {
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int i = 32 * a + 8 * (q >> 2) + 4 * kb + (q & 3);
+            const float f = i == j ? 1.f : fj;
+            if (i < A && j < A) {
+              SB_GLOBAL float* dst = out + ((long long)i * A + j) * C + col0;
+              if (vec4) {
+                __builtin_nontemporal_store(
+                    f32x4{f * acc[0][a][b][q], f * acc[1][a][b][q], f * acc[2][a][b][q], f * acc[3][a][b][q]},
+                    (SB_GLOBAL f32x4*)dst);
+              } else {
+#pragma unroll
+                for (int c = 0; c < kTC; ++c)
+                  if (col0 + c < C) dst[c] = f * acc[c][a][b][q];
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template __global__ void sb_f81w_kernel<32>(const SbLaunch*);
+template __global__ void sb_f81w_kernel<64>(const SbLaunch*);
+
+static void (*pick_f81w(int AP))(const SbLaunch*) {
+  if (AP <= 32) return sb_f81w_kernel<32>;
+  if (AP <= 64) return sb_f81w_kernel<64>;
+  return nullptr;
+}
+
+extern "C" int sb_f81w_tile_cols() { return kTC; }
+
+extern "C" long long sb_f81w_wave_stride(int AP, int nint_max) {
+  const int AW = AP <= 32 ? 32 : 64;
+  const long long NS = AW / 16;
+  return (long long)nint_max * (2 * NS * SB_WAVE * 4 + 2 * kTC * 8);
+}
+
+extern "C" int sb_f81w_occupancy(int AP) {
+  void (*k)(const SbLaunch*) = pick_f81w(AP);
+  int n = 0;
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_f81w(AP);
+  if (!k) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
+  return hipGetLastError();
+}

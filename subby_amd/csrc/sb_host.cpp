@@ -28,6 +28,10 @@ extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool 
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
+extern "C" int sb_f81w_tile_cols();
+extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
+extern "C" int sb_f81w_occupancy(int AP);
+extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
@@ -77,6 +81,7 @@ struct DevBuf {
 struct Model {
   bool alive = false;
   int A = 0, AP = 0;
+  bool f81s = false;  // F81-structured: M(t) = e I + (1 - e) 1 pi^T
   std::vector<double> mu, V, pi;   // host copies (plain path)
   double* d64 = nullptr;  // FP64 master
   void* rec[2] = {nullptr, nullptr};  // float / double records
@@ -328,7 +333,9 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     }
     SB_HIP(hipEventRecord(ev.first, s));
   }
-  if (key.large)
+  if (key.large == 2)
+    SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
+  else if (key.large)
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
                            b->header, s));
   else
@@ -431,7 +438,10 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
-  const int tile_cols = large ? sb_large_tile_cols(dtype, AP, f81) : SB_WAVE;
+  // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
+  bool f81w = large && f81 && dtype == SB_F32 && !grad;
+  for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
+  const int tile_cols = f81w ? sb_f81w_tile_cols() : large ? sb_large_tile_cols(dtype, AP, f81) : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
   // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights
@@ -472,8 +482,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   const int wpg = large ? 1 : b->waves_per_group;
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
-    const int occ = large ? sb_large_occupancy(dtype, AP, f81, grad)
-                          : sb_fused_small_occupancy(dtype, AP, f81, grad, 64 * wpg);
+    const int occ = f81w ? sb_f81w_occupancy(AP)
+                    : large ? sb_large_occupancy(dtype, AP, f81, grad)
+                            : sb_fused_small_occupancy(dtype, AP, f81, grad, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
   }
   int nwaves = ctx->num_cus * wpc;
@@ -486,7 +497,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   nwaves = (nwaves + wpg - 1) / wpg * wpg;
   long long wave_stride;
-  if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
+  if (f81w) wave_stride = sb_f81w_wave_stride(AP, nint_max);
+  else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
   rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);
@@ -589,7 +601,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
-  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, large ? 1 : 0, nwaves, wpg};
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, f81w ? 2 : large ? 1 : 0, nwaves, wpg};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -733,6 +745,7 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   // M through sqrt(pi_j/pi_i) on the identity block).
   std::vector<double> m64(sb_m64_size(AP), 0.0);
   double f81 = 0.0;
+  bool f81s = false;
   for (int a = 0; a < A; ++a) f81 += pi[a] * pi[a];
   f81 = 1.0 / (1.0 - f81);
   for (int k = 0; k < AP; ++k) m64[sb_m64_mu(AP) + k] = k < A ? eigenvalues[k] : 0.0;
@@ -792,6 +805,7 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
       ok = false;
     }
     rec[sb_mr_misc(AP) + 1] = ok ? 1.0 : 0.0;
+    f81s = ok;
     rec[sb_mr_misc(AP) + 2] = ok ? double(ks) : 0.0;
   }
   std::vector<float> recf(rec.begin(), rec.end());
@@ -800,6 +814,7 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.alive = true;
   m.A = A;
   m.AP = AP;
+  m.f81s = f81s;
   m.mu.assign(eigenvalues, eigenvalues + A);
   m.V.assign(eigenvectors, eigenvectors + A * A);
   m.pi.assign(pi, pi + A);

@@ -363,6 +363,32 @@ def test_large_alphabets(sb, eng, A):
     assert np.abs(d.cpu().numpy() - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize('A,C', [(64, 400), (64, 203), (32, 256), (20, 129), (16, 64)])
+def test_f81_structured_large_alphabets(sb, eng, A, C):
+    """F81 fast path on F81-structured models in FP32 (the matrix-core kernel): widths
+    with C % 4 == 0 (vector stores) and ragged tails, every mask mode, and a
+    non-F81-structured model with f81_fast (the general large-alphabet kernel)."""
+    say('F81-structured', A, C)
+    rng = np.random.default_rng(700 + A + C)
+    pi = rng.dirichlet(np.full(A, 2.0))
+    f = sb.f81_model(pi)
+    t = sb.synthetic.random_tree(40, rng, 0.1)
+    aln = sb.synthetic.simulate(t, f, C, rng, gap_frac=0.05, ambig_frac=0.02)
+    aln[3, : C // 3] = -1          # a ragged run of gaps on one internal row
+    for mask in ('auto', None):
+        check_all(sb, eng, aln, t, f, 'fp32', mask=mask, f81=True)
+    mk = rng.random((len(t.parentIndex), C)) < 0.6
+    ref = O.Counts(aln, otree(t), omodel(f), f81_fast=True, branch_mask=mk)
+    got = sb.Counts(aln, t, f, f81_fast_flag=True, branch_mask=mk, precision='fp32', engine=eng)
+    assert err_counts(got, ref) <= 1e-4
+    # JC is F81-structured too
+    jc = sb.jukes_cantor_model(A)
+    check_all(sb, eng, aln, t, jc, 'fp32', f81=True)
+    # f81_fast on a general reversible model (not F81-structured)
+    Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi)
+    check_all(sb, eng, aln[:, :37], t, sb.diagonalize_rate_matrix(Q, pi), 'fp32', f81=True)
+
+
 def test_autograd_functions(sb, eng):
     """torch.autograd.Function wrappers: branch lengths and log kappa."""
     say('autograd functions')
