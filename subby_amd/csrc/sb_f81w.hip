@@ -71,6 +71,15 @@ __device__ inline double row_sum(double v) {
   v += dppd<0x140>(v);
   return v;
 }
+// Orders this wave's LDS writes before its later LDS reads (and reads before
+// later writes).  The workgroup is a single wave and LDS executes a wave's
+// instructions in order, so no wait on global memory is needed, unlike
+// __syncthreads(), whose workgroup fence drains every load and store in flight.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ inline float row_max(float v) {
   v = fmaxf(v, dppf<0xb1>(v));
   v = fmaxf(v, dppf<0x4e>(v));
@@ -78,15 +87,6 @@ __device__ inline float row_max(float v) {
   v = fmaxf(v, dppf<0x140>(v));
   return v;
 }
-
-// Child data as loaded: internal node -> vector + per-column word and pi.u;
-// leaf -> token.
-template <int NS>
-struct Raw {
-  float u[NS];
-  int w;
-  float pu;
-};
 
 template <int NS>
 struct Child {
@@ -104,7 +104,6 @@ template <int AW>
 __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
   constexpr int NS = AW / 16;   // states per lane
   constexpr int NT = AW / 32;   // MFMA tiles per dimension
-  typedef Raw<NS> R_;
   typedef Child<NS> C_;
   const SbLaunch& L = *Lp;
   __shared__ float opnd[2][2][kTC][AW];   // [a|u][branch of the pair][column][state]
@@ -150,23 +149,51 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
       pis[j] = s < A ? mdc[sb_mr_pi(AP) + s] : 0.f;
       ones[j] = s < A ? 1.f : 0.f;
     }
-    __syncthreads();
+    wave_sync();
     if (lane < AW) piT[lane] = lane < A ? mdc[sb_mr_pi(AP) + (lane < A ? lane : 0)] : 0.f;
     if (lane == 0) piT[AW] = 1.f;   // pi . (all ones)
-    __syncthreads();
+    wave_sync();
 
     auto token = [&](int n) -> int { return aln[(long long)n * C + cc]; };
     auto emul = [&](int n) -> float { return bt[(long long)n * BTS + sb_bt_E(AP) + kstar]; };
-    auto load_raw = [&](R_& r, int n, int slot, const SB_GLOBAL float* vreg, const SB_GLOBAL i32x2* wreg) {
-      if (slot < 0) {
-        r.w = token(n);
-      } else {
+    const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
+    // rows of the explicit mask (the alignment stands in when there is none; never used then)
+    const SB_GLOBAL uint8_t* mrow = B->mask ? glb<uint8_t>(B->mask) : (const SB_GLOBAL uint8_t*)aln;
+
+    // Everything one tree record needs is fetched one record ahead (its node
+    // indices two ahead) with unconditional loads: slots of leaf children are
+    // clamped to 0 and the values discarded, so no load or store sits in a
+    // branch and the loops are unrolled by two over ping-pong buffers (no
+    // loop-carried copies of loaded registers).
+    struct Idx {
+      int p, l, r, sl, sr;
+    };
+    auto fetch_idx = [&](Idx& x, int k) {
+      const SB_CONST int* rec = sched + k * 8;
+      x.p = rec[0]; x.l = rec[1]; x.r = rec[2]; x.sl = rec[4]; x.sr = rec[5];
+    };
+    struct Rec {
+      Idx ix;
+      int tp, tl, tr;
+      float ul[NS], ur[NS];
+      i32x2 wl, wr;
+      float el, er;
+    };
+    auto fetch = [&](Rec& x, const Idx& ix) {
+      x.ix = ix;
+      x.tp = token(ix.p);
+      x.tl = token(ix.l);
+      x.tr = token(ix.r);
+      const long long a = ix.sl < 0 ? 0 : ix.sl, b = ix.sr < 0 ? 0 : ix.sr;
 #pragma unroll
-        for (int j = 0; j < NS; ++j) r.u[j] = vreg[((long long)slot * NS + j) * SB_WAVE + lane];
-        const i32x2 x = wreg[slot * kTC + cl];
-        r.w = x.x;
-        r.pu = __int_as_float(x.y);
+      for (int j = 0; j < NS; ++j) {
+        x.ul[j] = Ureg[(a * NS + j) * SB_WAVE + lane];
+        x.ur[j] = Ureg[(b * NS + j) * SB_WAVE + lane];
       }
+      x.wl = Uw[a * kTC + cl];
+      x.wr = Uw[b * kTC + cl];
+      x.el = emul(ix.l);
+      x.er = emul(ix.r);
     };
     // one-hot / all-ones vector of a token, and pi . u of it
     auto token_vec = [&](int tok, float* u) -> float {
@@ -175,20 +202,25 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
       for (int j = 0; j < NS; ++j) u[j] = obs ? ((rl + 16 * j) == tok ? 1.f : 0.f) : ones[j];
       return piT[obs ? tok : AW];
     };
-    auto decode = [&](const R_& r, int slot, C_& ch) {
+    // child of a record: leaf (token), internal computed by the previous
+    // record (forwarded in registers), or internal loaded from scratch
+    auto child = [&](int slot, int tok, const float* ul, i32x2 wl, int fslot, const float* fu, int fw,
+                     float fpu, C_& ch) {
       if (slot < 0) {
-        ch.pu = token_vec(r.w, ch.u);
+        ch.pu = token_vec(tok, ch.u);
         ch.e = 0;
-        ch.has = r.w >= 0 && r.w <= A;
+        ch.has = tok >= 0 && tok <= A;
         ch.branch = false;
         ch.leaf = true;
       } else {
+        const bool f = slot == fslot;
+        const int w = f ? fw : wl.x;
 #pragma unroll
-        for (int j = 0; j < NS; ++j) ch.u[j] = r.u[j];
-        ch.pu = r.pu;
-        ch.e = r.w >> 2;
-        ch.has = (r.w & 1) != 0;
-        ch.branch = (r.w & 2) != 0;
+        for (int j = 0; j < NS; ++j) ch.u[j] = f ? fu[j] : ul[j];
+        ch.pu = f ? fpu : __int_as_float(wl.y);
+        ch.e = w >> 2;
+        ch.has = (w & 1) != 0;
+        ch.branch = (w & 2) != 0;
         ch.leaf = false;
       }
     };
@@ -214,67 +246,59 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
       return row_sum(s);
     };
 
-    // ---------------- inside pass (postorder, one node ahead) ----------------
+    // ---------------- inside pass (postorder) ----------------
     float u0[NS];
     int e0 = 0;
     bool rbr = false;
     if (nint == 0) (void)token_vec(token(0), u0);
-    R_ nl, nr;
-    int np_ = 0, nl_ = 0, nr_ = 0, nsl = -1, nsr = -1;
     if (nint > 0) {
-      const SB_CONST int* rec = sched + (nint - 1) * 8;
-      np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
-      load_raw(nl, nl_, nsl, Ureg, Uw);
-      load_raw(nr, nr_, nsr, Ureg, Uw);
-    }
-    for (int i = nint - 1; i >= 0; --i) {
-      const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
-      const R_ rl_ = nl, rr_ = nr;
-      const int tokp = token(p);
-      bool fwd_l = false, fwd_r = false;
-      if (i > 0) {
-        const SB_CONST int* rec = sched + (i - 1) * 8;
-        np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
-        fwd_l = nsl == i;
-        fwd_r = nsr == i;
-        if (!fwd_l) load_raw(nl, nl_, nsl, Ureg, Uw);
-        if (!fwd_r) load_raw(nr, nr_, nsr, Ureg, Uw);
-      }
-      C_ fl, fr;
-      decode(rl_, sl, fl);
-      decode(rr_, sr, fr);
-      float wl[NS], up[NS], o[NS];
-      apply_M(emul(l), fl, wl);
-      apply_M(emul(r), fr, up);
-      (void)token_vec(tokp, o);
+      Rec RA, RB;
+      Idx i2;
+      fetch_idx(i2, nint - 1);
+      fetch(RA, i2);
+      fetch_idx(i2, nint > 1 ? nint - 2 : 0);
+      float pv[NS];       // result of the previous record (slot i + 1)
+      int pw = 0;
+      float ppu = 0.f;
 #pragma unroll
-      for (int j = 0; j < NS; ++j) up[j] = o[j] * wl[j] * up[j];
-      const int e = rescale(up);
-      const int ep = fl.e + fr.e + e;
-      const int word = (ep << 2) | ((fl.has && fr.has) ? 2 : 0) | ((fl.has || fr.has) ? 1 : 0);
-      if (i > 0) {
+      for (int j = 0; j < NS; ++j) pv[j] = 0.f;
+      auto step = [&](int i, const Rec& cur, Rec& nxt) {
+        fetch(nxt, i2);                          // record i - 1 (clamped)
+        fetch_idx(i2, i > 1 ? i - 2 : 0);        // record i - 2 (clamped)
+        C_ fl, fr;
+        child(cur.ix.sl, cur.tl, cur.ul, cur.wl, i + 1, pv, pw, ppu, fl);
+        child(cur.ix.sr, cur.tr, cur.ur, cur.wr, i + 1, pv, pw, ppu, fr);
+        float wl[NS], up[NS], o[NS];
+        apply_M(cur.el, fl, wl);
+        apply_M(cur.er, fr, up);
+        (void)token_vec(cur.tp, o);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) up[j] = o[j] * wl[j] * up[j];
+        const int e = rescale(up);
+        const int ep = fl.e + fr.e + e;
+        const int word = (ep << 2) | ((fl.has && fr.has) ? 2 : 0) | ((fl.has || fr.has) ? 1 : 0);
         const float pu = dot_pi(up);
+        // (the root's own slot 0 only ever serves as the clamped dummy)
 #pragma unroll
         for (int j = 0; j < NS; ++j) Ureg[((long long)i * NS + j) * SB_WAVE + lane] = up[j];
-        if (rl == 0) Uw[i * kTC + cl] = i32x2{word, __float_as_int(pu)};
-        if (fwd_l) {
+        Uw[i * kTC + cl] = i32x2{word, __float_as_int(pu)};   // same value from the whole row
 #pragma unroll
-          for (int j = 0; j < NS; ++j) nl.u[j] = up[j];
-          nl.w = word;
-          nl.pu = pu;
+        for (int j = 0; j < NS; ++j) pv[j] = up[j];
+        pw = word;
+        ppu = pu;
+        if (i == 0) {
+#pragma unroll
+          for (int j = 0; j < NS; ++j) u0[j] = up[j];
+          e0 = ep;
+          rbr = fl.has && fr.has;
         }
-        if (fwd_r) {
-#pragma unroll
-          for (int j = 0; j < NS; ++j) nr.u[j] = up[j];
-          nr.w = word;
-          nr.pu = pu;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) u0[j] = up[j];
-        e0 = ep;
-        rbr = fl.has && fr.has;
+      };
+      int i = nint - 1;
+      for (; i >= 1; i -= 2) {
+        step(i, RA, RB);
+        step(i - 1, RB, RA);
       }
+      if (i == 0) step(0, RA, RB);
     }
     // log-likelihood (FP64 pi . u), root posterior
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
@@ -297,9 +321,7 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
     }
     if (!(flags & SB_F_COUNTS)) continue;
 
-    // ---------------- outside pass + accumulation (preorder, one node ahead) ----------------
-    const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
-    const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) : nullptr;
+    // ---------------- outside pass + accumulation (preorder) ----------------
     f32x16 acc[kTC][NT][NT];
 #pragma unroll
     for (int c = 0; c < kTC; ++c)
@@ -311,116 +333,151 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
 #pragma unroll
     for (int j = 0; j < NS; ++j) { t2a[j] = 0.f; t2b[j] = 0.f; }
 
-    R_ ql, qr, qd;
-    int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
-    if (nint > 0) {
-      qp = sched[0]; qls = sched[1]; qrs = sched[2]; qsl = sched[4]; qsr = sched[5];
-      load_raw(ql, qls, qsl, Ureg, Uw);
-      load_raw(qr, qrs, qsr, Ureg, Uw);
-    }
-    for (int i = 0; i < nint; ++i) {
-      const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
-      const R_ rl_ = ql, rr_ = qr, rd_ = qd;
-      const int tokp = token(p);
-      bool fwd0 = false, fwd1 = false;
-      if (i + 1 < nint) {
-        const SB_CONST int* rec = sched + (i + 1) * 8;
-        qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
-        fwd0 = qp == l;
-        fwd1 = qp == r;
-        if (!fwd0 && !fwd1) load_raw(qd, 0, i + 1, Dreg, Dw);
-        load_raw(ql, qls, qsl, Ureg, Uw);
-        load_raw(qr, qrs, qsr, Ureg, Uw);
-      }
-      // x = obs(p) * (pi at the root, else M_p^T d_p)
-      float x[NS];
-      int ex;
-      bool stp;
-      {
-        float o[NS];
-        (void)token_vec(tokp, o);
-        if (i == 0) {
+    // outside-pass extras of a record: D of p (slot k), e of p's branch,
+    // F81 coefficients of the child branches, explicit-mask bytes
+    struct Out {
+      float dp[NS];
+      i32x2 wp;
+      float ep, al[2], be[2], ga[2];
+      int mk[2];
+    };
+    auto fetch_out = [&](Out& y, const Idx& ix, int k) {
 #pragma unroll
-          for (int j = 0; j < NS; ++j) x[j] = o[j] * pis[j];
-          ex = 0;
-          stp = rbr;
-        } else {
-          ex = rd_.w >> 1;
-          stp = (rd_.w & 1) != 0;
-          const float e = emul(p);
-          const float b = (1.f - e) * rd_.pu;   // pu slot holds sum d for D vectors
-#pragma unroll
-          for (int j = 0; j < NS; ++j) x[j] = o[j] * fmaf(e, rd_.u[j], b * pis[j]);
-        }
-      }
-      C_ ch[2];
-      decode(rl_, sl, ch[0]);
-      decode(rr_, sr, ch[1]);
+      for (int j = 0; j < NS; ++j) y.dp[j] = Dreg[((long long)k * NS + j) * SB_WAVE + lane];
+      y.wp = Dw[k * kTC + cl];
+      y.ep = emul(ix.p);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        const int n = s == 0 ? l : r, sn = s == 0 ? sl : sr;
-        const int sib = s == 0 ? r : l;
-        const C_& fs = ch[1 - s];
-        const C_& fn = ch[s];
-        float d[NS];
-        apply_M(emul(sib), fs, d);
-#pragma unroll
-        for (int j = 0; j < NS; ++j) d[j] *= x[j];
-        const int e = rescale(d);
-        float sd = 0.f;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) sd += d[j];
-        sd = row_sum(sd);
-        const int ed = fs.e + ex + e;
-        const bool st = (fn.leaf ? fn.has : fn.branch) || (stp && fn.has);
-        const int dword = (ed << 1) | (st ? 1 : 0);
-        if (sn >= 0) {
-#pragma unroll
-          for (int j = 0; j < NS; ++j) Dreg[((long long)sn * NS + j) * SB_WAVE + lane] = d[j];
-          if (rl == 0) Dw[sn * kTC + cl] = i32x2{dword, __float_as_int(sd)};
-        }
-        if (s == 0 ? fwd0 : fwd1) {
-#pragma unroll
-          for (int j = 0; j < NS; ++j) qd.u[j] = d[j];
-          qd.w = dword;
-          qd.pu = sd;
-        }
-        bool act;
-        if (mask_mode == SB_MASK_NONE) act = true;
-        else if (mask_mode == SB_MASK_AUTO) act = st && stp;
-        else act = emask[(long long)n * C + cc] != 0;
-        const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + fn.e - e0) : 0.f;
+        const int n = s == 0 ? ix.l : ix.r;
         const SB_CONST float* cf = bt + (long long)n * BTS + sb_bt_coef(AP);
-        const float al = cf[0], be = cf[1], ga = cf[2];
-        const float dsum = sd * sc;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-          const float ds = d[j] * sc;
-          t2a[j] = fmaf(be * ds, fn.pu, t2a[j]);
-          t2b[j] = fmaf(be * dsum, fn.u[j], t2b[j]);
-          opnd[0][s][cl][rl + 16 * j] = al * ds;
-          opnd[1][s][cl][rl + 16 * j] = fn.u[j];
-        }
-        t3 = fmaf(ga * dsum, fn.pu, t3);
+        y.al[s] = cf[0];
+        y.be[s] = cf[1];
+        y.ga[s] = cf[2];
+        y.mk[s] = mrow[(long long)n * C + cc];
       }
-      __syncthreads();
-      // acc_c += [a_l a_r] [u_l u_r]^T  (K = 2), per column and 32 x 32 tile
-      const int kb = lane >> 5, ii = lane & 31;
+    };
+    if (nint > 0) {
+      Rec RA, RB;
+      Out OA, OB;
+      Idx i2;
+      fetch_idx(i2, 0);
+      fetch(RA, i2);
+      fetch_out(OA, i2, 0);
+      int k2 = nint > 1 ? 1 : 0;   // record held in i2
+      fetch_idx(i2, k2);
+      // D vectors computed by the previous record for its two children
+      float pd[2][NS];
+      int pdw[2] = {0, 0}, pn[2] = {-1, -1};
+      float psd[2] = {0.f, 0.f};
 #pragma unroll
-      for (int c = 0; c < kTC; ++c) {
-        float av[NT], bv[NT];
+      for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int a = 0; a < NT; ++a) {
-          av[a] = opnd[0][kb][c][32 * a + ii];
-          bv[a] = opnd[1][kb][c][32 * a + ii];
+        for (int j = 0; j < NS; ++j) pd[s][j] = 0.f;
+      auto step = [&](int i, const Rec& cur, const Out& oc, Rec& nxt, Out& on) {
+        fetch(nxt, i2);                 // record i + 1 (clamped)
+        fetch_out(on, i2, k2);
+        k2 = i + 2 < nint ? i + 2 : nint - 1;
+        fetch_idx(i2, k2);              // record i + 2 (clamped)
+        // x = obs(p) * (pi at the root, else M_p^T d_p)
+        float x[NS];
+        int ex;
+        bool stp;
+        {
+          float o[NS];
+          (void)token_vec(cur.tp, o);
+          if (i == 0) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) x[j] = o[j] * pis[j];
+            ex = 0;
+            stp = rbr;
+          } else {
+            const bool f0 = pn[0] == cur.ix.p, f1 = pn[1] == cur.ix.p;
+            const int w = f0 ? pdw[0] : f1 ? pdw[1] : oc.wp.x;
+            const float sdp = f0 ? psd[0] : f1 ? psd[1] : __int_as_float(oc.wp.y);
+            ex = w >> 1;
+            stp = (w & 1) != 0;
+            const float e = oc.ep;
+            const float b = (1.f - e) * sdp;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+              const float dpj = f0 ? pd[0][j] : f1 ? pd[1][j] : oc.dp[j];
+              x[j] = o[j] * fmaf(e, dpj, b * pis[j]);
+            }
+          }
         }
+        C_ ch[2];
+        child(cur.ix.sl, cur.tl, cur.ul, cur.wl, -2, cur.ul, 0, 0.f, ch[0]);   // all U stored by now
+        child(cur.ix.sr, cur.tr, cur.ur, cur.wr, -2, cur.ur, 0, 0.f, ch[1]);
 #pragma unroll
-        for (int a = 0; a < NT; ++a)
+        for (int s = 0; s < 2; ++s) {
+          const int n = s == 0 ? cur.ix.l : cur.ix.r, sn = s == 0 ? cur.ix.sl : cur.ix.sr;
+          const C_& fs = ch[1 - s];
+          const C_& fn = ch[s];
+          float d[NS];
+          apply_M(s == 0 ? cur.er : cur.el, fs, d);
 #pragma unroll
-          for (int b = 0; b < NT; ++b)
-            acc[c][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[c][a][b], 0, 0, 0);
+          for (int j = 0; j < NS; ++j) d[j] *= x[j];
+          const int e = rescale(d);
+          float sd = 0.f;
+#pragma unroll
+          for (int j = 0; j < NS; ++j) sd += d[j];
+          sd = row_sum(sd);
+          const int ed = fs.e + ex + e;
+          const bool st = (fn.leaf ? fn.has : fn.branch) || (stp && fn.has);
+          const int dword = (ed << 1) | (st ? 1 : 0);
+          // leaf children write the never-read slot 0 (the root has no D)
+          const long long ds_slot = sn < 0 ? 0 : sn;
+#pragma unroll
+          for (int j = 0; j < NS; ++j) Dreg[(ds_slot * NS + j) * SB_WAVE + lane] = d[j];
+          Dw[ds_slot * kTC + cl] = i32x2{dword, __float_as_int(sd)};
+#pragma unroll
+          for (int j = 0; j < NS; ++j) pd[s][j] = d[j];
+          pdw[s] = dword;
+          psd[s] = sd;
+          pn[s] = n;
+          bool act;
+          if (mask_mode == SB_MASK_NONE) act = true;
+          else if (mask_mode == SB_MASK_AUTO) act = st && stp;
+          else act = oc.mk[s] != 0;
+          const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + fn.e - e0) : 0.f;
+          const float dsum = sd * sc;
+          const float al = oc.al[s], be = oc.be[s];
+#pragma unroll
+          for (int j = 0; j < NS; ++j) {
+            const float ds = d[j] * sc;
+            t2a[j] = fmaf(be * ds, fn.pu, t2a[j]);
+            t2b[j] = fmaf(be * dsum, fn.u[j], t2b[j]);
+            opnd[0][s][cl][rl + 16 * j] = al * ds;
+            opnd[1][s][cl][rl + 16 * j] = fn.u[j];
+          }
+          t3 = fmaf(oc.ga[s] * dsum, fn.pu, t3);
+        }
+        wave_sync();
+        // acc_c += [a_l a_r] [u_l u_r]^T  (K = 2), per column and 32 x 32 tile
+        const int kb = lane >> 5, ii = lane & 31;
+        float av[kTC][NT], bv[kTC][NT];
+#pragma unroll
+        for (int c = 0; c < kTC; ++c)
+#pragma unroll
+          for (int a = 0; a < NT; ++a) {
+            av[c][a] = opnd[0][kb][c][32 * a + ii];
+            bv[c][a] = opnd[1][kb][c][32 * a + ii];
+          }
+#pragma unroll
+        for (int c = 0; c < kTC; ++c)
+#pragma unroll
+          for (int a = 0; a < NT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+              acc[c][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[c][a], bv[c][b], acc[c][a][b], 0, 0, 0);
+        wave_sync();
+      };
+      int i = 0;
+      for (; i + 1 < nint; i += 2) {
+        step(i, RA, OA, RB, OB);
+        step(i + 1, RB, OB, RA, OA);
       }
-      __syncthreads();
+      if (i < nint) step(i, RA, OA, RB, OB);
     }
 
     // ---------------- epilogue: natural-basis counts ----------------
@@ -433,7 +490,7 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
       opnd[1][0][cl][rl + 16 * j] = 1.f;
       opnd[1][1][cl][rl + 16 * j] = t2b[j] + t3;
     }
-    __syncthreads();
+    wave_sync();
     const int kb = lane >> 5, ii = lane & 31;
 #pragma unroll
     for (int c = 0; c < kTC; ++c) {
@@ -481,7 +538,7 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
