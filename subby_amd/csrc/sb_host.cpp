@@ -32,6 +32,13 @@ extern "C" int sb_f81w_tile_cols();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
 extern "C" int sb_f81w_occupancy(int AP);
 extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s);
+extern "C" int sb_gen2_tile_cols();
+extern "C" long long sb_gen2_tile_bytes(int AP, int rmax);
+extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaunch* L, hipStream_t s);
+
+// Stage memory of the two-pass general path: tiles are processed in rounds
+// that fit this many bytes.
+static const size_t kGen2StageBudget = size_t(4) << 30;
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
@@ -143,6 +150,7 @@ struct sb_batch {
   bool staged_pending = false;
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
+  int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
   bool timing = false;
@@ -333,7 +341,10 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     }
     SB_HIP(hipEventRecord(ev.first, s));
   }
-  if (key.large == 2)
+  if (key.large == 3) {
+    for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
+      SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
+  } else if (key.large == 2)
     SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
   else if (key.large)
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
@@ -441,7 +452,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
   bool f81w = large && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
-  const int tile_cols = f81w ? sb_f81w_tile_cols() : large ? sb_large_tile_cols(dtype, AP, f81) : SB_WAVE;
+  // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
+  const bool gen2 = large && !f81 && dtype == SB_F32 && !grad;
+  const int tile_cols = gen2   ? sb_gen2_tile_cols()
+                        : f81w ? sb_f81w_tile_cols()
+                        : large ? sb_large_tile_cols(dtype, AP, f81)
+                                : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
   // ---- host staging layout: header | blocks | sched | tiles | jobs | rate weights
@@ -501,8 +517,19 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
-  rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);
-  if (rc) return rc;
+  if (!gen2) {
+    rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);
+    if (rc) return rc;
+  }
+  long long g2_tile_bytes = 0;
+  if (gen2) {
+    g2_tile_bytes = sb_gen2_tile_bytes(AP, R_max);
+    const size_t per_round = std::max<size_t>(1, kGen2StageBudget / size_t(g2_tile_bytes));
+    b->g2_ntiles = int(ntiles);
+    b->g2_round = int(std::min<size_t>(std::max<size_t>(ntiles, 1), per_round));
+    rc = ensure(b, b->stage, size_t(b->g2_round) * size_t(g2_tile_bytes));
+    if (rc) return rc;
+  }
   const int grad_stride = R_max;
   if (grad) {
     rc = ensure(b, b->part, std::max<size_t>(1, ntiles) * (grad_stride + SB_MAX_RATE) * sizeof(double));
@@ -585,6 +612,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   L->grad_part = grad ? static_cast<double*>(b->part.p) : nullptr;
   L->rate_part = grad ? static_cast<double*>(b->part.p) + ntiles * grad_stride : nullptr;
   L->stage = b->stage.p;
+  L->stage_stride = g2_tile_bytes;
+  L->rmax = R_max;
   L->wave_stride = wave_stride;
   L->nblocks = nb;
   L->ntiles = int(ntiles);
@@ -601,12 +630,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
-  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, f81w ? 2 : large ? 1 : 0, nwaves, wpg};
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, gen2 ? 3 : f81w ? 2 : large ? 1 : 0, nwaves, wpg};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
   const int nwg_reduce = 64;
-  if (use_graph) {
+  if (use_graph && !gen2) {   // the two-pass path launches per round, uncaptured
     hipGraphExec_t exec = nullptr;
     rc = get_graph(b, key, &exec, nwg_params, grid, block, nwaves, nwg_reduce);
     if (rc) return rc;

@@ -99,9 +99,9 @@ struct SbLaunch {
   char* scratch;           // nwaves * wave_stride bytes
   double* grad_part;       // [tile][grad_stride]: per-tile branch-length partials
   double* rate_part;       // [tile][SB_MAX_RATE]
-  void* stage;             // per-branch staging for large alphabets
+  void* stage;             // two-pass general path: per-tile node vectors
   long long wave_stride;
-  long long stage_stride;  // bytes per staged block column-branch vector set
+  long long stage_stride;  // bytes of stage per tile (two-pass general path)
   int nblocks, ntiles, njobs, nint_max;
-  int grad_stride, AP, tile_cols, pad1;
+  int grad_stride, AP, tile_cols, rmax;   // rmax: largest tree (nodes) in the launch
 };
