@@ -134,18 +134,25 @@ struct Kid {
 
 // ------------------------------------------------------------------ pass 1
 // Stage area of a tile (floats unless noted), R = rmax nodes, 64 columns:
-//   UU[R][64][AW]  U of internal nodes, then Ut of every branch
-//   DD[R][64][AW]  D of internal nodes
-//   DT[R][64][AW]  Dt * scale * mask of every branch
+//   UU[R][64][AW]  U_n of internal nodes (inside pass); W_n = M_n U_n of every
+//                  branch replaces it at the parent's record (U_n is dead then)
+//   UT[R][64][AW]  Ut_n = Vs^T U_n of every branch
+//   DD[R][64][AW]  D_n of internal nodes
+//   DT[R][64][AW]  Dt_n = Vi^T D_n * scale * mask of every branch
 //   UW[R][64], DW[R][64]  ints: U word {exp << 2 | both | any}, D word {exp << 1 | steiner}
+// Rows of leaves that are never written may still be loaded one record ahead
+// (unconditional loads); those values are discarded.
 template <int AW>
-__global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+__global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   constexpr int NV = AW / 4;
   constexpr int PS = AW + 4;
+  constexpr int NTH = 64 * kWPG;
+  constexpr int MPT = AW * AW / 4 / NTH;   // float4 of a staged matrix per thread
   __shared__ __attribute__((aligned(16))) float sVs[AW * PS];   // Vs^T
   __shared__ __attribute__((aligned(16))) float sVi[AW * PS];   // Vi^T
   __shared__ __attribute__((aligned(16))) float sP[2][AW * PS];
   __shared__ float sv1[AW];                                     // Vs^T (all ones on real states)
+  __shared__ float spi[AW];
   const SbLaunch& L = *Lp;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int jc = lane & 15, h = lane >> 4;
@@ -165,7 +172,8 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
   SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
   SB_GLOBAL float* UU = (SB_GLOBAL float*)st;
-  SB_GLOBAL float* DD = UU + vplane;
+  SB_GLOBAL float* UT = UU + vplane;
+  SB_GLOBAL float* DD = UT + vplane;
   SB_GLOBAL float* DT = DD + vplane;
   SB_GLOBAL int* UW = (SB_GLOBAL int*)(DT + vplane);
   SB_GLOBAL int* DWd = UW + (long long)RM * kTW;
@@ -174,27 +182,48 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
   const SB_GLOBAL float* md = glb<float>(B->model);
   const SB_CONST int* sched = cst<int>(B->sched);
   const int BTS = sb_bt_stride(AP);
+  const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
+  // rows of the explicit mask (the alignment stands in when there is none; never used then)
+  const SB_GLOBAL int8_t* mrow = B->mask ? glb<int8_t>(B->mask) : aln;
 
   auto state = [&](int v) { return 16 * (v >> 2) + 4 * h + (v & 3); };
-  float pis[NV], ones[NV];   // pi and 1 on this lane's real states
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    const int s = state(v);
-    pis[v] = s < A ? md[sb_mr_pi(AP) + s] : 0.f;
-    ones[v] = s < A ? 1.f : 0.f;
-  }
-  stage<AW, true>(sVs, md + sb_mr_Vs(AP), AP, tid, 256);
-  stage<AW, true>(sVi, md + sb_mr_Vi(AP), AP, tid, 256);
+  auto real = [&](int v) { return state(v) < A ? 1.f : 0.f; };
+  stage<AW, true>(sVs, md + sb_mr_Vs(AP), AP, tid, NTH);
+  stage<AW, true>(sVi, md + sb_mr_Vi(AP), AP, tid, NTH);
+  if (tid < AW) spi[tid] = tid < A ? md[sb_mr_pi(AP) + tid] : 0.f;
   lds_barrier();
   if (tid < AW) {
     float s = 0.f;
     for (int a = 0; a < A; ++a) s += sVs[tid * PS + a];
     sv1[tid] = s;
   }
-  lds_barrier();
-  float vs1r[NV];
+  // (sv1 is read after the next barrier)
+
+  // matrices are fetched into registers one record ahead and written to LDS
+  struct Mat {
+    f32x4 v[MPT];
+  };
+  auto mload = [&](Mat& m, const SB_GLOBAL float* src) {
 #pragma unroll
-  for (int v = 0; v < NV; ++v) vs1r[v] = sv1[state(v)];
+    for (int q = 0; q < MPT; ++q) {
+      const int e = 4 * (tid + q * NTH);
+      const int r = e / AW, c = e % AW;
+      if (AP == AW) {
+        m.v[q] = *(const SB_GLOBAL f32x4*)(src + r * AW + c);
+      } else {
+#pragma unroll
+        for (int z = 0; z < 4; ++z) m.v[q][z] = (r < AP && c + z < AP) ? src[r * AP + c + z] : 0.f;
+      }
+    }
+  };
+  auto mstore = [&](float* dst, const Mat& m) {
+#pragma unroll
+    for (int q = 0; q < MPT; ++q) {
+      const int e = 4 * (tid + q * NTH);
+      *(f32x4*)(dst + (e / AW) * PS + e % AW) = m.v[q];
+    }
+  };
+  auto mat = [&](int n, int off) { return bt + (long long)n * BTS + off; };
 
   auto vptr = [&](SB_GLOBAL float* base, int n) { return base + ((long long)n * kTW + wcol) * AW + 4 * h; };
   auto load_vec = [&](SB_GLOBAL float* base, int n, float* u) {
@@ -216,14 +245,7 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
   auto token_vec = [&](int tok, float* u) {
     const bool obs = tok >= 0 && tok < A;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) u[v] = obs ? (state(v) == tok ? 1.f : 0.f) : ones[v];
-  };
-  // w = P e_tok (a column of the staged matrix), or the all-ones image
-  // (rows of M(t) sum to one) for an unobserved token
-  auto gather = [&](const float* P, int tok, const float* all, float* w) {
-    const bool obs = tok >= 0 && tok < A;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) w[v] = obs ? P[state(v) * PS + tok] : all[v];
+    for (int v = 0; v < NV; ++v) u[v] = obs ? (state(v) == tok ? 1.f : 0.f) : real(v);
   };
   auto rescale = [&](float* v) -> int {
     float m = v[0];
@@ -234,33 +256,35 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
     for (int i = 0; i < NV; ++i) v[i] = __builtin_amdgcn_ldexpf(v[i], -x);
     return x;
   };
-  auto kid = [&](int n, int slot) {
+
+  // One tree record's indices, tokens and words, fetched one record ahead
+  // with unconditional loads.
+  struct Rec {
+    int p, l, r, sl, sr;
+    int tp, tl, tr, wl, wr, mkl, mkr;
+  };
+  auto fetch = [&](Rec& x, int k, bool outside) {
+    const SB_CONST int* rec = sched + k * 8;
+    x.p = rec[0]; x.l = rec[1]; x.r = rec[2]; x.sl = rec[4]; x.sr = rec[5];
+    x.tp = token(x.p);
+    x.tl = token(x.l);
+    x.tr = token(x.r);
+    x.wl = UW[(long long)x.l * kTW + wcol];
+    x.wr = UW[(long long)x.r * kTW + wcol];
+    if (outside) {
+      x.mkl = mrow[(long long)x.l * C + cc];
+      x.mkr = mrow[(long long)x.r * C + cc];
+    }
+  };
+  auto kid = [&](int n, int slot, int tok, int w) {
     Kid k;
     k.n = n;
     k.leaf = slot < 0;
-    if (k.leaf) {
-      k.tok = token(n);
-      k.e = 0;
-      k.has = k.tok >= 0 && k.tok <= A;
-      k.branch = false;
-    } else {
-      const int w = UW[(long long)n * kTW + wcol];
-      k.tok = 0;
-      k.e = w >> 2;
-      k.has = (w & 1) != 0;
-      k.branch = (w & 2) != 0;
-    }
+    k.tok = tok;
+    k.e = k.leaf ? 0 : w >> 2;
+    k.has = k.leaf ? (tok >= 0 && tok <= A) : (w & 1) != 0;
+    k.branch = !k.leaf && (w & 2) != 0;
     return k;
-  };
-  // u (the child's inside vector) and w = M u, with M staged in P
-  auto child_vecs = [&](const Kid& k, const float* P, float* u, float* w) {
-    if (k.leaf) {
-      token_vec(k.tok, u);
-      gather(P, k.tok, ones, w);
-    } else {
-      load_vec(UU, k.n, u);
-      gemm<AW>(P, u, w, lane);
-    }
   };
 
   // ---------------- inside pass (postorder) ----------------
@@ -268,33 +292,74 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
   int e0 = 0;
   bool rbr = false;
   if (nint == 0) token_vec(token(0), u0);
-  for (int i = nint - 1; i >= 0; --i) {
-    const SB_CONST int* rec = sched + i * 8;
-    const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
-    const Kid kl = kid(l, sl), kr = kid(r, sr);
-    const int tokp = token(p);
-    lds_barrier();   // previous readers of sP are done
-    stage<AW, false>(sP[0], bt + (long long)l * BTS + sb_bt_M(AP), AP, tid, 256);
-    stage<AW, false>(sP[1], bt + (long long)r * BTS + sb_bt_M(AP), AP, tid, 256);
-    lds_barrier();
-    float u[NV], wl[NV], wr[NV];
-    child_vecs(kl, sP[0], u, wl);
-    child_vecs(kr, sP[1], u, wr);
-    float up[NV];
-    token_vec(tokp, up);
+  if (nint > 0) {
+    Rec cur, nxt;
+    Mat ml, mr;
+    fetch(cur, nint - 1, false);
+    mload(ml, mat(cur.l, sb_bt_M(AP)));
+    mload(mr, mat(cur.r, sb_bt_M(AP)));
+    for (int i = nint - 1; i >= 0; --i) {
+      float wl[NV], wr[NV];
+      load_vec(UU, cur.l, wl);       // U of internal children (leaf rows: discarded)
+      load_vec(UU, cur.r, wr);
+      lds_barrier();                 // previous readers of sP are done
+      mstore(sP[0], ml);
+      mstore(sP[1], mr);
+      const int kn = i > 0 ? i - 1 : 0;   // next record (clamped)
+      fetch(nxt, kn, false);
+      {
+        const SB_CONST int* rn = sched + kn * 8;
+        mload(ml, mat(rn[1], sb_bt_M(AP)));
+        mload(mr, mat(rn[2], sb_bt_M(AP)));
+      }
+      lds_barrier();
+      const Kid kl = kid(cur.l, cur.sl, cur.tl, cur.wl), kr = kid(cur.r, cur.sr, cur.tr, cur.wr);
+      // W = M u for both children; leaves: a column of M (or ones), and their Ut
 #pragma unroll
-    for (int v = 0; v < NV; ++v) up[v] *= wl[v] * wr[v];
-    const int e = rescale(up);
-    const int ep = kl.e + kr.e + e;
-    if (i > 0) {
-      const int word = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
-      store_vec(UU, p, up);
-      if (h == 0) UW[(long long)p * kTW + wcol] = word;
-    } else {
+      for (int s = 0; s < 2; ++s) {
+        const Kid& k = s == 0 ? kl : kr;
+        float* w = s == 0 ? wl : wr;
+        const float* P = sP[s];
+        if (k.leaf) {
+          const bool obs = k.tok >= 0 && k.tok < A;
+          float t[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) u0[v] = up[v];
-      e0 = ep;
-      rbr = kl.has && kr.has;
+          for (int v = 0; v < NV; ++v) {
+            w[v] = obs ? P[state(v) * PS + k.tok] : real(v);
+            t[v] = obs ? sVs[state(v) * PS + k.tok] : sv1[state(v)];
+          }
+          store_vec(UT, k.n, t);
+        } else {
+          float u[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) u[v] = w[v];
+          gemm<AW>(P, u, w, lane);
+        }
+        store_vec(UU, k.n, w);       // W_n replaces U_n
+      }
+      float up[NV];
+      token_vec(cur.tp, up);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) up[v] *= wl[v] * wr[v];
+      const int e = rescale(up);
+      const int ep = kl.e + kr.e + e;
+      if (i > 0) {
+        const int word = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
+        store_vec(UU, cur.p, up);
+        if (h == 0) UW[(long long)cur.p * kTW + wcol] = word;
+        // the next record's words were fetched before this one was written
+        if (nxt.l == cur.p) nxt.wl = word;
+        if (nxt.r == cur.p) nxt.wr = word;
+        float t[NV];
+        gemm<AW>(sVs, up, t, lane);
+        store_vec(UT, cur.p, t);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u0[v] = up[v];
+        e0 = ep;
+        rbr = kl.has && kr.has;
+      }
+      cur = nxt;
     }
   }
   // log-likelihood (FP64 pi . u), root posterior
@@ -313,77 +378,83 @@ __global__ void __launch_bounds__(256) sb_gwalk_kernel(const SbLaunch* __restric
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int s = state(v);
-      if (s < A) glb<float>(B->root)[(long long)s * C + cc] = pis[v] * u0[v] * inv_s0;
+      if (s < A) glb<float>(B->root)[(long long)s * C + cc] = spi[s] * u0[v] * inv_s0;
     }
   }
   if (!(flags & SB_F_COUNTS)) return;
 
-  // ---------------- outside pass + projections (preorder) ----------------
-  const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
-  const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) : nullptr;
-  for (int i = 0; i < nint; ++i) {
-    const SB_CONST int* rec = sched + i * 8;
-    const int p = rec[0], l = rec[1], r = rec[2], sl = rec[4], sr = rec[5];
-    const Kid kl = kid(l, sl), kr = kid(r, sr);
-    // x = obs(p) * (pi at the root, else M_p^T d_p)
-    float x[NV];
-    int ex;
-    bool stp;
-    token_vec(token(p), x);
-    if (i == 0) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v) x[v] *= pis[v];
-      ex = 0;
-      stp = rbr;
-    } else {
-      float dp[NV], y[NV];
-      load_vec(DD, p, dp);
-      const int dw = DWd[(long long)p * kTW + wcol];
-      ex = dw >> 1;
-      stp = (dw & 1) != 0;
-      lds_barrier();
-      stage<AW, false>(sP[0], bt + (long long)p * BTS + sb_bt_Mt(AP), AP, tid, 256);
-      lds_barrier();
-      gemm<AW>(sP[0], dp, y, lane);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) x[v] *= y[v];
+  // ---------------- outside pass + Dt (preorder) ----------------
+  {
+    Rec cur, nxt;
+    Mat mt;              // M_p^T of the next record
+    fetch(cur, 0, true);
+    if (nint > 1) {
+      const SB_CONST int* rn = sched + 8;
+      mload(mt, mat(rn[0], sb_bt_Mt(AP)));
     }
-    lds_barrier();
-    stage<AW, false>(sP[0], bt + (long long)l * BTS + sb_bt_M(AP), AP, tid, 256);
-    stage<AW, false>(sP[1], bt + (long long)r * BTS + sb_bt_M(AP), AP, tid, 256);
-    lds_barrier();
-    float ul[NV], ur[NV], wl[NV], wr[NV];
-    child_vecs(kl, sP[0], ul, wl);
-    child_vecs(kr, sP[1], ur, wr);
+    for (int i = 0; i < nint; ++i) {
+      float x[NV], wl[NV], wr[NV];
+      load_vec(UU, cur.l, wl);
+      load_vec(UU, cur.r, wr);
+      int ex;
+      bool stp;
+      if (i == 0) {
+        token_vec(cur.tp, x);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const Kid& kn = s == 0 ? kl : kr;
-      const Kid& ks = s == 0 ? kr : kl;
-      const int sn = s == 0 ? sl : sr;
-      float d[NV];
+        for (int v = 0; v < NV; ++v) x[v] *= spi[state(v)];
+        ex = 0;
+        stp = rbr;
+      } else {
+        // x = obs(p) * M_p^T d_p, with M_p^T staged at the end of the previous record
+        float dp[NV], o[NV];
+        load_vec(DD, cur.p, dp);
+        const int dw = DWd[(long long)cur.p * kTW + wcol];
+        ex = dw >> 1;
+        stp = (dw & 1) != 0;
+        gemm<AW>(sP[0], dp, x, lane);
+        token_vec(cur.tp, o);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) d[v] = x[v] * (s == 0 ? wr[v] : wl[v]);
-      const int e = rescale(d);
-      const int ed = ks.e + ex + e;
-      const bool stn = (kn.leaf ? kn.has : kn.branch) || (stp && kn.has);
-      if (sn >= 0) {
-        store_vec(DD, kn.n, d);
-        if (h == 0) DWd[(long long)kn.n * kTW + wcol] = (ed << 1) | (stn ? 1 : 0);
+        for (int v = 0; v < NV; ++v) x[v] *= o[v];
       }
-      bool act;
-      if (mask_mode == SB_MASK_NONE) act = true;
-      else if (mask_mode == SB_MASK_AUTO) act = stn && stp;
-      else act = emask[(long long)kn.n * C + cc] != 0;
-      const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + kn.e - e0) : 0.f;
-      // projections: Ut = Vs^T u, Dt = Vi^T d * scale
-      float t[NV];
-      if (kn.leaf) gather(sVs, kn.tok, vs1r, t);
-      else gemm<AW>(sVs, s == 0 ? ul : ur, t, lane);
-      store_vec(UU, kn.n, t);
-      gemm<AW>(sVi, d, t, lane);
+      const int kn = i + 1 < nint ? i + 1 : i;   // next record (clamped)
+      fetch(nxt, kn, true);
+      const Kid kl = kid(cur.l, cur.sl, cur.tl, cur.wl), kr = kid(cur.r, cur.sr, cur.tr, cur.wr);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) t[v] *= sc;
-      store_vec(DT, kn.n, t);
+      for (int s = 0; s < 2; ++s) {
+        const Kid& kn_ = s == 0 ? kl : kr;
+        const Kid& ks = s == 0 ? kr : kl;
+        const int sn = s == 0 ? cur.sl : cur.sr;
+        float d[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) d[v] = x[v] * (s == 0 ? wr[v] : wl[v]);
+        const int e = rescale(d);
+        const int ed = ks.e + ex + e;
+        const bool stn = (kn_.leaf ? kn_.has : kn_.branch) || (stp && kn_.has);
+        if (sn >= 0) {
+          store_vec(DD, kn_.n, d);
+          if (h == 0) DWd[(long long)kn_.n * kTW + wcol] = (ed << 1) | (stn ? 1 : 0);
+        }
+        bool act;
+        if (mask_mode == SB_MASK_NONE) act = true;
+        else if (mask_mode == SB_MASK_AUTO) act = stn && stp;
+        else act = (s == 0 ? cur.mkl : cur.mkr) != 0;
+        const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + kn_.e - e0) : 0.f;
+        float t[NV];
+        gemm<AW>(sVi, d, t, lane);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) t[v] *= sc;
+        store_vec(DT, kn_.n, t);
+      }
+      if (i + 1 < nint) {
+        lds_barrier();               // every wave is done with M_p^T of this record
+        mstore(sP[0], mt);
+        if (i + 2 < nint) {
+          const SB_CONST int* rn = sched + (i + 2) * 8;
+          mload(mt, mat(rn[0], sb_bt_Mt(AP)));
+        }
+        lds_barrier();
+      }
+      cur = nxt;
     }
   }
 }
@@ -415,8 +486,8 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
   const int wcol0 = qq * (TC * kWPG2) + wv * TC;
   SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
-  const SB_GLOBAL float* UU = (const SB_GLOBAL float*)st;
-  const SB_GLOBAL float* DT = UU + 2 * vplane;
+  const SB_GLOBAL float* UT = (const SB_GLOBAL float*)st + vplane;
+  const SB_GLOBAL float* DT = UT + 2 * vplane;
   const SB_GLOBAL float* bt = glb<float>(B->bt);
   const SB_GLOBAL float* md = glb<float>(B->model);
   const SB_CONST float* mdc = cst<float>(B->model);
@@ -467,7 +538,7 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
     for (int c = 0; c < TC; ++c) {
       const long long base = ((long long)n * kTW + wcol0 + c) * AW;
       dk[c] = on ? DT[base + k] : 0.f;
-      if (on) sb[wv][c][k] = UU[base + k];
+      if (on) sb[wv][c][k] = UT[base + k];
     }
     wave_lds_sync();
 #pragma unroll
@@ -537,7 +608,7 @@ extern "C" int sb_gen2_tile_cols() { return kTW; }
 
 extern "C" long long sb_gen2_tile_bytes(int AP, int rmax) {
   const long long AW = AP <= 32 ? 32 : 64;
-  const long long b = 3LL * rmax * kTW * AW * 4 + 2LL * rmax * kTW * 4;
+  const long long b = 4LL * rmax * kTW * AW * 4 + 2LL * rmax * kTW * 4;
   return (b + 255) / 256 * 256;
 }
 

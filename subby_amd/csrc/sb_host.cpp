@@ -38,7 +38,7 @@ extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaun
 
 // Stage memory of the two-pass general path: tiles are processed in rounds
 // that fit this many bytes.
-static const size_t kGen2StageBudget = size_t(4) << 30;
+static const size_t kGen2StageBudget = size_t(8) << 30;
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
@@ -524,9 +524,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   long long g2_tile_bytes = 0;
   if (gen2) {
     g2_tile_bytes = sb_gen2_tile_bytes(AP, R_max);
-    const size_t per_round = std::max<size_t>(1, kGen2StageBudget / size_t(g2_tile_bytes));
+    const size_t cap = std::max<size_t>(1, kGen2StageBudget / size_t(g2_tile_bytes));
+    const size_t nt = std::max<size_t>(ntiles, 1);
+    const size_t rounds = (nt + cap - 1) / cap;
     b->g2_ntiles = int(ntiles);
-    b->g2_round = int(std::min<size_t>(std::max<size_t>(ntiles, 1), per_round));
+    b->g2_round = int((nt + rounds - 1) / rounds);   // equal rounds
     rc = ensure(b, b->stage, size_t(b->g2_round) * size_t(g2_tile_bytes));
     if (rc) return rc;
   }
