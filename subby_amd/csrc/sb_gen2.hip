@@ -501,21 +501,56 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
     sV[kk * AW + i] = (i < AP && kk < AP) ? md[sb_mr_V(AP) + i * AP + kk] : 0.f;
   }
   // J_n staged as sJ[q * AW + kk] = J_n[q][kk] (J is symmetric), pads zero
+  // thread tid holds elements 4 (tid + 512 m) + z (z < 4, m < JPT / 4) when
+  // JPT is a multiple of 4, else elements tid + 512 m (m < JPT)
   float jr[JPT];
   auto load_J = [&](int n) {
     const SB_GLOBAL float* J = bt + (long long)n * BTS + sb_bt_J(AP);
+    if constexpr (JPT % 4 == 0) {
 #pragma unroll
-    for (int m = 0; m < JPT; ++m) {
-      const int idx = tid + m * 64 * kWPG2;
-      const int q = idx / AW, kk = idx % AW;
-      jr[m] = (q < AP && kk < AP) ? J[q * AP + kk] : 0.f;
+      for (int m = 0; m < JPT / 4; ++m) {
+        const int idx = 4 * (tid + m * 64 * kWPG2);
+        const int q = idx / AW, kk = idx % AW;
+        if (AP == AW) {
+          const f32x4 v = *(const SB_GLOBAL f32x4*)(J + q * AW + kk);
+#pragma unroll
+          for (int z = 0; z < 4; ++z) jr[4 * m + z] = v[z];
+        } else {
+#pragma unroll
+          for (int z = 0; z < 4; ++z) jr[4 * m + z] = (q < AP && kk + z < AP) ? J[q * AP + kk + z] : 0.f;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < JPT; ++m) {
+        const int idx = tid + m * 64 * kWPG2;
+        const int q = idx / AW, kk = idx % AW;
+        jr[m] = (q < AP && kk < AP) ? J[q * AP + kk] : 0.f;
+      }
     }
   };
   auto put_J = [&](float* dst) {
+    if constexpr (JPT % 4 == 0) {
 #pragma unroll
-    for (int m = 0; m < JPT; ++m) dst[tid + m * 64 * kWPG2] = jr[m];
+      for (int m = 0; m < JPT / 4; ++m)
+        *(f32x4*)(dst + 4 * (tid + m * 64 * kWPG2)) =
+            f32x4{jr[4 * m], jr[4 * m + 1], jr[4 * m + 2], jr[4 * m + 3]};
+    } else {
+#pragma unroll
+      for (int m = 0; m < JPT; ++m) dst[tid + m * 64 * kWPG2] = jr[m];
+    }
   };
 
+  // Dt[k] and Ut[k] of branch n, fetched one branch ahead
+  float dkn[TC], ukn[TC];
+  auto load_vecs = [&](int n) {
+#pragma unroll
+    for (int c = 0; c < TC; ++c) {
+      const long long base = ((long long)n * kTW + wcol0 + c) * AW + (on ? k : 0);
+      dkn[c] = DT[base];
+      ukn[c] = UT[base];
+    }
+  };
   float acc[TC][AW];
 #pragma unroll
   for (int c = 0; c < TC; ++c)
@@ -525,21 +560,22 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
     load_J(1);
     put_J(sJ1);
     if (R > 2) load_J(2);
+    load_vecs(1);
   }
   for (int n = 1; n < R; ++n) {
     lds_barrier();                       // J_n is in buffer n & 1; buffer (n+1) & 1 is free
-    if (n + 1 < R) {
-      put_J((n + 1) & 1 ? sJ1 : sJ0);
-      if (n + 2 < R) load_J(n + 2);
-    }
-    const float* J = (n & 1) ? sJ1 : sJ0;
     float dk[TC];
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
-      const long long base = ((long long)n * kTW + wcol0 + c) * AW;
-      dk[c] = on ? DT[base + k] : 0.f;
-      if (on) sb[wv][c][k] = UT[base + k];
+      dk[c] = on ? dkn[c] : 0.f;
+      if (on) sb[wv][c][k] = ukn[c];
     }
+    if (n + 1 < R) {
+      put_J((n + 1) & 1 ? sJ1 : sJ0);
+      if (n + 2 < R) load_J(n + 2);
+      load_vecs(n + 1);
+    }
+    const float* J = (n & 1) ? sJ1 : sJ0;
     wave_lds_sync();
 #pragma unroll
     for (int q = 0; q < AW; q += 4) {
