@@ -230,53 +230,74 @@ struct Acc {
 }  // namespace
 
 // Branch parameters in FP64 (subby/jax/diagonalize.py:141-172, eigensub.py:7-50,
-// f81_fast.py:56-66), one workgroup of 64 threads per (block, node) record.
+// f81_fast.py:56-66), one workgroup of 256 threads per (block, node) record.
 template <typename T>
-__global__ void __launch_bounds__(64) sb_branch_params_kernel(const SbLaunch* __restrict__ L) {
+__global__ void __launch_bounds__(256) sb_branch_params_kernel(const SbLaunch* __restrict__ L) {
+  // One workgroup per branch job: V^T, mu, E = exp(mu t) and sqrt(pi) in LDS;
+  // wave w computes rows i = w, w + 4, ... four at a time, lane j one column,
+  // so each coalesced read of V^T[q][j] feeds four FMAs.
   const int AP = L->AP;
   const int njobs = L->njobs;
   SB_GLOBAL T* out = glb<T>(L->bt);
+  __shared__ double vt[64 * 64];          // vt[q * AP + j] = V[j][q]
   __shared__ double mu[64], E[64], sq[64];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *L->counter = 0;  // tile counter of the launch that follows
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (blockIdx.x == 0 && tid == 0) *L->counter = 0;  // tile counter of the launch that follows
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
-  const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
-  const SB_CONST double* m = cst<double>(job->model64);
-  const double t = job->t;
-  const int k = threadIdx.x;
-  __syncthreads();
-  if (k < AP) {
-    mu[k] = m[sb_m64_mu(AP) + k];
-    E[k] = exp(mu[k] * t);
-    sq[k] = sqrt(m[sb_m64_pi(AP) + k]);
-  }
-  __syncthreads();
-  SB_GLOBAL T* o = out + (long long)jb * sb_bt_stride(AP);
-  const SB_CONST double* V = m + sb_m64_V(AP);
-  for (int idx = threadIdx.x; idx < AP * AP; idx += 64) {
-    const int i = idx / AP, j = idx - (idx / AP) * AP;
-    double s = 0.0;
-    for (int q = 0; q < AP; ++q) s = fma(V[i * AP + q] * E[q], V[j * AP + q], s);
-    o[sb_bt_M(AP) + idx] = T(s * sq[j] / sq[i]);
-    if (AP > 8) o[sb_bt_Mt(AP) + j * AP + i] = T(s * sq[j] / sq[i]);
-    const double d = mu[i] - mu[j];
-    double J;
-    if (fabs(d) < 1e-8) J = t * E[i];
-    else J = E[j] * expm1(d * t) / d;
-    o[sb_bt_J(AP) + idx] = T(J);
-  }
-  if (k < AP) {
-    o[sb_bt_E(AP) + k] = T(E[k]);
-    o[sb_bt_dE(AP) + k] = T(mu[k] * E[k]);
-  }
-  if (k == 0) {
-    const double fm = m[sb_m64_misc(AP)];
-    const double e = exp(-fm * t);
-    const double p = -expm1(-fm * t);
-    o[sb_bt_coef(AP) + 0] = T(t * e);
-    o[sb_bt_coef(AP) + 1] = T(p / fm - t * e);
-    o[sb_bt_coef(AP) + 2] = T(t * (1.0 + e) - 2.0 * p / fm);
-    o[sb_bt_coef(AP) + 3] = T(t);
-  }
+    const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
+    const SB_CONST double* m = cst<double>(job->model64);
+    const double t = job->t;
+    const SB_CONST double* V = m + sb_m64_V(AP);
+    __syncthreads();
+    if (tid < AP) {
+      mu[tid] = m[sb_m64_mu(AP) + tid];
+      E[tid] = exp(mu[tid] * t);
+      sq[tid] = sqrt(m[sb_m64_pi(AP) + tid]);
+    }
+    for (int idx = tid; idx < AP * AP; idx += 256) {
+      const int j = idx / AP, q = idx - j * AP;
+      vt[q * AP + j] = V[idx];
+    }
+    __syncthreads();
+    SB_GLOBAL T* o = out + (long long)jb * sb_bt_stride(AP);
+    const int j = lane;
+    if (j < AP) {
+      for (int i0 = wv; i0 < AP; i0 += 16) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < AP; ++q) {
+          const double vjq = vt[q * AP + j];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + 4 * r;
+            if (i < AP) s[r] = fma(vt[q * AP + i] * E[q], vjq, s[r]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + 4 * r;
+          if (i >= AP) continue;
+          const T mij = T(s[r] * sq[j] / sq[i]);
+          o[sb_bt_M(AP) + i * AP + j] = mij;
+          if (AP > 8) o[sb_bt_Mt(AP) + j * AP + i] = mij;
+          const double d = mu[i] - mu[j];
+          const double J = fabs(d) < 1e-8 ? t * E[i] : E[j] * expm1(d * t) / d;
+          o[sb_bt_J(AP) + i * AP + j] = T(J);
+        }
+      }
+    }
+    if (tid < AP) {
+      o[sb_bt_E(AP) + tid] = T(E[tid]);
+      o[sb_bt_dE(AP) + tid] = T(mu[tid] * E[tid]);
+    }
+    if (tid == 0) {
+      const double fm = m[sb_m64_misc(AP)];
+      const double e = exp(-fm * t);
+      const double p = -expm1(-fm * t);
+      o[sb_bt_coef(AP) + 0] = T(t * e);
+      o[sb_bt_coef(AP) + 1] = T(p / fm - t * e);
+      o[sb_bt_coef(AP) + 2] = T(t * (1.0 + e) - 2.0 * p / fm);
+      o[sb_bt_coef(AP) + 3] = T(t);
+    }
   }
 }
 
@@ -676,9 +697,9 @@ extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool gr
 
 extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s) {
   if (dtype == 0)
-    hipLaunchKernelGGL(sb_branch_params_kernel<float>, dim3(nwg), dim3(64), 0, s, L);
+    hipLaunchKernelGGL(sb_branch_params_kernel<float>, dim3(nwg), dim3(256), 0, s, L);
   else
-    hipLaunchKernelGGL(sb_branch_params_kernel<double>, dim3(nwg), dim3(64), 0, s, L);
+    hipLaunchKernelGGL(sb_branch_params_kernel<double>, dim3(nwg), dim3(256), 0, s, L);
   return hipGetLastError();
 }
 
