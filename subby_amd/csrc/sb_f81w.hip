@@ -36,7 +36,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kTC = 4;   // columns per wave tile
+constexpr int kTC = 4;   // columns per wave
+constexpr int kWG = 4;   // waves per workgroup: a group tile is kTC * kWG = 16 consecutive columns
+constexpr int kXS = 20;  // row stride (floats) of the count exchange buffer: 16 columns + pad
 
 template <typename T>
 __device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
@@ -72,13 +74,19 @@ __device__ inline double row_sum(double v) {
   return v;
 }
 // Orders this wave's LDS writes before its later LDS reads (and reads before
-// later writes).  The workgroup is a single wave and LDS executes a wave's
-// instructions in order, so no wait on global memory is needed, unlike
-// __syncthreads(), whose workgroup fence drains every load and store in flight.
+// later writes).  LDS executes a wave's instructions in order, so no wait on
+// global memory is needed, unlike __syncthreads(), whose fence drains every
+// load and store in flight.
 __device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+// Workgroup barrier ordering LDS only (global loads and stores stay in flight).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 __device__ inline float row_max(float v) {
   v = fmaxf(v, dppf<0xb1>(v));
@@ -98,23 +106,31 @@ struct Child {
 
 }  // namespace
 
+// A workgroup of four waves takes a group tile of 16 consecutive columns of
+// one block; wave w owns columns 4w .. 4w + 3 and walks the tree on its own
+// (wave-scope LDS exchanges only).  At the end the four waves trade counts
+// through LDS so that each count plane gets 64-byte stores of 16 columns.
 // Scratch per wave: U and D vectors of internal nodes ([slot][NS][64] floats)
 // followed by per-column words ([slot][4] int pairs: {word, pi.u or sum d}).
 template <int AW>
-__global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
+__global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
   constexpr int NS = AW / 16;   // states per lane
   constexpr int NT = AW / 32;   // MFMA tiles per dimension
   typedef Child<NS> C_;
   const SbLaunch& L = *Lp;
-  __shared__ float opnd[2][2][kTC][AW];   // [a|u][branch of the pair][column][state]
+  __shared__ float opnd_all[kWG][2][2][kTC][AW];   // per wave: [a|u][branch of the pair][column][state]
+  __shared__ __attribute__((aligned(16))) float xb[256 * kXS];   // count exchange: [plane][column]
   __shared__ float piT[AW + 1];
-  const int lane = threadIdx.x;
+  __shared__ int gtile;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  auto& opnd = opnd_all[wv];
   const int cl = lane >> 4;      // column of this lane
   const int rl = lane & 15;
   const int AP = L.AP;
   const int NI = L.nint_max;
   const long long vbytes = (long long)NI * NS * SB_WAVE * 4;
-  SB_GLOBAL char* wscr = glb<char>(L.scratch) + (long long)blockIdx.x * L.wave_stride;
+  SB_GLOBAL char* wscr = glb<char>(L.scratch) + ((long long)blockIdx.x * kWG + wv) * L.wave_stride;
   SB_GLOBAL float* Ureg = (SB_GLOBAL float*)wscr;
   SB_GLOBAL float* Dreg = (SB_GLOBAL float*)(wscr + vbytes);
   SB_GLOBAL i32x2* Uw = (SB_GLOBAL i32x2*)(wscr + 2 * vbytes);
@@ -122,9 +138,10 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
   const int BTS = sb_bt_stride(AP);
 
   for (;;) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(L.counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if (threadIdx.x == 0) gtile = atomicAdd(L.counter, 1);
+    lds_barrier();
+    const int t = __builtin_amdgcn_readfirstlane(gtile);
+    lds_barrier();                 // everyone has read gtile before it is rewritten
     if (t >= L.ntiles) break;
     const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * t;
     const int bidx = tbp[0], tidx = tbp[1];
@@ -133,7 +150,8 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
     const int A = B->A;
     const int flags = B->flags;
     const int nint = B->nint;
-    const long long col0 = (long long)tidx * kTC;
+    const long long gcol0 = (long long)tidx * (kTC * kWG);   // first column of the group tile
+    const long long col0 = gcol0 + kTC * wv;
     const long long mycol = col0 + cl;
     const bool valid = mycol < C;
     const long long cc = valid ? mycol : C - 1;
@@ -149,10 +167,11 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
       pis[j] = s < A ? mdc[sb_mr_pi(AP) + s] : 0.f;
       ones[j] = s < A ? 1.f : 0.f;
     }
-    wave_sync();
-    if (lane < AW) piT[lane] = lane < A ? mdc[sb_mr_pi(AP) + (lane < A ? lane : 0)] : 0.f;
-    if (lane == 0) piT[AW] = 1.f;   // pi . (all ones)
-    wave_sync();
+    if (wv == 0) {                 // (the previous tile's readers passed the barrier above)
+      if (lane < AW) piT[lane] = lane < A ? mdc[sb_mr_pi(AP) + (lane < A ? lane : 0)] : 0.f;
+      if (lane == 0) piT[AW] = 1.f;   // pi . (all ones)
+    }
+    lds_barrier();
 
     auto token = [&](int n) -> int { return aln[(long long)n * C + cc]; };
     auto emul = [&](int n) -> float { return bt[(long long)n * BTS + sb_bt_E(AP) + kstar]; };
@@ -319,7 +338,7 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
         if (s < A) glb<float>(B->root)[(long long)s * C + cc] = pis[j] * u0[j] * inv_s0;
       }
     }
-    if (!(flags & SB_F_COUNTS)) continue;
+    if (!(flags & SB_F_COUNTS)) continue;   // (uniform over the workgroup: one block)
 
     // ---------------- outside pass + accumulation (preorder) ----------------
     f32x16 acc[kTC][NT][NT];
@@ -507,10 +526,13 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
           acc[c][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[c][a][b], 0, 0, 0);
     }
     // counts_ii = I_ii, counts_ij = mu pi_j I_ij; accumulator element q of tile
-    // (a, b) in this lane is I[32a + 8(q/4) + 4(lane/32) + q%4][32b + lane%32]
+    // (a, b) in this lane is I[32a + 8(q/4) + 4(lane/32) + q%4][32b + lane%32].
+    // Chunks of 256 planes (tile (a, b), q / 4 fixed) go through LDS: wave w
+    // writes its 4 columns of each plane, then the workgroup stores 16 columns
+    // per plane.
     const float fmu = mdc[sb_mr_misc(AP)];
     SB_GLOBAL float* out = glb<float>(B->counts);
-    const bool vec4 = (C % 4 == 0) && (col0 + kTC <= C);
+    const bool full = (C % 4 == 0) && (gcol0 + kTC * kWG <= C);
     if (out) {
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
@@ -519,26 +541,41 @@ __global__ void __launch_bounds__(64) sb_f81w_kernel(const SbLaunch* __restrict_
 #pragma unroll
         for (int a = 0; a < NT; ++a) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) {
-            const int i = 32 * a + 8 * (q >> 2) + 4 * kb + (q & 3);
-            const float f = i == j ? 1.f : fj;
-            if (i < A && j < A) {
-              SB_GLOBAL float* dst = out + ((long long)i * A + j) * C + col0;
-              if (vec4) {
-                __builtin_nontemporal_store(
-                    f32x4{f * acc[0][a][b][q], f * acc[1][a][b][q], f * acc[2][a][b][q], f * acc[3][a][b][q]},
-                    (SB_GLOBAL f32x4*)dst);
-              } else {
+          for (int qg = 0; qg < 4; ++qg) {
+            lds_barrier();           // previous chunk's readers are done
 #pragma unroll
-                for (int c = 0; c < kTC; ++c)
-                  if (col0 + c < C) dst[c] = f * acc[c][a][b][q];
+            for (int r = 0; r < 4; ++r) {
+              const int q = 4 * qg + r;
+              const int i = 32 * a + 8 * qg + 4 * kb + r;
+              const float f = i == j ? 1.f : fj;
+              *(f32x4*)&xb[(r * 64 + lane) * kXS + kTC * wv] =
+                  f32x4{f * acc[0][a][b][q], f * acc[1][a][b][q], f * acc[2][a][b][q], f * acc[3][a][b][q]};
+            }
+            lds_barrier();
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              const int e = threadIdx.x + 64 * kWG * m;   // float4 index: plane e / 4, part e % 4
+              const int pl = e >> 2, part = e & 3;
+              const int r = pl >> 6, Lp_ = pl & 63;
+              const int i = 32 * a + 8 * qg + 4 * (Lp_ >> 5) + r;
+              const int jj = 32 * b + (Lp_ & 31);
+              if (i < A && jj < A) {
+                const f32x4 v = *(const f32x4*)&xb[pl * kXS + 4 * part];
+                const long long c0 = gcol0 + 4 * part;
+                SB_GLOBAL float* dst = out + ((long long)i * A + jj) * C + c0;
+                if (full) {
+                  __builtin_nontemporal_store(v, (SB_GLOBAL f32x4*)dst);
+                } else {
+#pragma unroll
+                  for (int z = 0; z < 4; ++z)
+                    if (c0 + z < C) dst[z] = v[z];
+                }
               }
             }
           }
         }
       }
     }
-    wave_sync();
   }
 }
 
@@ -551,7 +588,8 @@ static void (*pick_f81w(int AP))(const SbLaunch*) {
   return nullptr;
 }
 
-extern "C" int sb_f81w_tile_cols() { return kTC; }
+extern "C" int sb_f81w_tile_cols() { return kTC * kWG; }
+extern "C" int sb_f81w_waves_per_group() { return kWG; }
 
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max) {
   const int AW = AP <= 32 ? 32 : 64;
@@ -562,13 +600,14 @@ extern "C" long long sb_f81w_wave_stride(int AP, int nint_max) {
 extern "C" int sb_f81w_occupancy(int AP) {
   void (*k)(const SbLaunch*) = pick_f81w(AP);
   int n = 0;
-  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
-  return n;
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64 * kWG, 0) != hipSuccess) return 0;
+  return n * kWG;   // resident waves per CU
 }
 
+// nwaves: a multiple of kWG (one workgroup per kWG waves)
 extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s) {
   void (*k)(const SbLaunch*) = pick_f81w(AP);
   if (!k) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
+  hipLaunchKernelGGL(k, dim3((nwaves + kWG - 1) / kWG), dim3(64 * kWG), 0, s, L);
   return hipGetLastError();
 }

@@ -29,6 +29,7 @@ extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" int sb_f81w_tile_cols();
+extern "C" int sb_f81w_waves_per_group();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
 extern "C" int sb_f81w_occupancy(int AP);
 extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s);
@@ -507,11 +508,16 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   {
     // small batches launch fewer waves (a power of two >= the tile count, so
     // few distinct grids and hence few captured graphs)
+    const long long wpt = f81w ? sb_f81w_waves_per_group() : 1;   // waves per tile
     long long want = 64;
-    while (want < (long long)ntiles) want *= 2;
+    while (want < (long long)ntiles * wpt) want *= 2;
     if (want < nwaves) nwaves = int(want);
   }
   nwaves = (nwaves + wpg - 1) / wpg * wpg;
+  if (f81w) {
+    const int g = sb_f81w_waves_per_group();
+    nwaves = (nwaves + g - 1) / g * g;
+  }
   long long wave_stride;
   if (f81w) wave_stride = sb_f81w_wave_stride(AP, nint_max);
   else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
