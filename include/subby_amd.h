@@ -143,6 +143,49 @@ int sb_batch_sync(sb_batch* b);
 int sb_batch_set_timing(sb_batch* b, int enable);
 int sb_batch_kernel_ms(sb_batch* b, double* total_ms, int* launches);
 
+/* Continuous-batching server.  Replaces the batched driver's host loop
+ * (scripts/batch_eigensub_maf.py:423-698: bucket blocks by size, pad, vmap
+ * Counts per bucket, copy features back) with a native admission loop: a
+ * worker thread admits queued requests first come first served into free
+ * lanes (launcher + HIP stream + pinned and device buffers each) while the
+ * bytes in flight stay within the HBM budget, uploads each batch with one
+ * copy, runs it as one persistent launch, and copies results straight into
+ * the callers' host buffers.  Results are bitwise those of the same request
+ * run alone.  All host pointers of a request must stay valid until it
+ * completes.  Outputs are FP32 counts / root and FP64 logLike. */
+typedef struct sb_server sb_server;
+typedef struct sb_request {
+  int model_id;                 /* resident model, or -1: the inline model below */
+  int A;                        /* inline model: alphabet size */
+  const double* eigenvalues;    /* inline model [A] */
+  const double* eigenvectors;   /* inline model [A*A], V[a*A + k] */
+  const double* pi;             /* inline model [A] */
+  int R, C;
+  const int8_t* alignment;      /* host [R][C] */
+  const int32_t* parent_index;  /* host [R] */
+  const double* distances;      /* host [R] */
+  int mask_mode;                /* SB_MASK_NONE or SB_MASK_AUTO */
+  int want;                     /* SB_WANT_* bits */
+  float* counts;                /* host [A][A][C] or NULL */
+  float* root;                  /* host [A][C] or NULL */
+  double* loglike;              /* host [C] or NULL */
+  long long tag;                /* returned on completion */
+} sb_request;
+int sb_server_create(sb_context* ctx, int lanes, long long hbm_budget, int max_blocks,
+                     long long max_cols, sb_server** out);
+int sb_server_destroy(sb_server* s);   /* waits for everything submitted */
+int sb_server_submit(sb_server* s, int n, const sb_request* requests);
+/* Up to `max` completed tags with completion times (seconds since the
+ * server was created); *n_done receives how many. */
+int sb_server_poll(sb_server* s, int max, long long* tags, double* done_s, int* n_done);
+int sb_server_drain(sb_server* s);     /* wait until every submitted request completed */
+/* Load generator: submit request i at arrival_s[i] seconds after the call
+ * (non-decreasing), wait for all, and write each request's completion time
+ * (seconds after the call) to done_s[i].  Request tags are overwritten with
+ * their index.  (No reference counterpart: measurement of the serving loop.) */
+int sb_server_replay(sb_server* s, int n, sb_request* requests, const double* arrival_s,
+                     double* done_s);
+
 /* Plain path: the same statistics through one launch per tree level per pass
  * with inside/outside partials materialised in HBM (the data flow of the
  * reference's WebGPU engine, subby/webgpu/phylo_gpu.js:611-724).  Kept as the

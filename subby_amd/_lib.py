@@ -20,6 +20,8 @@ EXPORTS = [
     'sb_model_drop', 'sb_loglike', 'sb_root_prob', 'sb_counts', 'sb_loglike_grad',
     'sb_batch_create', 'sb_batch_destroy', 'sb_batch_submit', 'sb_batch_set_tiling',
     'sb_batch_sync', 'sb_batch_set_timing', 'sb_batch_kernel_ms', 'sb_counts_plain',
+    'sb_server_create', 'sb_server_destroy', 'sb_server_submit', 'sb_server_poll',
+    'sb_server_drain', 'sb_server_replay',
 ]
 
 
@@ -39,6 +41,28 @@ class SbBlock(ctypes.Structure):
         ('d_counts', ctypes.c_void_p),
         ('d_root', ctypes.c_void_p),
         ('d_loglike', ctypes.c_void_p),
+    ]
+
+
+class SbRequest(ctypes.Structure):
+    """sb_request of include/subby_amd.h."""
+    _fields_ = [
+        ('model_id', ctypes.c_int),
+        ('A', ctypes.c_int),
+        ('eigenvalues', ctypes.c_void_p),
+        ('eigenvectors', ctypes.c_void_p),
+        ('pi', ctypes.c_void_p),
+        ('R', ctypes.c_int),
+        ('C', ctypes.c_int),
+        ('alignment', ctypes.c_void_p),
+        ('parent_index', ctypes.c_void_p),
+        ('distances', ctypes.c_void_p),
+        ('mask_mode', ctypes.c_int),
+        ('want', ctypes.c_int),
+        ('counts', ctypes.c_void_p),
+        ('root', ctypes.c_void_p),
+        ('loglike', ctypes.c_void_p),
+        ('tag', ctypes.c_longlong),
     ]
 
 
@@ -82,6 +106,13 @@ def load(path=LIB_PATH):
     lib.sb_batch_set_timing.argtypes = [vp, i]
     lib.sb_batch_kernel_ms.argtypes = [vp, P(d), P(i)]
     lib.sb_counts_plain.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp, vp]
+    ll = ctypes.c_longlong
+    lib.sb_server_create.argtypes = [vp, i, ll, i, ll, P(vp)]
+    lib.sb_server_destroy.argtypes = [vp]
+    lib.sb_server_submit.argtypes = [vp, i, vp]
+    lib.sb_server_poll.argtypes = [vp, i, vp, vp, P(i)]
+    lib.sb_server_drain.argtypes = [vp]
+    lib.sb_server_replay.argtypes = [vp, i, vp, vp, vp]
     for name in EXPORTS:
         if name not in ('sb_last_error',):
             getattr(lib, name).restype = i

@@ -106,6 +106,7 @@ struct sb_context {
   int device = 0;
   int num_cus = 256;
   std::vector<Model> models;
+  std::vector<int> free_ids;        // ids of dropped models, reused by sb_model_put
   std::mutex mu;
   // Model arena: records of every model live in large device chunks that never
   // move.  sb_model_put stages the records in pinned memory; the next
@@ -727,6 +728,14 @@ BlockIn make_block(int model_id, const int8_t* aln, const int32_t* parent, const
 extern "C" {
 
 const char* sb_last_error(void) { return g_err.c_str(); }
+
+// Internal hooks for the serving loop (sb_serve.cpp): set the calling
+// thread's error message; the device a context was created on.
+int sb_internal_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+int sb_internal_device(sb_context* ctx) { return ctx ? ctx->device : -1; }
 int sb_version(void) { return 1; }
 
 int sb_create(sb_context** out, int device) {
@@ -775,8 +784,8 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   for (int i = 0; i < A * A; ++i)
     if (!std::isfinite(eigenvectors[i])) return fail(SB_ERR_MODEL, "eigenvector entry %d not finite", i);
   if (!(psum < 1.0 + 1e-6 && psum > 1.0 - 1e-6)) return fail(SB_ERR_MODEL, "pi sums to %g, not 1", psum);
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  SB_HIP(hipSetDevice(ctx->device));
+  // (host math below runs outside the context lock; only the arena and the
+  // model table are shared)
 
   // FP64 master with inert pads: mu=0, V=identity block, pi=1 (pads only enter
   // M through sqrt(pi_j/pi_i) on the identity block).
@@ -857,26 +866,29 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.pi.assign(pi, pi + A);
   const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
                bd = align_up(rec.size() * 8, 256);
-  char* base = nullptr;
-  int rc = arena_alloc(ctx, b64 + bf + bd, &base);
-  if (rc) return rc;
   std::vector<char> blob(b64 + bf + bd, 0);
   std::memcpy(blob.data(), m64.data(), m64.size() * 8);
   std::memcpy(blob.data() + b64, recf.data(), recf.size() * 4);
   std::memcpy(blob.data() + b64 + bf, rec.data(), rec.size() * 8);
+
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  char* base = nullptr;
+  int rc = arena_alloc(ctx, b64 + bf + bd, &base);
+  if (rc) return rc;
   rc = stage_model_bytes(ctx, base, blob.data(), blob.size());
   if (rc) return rc;
   m.d64 = reinterpret_cast<double*>(base);
   m.rec[0] = base + b64;
   m.rec[1] = base + b64 + bf;
-  int id = -1;
-  for (size_t i = 0; i < ctx->models.size(); ++i)
-    if (!ctx->models[i].alive) { id = int(i); break; }
-  if (id < 0) {
+  int id;
+  if (!ctx->free_ids.empty()) {
+    id = ctx->free_ids.back();
+    ctx->free_ids.pop_back();
+    ctx->models[id] = m;
+  } else {
     id = int(ctx->models.size());
     ctx->models.push_back(m);
-  } else {
-    ctx->models[id] = m;
   }
   *model_id = id;
   return SB_OK;
@@ -891,6 +903,7 @@ int sb_model_drop(sb_context* ctx, int model_id) {
   // by queued launches)
   m->d64 = nullptr;
   m->alive = false;
+  ctx->free_ids.push_back(model_id);
   return SB_OK;
 }
 

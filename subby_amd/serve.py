@@ -237,6 +237,93 @@ class Server:
         return results, stats
 
 
+class NativeServer:
+    """The same continuous batching in the library's native loop (sb_server_*).
+
+    A C++ worker thread owns the lanes (launcher + HIP stream + pinned and
+    device buffers), admits queued requests first come first served within
+    the HBM budget, packs and unpacks batches with a small thread pool and
+    copies results straight into each request's host arrays; `run` replays a
+    request stream at its arrival times from native code, so no Python runs
+    per request while the clock is running.  Each request carries its model
+    inline (uploaded into the resident arena when admitted).  FP32 outputs.
+    """
+
+    def __init__(self, engine: Engine | None = None, hbm_budget: int = 8 << 30, lanes: int = 3,
+                 max_blocks: int = 2048, max_cols: int = 1 << 19):
+        self.eng = engine or default_engine()
+        self.lib = self.eng.lib
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.sb_server_create(self.eng.ctx, lanes, int(hbm_budget), max_blocks,
+                                             int(max_cols), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            _lib.check(self.lib.sb_server_destroy(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, requests):
+        """Host-side request records and output arrays (not timed)."""
+        n = len(requests)
+        arr = (_lib.SbRequest * n)()
+        keep, outs = [], []
+        for k, r in enumerate(requests):
+            m = ensure_diag(r.model)
+            ev = np.ascontiguousarray(m.eigenvalues, dtype=np.float64)
+            V = np.ascontiguousarray(m.eigenvectors, dtype=np.float64)
+            pi = np.ascontiguousarray(m.pi, dtype=np.float64)
+            aln = np.ascontiguousarray(r.alignment, dtype=np.int8)
+            parent = np.ascontiguousarray(r.tree.parentIndex, dtype=np.int32)
+            dist = np.ascontiguousarray(r.tree.distanceToParent, dtype=np.float64)
+            R, C = aln.shape
+            A = len(pi)
+            # output arrays are touched here, so no page is first faulted in
+            # while the server copies results into them
+            cnt = np.full((A, A, C), np.nan, dtype=np.float32)
+            root = np.full((A, C), np.nan, dtype=np.float32)
+            ll = np.full(C, np.nan, dtype=np.float64)
+            q = arr[k]
+            q.model_id = -1
+            q.A = A
+            q.eigenvalues, q.eigenvectors, q.pi = ev.ctypes.data, V.ctypes.data, pi.ctypes.data
+            q.R, q.C = R, C
+            q.alignment, q.parent_index, q.distances = aln.ctypes.data, parent.ctypes.data, dist.ctypes.data
+            q.mask_mode = _lib.SB_MASK_AUTO if r.branch_mask == 'auto' else _lib.SB_MASK_NONE
+            q.want = _lib.SB_WANT_LOGLIKE | _lib.SB_WANT_ROOT | _lib.SB_WANT_COUNTS
+            q.counts, q.root, q.loglike = cnt.ctypes.data, root.ctypes.data, ll.ctypes.data
+            keep += [ev, V, pi, aln, parent, dist]
+            outs.append((cnt, ll, root))
+        return arr, keep, outs
+
+    def run(self, requests, realtime: bool = True):
+        """Serve ``requests`` (sorted by arrival).  Returns (results, stats) like Server.run."""
+        n = len(requests)
+        arr, keep, outs = self.prepare(requests)
+        arrival = np.array([r.arrival if realtime else 0.0 for r in requests], dtype=np.float64)
+        done = np.zeros(n, dtype=np.float64)
+        _lib.check(self.lib.sb_server_replay(self.h, n, ctypes.cast(arr, ctypes.c_void_p),
+                                             arrival.ctypes.data, done.ctypes.data))
+        del keep
+        results = [Result(r.rid, cnt, ll, root, float(arrival[k]), float(done[k]))
+                   for k, (r, (cnt, ll, root)) in enumerate(zip(requests, outs))]
+        lat = done - arrival if n else np.zeros(1)
+        makespan = float(done.max()) if n else 0.0
+        units = sum(int(r.alignment.shape[1]) * (len(r.tree.parentIndex) - 1) for r in requests)
+        stats = {'requests': n, 'makespan_s': makespan,
+                 'throughput': units / makespan if makespan > 0 else 0.0,
+                 'p50_ms': float(np.percentile(lat, 50) * 1e3), 'p99_ms': float(np.percentile(lat, 99) * 1e3),
+                 'mean_ms': float(lat.mean() * 1e3)}
+        results.sort(key=lambda r: r.rid)
+        return results, stats
+
+
 def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000),
                 pool=None):
     """Synthetic request stream: Poisson arrivals at ``qps``; each request has a

@@ -63,3 +63,57 @@ def test_tight_budget_and_burst(eng):
     say('   stats', stats)
     assert stats['batches'] >= 120 // 16
     _check_alone(sb, eng, reqs, results, oracle_every=40)
+
+
+def test_native_served_equals_alone(eng):
+    """The native loop (sb_server_*), Poisson arrivals replayed from native code."""
+    import subby_amd as sb
+    from subby_amd.serve import NativeServer, make_stream
+    say('native serve: 400 requests, Poisson arrivals')
+    reqs = make_stream(400, qps=4000.0, seed=13, leaves=(4, 60), mean_cols=300)
+    srv = NativeServer(eng, hbm_budget=1 << 30, lanes=3)
+    try:
+        results, stats = srv.run(reqs)
+    finally:
+        srv.close()
+    say('   stats', stats)
+    assert stats['requests'] == 400
+    assert all(res.done >= res.arrival for res in results)
+    _check_alone(sb, eng, reqs, results)
+
+
+def test_native_tight_budget_burst_and_poll(eng):
+    """Burst under a budget that admits few requests at a time; then the
+    submit / drain / poll interface with caller tags."""
+    import ctypes
+    import subby_amd as sb
+    from subby_amd import _lib
+    from subby_amd.serve import NativeServer, make_stream
+    say('native serve: tight budget burst, submit/poll')
+    reqs = make_stream(120, qps=1e9, seed=14, leaves=(4, 40), mean_cols=200)
+    srv = NativeServer(eng, hbm_budget=200_000, lanes=2, max_blocks=16)
+    try:
+        results, stats = srv.run(reqs, realtime=False)
+        say('   stats', stats)
+        _check_alone(sb, eng, reqs, results, oracle_every=40)
+        sub = reqs[:10]
+        arr, keep, outs = srv.prepare(sub)
+        for k in range(len(sub)):
+            arr[k].tag = 1000 + k
+        _lib.check(srv.lib.sb_server_submit(srv.h, len(sub), ctypes.cast(arr, ctypes.c_void_p)))
+        _lib.check(srv.lib.sb_server_drain(srv.h))
+        tags = np.zeros(64, dtype=np.int64)
+        times = np.zeros(64, dtype=np.float64)
+        nd = ctypes.c_int()
+        _lib.check(srv.lib.sb_server_poll(srv.h, 64, tags.ctypes.data, times.ctypes.data, ctypes.byref(nd)))
+        assert sorted(tags[:nd.value].tolist()) == list(range(1000, 1010))
+        for (cnt, ll, root), res in zip(outs, results[:10]):
+            assert np.array_equal(cnt, res.counts) and np.array_equal(ll, res.loglike)
+        # a malformed request is refused at submission
+        bad = (_lib.SbRequest * 1)()
+        bad[0].model_id = -1
+        bad[0].A = 70
+        with pytest.raises(sb.SubbyError):
+            _lib.check(srv.lib.sb_server_submit(srv.h, 1, ctypes.cast(bad, ctypes.c_void_p)))
+    finally:
+        srv.close()
