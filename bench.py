@@ -400,28 +400,50 @@ def main():
             "unit": "column*branches/s"}
 
     if args.serve_requests > 0:
-        from subby_amd.serve import Server, make_stream as make_requests
+        from subby_amd.serve import NativeServer, Server, make_stream as make_requests
         log(f'bench: serving {args.serve_requests} requests (continuous batching)')
         reqs = make_requests(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool)
-        srv = Server(eng, hbm_budget=16 << 30, lanes=3)
-        srv.run(reqs[:200], realtime=False)                     # warm-up (graphs, buffers)
-        _, sat = srv.run(reqs, realtime=False)
+        units = sum(int(r.alignment.shape[1]) * (len(r.tree.parentIndex) - 1) for r in reqs)
+        # GPU-only reference: the same blocks as one persistent batch, inputs resident
+        from subby_amd.batch import Block
+        blocks = [Block(torch.from_numpy(r.alignment).to(dev), r.tree, r.model) for r in reqs]
+        Lg = Launcher(eng)
+        prep = Lg.prepare(blocks)
+        Lg.launch(prep, use_graph=False)
+        torch.cuda.synchronize()
+        s_ = time.perf_counter()
+        for _ in range(3):
+            Lg.launch(prep, use_graph=False)
+        torch.cuda.synchronize()
+        gpu_only = units / ((time.perf_counter() - s_) / 3)
+        del blocks, prep, Lg
+        nsrv = NativeServer(eng, hbm_budget=16 << 30, lanes=3)
+        nsrv.run(reqs[:500], realtime=False)                    # warm-up (graphs, buffers)
+        _, sat = nsrv.run(reqs, realtime=False)
         rate = sat['requests'] / sat['makespan_s']
         qps = 0.5 * rate
         arr = np.cumsum(np.random.default_rng(5).exponential(1.0 / qps, size=len(reqs)))
         for r, a in zip(reqs, arr):
             r.arrival = float(a)
-        _, lat = srv.run(reqs, realtime=True)
+        _, lat = nsrv.run(reqs, realtime=True)
+        nsrv.close()
+        psrv = Server(eng, hbm_budget=16 << 30, lanes=3)       # the Python loop, for comparison
+        psrv.run(reqs[:200], realtime=False)
+        _, psat = psrv.run(reqs, realtime=False)
         out["served"] = {
             "workload": "stream of HKY85 blocks: random trees with 4-100 leaves, block length LogNormal "
                         "(mean 500, sd 1.0) clipped to [20, 10000], kappa~U(1,5), pi~Dir(5,5,5,5) per request; "
                         f"{args.serve_pool} simulated (tree, alignment) pairs drawn with replacement; "
-                        "FP32 Counts+logLike+root, results copied back to host",
+                        "FP32 Counts+logLike+root, alignments from host memory, results copied back to host",
+            "server": "native loop (sb_server_*), 3 lanes, arrivals replayed from native code",
             "requests": len(reqs),
             "saturated": {"throughput": sat['throughput'], "requests_per_s": rate,
-                          "p50_ms": sat['p50_ms'], "p99_ms": sat['p99_ms'], "batches": sat['batches']},
+                          "p50_ms": sat['p50_ms'], "p99_ms": sat['p99_ms']},
             "poisson": {"qps": qps, "throughput": lat['throughput'], "p50_ms": lat['p50_ms'],
-                        "p99_ms": lat['p99_ms'], "batches": lat['batches']},
+                        "p99_ms": lat['p99_ms']},
+            "python_loop_saturated": {"throughput": psat['throughput'],
+                                      "requests_per_s": psat['requests'] / psat['makespan_s']},
+            "gpu_only_batch": gpu_only,
             "unit": "column*branches/s"}
 
     if not args.no_cpu:
