@@ -24,7 +24,9 @@ tests/test_oracle_invariants.py):
     small trees (counts via numerical quadrature of the I^{ab}_{ij}
     integral), Steiner mask vs brute-force graph definition, F81 fast path
     == general eigensub path, dwell sums == branch-length sums, and the
-    gradient vs central finite differences.
+    gradient vs central finite differences; per-branch counts and node /
+    branch posteriors against the same enumeration (and Counts = sum of
+    BranchCounts).
 No function here is "parity unpinned".
 """
 from __future__ import annotations
@@ -420,6 +422,120 @@ def counts_loglike_root(alignment, tree, model, f81_fast=False, branch_mask="aut
                                           branch_mask=mask), model)
     root = (model.pi[None, :] * U[0] * np.exp(lnU[0] - ll)[:, None]).T
     return cnt, ll, root
+
+
+# ---------------------------------------------------------------------------
+# Per-branch counts and posteriors  (oracle.py:1517-1568, 1618-1902;
+# subby/jax/__init__.py:450-540, 681-1040; eigensub.py:175-231;
+# f81_fast.py:144-...)
+# ---------------------------------------------------------------------------
+
+def _branch_scale_masked(lnU, lnD, ll, branch_mask):
+    sc = branch_scale(lnU, lnD, ll)
+    sc[0] = 0.0
+    if branch_mask is not None:
+        sc = np.where(np.asarray(branch_mask, dtype=bool), sc, 0.0)
+    return sc
+
+
+def accumulate_C_per_branch(D_tilde, U_tilde, J, logNormU, logNormD, logLike, branch_mask=None):
+    """C[n,k,l,c] = D~_k J_kl U~_l scale, no sum over n; branch 0 zeros (eigensub.py:175-203)."""
+    sc = _branch_scale_masked(logNormU, logNormD, logLike, branch_mask)
+    return np.einsum('nck,nkl,ncl->nklc', D_tilde * sc[..., None], J, U_tilde)
+
+
+def back_transform_per_branch(Cn, model):
+    """back_transform keeping the branch axis (eigensub.py:206-231)."""
+    V = model.eigenvectors
+    VCV = np.einsum('ik,nklc,jl->nijc', V, Cn, V)
+    S = rate_matrix_symmetric(model)
+    A = len(model.pi)
+    eye = np.eye(A, dtype=bool)[None, ..., None]
+    return np.where(eye, VCV, S[None, ..., None] * VCV)
+
+
+def f81_counts_per_branch(U, D, logNormU, logNormD, logLike, distances, pi, branch_mask=None):
+    """f81_counts keeping the branch axis; branch 0 zeros (f81_fast.py:144-...)."""
+    R, C, A = U.shape
+    pi = np.asarray(pi)
+    mu = 1.0 / (1.0 - np.sum(pi ** 2))
+    t = np.asarray(distances, dtype=np.float64)
+    e = np.exp(-mu * t)
+    p = 1.0 - e
+    alpha = t * e
+    beta = p / mu - t * e
+    gamma = t * (1.0 + e) - 2.0 * p / mu
+    sc = _branch_scale_masked(logNormU, logNormD, logLike, branch_mask)
+    Ds = D * sc[..., None]
+    piU = U @ pi                        # (R, C)
+    Dsum = Ds.sum(-1)                   # (R, C)
+    I = (np.einsum('n,nci,ncj->nijc', alpha, Ds, U)
+         + np.einsum('n,nci,nc->nic', beta, Ds, piU)[:, :, None, :]
+         + pi[None, :, None, None] * np.einsum('n,nc,ncj->njc', beta, Dsum, U)[:, None, :, :]
+         + pi[None, :, None, None] * np.einsum('n,nc,nc->nc', gamma, Dsum, piU)[:, None, None, :])
+    out = mu * pi[None, None, :, None] * I
+    idx = np.arange(A)
+    out[:, idx, idx, :] = I[:, idx, idx, :]
+    out[0] = 0.0
+    return out
+
+
+def BranchCounts(alignment, tree, model, f81_fast=False, branch_mask=None):
+    """(R,A,A,C) per-branch dwell (diagonal) and substitution counts, branch 0 zeros.
+
+    The JAX entry point applies no mask (subby/jax/__init__.py:450-540); the
+    reference oracle's takes branch_mask="auto" by default (oracle.py:1517-1568).
+    Here the default is None (the JAX behaviour); "auto" and arrays are honoured.
+    """
+    mask = _resolve_mask(alignment, tree, model, branch_mask)
+    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
+    if f81_fast:
+        return f81_counts_per_branch(U, D, lnU, lnD, ll, tree.distanceToParent, model.pi,
+                                     branch_mask=mask)
+    J = compute_J(model.eigenvalues, tree.distanceToParent)
+    Ut, Dt = eigenbasis_project(U, D, model)
+    return back_transform_per_branch(
+        accumulate_C_per_branch(Dt, Ut, J, lnU, lnD, ll, branch_mask=mask), model)
+
+
+def _with_root_outside(D, lnD, pi):
+    """D[0] = pi / max(pi), lnD[0] = log max(pi) (oracle.py:1648-1660)."""
+    D = D.copy()
+    lnD = lnD.copy()
+    mx = max(float(np.max(pi)), 1e-300)
+    D[0] = np.asarray(pi)[None, :] / mx
+    lnD[0] = np.log(mx)
+    return D, lnD
+
+
+def node_posterior(alignment, tree, model):
+    """(R,A,C) posterior state distribution of every node (oracle.py:1755-1832).
+
+    Root: pi_a U_0a normalised.  Non-root n: (sum_a D[n]_a M_n[a,j]) U_nj normalised,
+    D[n] being the outside vector of branch n (indexed by the parent's state).
+    """
+    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
+    D, lnD = _with_root_outside(D, lnD, model.pi)
+    with np.errstate(invalid='ignore', over='ignore'):
+        sc = np.exp(lnU + lnD - ll[None, :])
+    Dt = np.einsum('nca,naj->ncj', D, M)
+    post = Dt * U * sc[..., None]
+    post[0] = D[0] * U[0] * sc[0][:, None]
+    post = post / post.sum(-1, keepdims=True)
+    return np.moveaxis(post, -1, -2)
+
+
+def branch_posterior(alignment, tree, model):
+    """(R,A,A,C) joint posterior P(x_parent(n)=i, x_n=j | column), branch 0 zeros
+    (oracle.py:1833-1902): D[n]_i M_n[i,j] U[n]_j normalised over (i, j)."""
+    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
+    D, lnD = _with_root_outside(D, lnD, model.pi)
+    with np.errstate(invalid='ignore', over='ignore'):
+        sc = np.exp(lnU + lnD - ll[None, :])
+    joint = D[:, :, :, None] * M[:, None, :, :] * U[:, :, None, :] * sc[:, :, None, None]
+    joint = joint / joint.sum(axis=(-2, -1), keepdims=True)
+    joint[0] = 0.0
+    return np.moveaxis(joint, 1, -1)
 
 
 # ---------------------------------------------------------------------------

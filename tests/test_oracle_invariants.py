@@ -348,3 +348,40 @@ def test_J_expm1_form_is_the_accurate_one():
     err_ref = float(np.max(np.abs(ref - exact) / np.maximum(np.abs(exact), 1e-300)))
     assert err_alt < 1e-14
     assert err_ref > 10 * err_alt
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+@pytest.mark.parametrize('A', [3, 4])
+def test_branch_counts_and_posteriors_by_enumeration(seed, A):
+    """Per-branch counts, node posteriors and branch joint posteriors against
+    exhaustive enumeration (+ quadrature of I^{ab}_{ij}); Counts = sum of
+    BranchCounts under every mask."""
+    rng = np.random.default_rng(300 + seed + 10 * A)
+    tree = _rand_tree(5, rng)
+    R, C = 5, 4
+    aln = rng.integers(-1, A + 2, size=(R, C))
+    aln[:, 0] = rng.integers(0, A, size=R)
+    for name, m in _models(A, rng):
+        P, root, pair, M, Q = _enumerate(aln, tree, m)
+        bp = O.branch_posterior(aln, tree, m)
+        np.testing.assert_allclose(bp[1:], pair[1:], atol=1e-11, err_msg=name)
+        assert np.all(bp[0] == 0)
+        npost = O.node_posterior(aln, tree, m)
+        np.testing.assert_allclose(npost[0], root, atol=1e-11, err_msg=name)
+        for n in range(1, R):
+            np.testing.assert_allclose(npost[n], pair[n].sum(0), atol=1e-11, err_msg=name)
+        bc = O.BranchCounts(aln, tree, m)
+        assert np.all(bc[0] == 0)
+        off = ~np.eye(A, dtype=bool)
+        for n in range(1, R):
+            I = _I_integral(Q, tree.distanceToParent[n])
+            E = np.einsum('abc,abij->ijc', pair[n] / M[n][:, :, None], I)
+            np.testing.assert_allclose(bc[n], np.where(off[..., None], Q[..., None] * E, E),
+                                       atol=1e-9, err_msg=name)
+        for mask in (None, 'auto'):
+            np.testing.assert_allclose(O.BranchCounts(aln, tree, m, branch_mask=mask).sum(0),
+                                       O.Counts(aln, tree, m, branch_mask=mask), atol=1e-13)
+            if name in ('jc', 'f81'):
+                np.testing.assert_allclose(
+                    O.BranchCounts(aln, tree, m, f81_fast=True, branch_mask=mask),
+                    O.BranchCounts(aln, tree, m, branch_mask=mask), atol=1e-12, err_msg=name)
