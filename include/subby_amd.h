@@ -101,6 +101,21 @@ int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment,
                     int n_rate, const double* rate_weights, double* d_rate_grad,
                     double* d_loglike, void* stream);
 
+/* Per-branch statistics of the inside-outside tables: replaces
+ * BranchCounts (subby/jax/__init__.py:450-540) and the InsideOutside queries
+ * branch_counts / node_posterior / branch_posterior (:681-1040).  Any output
+ * may be NULL (at least one of the three per-branch outputs must be given):
+ *   d_branch_counts  T   [R][A][A][C]  per-branch counts (branch 0 zeros); mask_mode
+ *                                       as in sb_counts (subby.jax applies none)
+ *   d_node_post      T   [R][A][C]     node state posteriors (row 0: the root)
+ *   d_branch_post    T   [R][A][A][C]  joint (parent = i, node = j) posteriors (branch 0 zeros)
+ *   d_loglike        f64 [C]
+ * Alphabets of at most 8 states (SB_ERR_UNSUPPORTED otherwise). */
+int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                      const int32_t* parent_index, const double* distances, int R, int C, int dtype,
+                      int f81_fast, int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
+                      void* d_node_post, void* d_branch_post, double* d_loglike, void* stream);
+
 /* Batched calls: many (tree, alignment-block) pairs in one persistent launch.
  * Mirrors jax.vmap(Counts) of scripts/batch_eigensub_maf.py:528-532 without
  * requiring equal tree sizes. */

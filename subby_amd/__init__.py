@@ -1,20 +1,22 @@
 """subby_amd: MI355X (gfx950) engine for subby's per-column sufficient statistics.
 
 Public surface mirrors subby.jax for the single-model path:
-LogLike, Counts, RootProb, LogLikeCustomGrad, Tree, DiagModel, RateModel and
-the model builders.  Kernels live in libsubby_amd.so (built by
+LogLike, Counts, RootProb, BranchCounts, InsideOutside, LogLikeCustomGrad,
+Tree, DiagModel, RateModel and the model builders.  Kernels live in libsubby_amd.so (built by
 ``python -m subby_amd.build``); there is no CPU fallback.
 """
 from .models import (Tree, DiagModel, RateModel, hky85_diag, hky85_rate_matrix,
                      hky85_log_kappa_weights, jukes_cantor_model, f81_model,
                      diagonalize_rate_matrix, reversible_rate_matrix, ensure_diag)
 from ._lib import SubbyError
-from .api import Engine, default_engine, LogLike, Counts, RootProb, loglike_grad
+from .api import (Engine, default_engine, LogLike, Counts, RootProb, loglike_grad, BranchCounts,
+                  InsideOutside)
 from .grad import LogLikeCustomGrad, hky85_loglike
 
 __all__ = ['Tree', 'DiagModel', 'RateModel', 'hky85_diag', 'hky85_rate_matrix',
            'hky85_log_kappa_weights', 'jukes_cantor_model', 'f81_model',
            'diagonalize_rate_matrix', 'reversible_rate_matrix', 'ensure_diag', 'SubbyError',
            'Engine', 'default_engine', 'LogLike', 'Counts', 'RootProb', 'loglike_grad',
+           'BranchCounts', 'InsideOutside',
            'LogLikeCustomGrad', 'hky85_loglike']
 from . import synthetic, batch  # noqa: E402

@@ -199,6 +199,94 @@ def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool 
     return counts
 
 
+def _inside_outside(alignment, tree, model, want, f81_fast=False, branch_mask=None, precision='fp32',
+                    engine=None):
+    """One sb_inside_outside call; ``want`` is a set of 'bcounts', 'npost', 'bpost', 'll'."""
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    dt, tdt = _PREC[precision]
+    mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
+    dev = eng.device
+    out = {}
+    if 'bcounts' in want:
+        out['bcounts'] = torch.empty((R, A, A, C), dtype=tdt, device=dev)
+    if 'npost' in want:
+        out['npost'] = torch.empty((R, A, C), dtype=tdt, device=dev)
+    if 'bpost' in want:
+        out['bpost'] = torch.empty((R, A, A, C), dtype=tdt, device=dev)
+    if 'll' in want:
+        out['ll'] = torch.empty(C, dtype=torch.float64, device=dev)
+    ptr = lambda k: out[k].data_ptr() if k in out else None  # noqa: E731
+    _lib.check(eng.lib.sb_inside_outside(
+        eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data, dist.ctypes.data, R, C, dt,
+        1 if f81_fast else 0, mode, mptr, ptr('bcounts'), ptr('npost'), ptr('bpost'), ptr('ll'),
+        _stream_ptr(dev)))
+    return out
+
+
+def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool = False,
+                 branch_mask=None, precision: str = 'fp32', engine: Engine | None = None):
+    """Per-branch counts and dwell times, (R, A, A, C), branch 0 zeros
+    (subby/jax/__init__.py:450-540).  subby.jax applies no branch mask here; the
+    default ``branch_mask=None`` does the same ("auto" or an (R, C) array as in
+    Counts).  Alphabets of at most 8 states."""
+    return _inside_outside(alignment, tree, model, {'bcounts'}, f81_fast_flag, branch_mask,
+                           precision, engine)['bcounts']
+
+
+class InsideOutside:
+    """Posterior queries on the inside-outside tables (subby/jax/__init__.py:681-1040).
+
+    The reference stores U and D once and answers queries from them; here each
+    query runs the fused GPU passes again (the tables never leave the chip),
+    and results are torch tensors on the GPU.  Alphabets of at most 8 states
+    for the per-branch queries.
+    """
+
+    def __init__(self, alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
+                 engine: Engine | None = None):
+        self._eng = engine or default_engine()
+        m = ensure_diag(model)
+        self._args = (self._eng.alignment(alignment, len(m.pi)), tree, m)
+        self._precision = precision
+        self._ll = None
+        self._npost = None
+        self._bpost = None
+
+    @property
+    def log_likelihood(self):
+        """Per-column log-likelihoods, (C,) float64."""
+        if self._ll is None:
+            self._ll = LogLike(*self._args, precision=self._precision, engine=self._eng)
+        return self._ll
+
+    def counts(self, f81_fast_flag: bool = False, branch_mask='auto'):
+        """(A, A, C) counts (as Counts)."""
+        return Counts(*self._args, f81_fast_flag=f81_fast_flag, branch_mask=branch_mask,
+                      precision=self._precision, engine=self._eng)
+
+    def branch_counts(self, f81_fast_flag: bool = False):
+        """(R, A, A, C) per-branch counts, unmasked (as subby.jax)."""
+        return BranchCounts(*self._args, f81_fast_flag=f81_fast_flag, branch_mask=None,
+                            precision=self._precision, engine=self._eng)
+
+    def _posteriors(self):
+        if self._npost is None:
+            out = _inside_outside(*self._args, {'npost', 'bpost'}, precision=self._precision,
+                                  engine=self._eng)
+            self._npost, self._bpost = out['npost'], out['bpost']
+
+    def node_posterior(self, node: int | None = None):
+        """(A, C) posterior of one node, or (R, A, C) for all nodes (row 0: root)."""
+        self._posteriors()
+        return self._npost if node is None else self._npost[node]
+
+    def branch_posterior(self, node: int | None = None):
+        """(A, A, C) joint posterior P(x_parent = i, x_node = j) of branch ``node``
+        (> 0), or (R, A, A, C) for all branches (branch 0 zeros)."""
+        self._posteriors()
+        return self._bpost if node is None else self._bpost[node]
+
+
 def loglike_grad(alignment, tree, model, g=None, rate_weights=None, precision: str = 'fp32',
                  engine: Engine | None = None):
     """d(sum_c g_c logL_c)/d(distanceToParent) (R,) and, for each Fisher weight

@@ -33,6 +33,9 @@ extern "C" int sb_f81w_waves_per_group();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
 extern "C" int sb_f81w_occupancy(int AP);
 extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s);
+extern "C" int sb_post_occupancy(int dtype, int AP, bool f81);
+extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, int nwaves, const SbLaunch* L,
+                                     hipStream_t s);
 extern "C" int sb_gen2_tile_cols();
 extern "C" long long sb_gen2_tile_bytes(int AP, int rmax);
 extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaunch* L, hipStream_t s);
@@ -323,6 +326,9 @@ struct BlockIn {
   double* rate_grad;
   int n_rate;
   const double* rate_w_host;  // [n_rate][A*A]
+  void* bcounts;              // per-branch outputs (sb_inside_outside)
+  void* npost;
+  void* bpost;
 };
 
 int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
@@ -343,7 +349,9 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     }
     SB_HIP(hipEventRecord(ev.first, s));
   }
-  if (key.large == 3) {
+  if (key.large == 4) {
+    SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, nwaves, b->header, s));
+  } else if (key.large == 3) {
     for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
       SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
   } else if (key.large == 2)
@@ -451,6 +459,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
+  // per-branch outputs (branch counts, posteriors): their own small-alphabet kernel
+  bool post = false;
+  for (int i = 0; i < nb; ++i)
+    if (in[i].flags & (SB_F_BCOUNTS | SB_F_NPOST | SB_F_BPOST)) post = true;
+  if (post && (large || grad))
+    return fail(SB_ERR_UNSUPPORTED, "per-branch outputs need an alphabet of at most 8 and no gradient");
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
   bool f81w = large && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
@@ -497,10 +511,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
 
   // persistent grid: resident waves per CU from the kernel's occupancy unless
   // the launcher was given an explicit value; never more waves than tiles
-  const int wpg = large ? 1 : b->waves_per_group;
+  const int wpg = (large || post) ? 1 : b->waves_per_group;
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
-    const int occ = f81w ? sb_f81w_occupancy(AP)
+    const int occ = post ? sb_post_occupancy(dtype, AP, f81)
+                    : f81w ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad)
                             : sb_fused_small_occupancy(dtype, AP, f81, grad, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
@@ -574,6 +589,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     d.dist_grad = in[i].dist_grad;
     d.rate_grad = in[i].rate_grad;
     d.n_rate = in[i].n_rate;
+    d.bcounts = in[i].bcounts;
+    d.npost = in[i].npost;
+    d.bpost = in[i].bpost;
     d.rate_w = reinterpret_cast<const double*>(dev(o_rate) + pr * sizeof(double));
     d.R = k.R;
     d.C = k.C;
@@ -639,7 +657,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
-  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, gen2 ? 3 : f81w ? 2 : large ? 1 : 0, nwaves, wpg};
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
+               nwaves, wpg};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -939,6 +958,25 @@ int sb_counts(sb_context* ctx, int model_id, const int8_t* d_alignment, const in
   bi.blk.d_counts = d_counts;
   bi.blk.d_root = d_root;
   bi.blk.d_loglike = d_loglike;
+  return single(ctx, bi, dtype, stream);
+}
+
+int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                      const int32_t* parent_index, const double* distances, int R, int C, int dtype,
+                      int f81_fast, int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
+                      void* d_node_post, void* d_branch_post, double* d_loglike, void* stream) {
+  if (!d_branch_counts && !d_node_post && !d_branch_post && C > 0)
+    return fail(SB_ERR_ARG, "no per-branch output requested");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = (d_branch_counts ? SB_F_BCOUNTS : 0) | (d_node_post ? SB_F_NPOST : 0) |
+             (d_branch_post ? SB_F_BPOST : 0) | (d_loglike ? SB_F_LL : 0);
+  bi.blk.f81_fast = f81_fast ? 1 : 0;
+  bi.blk.mask_mode = mask_mode;
+  bi.blk.d_mask = d_mask;
+  bi.blk.d_loglike = d_loglike;
+  bi.bcounts = d_branch_counts;
+  bi.npost = d_node_post;
+  bi.bpost = d_branch_post;
   return single(ctx, bi, dtype, stream);
 }
 

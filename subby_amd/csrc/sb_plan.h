@@ -62,6 +62,9 @@ enum {
   SB_F_ROOT = 16,
   SB_F_LL = 32,
   SB_F_GRAD = 64,
+  SB_F_BCOUNTS = 128,      // per-branch counts   [R][A][A][C]
+  SB_F_NPOST = 256,        // node posteriors     [R][A][C]
+  SB_F_BPOST = 512,        // branch joint posteriors [R][A][A][C]
 };
 
 struct SbBlockDesc {
@@ -78,6 +81,9 @@ struct SbBlockDesc {
   const double* rate_w;    // [n_rate][AP*AP] Fisher weights (gradient)
   double* dist_grad;       // [R] output (gradient)
   double* rate_grad;       // [n_rate] output (gradient)
+  void* bcounts;           // [R][A][A][C] of T or null (per-branch outputs)
+  void* npost;             // [R][A][C] of T or null
+  void* bpost;             // [R][A][A][C] of T or null
   int R, C, nint, flags;
   int A, tile_base, ntiles, n_rate;
 };

@@ -1,0 +1,374 @@
+// Per-branch counts and posteriors of the inside-outside tables, small
+// alphabets (padded to 4 or 8), gfx950.
+//
+// Outputs (any subset), per column:
+//   branch counts  [R][A][A][C]  per-branch dwell / substitution counts
+//                  (accumulate_C_per_branch + back_transform_per_branch,
+//                  eigensub.py:175-231; f81_counts_per_branch, f81_fast.py:144-);
+//                  branch 0 zeros
+//   node posterior [R][A][C]     P(x_n = j | column) (InsideOutside.node_posterior,
+//                  subby/jax/__init__.py:864-946): root pi_a U_0a, others
+//                  (sum_a D_na M_n[a][j]) U_nj, normalised
+//   branch joint   [R][A][A][C]  P(x_parent = i, x_n = j | column)
+//                  (InsideOutside.branch_posterior, :948-1040): D_ni M_n[i][j] U_nj,
+//                  normalised; branch 0 zeros
+// Both posteriors share the normaliser Z = D_n . (M_n U_n), so the scales cancel.
+//
+// One thread = one column, one wave64 = 64 columns of one block, persistent
+// over tiles; the inside pass stores internal U (as the fused kernel does),
+// the outside pass forms D for both children of each internal node and writes
+// the requested outputs of both branches, all stores coalesced over columns.
+// Rescaling, Steiner flags and scales follow the fused kernel.
+#include <hip/hip_runtime.h>
+
+#include "../../include/subby_amd.h"
+#include "sb_plan.h"
+
+#define SB_CONST __attribute__((address_space(4)))
+#define SB_GLOBAL __attribute__((address_space(1)))
+
+namespace {
+
+template <typename T>
+__device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
+template <typename T>
+__device__ inline SB_GLOBAL T* glb(void* p) { return (SB_GLOBAL T*)(p); }
+template <typename T>
+__device__ inline const SB_GLOBAL T* glb(const void* p) { return (const SB_GLOBAL T*)(p); }
+
+__device__ inline int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
+__device__ inline int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
+__device__ inline float ldexp_(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
+__device__ inline double ldexp_(double x, int e) { return __builtin_amdgcn_ldexp(x, e); }
+
+template <typename T, int AP>
+__device__ inline void matvec(const SB_CONST T* M, const T* u, T* w) {   // w_a = sum_j M[a][j] u_j
+#pragma unroll
+  for (int a = 0; a < AP; ++a) {
+    T s = M[a * AP] * u[0];
+#pragma unroll
+    for (int j = 1; j < AP; ++j) s = fma(M[a * AP + j], u[j], s);
+    w[a] = s;
+  }
+}
+template <typename T, int AP>
+__device__ inline void vecmat(const T* d, const SB_CONST T* M, T* x) {   // x_a = sum_i d_i M[i][a]
+#pragma unroll
+  for (int a = 0; a < AP; ++a) {
+    T s = d[0] * M[a];
+#pragma unroll
+    for (int i = 1; i < AP; ++i) s = fma(d[i], M[i * AP + a], s);
+    x[a] = s;
+  }
+}
+template <typename T, int AP>
+__device__ inline int rescale(T* v) {
+  T m = v[0];
+#pragma unroll
+  for (int a = 1; a < AP; ++a) m = fmax(m, v[a]);
+  const int e = frexp_exp(m);
+#pragma unroll
+  for (int a = 0; a < AP; ++a) v[a] = ldexp_(v[a], -e);
+  return e;
+}
+
+template <typename T, int AP>
+struct Kid {
+  T u[AP];
+  int e, n, tok;
+  bool has, branch, leaf;
+};
+
+}  // namespace
+
+template <typename T, int AP, bool F81>
+__global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict__ Lp) {
+  const SbLaunch& L = *Lp;
+  const int lane = threadIdx.x;
+  const int NI = L.nint_max;
+  SB_GLOBAL char* wscr = glb<char>(L.scratch) + (long long)blockIdx.x * L.wave_stride;
+  SB_GLOBAL T* Ureg = (SB_GLOBAL T*)wscr;                    // [slot][AP][64]
+  SB_GLOBAL T* Dreg = Ureg + (long long)NI * AP * SB_WAVE;   // [slot][AP][64]
+  SB_GLOBAL int* Uw = (SB_GLOBAL int*)(Dreg + (long long)NI * AP * SB_WAVE);
+  SB_GLOBAL int* Dw = Uw + (long long)NI * SB_WAVE;
+  const int BTS = sb_bt_stride(AP);
+
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(L.counter, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if (t >= L.ntiles) break;
+    const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * t;
+    const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(L.blocks) + tbp[0];
+    const long long C = B->C;
+    const int A = B->A;
+    const int flags = B->flags;
+    const int nint = B->nint;
+    const long long col = (long long)tbp[1] * SB_WAVE + lane;
+    const bool valid = col < C;
+    const long long cc = valid ? col : C - 1;
+    const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln) + cc;
+    const SB_CONST T* bt = cst<T>(B->bt);
+    const SB_CONST T* md = cst<T>(B->model);
+    const SB_CONST int* sched = cst<int>(B->sched);
+    const SB_CONST T* pi = md + sb_mr_pi(AP);
+    const SB_CONST T* ones = md + sb_mr_ones(AP);
+    auto Mat = [&](int n) { return bt + (long long)n * BTS + sb_bt_M(AP); };
+    auto vstore = [&](SB_GLOBAL T* base, int slot, const T* v) {
+#pragma unroll
+      for (int a = 0; a < AP; ++a) base[((long long)slot * AP + a) * SB_WAVE + lane] = v[a];
+    };
+    auto vload = [&](const SB_GLOBAL T* base, int slot, T* v) {
+#pragma unroll
+      for (int a = 0; a < AP; ++a) v[a] = base[((long long)slot * AP + a) * SB_WAVE + lane];
+    };
+    auto token_vec = [&](int tok, T* u) {
+      const bool obs = tok >= 0 && tok < A;
+#pragma unroll
+      for (int a = 0; a < AP; ++a) u[a] = obs ? (a == tok ? T(1) : T(0)) : ones[a];
+    };
+    auto kid = [&](Kid<T, AP>& k, int n, int slot) {
+      k.n = n;
+      k.leaf = slot < 0;
+      if (k.leaf) {
+        k.tok = aln[(long long)n * C];
+        token_vec(k.tok, k.u);
+        k.e = 0;
+        k.has = k.tok >= 0 && k.tok <= A;
+        k.branch = false;
+      } else {
+        vload(Ureg, slot, k.u);
+        const int w = Uw[slot * SB_WAVE + lane];
+        k.tok = 0;
+        k.e = w >> 2;
+        k.has = (w & 1) != 0;
+        k.branch = (w & 2) != 0;
+      }
+    };
+
+    // ---------------- inside pass (postorder) ----------------
+    T u0[AP];
+    int e0 = 0;
+    bool rbr = false;
+    if (nint == 0) token_vec(aln[0], u0);
+    for (int i = nint - 1; i >= 0; --i) {
+      const SB_CONST int* rec = sched + i * 8;
+      Kid<T, AP> kl, kr;
+      kid(kl, rec[1], rec[4]);
+      kid(kr, rec[2], rec[5]);
+      T wl[AP], up[AP];
+      matvec<T, AP>(Mat(kl.n), kl.u, wl);
+      matvec<T, AP>(Mat(kr.n), kr.u, up);
+      T o[AP];
+      token_vec(aln[(long long)rec[0] * C], o);
+#pragma unroll
+      for (int a = 0; a < AP; ++a) up[a] *= o[a] * wl[a];
+      const int ep = kl.e + kr.e + rescale<T, AP>(up);
+      if (i > 0) {
+        vstore(Ureg, i, up);
+        Uw[i * SB_WAVE + lane] = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
+      } else {
+#pragma unroll
+        for (int a = 0; a < AP; ++a) u0[a] = up[a];
+        e0 = ep;
+        rbr = kl.has && kr.has;
+      }
+    }
+    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    double s0 = 0.0;
+#pragma unroll
+    for (int a = 0; a < AP; ++a)
+      if (a < A) s0 = fma(pi64[a], double(u0[a]), s0);
+    const T inv_s0 = T(1.0 / s0);
+    SB_GLOBAL T* bcnt = (flags & SB_F_BCOUNTS) ? glb<T>(B->bcounts) : nullptr;
+    SB_GLOBAL T* npost = (flags & SB_F_NPOST) ? glb<T>(B->npost) : nullptr;
+    SB_GLOBAL T* bpost = (flags & SB_F_BPOST) ? glb<T>(B->bpost) : nullptr;
+    if (valid) {
+      if ((flags & SB_F_LL) && B->loglike)
+        glb<double>(B->loglike)[cc] = double(e0) * 0.69314718055994530942 + log(s0);
+      if (npost) {
+#pragma unroll
+        for (int a = 0; a < AP; ++a)
+          if (a < A) npost[(long long)a * C + cc] = pi[a] * u0[a] * inv_s0;
+      }
+      for (int ij = 0; ij < A * A; ++ij) {     // branch 0 (above the root) is zero
+        if (bcnt) bcnt[(long long)ij * C + cc] = T(0);
+        if (bpost) bpost[(long long)ij * C + cc] = T(0);
+      }
+    }
+    if (!(bcnt || npost || bpost)) continue;
+
+    // ---------------- outside pass (preorder) ----------------
+    const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
+    const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) + cc : nullptr;
+    for (int i = 0; i < nint; ++i) {
+      const SB_CONST int* rec = sched + i * 8;
+      const int p = rec[0];
+      T x[AP];
+      int ex;
+      bool stp;
+      {
+        T o[AP];
+        token_vec(aln[(long long)p * C], o);
+        if (i == 0) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) x[a] = o[a] * pi[a];
+          ex = 0;
+          stp = rbr;
+        } else {
+          T dp[AP];
+          vload(Dreg, i, dp);
+          const int w = Dw[i * SB_WAVE + lane];
+          ex = w >> 1;
+          stp = (w & 1) != 0;
+          vecmat<T, AP>(dp, Mat(p), x);
+#pragma unroll
+          for (int a = 0; a < AP; ++a) x[a] *= o[a];
+        }
+      }
+      Kid<T, AP> ch[2];
+      kid(ch[0], rec[1], rec[4]);
+      kid(ch[1], rec[2], rec[5]);
+      T w2[2][AP];
+      matvec<T, AP>(Mat(ch[0].n), ch[0].u, w2[0]);
+      matvec<T, AP>(Mat(ch[1].n), ch[1].u, w2[1]);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const Kid<T, AP>& kn = ch[s];
+        const Kid<T, AP>& ks = ch[1 - s];
+        const int n = kn.n, sn = s == 0 ? rec[4] : rec[5];
+        T d[AP];
+#pragma unroll
+        for (int a = 0; a < AP; ++a) d[a] = x[a] * w2[1 - s][a];
+        const int ed = ks.e + ex + rescale<T, AP>(d);
+        const bool st = (kn.leaf ? kn.has : kn.branch) || (stp && kn.has);
+        if (sn >= 0) {
+          vstore(Dreg, sn, d);
+          Dw[sn * SB_WAVE + lane] = (ed << 1) | (st ? 1 : 0);
+        }
+        if (!valid) continue;
+        const SB_CONST T* Mn = Mat(n);
+        if (npost || bpost) {
+          T z = T(0);
+#pragma unroll
+          for (int a = 0; a < AP; ++a) z = fma(d[a], w2[s][a], z);
+          const T iz = T(1) / z;
+          if (npost) {
+            T y[AP];
+            vecmat<T, AP>(d, Mn, y);
+#pragma unroll
+            for (int j = 0; j < AP; ++j)
+              if (j < A) npost[((long long)n * A + j) * C + cc] = y[j] * kn.u[j] * iz;
+          }
+          if (bpost) {
+#pragma unroll
+            for (int a = 0; a < AP; ++a) {
+              if (a >= A) continue;
+              const T da = d[a] * iz;
+#pragma unroll
+              for (int j = 0; j < AP; ++j)
+                if (j < A) bpost[(((long long)n * A + a) * A + j) * C + cc] = da * Mn[a * AP + j] * kn.u[j];
+            }
+          }
+        }
+        if (bcnt) {
+          bool act;
+          if (mask_mode == SB_MASK_NONE) act = true;
+          else if (mask_mode == SB_MASK_AUTO) act = st && stp;
+          else act = emask[(long long)n * C] != 0;
+          const T sc = act ? ldexp_(inv_s0, ed + kn.e - e0) : T(0);
+          SB_GLOBAL T* out = bcnt + (long long)n * A * A * C + cc;
+          if constexpr (F81) {
+            const SB_CONST T* cf = bt + (long long)n * BTS + sb_bt_coef(AP);
+            const T al = cf[0], be = cf[1], ga = cf[2];
+            const T fmu = md[sb_mr_misc(AP)];
+            T Ds[AP], piU = T(0), Dsum = T(0);
+#pragma unroll
+            for (int a = 0; a < AP; ++a) {
+              Ds[a] = d[a] * sc;
+              piU = fma(pi[a], kn.u[a], piU);
+              Dsum += Ds[a];
+            }
+#pragma unroll
+            for (int a = 0; a < AP; ++a) {
+              if (a >= A) continue;
+#pragma unroll
+              for (int j = 0; j < AP; ++j) {
+                if (j >= A) continue;
+                const T I = al * Ds[a] * kn.u[j] + be * (Ds[a] * piU + pi[a] * Dsum * kn.u[j]) +
+                            ga * pi[a] * Dsum * piU;
+                out[(long long)(a * A + j) * C] = a == j ? I : fmu * pi[j] * I;
+              }
+            }
+          } else {
+            const SB_CONST T* Vs = md + sb_mr_Vs(AP);
+            const SB_CONST T* Vi = md + sb_mr_Vi(AP);
+            const SB_CONST T* V = md + sb_mr_V(AP);
+            const SB_CONST T* S = md + sb_mr_S(AP);
+            const SB_CONST T* J = bt + (long long)n * BTS + sb_bt_J(AP);
+            T Ut[AP], Dt[AP];
+            vecmat<T, AP>(kn.u, Vs, Ut);   // Ut_q = sum_a Vs[a][q] u_a
+            vecmat<T, AP>(d, Vi, Dt);      // Dt_k = sum_a Vi[a][k] d_a
+            T Ce[AP * AP];
+#pragma unroll
+            for (int k = 0; k < AP; ++k)
+#pragma unroll
+              for (int q = 0; q < AP; ++q) Ce[k * AP + q] = Dt[k] * sc * J[k * AP + q] * Ut[q];
+#pragma unroll
+            for (int a = 0; a < AP; ++a) {
+              if (a >= A) continue;
+              T tmp[AP];
+#pragma unroll
+              for (int q = 0; q < AP; ++q) {
+                T v = T(0);
+#pragma unroll
+                for (int k = 0; k < AP; ++k) v = fma(V[a * AP + k], Ce[k * AP + q], v);
+                tmp[q] = v;
+              }
+#pragma unroll
+              for (int j = 0; j < AP; ++j) {
+                if (j >= A) continue;
+                T v = T(0);
+#pragma unroll
+                for (int q = 0; q < AP; ++q) v = fma(tmp[q], V[j * AP + q], v);
+                out[(long long)(a * A + j) * C] = a == j ? v : S[a * AP + j] * v;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+#define SB_PINST(T, AP)                                                   \
+  template __global__ void sb_post_kernel<T, AP, false>(const SbLaunch*); \
+  template __global__ void sb_post_kernel<T, AP, true>(const SbLaunch*);
+SB_PINST(float, 4)
+SB_PINST(float, 8)
+SB_PINST(double, 4)
+SB_PINST(double, 8)
+
+static void (*pick_post(int dtype, int AP, bool f81))(const SbLaunch*) {
+#define SB_PP(T, APV) \
+  if (AP == APV) return f81 ? sb_post_kernel<T, APV, true> : sb_post_kernel<T, APV, false>;
+  if (dtype == 0) { SB_PP(float, 4) SB_PP(float, 8) }
+  else { SB_PP(double, 4) SB_PP(double, 8) }
+#undef SB_PP
+  return nullptr;
+}
+
+extern "C" int sb_post_occupancy(int dtype, int AP, bool f81) {
+  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81);
+  int n = 0;
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, int nwaves, const SbLaunch* L,
+                                     hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81);
+  if (!k) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
+  return hipGetLastError();
+}

@@ -1,0 +1,117 @@
+"""BranchCounts and InsideOutside posteriors on the GPU vs the FP64 oracle.
+
+The oracle functions are pinned by exhaustive enumeration
+(tests/test_oracle_invariants.py); here the GPU path is compared with them,
+and the per-branch counts are tied to the reference's own golden counts
+(Counts = sum over branches of BranchCounts, unmasked)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import subby_oracle as O
+from tests.golden_cases import CASES, load
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope='module')
+def sb():
+    import subby_amd
+    return subby_amd
+
+
+@pytest.fixture(scope='module')
+def eng(sb):
+    return sb.Engine(0)
+
+
+def otree(t):
+    return O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent, dtype=np.float64))
+
+
+def omodel(m):
+    return O.DiagModel(*(np.asarray(x, dtype=np.float64) for x in m))
+
+
+def per_branch_err(got, ref):
+    """max over branches and columns of max_ij |d| / max_ij |ref| (branches with ref == 0 skipped)."""
+    got = got.detach().cpu().double().numpy()
+    num = np.abs(got - ref).max(axis=(1, 2))
+    den = np.abs(ref).max(axis=(1, 2))
+    ok = den > 0
+    assert np.all(num[~ok] == 0)
+    return float(np.max(num[ok] / den[ok])) if ok.any() else 0.0
+
+
+def abs_err(got, ref):
+    return float(np.max(np.abs(got.detach().cpu().double().numpy() - ref)))
+
+
+def _block(sb, A, C, seed, leaves=12):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(np.full(A, 3.0))
+    if A == 4:
+        m = sb.hky85_diag(2.3, pi)
+    else:
+        Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi)
+        m = sb.diagonalize_rate_matrix(Q, pi)
+    t = sb.synthetic.random_tree(leaves, rng, 0.15)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.08, ambig_frac=0.03)
+    return t, m, aln, pi
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('A', [3, 4, 6])
+def test_branch_counts_and_posteriors(sb, eng, A, precision):
+    say('posteriors', A, precision)
+    t, m, aln, pi = _block(sb, A, 333, seed=60 + A)
+    ot, om = otree(t), omodel(m)
+    tol_c = 1e-11 if precision == 'fp64' else 1e-4
+    tol_p = 1e-12 if precision == 'fp64' else 1e-5
+    for mask in (None, 'auto'):
+        ref = O.BranchCounts(aln, ot, om, branch_mask=mask)
+        got = sb.BranchCounts(aln, t, m, branch_mask=mask, precision=precision, engine=eng)
+        e = per_branch_err(got, ref)
+        say('   branch counts', mask, f'{e:.2e}')
+        assert e <= tol_c
+    mk = np.random.default_rng(5).random((len(t.parentIndex), aln.shape[1])) < 0.5
+    ref = O.BranchCounts(aln, ot, om, branch_mask=mk)
+    got = sb.BranchCounts(aln, t, m, branch_mask=mk, precision=precision, engine=eng)
+    assert per_branch_err(got, ref) <= tol_c
+    io = sb.InsideOutside(aln, t, m, precision=precision, engine=eng)
+    en = abs_err(io.node_posterior(), O.node_posterior(aln, ot, om))
+    eb = abs_err(io.branch_posterior(), O.branch_posterior(aln, ot, om))
+    say('   node / branch posterior', f'{en:.2e} {eb:.2e}')
+    assert en <= tol_p and eb <= tol_p
+    assert abs_err(io.node_posterior(0), O.RootProb(aln, ot, om)) <= tol_p
+    ll = io.log_likelihood.cpu().numpy()
+    assert np.max(np.abs(ll - O.LogLike(aln, ot, om)) / np.maximum(np.abs(ll), 1)) <= (1e-12 if precision == 'fp64' else 1e-5)
+    # F81 closed form per branch (F81 model)
+    f = sb.f81_model(pi)
+    ref = O.BranchCounts(aln, ot, omodel(f), f81_fast=True)
+    got = sb.BranchCounts(aln, t, f, f81_fast_flag=True, precision=precision, engine=eng)
+    assert per_branch_err(got, ref) <= tol_c
+
+
+@pytest.mark.parametrize('case', [c for c in CASES if c != '7node_jc64'])
+def test_branch_counts_sum_to_golden_counts(sb, eng, case):
+    """sum_n BranchCounts = the reference's golden (unmasked) counts."""
+    g = load(case)
+    m = sb.DiagModel(g['inputs.model.eigenvalues'], g['inputs.model.eigenvectors'], g['inputs.model.pi'])
+    t = sb.Tree(g['inputs.parentIndex'], g['inputs.distanceToParent'])
+    aln = g['inputs.alignment']
+    bc = sb.BranchCounts(aln, t, m, precision='fp64', engine=eng).sum(0).cpu().numpy()
+    ref = g['outputs.counts']
+    num = np.abs(bc - ref).max(axis=(0, 1))
+    den = np.maximum(np.abs(ref).max(axis=(0, 1)), 1e-300)
+    assert np.max(num / den) <= 1e-12
+
+
+def test_per_branch_outputs_reject_large_alphabets(sb, eng):
+    t, m, aln, pi = _block(sb, 12, 40, seed=70)
+    with pytest.raises(sb.SubbyError, match='at most 8'):
+        sb.BranchCounts(aln, t, m, engine=eng)
