@@ -178,6 +178,61 @@ def other_workload(args, log):
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
                "config": {"workload": w, "A": A, "R": R, "C": C}, "roofline": roof,
                "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": len(cols)}}
+    elif w == 'io':
+        # per-branch outputs on the headline block: BranchCounts, then node + branch posteriors
+        rng = np.random.default_rng(2000)
+        pi = rng.dirichlet(np.full(4, 5.0))
+        m = sb.hky85_diag(2.0, pi)
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        C = 200_000
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        aln = torch.from_numpy(aln_np).to(dev)
+        R = len(t.parentIndex)
+        A = 4
+
+        def timed(fn, reps):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps
+
+        reps = max(5, args.steps // 5)
+        bc_holder = {}
+        t_bc = timed(lambda: bc_holder.__setitem__('bc', sb.BranchCounts(aln, t, m, engine=eng)), reps)
+        io = sb.InsideOutside(aln, t, m, engine=eng)
+
+        def post():
+            io._npost = None
+            io.node_posterior()
+        t_post = timed(post, reps)
+        cols = np.arange(0, C, max(1, C // 300))[:300]
+        ot, om = O.Tree(*t), O.DiagModel(*m)
+        sub = aln_np[:, cols]
+        rbc = O.BranchCounts(sub, ot, om)
+        gbc = bc_holder['bc'][:, :, :, cols].double().cpu().numpy()
+        num = np.abs(gbc - rbc).max(axis=(1, 2))
+        den = np.abs(rbc).max(axis=(1, 2))
+        err_bc = float(np.max(num[1:] / den[1:]))
+        err_np = float(np.abs(io.node_posterior()[:, :, cols].double().cpu().numpy() - O.node_posterior(sub, ot, om)).max())
+        err_bp = float(np.abs(io.branch_posterior()[:, :, :, cols].double().cpu().numpy() - O.branch_posterior(sub, ot, om)).max())
+        units = C * (R - 1)
+        out_bc = R * A * A * C * 4
+        out_post = R * A * C * 4 + R * A * A * C * 4
+        out = {"metric": "BranchCounts column*branches/sec (FP32, HKY85 R=127, unmasked, (R,A,A,C) output)",
+               "value": units / t_bc, "unit": "column*branches/s", "n_gpus": 1, "steps": reps,
+               "ms_per_step": t_bc * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": "io", "A": A, "R": R, "C": C},
+               "roofline": {"bound": "hbm", "note": "write-dominated: output bytes / call time",
+                            "achieved": out_bc / t_bc / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": out_bc / t_bc / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                            "output_bytes_per_call": out_bc},
+               "posteriors": {"ms_per_call": t_post * 1e3, "column*branches/s": units / t_post,
+                              "output_GB_per_s": out_post / t_post / 1e9},
+               "parity": {"branch_counts_per_branch_normwise": err_bc, "node_posterior_abs": err_np,
+                          "branch_posterior_abs": err_bp, "columns_checked": len(cols)}}
     else:  # c5 gradient
         rng = np.random.default_rng(5000)
         pi = rng.dirichlet(np.full(4, 5.0))
@@ -258,10 +313,11 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io'],
                     help='c2 (default, the headline line) or one of the other configurations: '
                          'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
-                         'c5 HKY85 gradient R=127 C=5e4')
+                         'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
+                         'node and branch posteriors) HKY85 R=127 C=2e5')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()

@@ -26,6 +26,7 @@
 
 #define SB_CONST __attribute__((address_space(4)))
 #define SB_GLOBAL __attribute__((address_space(1)))
+#define SB_INLINE __attribute__((always_inline))
 
 namespace {
 
@@ -79,10 +80,13 @@ struct Kid {
   bool has, branch, leaf;
 };
 
+template <typename T>
+__device__ inline void out_store(SB_GLOBAL T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 }  // namespace
 
 template <typename T, int AP, bool F81>
-__global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict__ Lp) {
+__global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_post_kernel(const SbLaunch* __restrict__ Lp) {
   const SbLaunch& L = *Lp;
   const int lane = threadIdx.x;
   const int NI = L.nint_max;
@@ -113,32 +117,44 @@ __global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict_
     const SB_CONST int* sched = cst<int>(B->sched);
     const SB_CONST T* pi = md + sb_mr_pi(AP);
     const SB_CONST T* ones = md + sb_mr_ones(AP);
-    auto Mat = [&](int n) { return bt + (long long)n * BTS + sb_bt_M(AP); };
-    auto vstore = [&](SB_GLOBAL T* base, int slot, const T* v) {
+    auto Mat = [&](int n) SB_INLINE { return bt + (long long)n * BTS + sb_bt_M(AP); };
+    auto vstore = [&](SB_GLOBAL T* base, int slot, const T* v) SB_INLINE {
 #pragma unroll
       for (int a = 0; a < AP; ++a) base[((long long)slot * AP + a) * SB_WAVE + lane] = v[a];
     };
-    auto vload = [&](const SB_GLOBAL T* base, int slot, T* v) {
+    auto vload = [&](const SB_GLOBAL T* base, int slot, T* v) SB_INLINE {
 #pragma unroll
       for (int a = 0; a < AP; ++a) v[a] = base[((long long)slot * AP + a) * SB_WAVE + lane];
     };
-    auto token_vec = [&](int tok, T* u) {
+    auto token_vec = [&](int tok, T* u) SB_INLINE {
       const bool obs = tok >= 0 && tok < A;
 #pragma unroll
       for (int a = 0; a < AP; ++a) u[a] = obs ? (a == tok ? T(1) : T(0)) : ones[a];
     };
-    auto kid = [&](Kid<T, AP>& k, int n, int slot) {
+    // Loads of one child, issued a record ahead and decoded when used (so a
+    // wait on them never waits for the previous record's stores).  slot < 0:
+    // a leaf (token of row n); else the inside vector and word of that slot.
+    // The branch is uniform over the wave.
+    auto fetch = [&](int n, int slot, T* u, int& w, int& tok) SB_INLINE {
+      if (slot < 0) {
+        tok = aln[(long long)n * C];
+      } else {
+        vload(Ureg, slot, u);
+        w = Uw[slot * SB_WAVE + lane];
+      }
+    };
+    auto decode = [&](Kid<T, AP>& k, int n, int slot, const T* u, int w, int tok) SB_INLINE {
       k.n = n;
       k.leaf = slot < 0;
       if (k.leaf) {
-        k.tok = aln[(long long)n * C];
-        token_vec(k.tok, k.u);
+        k.tok = tok;
+        token_vec(tok, k.u);
         k.e = 0;
-        k.has = k.tok >= 0 && k.tok <= A;
+        k.has = tok >= 0 && tok <= A;
         k.branch = false;
       } else {
-        vload(Ureg, slot, k.u);
-        const int w = Uw[slot * SB_WAVE + lane];
+#pragma unroll
+        for (int a = 0; a < AP; ++a) k.u[a] = u[a];
         k.tok = 0;
         k.e = w >> 2;
         k.has = (w & 1) != 0;
@@ -151,28 +167,66 @@ __global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict_
     int e0 = 0;
     bool rbr = false;
     if (nint == 0) token_vec(aln[0], u0);
+    T lu[AP], ru[AP];
+    int lw = 0, rw = 0, lt = 0, rt = 0, pt = 0;
+    if (nint > 0) {
+      const SB_CONST int* rec = sched + (nint - 1) * 8;
+      fetch(rec[1], rec[4], lu, lw, lt);
+      fetch(rec[2], rec[5], ru, rw, rt);
+      pt = aln[(long long)rec[0] * C];
+    }
     for (int i = nint - 1; i >= 0; --i) {
       const SB_CONST int* rec = sched + i * 8;
+      const SB_CONST int* rn = sched + (i > 0 ? i - 1 : 0) * 8;
+      T nlu[AP], nru[AP];
+      int nlw = 0, nrw = 0, nlt = 0, nrt = 0, npt = 0;
+      if (i > 0) {
+        fetch(rn[1], rn[4], nlu, nlw, nlt);
+        fetch(rn[2], rn[5], nru, nrw, nrt);
+        npt = aln[(long long)rn[0] * C];
+      }
       Kid<T, AP> kl, kr;
-      kid(kl, rec[1], rec[4]);
-      kid(kr, rec[2], rec[5]);
+      decode(kl, rec[1], rec[4], lu, lw, lt);
+      decode(kr, rec[2], rec[5], ru, rw, rt);
       T wl[AP], up[AP];
       matvec<T, AP>(Mat(kl.n), kl.u, wl);
       matvec<T, AP>(Mat(kr.n), kr.u, up);
       T o[AP];
-      token_vec(aln[(long long)rec[0] * C], o);
+      token_vec(pt, o);
 #pragma unroll
       for (int a = 0; a < AP; ++a) up[a] *= o[a] * wl[a];
       const int ep = kl.e + kr.e + rescale<T, AP>(up);
       if (i > 0) {
+        const int word = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
         vstore(Ureg, i, up);
-        Uw[i * SB_WAVE + lane] = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
+        Uw[i * SB_WAVE + lane] = word;
+        // the next record's loads of slot i went out before this store
+        if (rn[4] == i) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) nlu[a] = up[a];
+          nlw = word;
+        }
+        if (rn[5] == i) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) nru[a] = up[a];
+          nrw = word;
+        }
       } else {
 #pragma unroll
         for (int a = 0; a < AP; ++a) u0[a] = up[a];
         e0 = ep;
         rbr = kl.has && kr.has;
       }
+#pragma unroll
+      for (int a = 0; a < AP; ++a) {
+        lu[a] = nlu[a];
+        ru[a] = nru[a];
+      }
+      lw = nlw;
+      rw = nrw;
+      lt = nlt;
+      rt = nrt;
+      pt = npt;
     }
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
     double s0 = 0.0;
@@ -189,11 +243,11 @@ __global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict_
       if (npost) {
 #pragma unroll
         for (int a = 0; a < AP; ++a)
-          if (a < A) npost[(long long)a * C + cc] = pi[a] * u0[a] * inv_s0;
+          if (a < A) out_store(npost + (long long)a * C + cc, pi[a] * u0[a] * inv_s0);
       }
       for (int ij = 0; ij < A * A; ++ij) {     // branch 0 (above the root) is zero
-        if (bcnt) bcnt[(long long)ij * C + cc] = T(0);
-        if (bpost) bpost[(long long)ij * C + cc] = T(0);
+        if (bcnt) out_store(bcnt + (long long)ij * C + cc, T(0));
+        if (bpost) out_store(bpost + (long long)ij * C + cc, T(0));
       }
     }
     if (!(bcnt || npost || bpost)) continue;
@@ -201,142 +255,186 @@ __global__ void __launch_bounds__(64) sb_post_kernel(const SbLaunch* __restrict_
     // ---------------- outside pass (preorder) ----------------
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
     const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) + cc : nullptr;
+
+    // Branch above child kn (sibling ks) of a parent with outside product x:
+    // its outside vector d (stored for an internal child), then the outputs.
+    auto branch = [&](const Kid<T, AP>& kn, const T* wn, const Kid<T, AP>& ks, const T* ws, int sn,
+                      const T* x, int ex, bool stp, T* d, int& word) SB_INLINE {
+#pragma unroll
+      for (int a = 0; a < AP; ++a) d[a] = x[a] * ws[a];
+      const int ed = ks.e + ex + rescale<T, AP>(d);
+      const bool st = (kn.leaf ? kn.has : kn.branch) || (stp && kn.has);
+      word = (ed << 1) | (st ? 1 : 0);
+      if (sn >= 0) {
+        vstore(Dreg, sn, d);
+        Dw[sn * SB_WAVE + lane] = word;
+      }
+      if (!valid) return;
+      const int n = kn.n;
+      const SB_CONST T* Mn = Mat(n);
+      if (npost || bpost) {
+        T z = T(0);
+#pragma unroll
+        for (int a = 0; a < AP; ++a) z = fma(d[a], wn[a], z);
+        const T iz = T(1) / z;
+        if (npost) {
+          T y[AP];
+          vecmat<T, AP>(d, Mn, y);
+#pragma unroll
+          for (int j = 0; j < AP; ++j)
+            if (j < A) out_store(npost + ((long long)n * A + j) * C + cc, y[j] * kn.u[j] * iz);
+        }
+        if (bpost) {
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            if (a >= A) continue;
+            const T da = d[a] * iz;
+#pragma unroll
+            for (int j = 0; j < AP; ++j)
+              if (j < A) out_store(bpost + (((long long)n * A + a) * A + j) * C + cc, da * Mn[a * AP + j] * kn.u[j]);
+          }
+        }
+      }
+      if (bcnt) {
+        bool act;
+        if (mask_mode == SB_MASK_NONE) act = true;
+        else if (mask_mode == SB_MASK_AUTO) act = st && stp;
+        else act = emask[(long long)n * C] != 0;
+        const T sc = act ? ldexp_(inv_s0, ed + kn.e - e0) : T(0);
+        SB_GLOBAL T* out = bcnt + (long long)n * A * A * C + cc;
+        if constexpr (F81) {
+          const SB_CONST T* cf = bt + (long long)n * BTS + sb_bt_coef(AP);
+          const T al = cf[0], be = cf[1], ga = cf[2];
+          const T fmu = md[sb_mr_misc(AP)];
+          T Ds[AP], piU = T(0), Dsum = T(0);
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            Ds[a] = d[a] * sc;
+            piU = fma(pi[a], kn.u[a], piU);
+            Dsum += Ds[a];
+          }
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            if (a >= A) continue;
+#pragma unroll
+            for (int j = 0; j < AP; ++j) {
+              if (j >= A) continue;
+              const T I = al * Ds[a] * kn.u[j] + be * (Ds[a] * piU + pi[a] * Dsum * kn.u[j]) +
+                          ga * pi[a] * Dsum * piU;
+              out_store(out + (long long)(a * A + j) * C, a == j ? I : fmu * pi[j] * I);
+            }
+          }
+        } else {
+          const SB_CONST T* Vs = md + sb_mr_Vs(AP);
+          const SB_CONST T* Vi = md + sb_mr_Vi(AP);
+          const SB_CONST T* V = md + sb_mr_V(AP);
+          const SB_CONST T* S = md + sb_mr_S(AP);
+          const SB_CONST T* J = bt + (long long)n * BTS + sb_bt_J(AP);
+          T Ut[AP], Dt[AP];
+          vecmat<T, AP>(kn.u, Vs, Ut);   // Ut_q = sum_a Vs[a][q] u_a
+          vecmat<T, AP>(d, Vi, Dt);      // Dt_k = sum_a Vi[a][k] d_a
+          T Ce[AP * AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k)
+#pragma unroll
+            for (int q = 0; q < AP; ++q) Ce[k * AP + q] = Dt[k] * sc * J[k * AP + q] * Ut[q];
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            if (a >= A) continue;
+            T tmp[AP];
+#pragma unroll
+            for (int q = 0; q < AP; ++q) {
+              T v = T(0);
+#pragma unroll
+              for (int k = 0; k < AP; ++k) v = fma(V[a * AP + k], Ce[k * AP + q], v);
+              tmp[q] = v;
+            }
+#pragma unroll
+            for (int j = 0; j < AP; ++j) {
+              if (j >= A) continue;
+              T v = T(0);
+#pragma unroll
+              for (int q = 0; q < AP; ++q) v = fma(tmp[q], V[j * AP + q], v);
+              out_store(out + (long long)(a * A + j) * C, a == j ? v : S[a * AP + j] * v);
+            }
+          }
+        }
+      }
+    };
+
+    // loads of record i: parent token, its outside vector (slot i, i > 0), both children
+    T xd[AP], au[AP], bu[AP];
+    int xdw = 0, xt = 0, aw = 0, bw = 0, at = 0, btk = 0;
+    auto ofetch = [&](int i, T* dp, int& dw, int& tp, T* u0_, int& w0, int& t0, T* u1_, int& w1,
+                      int& t1) SB_INLINE {
+      const SB_CONST int* rq = sched + i * 8;
+      tp = aln[(long long)rq[0] * C];
+      if (i > 0) {
+        vload(Dreg, i, dp);
+        dw = Dw[i * SB_WAVE + lane];
+      }
+      fetch(rq[1], rq[4], u0_, w0, t0);
+      fetch(rq[2], rq[5], u1_, w1, t1);
+    };
+    if (nint > 0) ofetch(0, xd, xdw, xt, au, aw, at, bu, bw, btk);
     for (int i = 0; i < nint; ++i) {
       const SB_CONST int* rec = sched + i * 8;
       const int p = rec[0];
+      T nxd[AP], nau[AP], nbu[AP];
+      int nxdw = 0, nxt = 0, naw = 0, nbw = 0, nat = 0, nbt = 0;
+      if (i + 1 < nint) ofetch(i + 1, nxd, nxdw, nxt, nau, naw, nat, nbu, nbw, nbt);
       T x[AP];
       int ex;
       bool stp;
       {
         T o[AP];
-        token_vec(aln[(long long)p * C], o);
+        token_vec(xt, o);
         if (i == 0) {
 #pragma unroll
           for (int a = 0; a < AP; ++a) x[a] = o[a] * pi[a];
           ex = 0;
           stp = rbr;
         } else {
-          T dp[AP];
-          vload(Dreg, i, dp);
-          const int w = Dw[i * SB_WAVE + lane];
-          ex = w >> 1;
-          stp = (w & 1) != 0;
-          vecmat<T, AP>(dp, Mat(p), x);
+          ex = xdw >> 1;
+          stp = (xdw & 1) != 0;
+          vecmat<T, AP>(xd, Mat(p), x);
 #pragma unroll
           for (int a = 0; a < AP; ++a) x[a] *= o[a];
         }
       }
-      Kid<T, AP> ch[2];
-      kid(ch[0], rec[1], rec[4]);
-      kid(ch[1], rec[2], rec[5]);
-      T w2[2][AP];
-      matvec<T, AP>(Mat(ch[0].n), ch[0].u, w2[0]);
-      matvec<T, AP>(Mat(ch[1].n), ch[1].u, w2[1]);
+      Kid<T, AP> k0, k1;
+      decode(k0, rec[1], rec[4], au, aw, at);
+      decode(k1, rec[2], rec[5], bu, bw, btk);
+      T w0[AP], w1[AP];
+      matvec<T, AP>(Mat(k0.n), k0.u, w0);
+      matvec<T, AP>(Mat(k1.n), k1.u, w1);
+      T d0[AP], d1[AP];
+      int word0, word1;
+      branch(k0, w0, k1, w1, rec[4], x, ex, stp, d0, word0);
+      branch(k1, w1, k0, w0, rec[5], x, ex, stp, d1, word1);
+      // the next record's load of slot i + 1 went out before this record stored it
+      if (rec[4] == i + 1) {
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const Kid<T, AP>& kn = ch[s];
-        const Kid<T, AP>& ks = ch[1 - s];
-        const int n = kn.n, sn = s == 0 ? rec[4] : rec[5];
-        T d[AP];
-#pragma unroll
-        for (int a = 0; a < AP; ++a) d[a] = x[a] * w2[1 - s][a];
-        const int ed = ks.e + ex + rescale<T, AP>(d);
-        const bool st = (kn.leaf ? kn.has : kn.branch) || (stp && kn.has);
-        if (sn >= 0) {
-          vstore(Dreg, sn, d);
-          Dw[sn * SB_WAVE + lane] = (ed << 1) | (st ? 1 : 0);
-        }
-        if (!valid) continue;
-        const SB_CONST T* Mn = Mat(n);
-        if (npost || bpost) {
-          T z = T(0);
-#pragma unroll
-          for (int a = 0; a < AP; ++a) z = fma(d[a], w2[s][a], z);
-          const T iz = T(1) / z;
-          if (npost) {
-            T y[AP];
-            vecmat<T, AP>(d, Mn, y);
-#pragma unroll
-            for (int j = 0; j < AP; ++j)
-              if (j < A) npost[((long long)n * A + j) * C + cc] = y[j] * kn.u[j] * iz;
-          }
-          if (bpost) {
-#pragma unroll
-            for (int a = 0; a < AP; ++a) {
-              if (a >= A) continue;
-              const T da = d[a] * iz;
-#pragma unroll
-              for (int j = 0; j < AP; ++j)
-                if (j < A) bpost[(((long long)n * A + a) * A + j) * C + cc] = da * Mn[a * AP + j] * kn.u[j];
-            }
-          }
-        }
-        if (bcnt) {
-          bool act;
-          if (mask_mode == SB_MASK_NONE) act = true;
-          else if (mask_mode == SB_MASK_AUTO) act = st && stp;
-          else act = emask[(long long)n * C] != 0;
-          const T sc = act ? ldexp_(inv_s0, ed + kn.e - e0) : T(0);
-          SB_GLOBAL T* out = bcnt + (long long)n * A * A * C + cc;
-          if constexpr (F81) {
-            const SB_CONST T* cf = bt + (long long)n * BTS + sb_bt_coef(AP);
-            const T al = cf[0], be = cf[1], ga = cf[2];
-            const T fmu = md[sb_mr_misc(AP)];
-            T Ds[AP], piU = T(0), Dsum = T(0);
-#pragma unroll
-            for (int a = 0; a < AP; ++a) {
-              Ds[a] = d[a] * sc;
-              piU = fma(pi[a], kn.u[a], piU);
-              Dsum += Ds[a];
-            }
-#pragma unroll
-            for (int a = 0; a < AP; ++a) {
-              if (a >= A) continue;
-#pragma unroll
-              for (int j = 0; j < AP; ++j) {
-                if (j >= A) continue;
-                const T I = al * Ds[a] * kn.u[j] + be * (Ds[a] * piU + pi[a] * Dsum * kn.u[j]) +
-                            ga * pi[a] * Dsum * piU;
-                out[(long long)(a * A + j) * C] = a == j ? I : fmu * pi[j] * I;
-              }
-            }
-          } else {
-            const SB_CONST T* Vs = md + sb_mr_Vs(AP);
-            const SB_CONST T* Vi = md + sb_mr_Vi(AP);
-            const SB_CONST T* V = md + sb_mr_V(AP);
-            const SB_CONST T* S = md + sb_mr_S(AP);
-            const SB_CONST T* J = bt + (long long)n * BTS + sb_bt_J(AP);
-            T Ut[AP], Dt[AP];
-            vecmat<T, AP>(kn.u, Vs, Ut);   // Ut_q = sum_a Vs[a][q] u_a
-            vecmat<T, AP>(d, Vi, Dt);      // Dt_k = sum_a Vi[a][k] d_a
-            T Ce[AP * AP];
-#pragma unroll
-            for (int k = 0; k < AP; ++k)
-#pragma unroll
-              for (int q = 0; q < AP; ++q) Ce[k * AP + q] = Dt[k] * sc * J[k * AP + q] * Ut[q];
-#pragma unroll
-            for (int a = 0; a < AP; ++a) {
-              if (a >= A) continue;
-              T tmp[AP];
-#pragma unroll
-              for (int q = 0; q < AP; ++q) {
-                T v = T(0);
-#pragma unroll
-                for (int k = 0; k < AP; ++k) v = fma(V[a * AP + k], Ce[k * AP + q], v);
-                tmp[q] = v;
-              }
-#pragma unroll
-              for (int j = 0; j < AP; ++j) {
-                if (j >= A) continue;
-                T v = T(0);
-#pragma unroll
-                for (int q = 0; q < AP; ++q) v = fma(tmp[q], V[j * AP + q], v);
-                out[(long long)(a * A + j) * C] = a == j ? v : S[a * AP + j] * v;
-              }
-            }
-          }
-        }
+        for (int a = 0; a < AP; ++a) nxd[a] = d0[a];
+        nxdw = word0;
       }
+      if (rec[5] == i + 1) {
+#pragma unroll
+        for (int a = 0; a < AP; ++a) nxd[a] = d1[a];
+        nxdw = word1;
+      }
+#pragma unroll
+      for (int a = 0; a < AP; ++a) {
+        xd[a] = nxd[a];
+        au[a] = nau[a];
+        bu[a] = nbu[a];
+      }
+      xdw = nxdw;
+      xt = nxt;
+      aw = naw;
+      bw = nbw;
+      at = nat;
+      btk = nbt;
     }
   }
 }
