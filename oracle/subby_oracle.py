@@ -26,7 +26,12 @@ tests/test_oracle_invariants.py):
     == general eigensub path, dwell sums == branch-length sums, and the
     gradient vs central finite differences; per-branch counts and node /
     branch posteriors against the same enumeration (and Counts = sum of
-    BranchCounts).
+    BranchCounts);
+  * irreversible models (tests/test_oracle_irrev.py): M(t) = expm(Q t),
+    complex J(t) by quadrature, logLike / root / posteriors by enumeration,
+    counts of both conventions by quadrature (the reference convention with
+    Q^T in the integrand, the exact one with Q), exact convention on a
+    reversible Q == the reversible path.
 No function here is "parity unpinned".
 """
 from __future__ import annotations
@@ -47,6 +52,26 @@ class DiagModel(NamedTuple):
     eigenvalues: np.ndarray   # (A,)
     eigenvectors: np.ndarray  # (A, A), V[a, k] = component a of eigenvector k
     pi: np.ndarray            # (A,)
+
+
+class IrrevModel(NamedTuple):
+    """Eigensystem of an irreversible rate matrix Q = V diag(mu) V^{-1}
+    (``subby/jax/types.py:14-18`` IrrevDiagModel).
+
+    ``exact`` selects the counts convention: False (default) restates the
+    reference's formulas (eigensub.py:271-385), whose integrand is that of
+    Q^T; True projects with (V^T, V^{-1}) so that the counts are the expected
+    counts under Q itself.  The two agree on LogLike, RootProb and posteriors.
+    """
+    eigenvalues: np.ndarray       # (A,) complex
+    eigenvectors: np.ndarray      # (A, A) complex, right eigenvectors V[a, k]
+    eigenvectors_inv: np.ndarray  # (A, A) complex, V^{-1}
+    pi: np.ndarray                # (A,) real root distribution
+    exact: bool = False
+
+
+def is_irrev(model):
+    return isinstance(model, IrrevModel)
 
 
 # ---------------------------------------------------------------------------
@@ -181,7 +206,11 @@ def reconstruct_rate_matrix(model):
 # ---------------------------------------------------------------------------
 
 def compute_sub_matrices(model, distances):
-    """M_ij(t) = sqrt(pi_j/pi_i) sum_k V_ik e^{mu_k t} V_jk, (R,A,A) (oracle.py:747-777)."""
+    """M_ij(t) = sqrt(pi_j/pi_i) sum_k V_ik e^{mu_k t} V_jk, (R,A,A) (oracle.py:747-777).
+
+    Irreversible models: compute_sub_matrices_irrev."""
+    if is_irrev(model):
+        return compute_sub_matrices_irrev(model, distances)
     V, mu, pi = model.eigenvectors, model.eigenvalues, model.pi
     t = np.asarray(distances, dtype=np.float64)
     e = np.exp(mu[None, :] * t[:, None])                      # (R, A)
@@ -364,6 +393,107 @@ def compute_branch_mask(alignment, parentIndex, A):
 
 
 # ---------------------------------------------------------------------------
+# Irreversible models  (oracle.py:265-288, 474-740; eigensub.py:234-434)
+# ---------------------------------------------------------------------------
+
+def diagonalize_irreversible(subRate, rootProb, exact=False):
+    """Q = V diag(mu) V^{-1} by eig, no symmetrisation (oracle.py:265-288)."""
+    Q = np.asarray(subRate, dtype=np.float64)
+    mu, V = np.linalg.eig(Q)
+    return IrrevModel(mu.astype(np.complex128), V.astype(np.complex128),
+                      np.linalg.inv(V).astype(np.complex128),
+                      np.asarray(rootProb, dtype=np.float64), exact)
+
+
+def compute_sub_matrices_irrev(model, distances):
+    """M_ij(t) = Re sum_k V_ik e^{mu_k t} Vinv_kj, (R,A,A) (oracle.py:474-502)."""
+    t = np.asarray(distances, dtype=np.float64)
+    e = np.exp(model.eigenvalues[None, :] * t[:, None])            # (R, A) complex
+    return np.einsum('ik,rk,kj->rij', model.eigenvectors, e, model.eigenvectors_inv).real
+
+
+def compute_J_complex(eigenvalues, distances):
+    """Complex J_kl(t); degenerate when |mu_k - mu_l| < 1e-8 (oracle.py:509-535)."""
+    mu = np.asarray(eigenvalues, dtype=np.complex128)
+    t = np.asarray(distances, dtype=np.float64)[:, None, None]
+    ek = np.exp(mu[None, :, None] * t)
+    el = np.exp(mu[None, None, :] * t)
+    diff = (mu[:, None] - mu[None, :])[None]
+    deg = np.abs(diff) < 1e-8
+    with np.errstate(divide='ignore', invalid='ignore'):
+        nondeg = (ek - el) / np.where(deg, 1.0, diff)
+    return np.where(deg, t * ek, nondeg)
+
+
+def eigenbasis_project_irrev(U, D, model):
+    """Reference convention (oracle.py:538-571): U~_k = sum_b U_b V_bk,
+    D~_k = sum_a D_a Vinv_ka.  Exact convention: U^_l = sum_b Vinv_lb U_b,
+    D^_k = sum_a D_a V_ak (the factors of M_ai(s) and M_jb(t-s))."""
+    V, Vi = model.eigenvectors, model.eigenvectors_inv
+    if model.exact:
+        return U @ Vi.T, D @ V
+    return U @ V, D @ Vi.T
+
+
+def accumulate_C_complex(D_tilde, U_tilde, J, logNormU, logNormD, logLike, parentIndex,
+                         branch_mask=None):
+    """C_kl,c = sum_{n>0} D~_k J_kl U~_l scale, complex (oracle.py:574-611)."""
+    sc = branch_scale(logNormU, logNormD, logLike)[1:]
+    if branch_mask is not None:
+        sc = np.where(np.asarray(branch_mask, dtype=bool)[1:], sc, 0.0)
+    return np.einsum('nck,nkl,ncl->klc', D_tilde[1:] * sc[..., None], J[1:], U_tilde[1:])
+
+
+def accumulate_C_complex_per_branch(D_tilde, U_tilde, J, logNormU, logNormD, logLike,
+                                    branch_mask=None):
+    """accumulate_C_complex keeping the branch axis, branch 0 zeros (oracle.py:614-648)."""
+    sc = _branch_scale_masked(logNormU, logNormD, logLike, branch_mask)
+    return np.einsum('nck,nkl,ncl->nklc', D_tilde * sc[..., None], J, U_tilde)
+
+
+def rate_matrix_irrev(model):
+    """R = V diag(mu) V^{-1} (complex; real up to rounding) (oracle.py:668-675)."""
+    return (model.eigenvectors * model.eigenvalues) @ model.eigenvectors_inv
+
+
+def _vcv_irrev(Ceig, model, per_branch):
+    V, Vi = model.eigenvectors, model.eigenvectors_inv
+    if model.exact:     # VCV_ij = sum_kl Vinv_ki C_kl V_jl
+        return np.einsum('ki,...klc,jl->...ijc', Vi, Ceig, V)
+    return np.einsum('ik,...klc,lj->...ijc', V, Ceig, Vi)   # V C V^{-1}
+
+
+def back_transform_irrev(Ceig, model):
+    """counts_ii = Re VCV_ii, counts_ij = Re(R_ij VCV_ij) (oracle.py:651-696)."""
+    VCV = _vcv_irrev(Ceig, model, False)
+    Rm = rate_matrix_irrev(model)
+    A = len(model.pi)
+    eye = np.eye(A, dtype=bool)[..., None]
+    return np.where(eye, VCV, Rm[..., None] * VCV).real
+
+
+def back_transform_irrev_per_branch(Cn, model):
+    """back_transform_irrev keeping the branch axis (oracle.py:699-740)."""
+    VCV = _vcv_irrev(Cn, model, True)
+    Rm = rate_matrix_irrev(model)
+    A = len(model.pi)
+    eye = np.eye(A, dtype=bool)[None, ..., None]
+    return np.where(eye, VCV, Rm[None, ..., None] * VCV).real
+
+
+def _eigensub_counts(U, D, lnU, lnD, ll, tree, model, mask):
+    if is_irrev(model):
+        J = compute_J_complex(model.eigenvalues, tree.distanceToParent)
+        Ut, Dt = eigenbasis_project_irrev(U, D, model)
+        return back_transform_irrev(
+            accumulate_C_complex(Dt, Ut, J, lnU, lnD, ll, tree.parentIndex, branch_mask=mask), model)
+    J = compute_J(model.eigenvalues, tree.distanceToParent)
+    Ut, Dt = eigenbasis_project(U, D, model)
+    return back_transform(accumulate_C(Dt, Ut, J, lnU, lnD, ll, tree.parentIndex, branch_mask=mask),
+                          model)
+
+
+# ---------------------------------------------------------------------------
 # Public API  (oracle.py:1454-1595; subby/jax/__init__.py:207-650)
 # ---------------------------------------------------------------------------
 
@@ -400,12 +530,10 @@ def Counts(alignment, tree, model, f81_fast=False, branch_mask="auto"):
     mask = _resolve_mask(alignment, tree, model, branch_mask)
     M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
     if f81_fast:
+        assert not is_irrev(model), "F81 fast path is reversible-only"
         return f81_counts(U, D, lnU, lnD, ll, tree.distanceToParent, model.pi,
                           tree.parentIndex, branch_mask=mask)
-    J = compute_J(model.eigenvalues, tree.distanceToParent)
-    Ut, Dt = eigenbasis_project(U, D, model)
-    Ce = accumulate_C(Dt, Ut, J, lnU, lnD, ll, tree.parentIndex, branch_mask=mask)
-    return back_transform(Ce, model)
+    return _eigensub_counts(U, D, lnU, lnD, ll, tree, model, mask)
 
 
 def counts_loglike_root(alignment, tree, model, f81_fast=False, branch_mask="auto"):
@@ -413,13 +541,11 @@ def counts_loglike_root(alignment, tree, model, f81_fast=False, branch_mask="aut
     mask = _resolve_mask(alignment, tree, model, branch_mask)
     M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
     if f81_fast:
+        assert not is_irrev(model), "F81 fast path is reversible-only"
         cnt = f81_counts(U, D, lnU, lnD, ll, tree.distanceToParent, model.pi,
                          tree.parentIndex, branch_mask=mask)
     else:
-        J = compute_J(model.eigenvalues, tree.distanceToParent)
-        Ut, Dt = eigenbasis_project(U, D, model)
-        cnt = back_transform(accumulate_C(Dt, Ut, J, lnU, lnD, ll, tree.parentIndex,
-                                          branch_mask=mask), model)
+        cnt = _eigensub_counts(U, D, lnU, lnD, ll, tree, model, mask)
     root = (model.pi[None, :] * U[0] * np.exp(lnU[0] - ll)[:, None]).T
     return cnt, ll, root
 
@@ -490,8 +616,14 @@ def BranchCounts(alignment, tree, model, f81_fast=False, branch_mask=None):
     mask = _resolve_mask(alignment, tree, model, branch_mask)
     M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
     if f81_fast:
+        assert not is_irrev(model), "F81 fast path is reversible-only"
         return f81_counts_per_branch(U, D, lnU, lnD, ll, tree.distanceToParent, model.pi,
                                      branch_mask=mask)
+    if is_irrev(model):
+        J = compute_J_complex(model.eigenvalues, tree.distanceToParent)
+        Ut, Dt = eigenbasis_project_irrev(U, D, model)
+        return back_transform_irrev_per_branch(
+            accumulate_C_complex_per_branch(Dt, Ut, J, lnU, lnD, ll, branch_mask=mask), model)
     J = compute_J(model.eigenvalues, tree.distanceToParent)
     Ut, Dt = eigenbasis_project(U, D, model)
     return back_transform_per_branch(
