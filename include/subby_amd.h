@@ -64,6 +64,22 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues,
                  const double* eigenvectors, const double* pi, int* model_id);
 int sb_model_drop(sb_context* ctx, int model_id);
 
+/* Upload an irreversible eigensystem (IrrevDiagModel, subby/jax/types.py:14-18;
+ * built by diagonalize_irreversible, subby/jax/diagonalize.py:49-67):
+ * Q = V diag(mu) V^{-1}.  Complex arrays are interleaved (re, im) pairs, as
+ * C99 double complex / numpy complex128: eigenvalues[2A], eigenvectors[2*A*A]
+ * with V[a][k] at 2*(a*A+k), eigenvectors_inv[2*A*A] = V^{-1}; pi[A] real.
+ * exact == 0 reproduces the reference's counts (eigensub.py:271-434:
+ * projections V / V^{-T}, back-transform V C V^{-1}, whose integrand is that
+ * of Q^T); exact != 0 gives the expected counts under Q itself (projections
+ * V^{-T} / V, back-transform V^{-T} C V^T).  LogLike, RootProb and the
+ * gradient do not depend on it.  Such models run on alphabets of at most 8
+ * states; the F81 fast path and the per-branch outputs reject them
+ * (SB_ERR_UNSUPPORTED). */
+int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues,
+                       const double* eigenvectors, const double* eigenvectors_inv,
+                       const double* pi, int exact, int* model_id);
+
 /* Single-block calls (drop-in for subby.jax LogLike / RootProb / Counts). */
 
 /* subby/jax/__init__.py:207-291 (LogLike, single DiagModel branch :285-291). */

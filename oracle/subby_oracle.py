@@ -678,14 +678,20 @@ def loglike_branch_grad(alignment, tree, model, g=None):
     """d(sum_c g_c logL_c)/d t_n, n = 0..R-1 (root entry 0) (vjp.py:62-133).
 
     per_branch[n,c] = scale[n,c] sum_k D~_k mu_k e^{mu_k t_n} U~_k.
+    Irreversible models project D with V and U with V^{-1} and keep the real
+    part (vjp.py:80-88, 113-131).
     """
     M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
-    Ut, Dt = eigenbasis_project(U, D, model)
+    if is_irrev(model):
+        Dt = D @ model.eigenvectors
+        Ut = U @ model.eigenvectors_inv.T
+    else:
+        Ut, Dt = eigenbasis_project(U, D, model)
     mu = model.eigenvalues
     t = np.asarray(tree.distanceToParent, dtype=np.float64)
     me = mu[None, :] * np.exp(mu[None, :] * t[:, None])          # (R, A)
     sc = branch_scale(lnU, lnD, ll)
-    per = np.einsum('nck,nk,nck->nc', Dt, me, Ut) * sc
+    per = (np.einsum('nck,nk,nck->nc', Dt, me, Ut) * sc).real
     per[0] = 0.0
     C = U.shape[1]
     g = np.ones(C) if g is None else np.asarray(g, dtype=np.float64)

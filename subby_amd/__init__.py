@@ -5,15 +5,19 @@ LogLike, Counts, RootProb, BranchCounts, InsideOutside, LogLikeCustomGrad,
 Tree, DiagModel, RateModel and the model builders.  Kernels live in libsubby_amd.so (built by
 ``python -m subby_amd.build``); there is no CPU fallback.
 """
-from .models import (Tree, DiagModel, RateModel, hky85_diag, hky85_rate_matrix,
+from .models import (Tree, DiagModel, IrrevDiagModel, RateModel, hky85_diag, hky85_rate_matrix,
                      hky85_log_kappa_weights, jukes_cantor_model, f81_model,
-                     diagonalize_rate_matrix, reversible_rate_matrix, ensure_diag)
+                     diagonalize_rate_matrix, reversible_rate_matrix, ensure_diag,
+                     diagonalize_irreversible, irrev_model_from_rate_matrix,
+                     model_from_rate_matrix, check_detailed_balance)
 from ._lib import SubbyError
 from .api import (Engine, default_engine, LogLike, Counts, RootProb, loglike_grad, BranchCounts,
                   InsideOutside)
 from .grad import LogLikeCustomGrad, hky85_loglike
 
-__all__ = ['Tree', 'DiagModel', 'RateModel', 'hky85_diag', 'hky85_rate_matrix',
+__all__ = ['Tree', 'DiagModel', 'IrrevDiagModel', 'RateModel',
+           'diagonalize_irreversible', 'irrev_model_from_rate_matrix', 'model_from_rate_matrix',
+           'check_detailed_balance', 'hky85_diag', 'hky85_rate_matrix',
            'hky85_log_kappa_weights', 'jukes_cantor_model', 'f81_model',
            'diagonalize_rate_matrix', 'reversible_rate_matrix', 'ensure_diag', 'SubbyError',
            'Engine', 'default_engine', 'LogLike', 'Counts', 'RootProb', 'loglike_grad',

@@ -21,7 +21,7 @@ EXPORTS = [
     'sb_batch_create', 'sb_batch_destroy', 'sb_batch_submit', 'sb_batch_set_tiling',
     'sb_batch_sync', 'sb_batch_set_timing', 'sb_batch_kernel_ms', 'sb_counts_plain',
     'sb_server_create', 'sb_server_destroy', 'sb_server_submit', 'sb_server_poll',
-    'sb_server_drain', 'sb_server_replay', 'sb_inside_outside',
+    'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
 ]
 
 
@@ -93,6 +93,7 @@ def load(path=LIB_PATH):
     lib.sb_create.argtypes = [P(vp), i]
     lib.sb_destroy.argtypes = [vp]
     lib.sb_model_put.argtypes = [vp, i, vp, vp, vp, P(i)]
+    lib.sb_model_put_irrev.argtypes = [vp, i, vp, vp, vp, vp, i, P(i)]
     lib.sb_model_drop.argtypes = [vp, i]
     lib.sb_loglike.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp]
     lib.sb_root_prob.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp]

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .models import Tree, DiagModel, RateModel, ensure_diag
+from .models import Tree, DiagModel, IrrevDiagModel, RateModel, ensure_diag
 
 _PREC = {'fp32': (_lib.SB_F32, torch.float32), 'fp64': (_lib.SB_F64, torch.float64)}
 
@@ -58,21 +58,33 @@ class Engine:
             pass
 
     # ---- resident models
-    def model_id(self, model) -> int:
+    def model_id(self, model, exact: bool = False) -> int:
+        """Resident id of a model (uploaded once per engine).  For an
+        IrrevDiagModel, ``exact`` selects the counts convention (False: the
+        reference's formulas, True: expected counts under Q itself)."""
         m = ensure_diag(model)
-        ev = np.ascontiguousarray(m.eigenvalues, dtype=np.float64)
-        V = np.ascontiguousarray(m.eigenvectors, dtype=np.float64)
         pi = np.ascontiguousarray(m.pi, dtype=np.float64)
         A = len(pi)
-        if ev.shape != (A,) or V.shape != (A, A):
+        irr = isinstance(m, IrrevDiagModel)
+        cdt = np.complex128 if irr else np.float64
+        ev = np.ascontiguousarray(m.eigenvalues, dtype=cdt)
+        V = np.ascontiguousarray(m.eigenvectors, dtype=cdt)
+        Vi = np.ascontiguousarray(m.eigenvectors_inv, dtype=cdt) if irr else None
+        if ev.shape != (A,) or V.shape != (A, A) or (irr and Vi.shape != (A, A)):
             raise ValueError(f'model shapes {ev.shape}, {V.shape}, {pi.shape} are inconsistent')
-        key = (A, ev.tobytes(), V.tobytes(), pi.tobytes())
+        key = (A, irr, bool(exact) and irr, ev.tobytes(), V.tobytes(),
+               Vi.tobytes() if irr else b'', pi.tobytes())
         with self._lock:
             mid = self._models.get(key)
             if mid is None:
                 out = ctypes.c_int()
-                _lib.check(self.lib.sb_model_put(self.ctx, A, ev.ctypes.data, V.ctypes.data,
-                                                 pi.ctypes.data, ctypes.byref(out)))
+                if irr:
+                    _lib.check(self.lib.sb_model_put_irrev(self.ctx, A, ev.ctypes.data, V.ctypes.data,
+                                                           Vi.ctypes.data, pi.ctypes.data,
+                                                           1 if exact else 0, ctypes.byref(out)))
+                else:
+                    _lib.check(self.lib.sb_model_put(self.ctx, A, ev.ctypes.data, V.ctypes.data,
+                                                     pi.ctypes.data, ctypes.byref(out)))
                 mid = out.value
                 self._models[key] = mid
         return mid
@@ -175,17 +187,22 @@ def _mask_args(branch_mask, eng, R, C):
 
 def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool = False,
            branch_mask='auto', precision: str = 'fp32', engine: Engine | None = None,
-           return_loglike: bool = False, return_root: bool = False):
+           return_loglike: bool = False, return_root: bool = False,
+           irrev_convention: str = 'reference'):
     """Expected substitution counts (off-diagonal) and dwell times (diagonal), (A, A, C)
     (subby/jax/__init__.py:338-447).  ``branch_mask``: "auto" (Steiner mask of
-    the ungapped leaves), None, or an (R, C) bool array."""
+    the ungapped leaves), None, or an (R, C) bool array.  For an IrrevDiagModel
+    (alphabets of at most 8 states) ``irrev_convention`` is "reference" (the
+    reference's formulas, eigensub.py:271-434) or "exact" (expected counts
+    under Q itself)."""
     eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
     dt, tdt = _PREC[precision]
     mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
     counts = torch.empty((A, A, C), dtype=tdt, device=eng.device)
     ll = torch.empty(C, dtype=torch.float64, device=eng.device) if return_loglike else None
     root = torch.empty((A, C), dtype=tdt, device=eng.device) if return_root else None
-    _lib.check(eng.lib.sb_counts(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
+    _lib.check(eng.lib.sb_counts(eng.ctx, eng.model_id(m, _exact(irrev_convention)), aln.data_ptr(),
+                                 parent.ctypes.data,
                                  dist.ctypes.data, R, C, dt, 1 if f81_fast_flag else 0, mode, mptr,
                                  counts.data_ptr(), root.data_ptr() if root is not None else None,
                                  ll.data_ptr() if ll is not None else None, _stream_ptr(eng.device)))
@@ -197,6 +214,12 @@ def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool 
             res.append(root)
         return tuple(res)
     return counts
+
+
+def _exact(irrev_convention):
+    if irrev_convention not in ('reference', 'exact'):
+        raise ValueError(f'irrev_convention must be "reference" or "exact", got {irrev_convention!r}')
+    return irrev_convention == 'exact'
 
 
 def _inside_outside(alignment, tree, model, want, f81_fast=False, branch_mask=None, precision='fp32',
@@ -259,10 +282,12 @@ class InsideOutside:
             self._ll = LogLike(*self._args, precision=self._precision, engine=self._eng)
         return self._ll
 
-    def counts(self, f81_fast_flag: bool = False, branch_mask='auto'):
+    def counts(self, f81_fast_flag: bool = False, branch_mask='auto',
+               irrev_convention: str = 'reference'):
         """(A, A, C) counts (as Counts)."""
         return Counts(*self._args, f81_fast_flag=f81_fast_flag, branch_mask=branch_mask,
-                      precision=self._precision, engine=self._eng)
+                      precision=self._precision, engine=self._eng,
+                      irrev_convention=irrev_convention)
 
     def branch_counts(self, f81_fast_flag: bool = False):
         """(R, A, A, C) per-branch counts, unmasked (as subby.jax)."""

@@ -211,14 +211,16 @@ __device__ inline void decode_child(const Raw<T, AP>& r, int slot, int A, const 
   }
 }
 
-template <typename T, int AP, bool F81>
+template <typename T, int AP, bool F81, bool IRR>
 struct Acc {
-  // eigen: c[k][l];  F81: c = term1[i][j], t2a[i], t2b[j], t3
-  T c[AP * AP];
+  // eigen: c[k][l];  irreversible: c = Re C[k][l], then Im C[k][l];
+  // F81: c = term1[i][j], t2a[i], t2b[j], t3
+  static constexpr int NC = (IRR ? 2 : 1) * AP * AP;
+  T c[NC];
   T t2a[F81 ? AP : 1], t2b[F81 ? AP : 1], t3;
   __device__ inline void zero() {
 #pragma unroll
-    for (int i = 0; i < AP * AP; ++i) c[i] = T(0);
+    for (int i = 0; i < NC; ++i) c[i] = T(0);
     if constexpr (F81) {
 #pragma unroll
       for (int i = 0; i < AP; ++i) { t2a[i] = T(0); t2b[i] = T(0); }
@@ -301,11 +303,84 @@ __global__ void __launch_bounds__(256) sb_branch_params_kernel(const SbLaunch* _
   }
 }
 
+// Branch parameters of an irreversible model in FP64 (oracle.py:474-535,
+// eigensub.py:234-268): E_q = e^{mu_q t} (complex), M_ij = Re sum_q V_iq E_q
+// Vinv_qj, J_kl = E_l expm1((mu_k - mu_l) t) / (mu_k - mu_l) (t E_k when
+// |mu_k - mu_l| < 1e-8), dE_k = mu_k E_k.  One 256-thread workgroup per job,
+// one thread per matrix entry.
+template <typename T>
+__global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLaunch* __restrict__ L) {
+  const int AP = L->AP;
+  const int njobs = L->njobs;
+  SB_GLOBAL T* out = glb<T>(L->bt);
+  __shared__ double er[64], ei[64], mr[64], mi[64];
+  const int tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid == 0) *L->counter = 0;  // tile counter of the launch that follows
+  for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+    const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
+    const SB_CONST double* m = cst<double>(job->model64);
+    const double t = job->t;
+    __syncthreads();
+    if (tid < AP) {
+      const double a = m[sb_m64i_mu(AP) + tid], b = m[sb_m64i_mu(AP) + AP + tid];
+      const double mag = exp(a * t);
+      double sn, cs;
+      sincos(b * t, &sn, &cs);
+      mr[tid] = a;
+      mi[tid] = b;
+      er[tid] = mag * cs;
+      ei[tid] = mag * sn;
+    }
+    __syncthreads();
+    const SB_CONST double* Vr = m + sb_m64i_V(AP);
+    const SB_CONST double* Vi = Vr + AP * AP;
+    const SB_CONST double* Wr = m + sb_m64i_Vi(AP);
+    const SB_CONST double* Wi = Wr + AP * AP;
+    SB_GLOBAL T* o = out + (long long)jb * sb_bti_stride(AP);
+    for (int idx = tid; idx < AP * AP; idx += 256) {
+      const int i = idx / AP, j = idx - i * AP;
+      double s = 0.0;
+      for (int q = 0; q < AP; ++q) {
+        // Re(V_iq E_q Vinv_qj)
+        const double xr = Vr[i * AP + q] * er[q] - Vi[i * AP + q] * ei[q];
+        const double xi = Vr[i * AP + q] * ei[q] + Vi[i * AP + q] * er[q];
+        s += xr * Wr[q * AP + j] - xi * Wi[q * AP + j];
+      }
+      o[i * AP + j] = T(s);
+      // J_ij (k = i, l = j)
+      const double dr = mr[i] - mr[j], di = mi[i] - mi[j];
+      double jr, ji;
+      if (sqrt(dr * dr + di * di) < 1e-8) {
+        jr = t * er[i];
+        ji = t * ei[i];
+      } else {
+        // expm1(z), z = (dr + i di) t: Re = expm1(x) cos y - 2 sin^2(y/2), Im = e^x sin y
+        const double x = dr * t, y = di * t;
+        double sy, cy, sh, ch;
+        sincos(y, &sy, &cy);
+        sincos(0.5 * y, &sh, &ch);
+        const double zr = expm1(x) * cy - 2.0 * sh * sh, zi = exp(x) * sy;
+        const double den = dr * dr + di * di;            // z / (mu_i - mu_j)
+        const double qr = (zr * dr + zi * di) / den, qi = (zi * dr - zr * di) / den;
+        jr = er[j] * qr - ei[j] * qi;                    // times E_j
+        ji = er[j] * qi + ei[j] * qr;
+      }
+      o[sb_bti_Jre(AP) + idx] = T(jr);
+      o[sb_bti_Jim(AP) + idx] = T(ji);
+    }
+    if (tid < AP) {
+      o[sb_bti_dEre(AP) + tid] = T(mr[tid] * er[tid] - mi[tid] * ei[tid]);
+      o[sb_bti_dEim(AP) + tid] = T(mr[tid] * ei[tid] + mi[tid] * er[tid]);
+    }
+    if (tid < 4) o[sb_bti_coef(AP) + tid] = T(tid == 3 ? t : 0.0);
+  }
+}
+
 __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *L->counter = 0;
 }
 
-template <typename T, int AP, bool F81, bool GRAD>
+template <typename T, int AP, bool F81, bool GRAD, bool IRR>
 __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
   typedef Tile<T, AP> TL;
   const SbLaunch& args = *Lp;
@@ -341,10 +416,10 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     T ones[AP], pi[AP];
 #pragma unroll
     for (int a = 0; a < AP; ++a) {
-      ones[a] = md[sb_mr_ones(AP) + a];
-      pi[a] = md[sb_mr_pi(AP) + a];
+      ones[a] = md[sb_mr_ones_k(AP, IRR) + a];
+      pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
     }
-    const int BTS = sb_bt_stride(AP);
+    const int BTS = sb_bt_stride_k(AP, IRR);
 
     // ---------------- inside pass ----------------
     // Internal nodes in postorder.  The next node's children are fetched one
@@ -410,7 +485,7 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
       }
     }
     // log-likelihood and root posterior (pruning.py:84-87; __init__.py:642-650)
-    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
     double s0d = 0.0;
 #pragma unroll
     for (int a = 0; a < AP; ++a)
@@ -435,7 +510,7 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     SB_GLOBAL double* gpart =
         GRAD ? glb<double>(args.grad_part) + (long long)(B->tile_base + tb.y) * args.grad_stride : nullptr;
 
-    Acc<T, AP, F81> acc;
+    Acc<T, AP, F81, IRR> acc;
     acc.zero();
 
     // Internal nodes in preorder.  The next node's children (read-only U
@@ -533,6 +608,57 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
             acc.t2b[a] = fma(be * Dsum, u[a], acc.t2b[a]);
           }
           acc.t3 = fma(ga * Dsum, piU, acc.t3);
+        } else if constexpr (IRR) {
+          // Complex eigenbasis (eigensub.py:271-385): U~ = u Pu, D~ = d Pd,
+          // C[k][q] += D~_k sc J_kq U~_q.  Gradients use the exact set, whose
+          // projections are those of vjp.py:80-88.
+          const SB_CONST T* P = md + sb_mri_set(AP, GRAD ? 1 : 0);
+          const SB_CONST T* Pu = P + sb_mri_Pu(AP);
+          const SB_CONST T* Pd = P + sb_mri_Pd(AP);
+          T Ur[AP], Ui[AP], Dr[AP], Di[AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            T ur = T(0), ui = T(0), dr = T(0), di = T(0);
+#pragma unroll
+            for (int b = 0; b < AP; ++b) {
+              ur = fma(u[b], Pu[b * AP + k], ur);
+              ui = fma(u[b], Pu[AP * AP + b * AP + k], ui);
+              dr = fma(d[b], Pd[b * AP + k], dr);
+              di = fma(d[b], Pd[AP * AP + b * AP + k], di);
+            }
+            Ur[k] = ur;
+            Ui[k] = ui;
+            Dr[k] = dr;
+            Di[k] = di;
+          }
+          const SB_CONST T* Jr = bn + sb_bti_Jre(AP);
+          const SB_CONST T* Ji = bn + sb_bti_Jim(AP);
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            const T dr = Dr[k] * sc, di = Di[k] * sc;
+#pragma unroll
+            for (int q = 0; q < AP; ++q) {
+              const T jr = Jr[k * AP + q], ji = Ji[k * AP + q];
+              const T xr = fma(dr, jr, -di * ji), xi = fma(dr, ji, di * jr);
+              T& cr = acc.c[k * AP + q];
+              T& ci = acc.c[AP * AP + k * AP + q];
+              cr = fma(xr, Ur[q], fma(-xi, Ui[q], cr));
+              ci = fma(xr, Ui[q], fma(xi, Ur[q], ci));
+            }
+          }
+          if constexpr (GRAD) {
+            // d logL / d t_n = scale * Re sum_k D~_k mu_k e^{mu_k t} U~_k  (vjp.py:92-125)
+            const SB_CONST T* Er = bn + sb_bti_dEre(AP);
+            const SB_CONST T* Ei = bn + sb_bti_dEim(AP);
+            T s = T(0);
+#pragma unroll
+            for (int k = 0; k < AP; ++k) {
+              const T yr = fma(Dr[k], Er[k], -Di[k] * Ei[k]), yi = fma(Dr[k], Ei[k], Di[k] * Er[k]);
+              s = fma(yr, Ur[k], fma(-yi, Ui[k], s));
+            }
+            const double v = wave_sum(gc * double(s) * double(sc_raw));
+            if (lane == 0) gpart[n] = v;
+          }
         } else {
           T Ut[AP], Dt[AP];
 #pragma unroll
@@ -577,6 +703,44 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
           const T I = acc.c[i * AP + j] + acc.t2a[i] + pi[i] * (acc.t2b[j] + acc.t3);
           cnt[i * AP + j] = (i == j) ? I : fmu * pi[j] * I;
         }
+    } else if constexpr (IRR) {
+      // VCV = L C Rt (complex), one row of L C at a time; counts_ii = Re VCV_ii,
+      // counts_ij = Re(Q_ij VCV_ij)  (eigensub.py:388-434)
+      const SB_CONST T* P = md + sb_mri_set(AP, GRAD ? 1 : 0);
+      const SB_CONST T* Lr = P + sb_mri_L(AP);
+      const SB_CONST T* Li = Lr + AP * AP;
+      const SB_CONST T* Rr = P + sb_mri_Rt(AP);
+      const SB_CONST T* Ri = Rr + AP * AP;
+      const SB_CONST T* Wr = md + sb_mri_W(AP);
+      const SB_CONST T* Wi = Wr + AP * AP;
+#pragma unroll
+      for (int i = 0; i < AP; ++i) {
+        T tr[AP], ti[AP];
+#pragma unroll
+        for (int q = 0; q < AP; ++q) {
+          T sr = T(0), si = T(0);
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            const T lr = Lr[i * AP + k], li = Li[i * AP + k];
+            const T cr = acc.c[k * AP + q], ci = acc.c[AP * AP + k * AP + q];
+            sr = fma(lr, cr, fma(-li, ci, sr));
+            si = fma(lr, ci, fma(li, cr, si));
+          }
+          tr[q] = sr;
+          ti[q] = si;
+        }
+#pragma unroll
+        for (int j = 0; j < AP; ++j) {
+          T vr = T(0), vi = T(0);
+#pragma unroll
+          for (int q = 0; q < AP; ++q) {
+            const T rr = Rr[q * AP + j], ri = Ri[q * AP + j];
+            vr = fma(tr[q], rr, fma(-ti[q], ri, vr));
+            vi = fma(tr[q], ri, fma(ti[q], rr, vi));
+          }
+          cnt[i * AP + j] = (i == j) ? vr : fma(Wr[i * AP + j], vr, -Wi[i * AP + j] * vi);
+        }
+      }
     } else {
       const SB_CONST T* V = md + sb_mr_V(AP);
       const SB_CONST T* S = md + sb_mr_S(AP);
@@ -647,30 +811,33 @@ __global__ void sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
 }
 
 // Explicit instantiations and host-visible launch helpers.
-#define SB_INST(T, AP, F81, GRAD) \
-  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD>(const SbLaunch*);
-SB_INST(float, 4, false, false)
-SB_INST(float, 4, true, false)
-SB_INST(float, 4, false, true)
-SB_INST(float, 8, false, false)
-SB_INST(float, 8, true, false)
-SB_INST(float, 8, false, true)
-SB_INST(double, 4, false, false)
-SB_INST(double, 4, true, false)
-SB_INST(double, 4, false, true)
-SB_INST(double, 8, false, false)
-SB_INST(double, 8, true, false)
-SB_INST(double, 8, false, true)
+#define SB_INST(T, AP, F81, GRAD, IRR) \
+  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD, IRR>(const SbLaunch*);
+#define SB_INST_T(T, AP)                \
+  SB_INST(T, AP, false, false, false)   \
+  SB_INST(T, AP, true, false, false)    \
+  SB_INST(T, AP, false, true, false)    \
+  SB_INST(T, AP, false, false, true)    \
+  SB_INST(T, AP, false, true, true)
+SB_INST_T(float, 4)
+SB_INST_T(float, 8)
+SB_INST_T(double, 4)
+SB_INST_T(double, 8)
 template __global__ void sb_branch_params_kernel<float>(const SbLaunch*);
 template __global__ void sb_branch_params_kernel<double>(const SbLaunch*);
+template __global__ void sb_branch_params_irrev_kernel<float>(const SbLaunch*);
+template __global__ void sb_branch_params_irrev_kernel<double>(const SbLaunch*);
 
-static void (*pick_small(int dtype, int AP, bool f81, bool grad))(const SbLaunch*) {
+static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
-#define SB_PICK(T, APV)                                                          \
-  if (AP == APV) {                                                               \
-    if (grad) k = sb_fused_small_kernel<T, APV, false, true>;                    \
-    else if (f81) k = sb_fused_small_kernel<T, APV, true, false>;                \
-    else k = sb_fused_small_kernel<T, APV, false, false>;                        \
+  if (irr && f81) return nullptr;
+#define SB_PICK(T, APV)                                                               \
+  if (AP == APV) {                                                                    \
+    if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true>              \
+                      : sb_fused_small_kernel<T, APV, false, false, true>;            \
+    else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false>;             \
+    else if (f81) k = sb_fused_small_kernel<T, APV, true, false, false>;              \
+    else k = sb_fused_small_kernel<T, APV, false, false, false>;                      \
   }
   if (dtype == 0) { SB_PICK(float, 4) SB_PICK(float, 8) }
   else { SB_PICK(double, 4) SB_PICK(double, 8) }
@@ -679,23 +846,33 @@ static void (*pick_small(int dtype, int AP, bool f81, bool grad))(const SbLaunch
 }
 
 // Resident workgroups per CU for a block of `threads` threads (0 on error).
-extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad);
+extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
+                                        int threads) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr);
   if (!k) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess) return 0;
   return n;
 }
 
-extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
-                                            dim3 block, const SbLaunch* L, hipStream_t s) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad);
+extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
+                                            dim3 grid, dim3 block, const SbLaunch* L,
+                                            hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr);
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, grid, block, 0, s, L);
   return hipGetLastError();
 }
 
-extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s) {
+extern "C" hipError_t sb_launch_branch_params(int dtype, bool irr, int nwg, const SbLaunch* L,
+                                              hipStream_t s) {
+  if (irr) {
+    if (dtype == 0)
+      hipLaunchKernelGGL(sb_branch_params_irrev_kernel<float>, dim3(nwg), dim3(256), 0, s, L);
+    else
+      hipLaunchKernelGGL(sb_branch_params_irrev_kernel<double>, dim3(nwg), dim3(256), 0, s, L);
+    return hipGetLastError();
+  }
   if (dtype == 0)
     hipLaunchKernelGGL(sb_branch_params_kernel<float>, dim3(nwg), dim3(256), 0, s, L);
   else

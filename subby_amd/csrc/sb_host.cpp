@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -20,13 +21,15 @@
 #include "../../include/subby_amd.h"
 #include "sb_plan.h"
 
-extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, dim3 grid,
-                                            dim3 block, const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
+                                            dim3 grid, dim3 block, const SbLaunch* L,
+                                            hipStream_t s);
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81);
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
-extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, int threads);
+extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
+                                        int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
@@ -43,7 +46,8 @@ extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaun
 // Stage memory of the two-pass general path: tiles are processed in rounds
 // that fit this many bytes.
 static const size_t kGen2StageBudget = size_t(8) << 30;
-extern "C" hipError_t sb_launch_branch_params(int dtype, int nwg, const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_branch_params(int dtype, bool irr, int nwg, const SbLaunch* L,
+                                              hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
 struct SbPlainBuffers {
@@ -93,6 +97,7 @@ struct Model {
   bool alive = false;
   int A = 0, AP = 0;
   bool f81s = false;  // F81-structured: M(t) = e I + (1 - e) 1 pi^T
+  bool irr = false;   // irreversible (complex eigensystem) model
   std::vector<double> mu, V, pi;   // host copies (plain path)
   double* d64 = nullptr;  // FP64 master
   void* rec[2] = {nullptr, nullptr};  // float / double records
@@ -135,10 +140,10 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg;
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr;
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr;
   }
 };
 
@@ -337,7 +342,7 @@ int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_par
 int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves, int nwg_params,
             int nwg_reduce) {
   hipStream_t s = b->stream;
-  SB_HIP(sb_launch_branch_params(key.dtype, nwg_params, b->header, s));  // also zeroes the tile counter
+  SB_HIP(sb_launch_branch_params(key.dtype, key.irr != 0, nwg_params, b->header, s));  // also zeroes the tile counter
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if (b->timing) {
     if (!b->tfree.empty()) {
@@ -360,7 +365,8 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
                            b->header, s));
   else
-    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, grid, block, b->header, s));
+    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, grid, block,
+                                 b->header, s));
   if (b->timing) {
     SB_HIP(hipEventRecord(ev.second, s));
     b->tev.push_back(ev);
@@ -465,6 +471,15 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     if (in[i].flags & (SB_F_BCOUNTS | SB_F_NPOST | SB_F_BPOST)) post = true;
   if (post && (large || grad))
     return fail(SB_ERR_UNSUPPORTED, "per-branch outputs need an alphabet of at most 8 and no gradient");
+  const bool irr = mods[0]->irr;
+  for (int i = 1; i < nb; ++i)
+    if (mods[i]->irr != irr)
+      return fail(SB_ERR_UNSUPPORTED, "irreversible and reversible models cannot share a batch");
+  if (irr && f81) return fail(SB_ERR_UNSUPPORTED, "F81 fast path is reversible-only");
+  if (irr && large)
+    return fail(SB_ERR_UNSUPPORTED, "irreversible models need an alphabet of at most 8 states");
+  if (irr && post)
+    return fail(SB_ERR_UNSUPPORTED, "per-branch outputs are not implemented for irreversible models");
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
   bool f81w = large && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
@@ -505,7 +520,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
       b->header = static_cast<SbLaunch*>(b->plan.p);
     }
   }
-  const int bts = sb_bt_stride(AP);
+  const int bts = sb_bt_stride_k(AP, irr);
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
   if (rc) return rc;
 
@@ -517,7 +532,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     const int occ = post ? sb_post_occupancy(dtype, AP, f81)
                     : f81w ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad)
-                            : sb_fused_small_occupancy(dtype, AP, f81, grad, 64 * wpg);
+                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
   }
   int nwaves = ctx->num_cus * wpc;
@@ -658,7 +673,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   b->staged_pending = true;
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg};
+               nwaves, wpg, irr ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -755,6 +770,13 @@ int sb_internal_error(int code, const char* msg) {
   return code;
 }
 int sb_internal_device(sb_context* ctx) { return ctx ? ctx->device : -1; }
+// 1 for an irreversible model, 0 for a reversible one, -1 for an unknown id.
+int sb_internal_model_irr(sb_context* ctx, int model_id) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const Model* m = get_model(ctx, model_id);
+  return m ? (m->irr ? 1 : 0) : -1;
+}
 int sb_version(void) { return 1; }
 
 int sb_create(sb_context** out, int device) {
@@ -883,6 +905,123 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.mu.assign(eigenvalues, eigenvalues + A);
   m.V.assign(eigenvectors, eigenvectors + A * A);
   m.pi.assign(pi, pi + A);
+  const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
+               bd = align_up(rec.size() * 8, 256);
+  std::vector<char> blob(b64 + bf + bd, 0);
+  std::memcpy(blob.data(), m64.data(), m64.size() * 8);
+  std::memcpy(blob.data() + b64, recf.data(), recf.size() * 4);
+  std::memcpy(blob.data() + b64 + bf, rec.data(), rec.size() * 8);
+
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  char* base = nullptr;
+  int rc = arena_alloc(ctx, b64 + bf + bd, &base);
+  if (rc) return rc;
+  rc = stage_model_bytes(ctx, base, blob.data(), blob.size());
+  if (rc) return rc;
+  m.d64 = reinterpret_cast<double*>(base);
+  m.rec[0] = base + b64;
+  m.rec[1] = base + b64 + bf;
+  int id;
+  if (!ctx->free_ids.empty()) {
+    id = ctx->free_ids.back();
+    ctx->free_ids.pop_back();
+    ctx->models[id] = m;
+  } else {
+    id = int(ctx->models.size());
+    ctx->models.push_back(m);
+  }
+  *model_id = id;
+  return SB_OK;
+}
+
+int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues, const double* eigenvectors,
+                       const double* eigenvectors_inv, const double* pi, int exact, int* model_id) {
+  typedef std::complex<double> cd;
+  if (!ctx || !eigenvalues || !eigenvectors || !eigenvectors_inv || !pi || !model_id)
+    return fail(SB_ERR_ARG, "null argument");
+  const int AP = pad_alphabet(A);
+  if (A < 2 || AP < 0) return fail(SB_ERR_ALPHABET, "alphabet size %d outside 2..64", A);
+  double psum = 0.0;
+  for (int a = 0; a < A; ++a) {
+    if (!(pi[a] > 0.0) || !std::isfinite(pi[a])) return fail(SB_ERR_MODEL, "pi[%d]=%g must be > 0", a, pi[a]);
+    psum += pi[a];
+  }
+  if (!(psum < 1.0 + 1e-6 && psum > 1.0 - 1e-6)) return fail(SB_ERR_MODEL, "pi sums to %g, not 1", psum);
+  for (int i = 0; i < 2 * A; ++i)
+    if (!std::isfinite(eigenvalues[i])) return fail(SB_ERR_MODEL, "eigenvalue entry %d not finite", i);
+  for (int i = 0; i < 2 * A * A; ++i)
+    if (!std::isfinite(eigenvectors[i]) || !std::isfinite(eigenvectors_inv[i]))
+      return fail(SB_ERR_MODEL, "eigenvector entry %d not finite", i);
+  // padded complex matrices: V, V^{-1} extended by an identity block, mu by zeros
+  std::vector<cd> mu(AP), V(AP * AP), Vi(AP * AP);
+  for (int k = 0; k < AP; ++k) mu[k] = k < A ? cd(eigenvalues[2 * k], eigenvalues[2 * k + 1]) : cd(0.0);
+  for (int a = 0; a < AP; ++a)
+    for (int k = 0; k < AP; ++k) {
+      const bool in = a < A && k < A;
+      const int s = 2 * (a * A + k);
+      V[a * AP + k] = in ? cd(eigenvectors[s], eigenvectors[s + 1]) : cd(a == k ? 1.0 : 0.0);
+      Vi[a * AP + k] = in ? cd(eigenvectors_inv[s], eigenvectors_inv[s + 1]) : cd(a == k ? 1.0 : 0.0);
+    }
+  double err = 0.0;
+  for (int i = 0; i < A; ++i)
+    for (int j = 0; j < A; ++j) {
+      cd s = 0.0;
+      for (int k = 0; k < A; ++k) s += V[i * AP + k] * Vi[k * AP + j];
+      err = std::max(err, std::abs(s - cd(i == j ? 1.0 : 0.0)));
+    }
+  if (!(err < 1e-6)) return fail(SB_ERR_MODEL, "eigenvectors_inv is not the inverse (|V Vinv - I| = %g)", err);
+
+  std::vector<double> m64(sb_m64i_size(AP), 0.0);
+  for (int k = 0; k < AP; ++k) {
+    m64[sb_m64i_mu(AP) + k] = mu[k].real();
+    m64[sb_m64i_mu(AP) + AP + k] = mu[k].imag();
+  }
+  for (int i = 0; i < AP * AP; ++i) {
+    m64[sb_m64i_V(AP) + i] = V[i].real();
+    m64[sb_m64i_V(AP) + AP * AP + i] = V[i].imag();
+    m64[sb_m64i_Vi(AP) + i] = Vi[i].real();
+    m64[sb_m64i_Vi(AP) + AP * AP + i] = Vi[i].imag();
+  }
+  for (int a = 0; a < AP; ++a) m64[sb_m64i_pi(AP) + a] = a < A ? pi[a] : 1.0;
+  m64[sb_m64i_misc(AP) + 1] = A;
+
+  // record: set 0 (model's convention), set 1 (exact convention), W, pi, ones
+  std::vector<double> rec(sb_mri_size(AP), 0.0);
+  auto put = [&](int off, int i, int j, cd v) {
+    rec[off + i * AP + j] = v.real();
+    rec[off + AP * AP + i * AP + j] = v.imag();
+  };
+  for (int s = 0; s < 2; ++s) {
+    const bool ex = s == 1 || exact;
+    const int base = sb_mri_set(AP, s);
+    for (int i = 0; i < AP; ++i)
+      for (int j = 0; j < AP; ++j) {
+        const cd v = V[i * AP + j], vt = V[j * AP + i], w = Vi[i * AP + j], wt = Vi[j * AP + i];
+        put(base + sb_mri_Pu(AP), i, j, ex ? wt : v);   // Pu = V^{-T} | V
+        put(base + sb_mri_Pd(AP), i, j, ex ? v : wt);   // Pd = V | V^{-T}
+        put(base + sb_mri_L(AP), i, j, ex ? wt : v);    // L  = V^{-T} | V
+        put(base + sb_mri_Rt(AP), i, j, ex ? vt : w);   // Rt = V^T | V^{-1}
+      }
+  }
+  for (int i = 0; i < AP; ++i)
+    for (int j = 0; j < AP; ++j) {
+      cd q = 0.0;
+      if (i < A && j < A)
+        for (int k = 0; k < A; ++k) q += V[i * AP + k] * mu[k] * Vi[k * AP + j];
+      put(sb_mri_W(AP), i, j, i == j ? cd(1.0) : q);
+    }
+  for (int a = 0; a < AP; ++a) {
+    rec[sb_mri_pi(AP) + a] = a < A ? pi[a] : 0.0;
+    rec[sb_mri_ones(AP) + a] = a < A ? 1.0 : 0.0;
+  }
+  std::vector<float> recf(rec.begin(), rec.end());
+
+  Model m;
+  m.alive = true;
+  m.A = A;
+  m.AP = AP;
+  m.irr = true;
   const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
                bd = align_up(rec.size() * 8, 256);
   std::vector<char> blob(b64 + bf + bd, 0);
@@ -1053,7 +1192,8 @@ int sb_counts_plain(sb_context* ctx, int model_id, const int8_t* d_alignment, co
   SB_HIP(hipSetDevice(ctx->device));
   const Model* m = get_model(ctx, model_id);
   if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
-  if (m->A != 4) return fail(SB_ERR_UNSUPPORTED, "plain path is implemented for A = 4 only");
+  if (m->A != 4 || m->irr)
+    return fail(SB_ERR_UNSUPPORTED, "plain path is implemented for reversible A = 4 models only");
   TreePlan tp;
   int rc = plan_tree(parent_index, R, tp);
   if (rc) return rc;

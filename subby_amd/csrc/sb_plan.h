@@ -55,6 +55,54 @@ __host__ __device__ inline int sb_m64_pi(int AP) { return AP + AP * AP; }
 __host__ __device__ inline int sb_m64_misc(int AP) { return 2 * AP + AP * AP; }
 __host__ __device__ inline int sb_m64_size(int AP) { return 2 * AP + AP * AP + 4; }
 
+// ---- Irreversible models: Q = V diag(mu) V^{-1} with complex mu, V
+// (subby/jax/types.py:14-18).  Complex matrices are stored as a real plane
+// followed by an imaginary plane.
+//
+// Branch record (element T):  M[AP*AP] (real) | Jre[AP*AP] | Jim[AP*AP] |
+//   coef[4] = {0, 0, 0, t} | dEre[AP] | dEim[AP]   (dE_k = mu_k e^{mu_k t})
+__host__ __device__ inline int sb_bti_stride(int AP) { return 3 * AP * AP + 4 + 2 * AP; }
+__host__ __device__ inline int sb_bti_Jre(int AP) { return AP * AP; }
+__host__ __device__ inline int sb_bti_Jim(int AP) { return 2 * AP * AP; }
+__host__ __device__ inline int sb_bti_coef(int AP) { return 3 * AP * AP; }
+__host__ __device__ inline int sb_bti_dEre(int AP) { return 3 * AP * AP + 4; }
+__host__ __device__ inline int sb_bti_dEim(int AP) { return 3 * AP * AP + 4 + AP; }
+__host__ __device__ inline int sb_bt_stride_k(int AP, bool irr) {
+  return irr ? sb_bti_stride(AP) : sb_bt_stride(AP);
+}
+
+// Model record (element T) of an irreversible model: two sets of complex
+// matrices, set 0 for Counts in the model's convention and set 1 for the
+// exact convention (gradients), each
+//   Pu (AP x AP: U~_l = sum_b U_b Pu[b][l]) | Pd (D~_k = sum_a D_a Pd[a][k]) |
+//   L, Rt (VCV = L C Rt)
+// then W (counts_ij = Re(W_ij VCV_ij), W = Q off the diagonal, 1 on it),
+// pi[AP], ones[AP], misc[4].
+//   reference convention (eigensub.py:271-385): Pu = V, Pd = V^{-T}, L = V, Rt = V^{-1}
+//   exact convention:                           Pu = V^{-T}, Pd = V, L = V^{-T}, Rt = V^T
+__host__ __device__ inline int sb_mri_set(int AP, int s) { return s * 8 * AP * AP; }
+__host__ __device__ inline int sb_mri_Pu(int) { return 0; }
+__host__ __device__ inline int sb_mri_Pd(int AP) { return 2 * AP * AP; }
+__host__ __device__ inline int sb_mri_L(int AP) { return 4 * AP * AP; }
+__host__ __device__ inline int sb_mri_Rt(int AP) { return 6 * AP * AP; }
+__host__ __device__ inline int sb_mri_W(int AP) { return 16 * AP * AP; }
+__host__ __device__ inline int sb_mri_pi(int AP) { return 18 * AP * AP; }
+__host__ __device__ inline int sb_mri_ones(int AP) { return 18 * AP * AP + AP; }
+__host__ __device__ inline int sb_mri_misc(int AP) { return 18 * AP * AP + 2 * AP; }
+__host__ __device__ inline int sb_mri_size(int AP) { return 18 * AP * AP + 2 * AP + 4; }
+__host__ __device__ inline int sb_mr_pi_k(int AP, bool irr) { return irr ? sb_mri_pi(AP) : sb_mr_pi(AP); }
+__host__ __device__ inline int sb_mr_ones_k(int AP, bool irr) { return irr ? sb_mri_ones(AP) : sb_mr_ones(AP); }
+
+// FP64 master of an irreversible model:
+//   mu_re[AP] | mu_im[AP] | V_re, V_im [AP*AP] | Vinv_re, Vinv_im [AP*AP] | pi[AP] | misc[4]
+__host__ __device__ inline int sb_m64i_mu(int) { return 0; }
+__host__ __device__ inline int sb_m64i_V(int AP) { return 2 * AP; }
+__host__ __device__ inline int sb_m64i_Vi(int AP) { return 2 * AP + 2 * AP * AP; }
+__host__ __device__ inline int sb_m64i_pi(int AP) { return 2 * AP + 4 * AP * AP; }
+__host__ __device__ inline int sb_m64i_misc(int AP) { return 3 * AP + 4 * AP * AP; }
+__host__ __device__ inline int sb_m64i_size(int AP) { return 3 * AP + 4 * AP * AP + 4; }
+__host__ __device__ inline int sb_m64_pi_k(int AP, bool irr) { return irr ? sb_m64i_pi(AP) : sb_m64_pi(AP); }
+
 enum {
   SB_F_F81 = 1,
   SB_F_MASK_SHIFT = 1,     // 2 bits: 0 none, 1 auto, 2 explicit

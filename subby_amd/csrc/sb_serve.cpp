@@ -35,6 +35,7 @@
 
 extern "C" int sb_internal_error(int code, const char* msg);
 extern "C" int sb_internal_device(sb_context* ctx);
+extern "C" int sb_internal_model_irr(sb_context* ctx, int model_id);
 
 namespace {
 
@@ -391,6 +392,13 @@ int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
     it.r = reqs[i];
     const int rc = validate(it.r, &it.A);
     if (rc) return rc;
+    if (it.r.model_id >= 0 && sb_internal_model_irr(s->ctx, it.r.model_id) != 0) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf,
+                    "request tag %lld: unknown or irreversible model id %d (the server runs "
+                    "reversible models)", it.r.tag, it.r.model_id);
+      return sb_internal_error(SB_ERR_UNSUPPORTED, buf);
+    }
     it.in_bytes = align256(size_t(it.r.R) * size_t(it.r.C));
     size_t oc, ol, orr;
     it.out_bytes = out_layout(it, &oc, &ol, &orr);

@@ -24,6 +24,15 @@ class DiagModel(NamedTuple):
     pi: object
 
 
+class IrrevDiagModel(NamedTuple):
+    """subby/jax/types.py:14-18: Q = V diag(mu) V^{-1} with complex eigenvalues (A,),
+    right eigenvectors V[a,k] (A,A), V^{-1} (A,A) and a real root distribution pi (A,)."""
+    eigenvalues: object
+    eigenvectors: object
+    eigenvectors_inv: object
+    pi: object
+
+
 class RateModel(NamedTuple):
     """subby/jax/types.py:21-23: reversible rate matrix and root distribution."""
     subRate: object
@@ -114,10 +123,48 @@ def reversible_rate_matrix(exchangeabilities, pi) -> np.ndarray:
     return Q / -(pi @ np.diag(Q))
 
 
-def ensure_diag(model) -> DiagModel:
-    """RateModel -> DiagModel (subby/jax/__init__.py:44-50)."""
+def diagonalize_irreversible(subRate, rootProb) -> IrrevDiagModel:
+    """Q = V diag(mu) V^{-1} by eig, no symmetrisation (diagonalize.py:49-67)."""
+    Q = np.asarray(subRate, dtype=np.float64)
+    mu, V = np.linalg.eig(Q)
+    return IrrevDiagModel(mu.astype(np.complex128), V.astype(np.complex128),
+                          np.linalg.inv(V).astype(np.complex128),
+                          np.asarray(rootProb, dtype=np.float64))
+
+
+def check_detailed_balance(subRate, pi, tol=1e-10) -> bool:
+    """|pi_i Q_ij - pi_j Q_ji| < tol for all i, j (diagonalize.py:70-86)."""
+    Q = np.asarray(subRate, dtype=np.float64)
+    pi = np.asarray(pi, dtype=np.float64)
+    F = pi[:, None] * Q
+    return bool(np.all(np.abs(F - F.T) < tol))
+
+
+def irrev_model_from_rate_matrix(subRate, pi) -> IrrevDiagModel:
+    """Always the irreversible decomposition (models.py:234-248)."""
+    return diagonalize_irreversible(subRate, pi)
+
+
+def model_from_rate_matrix(subRate, pi, reversible=None, tol=1e-10):
+    """DiagModel when reversible (eigh of the symmetrised matrix), IrrevDiagModel
+    otherwise; reversible=None detects detailed balance (models.py:251-271)."""
+    if reversible is None:
+        reversible = check_detailed_balance(subRate, pi, tol)
+    if reversible:
+        return diagonalize_rate_matrix(subRate, pi)
+    return diagonalize_irreversible(subRate, pi)
+
+
+def ensure_diag(model):
+    """RateModel -> DiagModel (subby/jax/__init__.py:44-50); IrrevDiagModel passes
+    through with complex128 arrays."""
     if isinstance(model, RateModel):
         return diagonalize_rate_matrix(model.subRate, model.rootProb)
+    if isinstance(model, IrrevDiagModel):
+        return IrrevDiagModel(np.asarray(model.eigenvalues, dtype=np.complex128),
+                              np.asarray(model.eigenvectors, dtype=np.complex128),
+                              np.asarray(model.eigenvectors_inv, dtype=np.complex128),
+                              np.asarray(model.pi, dtype=np.float64))
     if isinstance(model, DiagModel):
         return DiagModel(np.asarray(model.eigenvalues, dtype=np.float64),
                          np.asarray(model.eigenvectors, dtype=np.float64),

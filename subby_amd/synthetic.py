@@ -11,7 +11,24 @@ from __future__ import annotations
 
 import numpy as np
 
-from .models import Tree, DiagModel
+from .models import Tree, DiagModel, IrrevDiagModel
+
+
+def _sub_matrices(model, t) -> np.ndarray:
+    """(R, A, A) M(t) of a DiagModel (symmetrised basis) or an IrrevDiagModel
+    (Re V e^{mu t} V^{-1}), clipped at 0 for sampling."""
+    t = np.asarray(t, dtype=np.float64)
+    if isinstance(model, IrrevDiagModel):
+        mu = np.asarray(model.eigenvalues, dtype=np.complex128)
+        V = np.asarray(model.eigenvectors, dtype=np.complex128)
+        Vi = np.asarray(model.eigenvectors_inv, dtype=np.complex128)
+        M = np.einsum('ak,rk,kb->rab', V, np.exp(mu[None, :] * t[:, None]), Vi).real
+    else:
+        mu, V, pi = (np.asarray(x, dtype=np.float64) for x in model)
+        s = np.sqrt(pi)
+        M = np.einsum('ak,rk,bk->rab', V, np.exp(mu[None, :] * t[:, None]), V) / s[None, :, None] \
+            * s[None, None, :]
+    return np.clip(M, 0.0, None)
 
 
 def random_tree(n_leaves: int, rng, mean_branch: float = 0.1) -> Tree:
@@ -50,24 +67,18 @@ def random_tree(n_leaves: int, rng, mean_branch: float = 0.1) -> Tree:
     return Tree(parent, dist.astype(np.float64))
 
 
-def sub_matrix(model: DiagModel, t: float) -> np.ndarray:
-    mu, V, pi = (np.asarray(x, dtype=np.float64) for x in model)
-    s = np.sqrt(pi)
-    M = (V * np.exp(mu * t)) @ V.T / s[:, None] * s[None, :]
-    return np.clip(M, 0.0, None)
+def sub_matrix(model, t: float) -> np.ndarray:
+    return _sub_matrices(model, [t])[0]
 
 
-def simulate(tree: Tree, model: DiagModel, C: int, rng, gap_frac: float = 0.0,
+def simulate(tree: Tree, model, C: int, rng, gap_frac: float = 0.0,
              ambig_frac: float = 0.0, dtype=np.int8) -> np.ndarray:
     """(R, C) tokens simulated down the tree; internal nodes get token A."""
     parent = np.asarray(tree.parentIndex)
     R = len(parent)
-    mu, V, pi = (np.asarray(x, dtype=np.float64) for x in model)
+    pi = np.asarray(model.pi, dtype=np.float64)
     A = len(pi)
-    s = np.sqrt(pi)
-    t = np.asarray(tree.distanceToParent, dtype=np.float64)
-    M = np.einsum('ak,rk,bk->rab', V, np.exp(mu[None, :] * t[:, None]), V) / s[None, :, None] * s[None, None, :]
-    M = np.clip(M, 0.0, None)
+    M = _sub_matrices(model, tree.distanceToParent)
     cum = np.cumsum(M / M.sum(2, keepdims=True), axis=2)
     cum[..., -1] = 1.0
     states = np.empty((R, C), dtype=np.int16)

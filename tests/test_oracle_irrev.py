@@ -182,3 +182,51 @@ def test_irrev_counts_real_and_nonnegative(seed):
         full = np.where(np.eye(4, dtype=bool)[..., None], VCV, O.rate_matrix_irrev(m)[..., None] * VCV)
         assert np.max(np.abs(full.imag)) < 1e-12 * max(1.0, np.max(np.abs(full.real)))
         assert O.Counts(aln, tree, m, branch_mask=None).min() >= -1e-10
+
+
+@pytest.mark.parametrize('seed', [0, 1])
+def test_irrev_branch_gradient_vs_finite_differences(seed):
+    """d logL / d t_n of an irreversible model (vjp.py:80-131: D projected with
+    V, U with V^{-1}, real part) against central differences of LogLike, whose
+    M(t) = expm(Q t) is pinned above."""
+    rng = np.random.default_rng(40 + seed)
+    A = 4
+    tree = _rand_tree(9, rng)
+    aln = rng.integers(-1, A + 2, size=(9, 20))
+    Q = _irrev_Q(A, rng)
+    pi = rng.dirichlet(np.ones(A) * 3)
+    m = O.diagonalize_irreversible(Q, pi)
+    assert np.max(np.abs(m.eigenvalues.imag)) > 0 or seed == 1
+    g = rng.normal(size=20)
+    grad, _ = O.loglike_branch_grad(aln, tree, m, g)
+    assert grad[0] == 0.0
+    h = 1e-5
+    for n in range(1, 9):
+        tp = tree.distanceToParent.copy()
+        tm = tp.copy()
+        tp[n] += h
+        tm[n] -= h
+        fd = (O.LogLike(aln, O.Tree(tree.parentIndex, tp), m)
+              - O.LogLike(aln, O.Tree(tree.parentIndex, tm), m)) @ g / (2 * h)
+        np.testing.assert_allclose(grad[n], fd, rtol=1e-7, atol=1e-9)
+
+
+def test_irrev_scaling_identity_links_gradient_and_exact_counts():
+    """sum_n t_n dlogL/dt_n = sum_{i!=j} N_ij + sum_i T_i Q_ii with the exact
+    (Q-integrand) unmasked counts; the reference convention does not satisfy it."""
+    rng = np.random.default_rng(77)
+    A = 4
+    tree = _rand_tree(11, rng)
+    aln = rng.integers(-1, A + 2, size=(11, 15))
+    Q = _irrev_Q(A, rng)
+    pi = rng.dirichlet(np.ones(A) * 3)
+    off = ~np.eye(A, dtype=bool)
+    grad, _ = O.loglike_branch_grad(aln, tree, O.diagonalize_irreversible(Q, pi), np.ones(15))
+    lhs = tree.distanceToParent @ grad
+    for exact, holds in ((True, True), (False, False)):
+        N = O.Counts(aln, tree, O.diagonalize_irreversible(Q, pi, exact=exact), branch_mask=None)
+        rhs = (np.where(off[..., None], N, 0).sum((0, 1)) + np.einsum('i,iic->c', np.diag(Q), N)).sum()
+        if holds:
+            np.testing.assert_allclose(lhs, rhs, rtol=1e-10)
+        else:
+            assert abs(lhs - rhs) > 1e-6 * abs(lhs)
