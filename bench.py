@@ -178,6 +178,65 @@ def other_workload(args, log):
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
                "config": {"workload": w, "A": A, "R": R, "C": C}, "roofline": roof,
                "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": len(cols)}}
+    elif w == 'irrev':
+        # irreversible A=4 model (complex eigensystem) on a block of the headline's shape
+        rng = np.random.default_rng(6000)
+        A = 4
+        Q = rng.uniform(0.05, 1.0, (A, A))
+        for a in range(A):
+            Q[a, (a + 1) % A] += 1.5           # rotational part: complex eigenvalue pair
+        np.fill_diagonal(Q, 0.0)
+        np.fill_diagonal(Q, -Q.sum(1))
+        pi = rng.dirichlet(np.full(A, 5.0))
+        Q /= -np.sum(pi * np.diag(Q))
+        m = sb.diagonalize_irreversible(Q, pi)
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        C = 200_000
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        R = len(t.parentIndex)
+        aln = torch.from_numpy(aln_np).to(dev)
+        L = Launcher(eng)
+        blk = Block(aln, t, m)
+        prep = L.prepare([blk])
+        L.launch(prep, use_graph=False)
+        torch.cuda.synchronize()
+        steps = max(5, args.steps)
+        for _ in range(3):
+            L.launch(prep, use_graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            L.launch(prep, use_graph=True)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        L.set_timing(True)
+        for _ in range(5):
+            L.launch(prep, use_graph=False)
+        kms, nl = L.kernel_ms()
+        kavg = kms / nl
+        cols = np.arange(0, C, max(1, C // 1000))[:1000]
+        om = O.IrrevModel(*(np.asarray(x) for x in m), False)
+        rc, rl, rr = O.counts_loglike_root(aln_np[:, cols], O.Tree(*t), om)
+        gc = blk.counts[:, :, cols].double().cpu().numpy()
+        err_c = float(np.max(np.abs(gc - rc).max((0, 1)) / np.abs(rc).max((0, 1))))
+        err_l = float(np.max(np.abs(blk.loglike[cols].cpu().numpy() - rl) / np.maximum(np.abs(rl), 1)))
+        err_r = float(np.abs(blk.root[:, cols].double().cpu().numpy() - rr).max())
+        balg = b_alg_counts(R, A, C)
+        units = C * (R - 1)
+        out = {"metric": "Counts column*branches/sec (FP32, irreversible A=4 model, complex eigenbasis, "
+                         "127-node tree, batched persistent path)",
+               "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
+               "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": "irrev", "A": A, "R": R, "C": C,
+                          "model": "random irreversible Q with a complex eigenvalue pair, "
+                                   "pi~Dir(5,5,5,5); counts in the reference convention"},
+               "roofline": {"bound": "hbm", "achieved": balg / (kavg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": balg / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "traffic": None, "alg_bytes_per_launch": balg,
+                            "kernel": "sb_fused_small_kernel<float,4,false,false,true>",
+                            "kernel_ms": kavg},
+               "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "root_abs": err_r,
+                          "columns_checked": len(cols)}}
     elif w == 'io':
         # per-branch outputs on the headline block: BranchCounts, then node + branch posteriors
         rng = np.random.default_rng(2000)
@@ -313,7 +372,7 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev'],
                     help='c2 (default, the headline line) or one of the other configurations: '
                          'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
                          'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
