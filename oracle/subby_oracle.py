@@ -731,6 +731,171 @@ def hky85_loglike_kappa_grad(alignment, tree, kappa, pi, g=None):
 # Reference data flow for CPU timing (subby/jax/pruning.py:41-56)
 # ---------------------------------------------------------------------------
 
+# ---------------------------------------------------------------------------
+# Rate categories, mixtures, per-column model lists and per-branch model grids
+# (subby/jax/__init__.py:52-204, 207-283, 338-411, 543-632, 653-678;
+#  subby/jax/models.py:126-186; subby/jax/mixture.py; oracle.py:208-262, 1334-1370, 1598-1616)
+# ---------------------------------------------------------------------------
+
+def gamma_rate_categories(alpha, K):
+    """Yang (1994) discretised gamma: medians of K equal-probability bins of
+    Gamma(alpha, 1/alpha) by 64 bisection steps on gammainc, normalised to mean
+    rate 1; uniform weights (oracle.py:208-241, models.py:126-158)."""
+    from scipy.special import gammainc
+    p = (np.arange(K) + 0.5) / K
+    lo = np.zeros(K)
+    hi = np.full(K, max(50.0, 10.0 / alpha))
+    for _ in range(64):
+        mid = 0.5 * (lo + hi)
+        below = gammainc(alpha, mid * alpha) < p
+        lo = np.where(below, mid, lo)
+        hi = np.where(below, hi, mid)
+    r = 0.5 * (lo + hi)
+    return r * K / r.sum(), np.full(K, 1.0 / K)
+
+
+def scale_model(model, rate_multiplier):
+    """Eigenvalues times a rate multiplier (oracle.py:244-262, models.py:161-186)."""
+    if is_irrev(model):
+        return model._replace(eigenvalues=model.eigenvalues * rate_multiplier)
+    return DiagModel(model.eigenvalues * rate_multiplier, model.eigenvectors, model.pi)
+
+
+def mixture_posterior(log_likes, log_weights):
+    """softmax_k(log_likes[k, c] + log_weights[k]) (mixture.py:6-30, oracle.py:1334-1365)."""
+    lj = np.asarray(log_likes, dtype=np.float64) + np.asarray(log_weights, dtype=np.float64)[:, None]
+    e = np.exp(lj - lj.max(0, keepdims=True))
+    return e / e.sum(0, keepdims=True)
+
+
+def MixturePosterior(alignment, tree, models, log_weights):
+    """(K, C) posterior over mixture components (__init__.py:653-678, oracle.py:1598-1616)."""
+    return mixture_posterior(np.stack([LogLike(alignment, tree, m) for m in models]), log_weights)
+
+
+# --- per-column model lists: model[c] for column c ---
+
+def LogLike_columns(alignment, tree, models):
+    """Per-column models, with the root distribution of models[0] for every
+    column, as the reference stacks the models and passes stacked.pi[0] to the
+    inside pass (__init__.py:264-274)."""
+    aln = np.asarray(alignment)
+    pi0 = models[0].pi
+    out = np.empty(aln.shape[1])
+    for c, m in enumerate(models):
+        out[c] = upward_pass(aln[:, c:c + 1], tree, compute_sub_matrices(m, tree.distanceToParent), pi0)[2][0]
+    return out
+
+
+def RootProb_columns(alignment, tree, models):
+    """q[:, c] = pi_c U0 exp(lnU0 - ll_c) with ll_c from LogLike_columns
+    (root distribution of models[0]) (__init__.py:617-632)."""
+    aln = np.asarray(alignment)
+    pi0 = models[0].pi
+    out = np.empty((len(pi0), aln.shape[1]))
+    for c, m in enumerate(models):
+        U, lnU, ll = upward_pass(aln[:, c:c + 1], tree, compute_sub_matrices(m, tree.distanceToParent), pi0)
+        out[:, c] = m.pi * U[0, 0] * np.exp(lnU[0, 0] - ll[0])
+    return out
+
+
+def _per_column_mask(branch_mask, c):
+    if branch_mask is None or isinstance(branch_mask, str):
+        return branch_mask
+    return np.asarray(branch_mask)[:, c:c + 1]
+
+
+def Counts_columns(alignment, tree, models, f81_fast=False, branch_mask="auto"):
+    """Column c is Counts of that column alone under models[c] (__init__.py:395-411)."""
+    aln = np.asarray(alignment)
+    return np.concatenate([Counts(aln[:, c:c + 1], tree, m, f81_fast, _per_column_mask(branch_mask, c))
+                           for c, m in enumerate(models)], axis=-1)
+
+
+def BranchCounts_columns(alignment, tree, models, f81_fast=False, branch_mask=None):
+    """Column c is BranchCounts of that column alone under models[c] (__init__.py:484-500)."""
+    aln = np.asarray(alignment)
+    return np.concatenate([BranchCounts(aln[:, c:c + 1], tree, m, f81_fast, _per_column_mask(branch_mask, c))
+                           for c, m in enumerate(models)], axis=-1)
+
+
+# --- per-branch models: model grid (1, R), models_row[n] for branch n, models_row[0] the root ---
+
+def sub_matrices_rows(models_row, distances):
+    """(R, A, A): M of branch n from its own model (__init__.py:73-99)."""
+    t = np.asarray(distances, dtype=np.float64)
+    return np.stack([compute_sub_matrices(m, t[n:n + 1])[0] for n, m in enumerate(models_row)])
+
+
+def LogLike_rows(alignment, tree, models_row):
+    """Inside pass with per-branch M and the root distribution of models_row[0]
+    (__init__.py:230-243)."""
+    return upward_pass(alignment, tree, sub_matrices_rows(models_row, tree.distanceToParent),
+                       models_row[0].pi)[2]
+
+
+def RootProb_rows(alignment, tree, models_row):
+    """(__init__.py:562-583)"""
+    U, lnU, ll = upward_pass(alignment, tree, sub_matrices_rows(models_row, tree.distanceToParent),
+                             models_row[0].pi)
+    return (models_row[0].pi[None, :] * U[0] * np.exp(lnU[0] - ll)[:, None]).T
+
+
+def BranchCounts_rows(alignment, tree, models_row, branch_mask=None):
+    """(R, A, A, C) per-branch counts, branch n projected, accumulated with its
+    own J and back-transformed in the eigenbasis of models_row[n]
+    (_branch_counts_per_row, __init__.py:102-144; eigensub.py:440-650)."""
+    M = sub_matrices_rows(models_row, tree.distanceToParent)
+    pi0 = models_row[0].pi
+    U, lnU, ll = upward_pass(alignment, tree, M, pi0)
+    D, lnD = downward_pass(U, lnU, tree, M, pi0, alignment)
+    mask = _resolve_mask(alignment, tree, models_row[0], branch_mask)
+    sc = _branch_scale_masked(lnU, lnD, ll, mask)
+    R, C, A = U.shape
+    t = np.asarray(tree.distanceToParent, dtype=np.float64)
+    out = np.zeros((R, A, A, C))
+    for n in range(1, R):
+        m = models_row[n]
+        sl = slice(n, n + 1)
+        if is_irrev(m):
+            J = compute_J_complex(m.eigenvalues, t[sl])
+            Ut, Dt = eigenbasis_project_irrev(U[sl], D[sl], m)
+            Cn = np.einsum('nck,nkl,ncl->nklc', Dt * sc[sl, :, None], J, Ut)
+            out[n] = back_transform_irrev_per_branch(Cn, m)[0]
+        else:
+            J = compute_J(m.eigenvalues, t[sl])
+            Ut, Dt = eigenbasis_project(U[sl], D[sl], m)
+            Cn = np.einsum('nck,nkl,ncl->nklc', Dt * sc[sl, :, None], J, Ut)
+            out[n] = back_transform_per_branch(Cn, m)[0]
+    return out
+
+
+def Counts_rows(alignment, tree, models_row, branch_mask="auto"):
+    """Sum over branches of the masked per-branch counts (__init__.py:362-377)."""
+    return BranchCounts_rows(alignment, tree, models_row, branch_mask).sum(0)
+
+
+# --- model grid (C, R): grid[c][n] ---
+
+def LogLike_grid(alignment, tree, grid):
+    """Column c: LogLike_rows of that column with grid[c] (__init__.py:244-263)."""
+    aln = np.asarray(alignment)
+    return np.concatenate([LogLike_rows(aln[:, c:c + 1], tree, row) for c, row in enumerate(grid)])
+
+
+def RootProb_grid(alignment, tree, grid):
+    """(__init__.py:584-615)"""
+    aln = np.asarray(alignment)
+    return np.concatenate([RootProb_rows(aln[:, c:c + 1], tree, row) for c, row in enumerate(grid)], axis=1)
+
+
+def Counts_grid(alignment, tree, grid, branch_mask="auto"):
+    """Column c: Counts of that column with the per-branch models grid[c] (__init__.py:378-392)."""
+    aln = np.asarray(alignment)
+    return np.concatenate([Counts_rows(aln[:, c:c + 1], tree, row, _per_column_mask(branch_mask, c))
+                           for c, row in enumerate(grid)], axis=-1)
+
+
 def counts_chunked(alignment, tree, model, chunk=128, f81_fast=False, branch_mask="auto"):
     """Counts with the inside pass run over 128-column chunks, as subby.jax does.
 
