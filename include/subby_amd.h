@@ -132,6 +132,44 @@ int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
                       int f81_fast, int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
                       void* d_node_post, void* d_branch_post, double* d_loglike, void* stream);
 
+/* Several models in one call: the reference's per-column model lists and
+ * model grids (subby/jax/__init__.py:52-204; LogLike :230-283, Counts
+ * :362-411, RootProb :562-632).  model_ids is a host table
+ * [n_cols][n_rows] of resident model ids, n_cols = 1 or C, n_rows = 1 or R:
+ *   (1, 1) one model (as sb_counts);
+ *   (C, 1) model_ids[c] for column c (model list): counts use each column's
+ *          own model; loglike and root use the root distribution of column
+ *          0's model, as the reference's stacked models do (:264-274, :617-632);
+ *   (1, R) model_ids[n] for branch n (model grid (1, R)); row 0 gives the root
+ *          distribution; counts are back-transformed per branch in each
+ *          branch's eigenbasis (:102-144);
+ *   (C, R) both (model grid (C, R)).
+ * All models share the alphabet and reversibility.  Per-branch models need
+ * A <= 8, no F81 path, and reversible models for counts (SB_ERR_UNSUPPORTED
+ * otherwise).  Outputs as sb_counts; any may be NULL, at least one given. */
+int sb_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* model_ids,
+                    const int8_t* d_alignment, const int32_t* parent_index, const double* distances,
+                    int R, int C, int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
+                    void* d_counts, void* d_root, double* d_loglike, void* stream);
+
+/* BranchCounts with a model table as sb_counts_multi (subby/jax/__init__.py:474-500):
+ * d_branch_counts T [R][A][A][C]; mask_mode as in sb_counts (the reference applies none). */
+int sb_branch_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* model_ids,
+                           const int8_t* d_alignment, const int32_t* parent_index,
+                           const double* distances, int R, int C, int dtype, int f81_fast,
+                           int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
+                           double* d_loglike, void* stream);
+
+/* MixturePosterior (subby/jax/__init__.py:653-678, mixture.py:6-30): the
+ * log-likelihoods of n_models (1..64) resident models over one alignment in
+ * one persistent launch, then d_post[k][c] = softmax_k(ll[k][c] + log_weights[k]).
+ * log_weights: host [n_models]; d_post f64 [n_models][C]; d_loglikes f64
+ * [n_models][C] or NULL. */
+int sb_mixture_posterior(sb_context* ctx, int n_models, const int* model_ids,
+                         const int8_t* d_alignment, const int32_t* parent_index,
+                         const double* distances, int R, int C, int dtype, const double* log_weights,
+                         double* d_post, double* d_loglikes, void* stream);
+
 /* Batched calls: many (tree, alignment-block) pairs in one persistent launch.
  * Mirrors jax.vmap(Counts) of scripts/batch_eigensub_maf.py:528-532 without
  * requiring equal tree sizes. */

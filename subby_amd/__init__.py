@@ -9,15 +9,16 @@ from .models import (Tree, DiagModel, IrrevDiagModel, RateModel, hky85_diag, hky
                      hky85_log_kappa_weights, jukes_cantor_model, f81_model,
                      diagonalize_rate_matrix, reversible_rate_matrix, ensure_diag,
                      diagonalize_irreversible, irrev_model_from_rate_matrix,
-                     model_from_rate_matrix, check_detailed_balance)
+                     model_from_rate_matrix, check_detailed_balance, gamma_rate_categories,
+                     scale_model)
 from ._lib import SubbyError
 from .api import (Engine, default_engine, LogLike, Counts, RootProb, loglike_grad, BranchCounts,
-                  InsideOutside)
+                  InsideOutside, MixturePosterior)
 from .grad import LogLikeCustomGrad, hky85_loglike
 
 __all__ = ['Tree', 'DiagModel', 'IrrevDiagModel', 'RateModel',
            'diagonalize_irreversible', 'irrev_model_from_rate_matrix', 'model_from_rate_matrix',
-           'check_detailed_balance', 'hky85_diag', 'hky85_rate_matrix',
+           'check_detailed_balance', 'gamma_rate_categories', 'scale_model', 'MixturePosterior', 'hky85_diag', 'hky85_rate_matrix',
            'hky85_log_kappa_weights', 'jukes_cantor_model', 'f81_model',
            'diagonalize_rate_matrix', 'reversible_rate_matrix', 'ensure_diag', 'SubbyError',
            'Engine', 'default_engine', 'LogLike', 'Counts', 'RootProb', 'loglike_grad',

@@ -44,9 +44,13 @@ class Engine:
         self.ctx = ctypes.c_void_p()
         _lib.check(self.lib.sb_create(ctypes.byref(self.ctx), device))
         self._models = {}
+        self._launchers = {}
         self._lock = threading.Lock()
 
     def close(self):
+        for L in getattr(self, '_launchers', {}).values():
+            L.close()
+        self._launchers = {}
         if self.ctx:
             self.lib.sb_destroy(self.ctx)
             self.ctx = ctypes.c_void_p()
@@ -88,6 +92,18 @@ class Engine:
                 mid = out.value
                 self._models[key] = mid
         return mid
+
+    def launcher(self):
+        """A batch launcher on the current stream (created once per stream)."""
+        from .batch import Launcher
+        st = torch.cuda.current_stream(self.device)
+        key = st.cuda_stream
+        with self._lock:
+            L = self._launchers.get(key)
+            if L is None:
+                L = Launcher(self, stream=st)
+                self._launchers[key] = L
+        return L
 
     def drop_models(self):
         with self._lock:
@@ -144,9 +160,115 @@ def _prep(alignment, tree, model, engine):
     return eng, m, A, aln, R, C, parent, dist
 
 
+_MODEL_TYPES = (Tree, DiagModel, IrrevDiagModel, RateModel)
+
+
+def _is_model_list(model) -> bool:
+    """A list of per-column models (subby/jax/__init__.py:52-55)."""
+    return isinstance(model, (list, tuple)) and len(model) > 0 and not isinstance(model, _MODEL_TYPES)
+
+
+def _is_model_grid(model) -> bool:
+    """A list of lists: grid[c][n] (subby/jax/__init__.py:58-70)."""
+    return _is_model_list(model) and isinstance(model[0], (list, tuple)) \
+        and not isinstance(model[0], _MODEL_TYPES)
+
+
+def _stacked(model):
+    """Models stacked along leading *H axes (e.g. scale_model with a vector of
+    rates) as (H shape, list of models), or None for a single model."""
+    if _is_model_list(model) or isinstance(model, dict):
+        return None
+    m = ensure_diag(model)
+    H = np.shape(m.eigenvalues)[:-1]
+    if not H:
+        return None
+    flat = [type(m)(*(np.asarray(x).reshape((-1,) + np.shape(x)[len(H):])[h] for x in m))
+            for h in range(int(np.prod(H)))]
+    return H, flat
+
+
+def _multi(alignment, tree, model, want, precision, engine, f81=False, branch_mask='auto',
+           irrev_convention='reference'):
+    """One sb_counts_multi / sb_branch_counts_multi call for a model list (C, 1)
+    or a model grid (1 or C, R).  ``want``: subset of {'counts', 'root', 'll', 'bcounts'}."""
+    eng = engine or default_engine()
+    exact = _exact(irrev_convention)
+    if _is_model_grid(model):
+        table = [list(row) for row in model]
+    else:
+        table = [[m] for m in model]
+    cache = {}
+
+    def mid(m):
+        k = id(m)
+        if k not in cache:
+            cache[k] = (m, eng.model_id(m, exact))
+        return cache[k][1]
+
+    ids = np.ascontiguousarray([[mid(m) for m in row] for row in table], dtype=np.int32)
+    m0 = ensure_diag(table[0][0])
+    A = len(m0.pi)
+    aln = eng.alignment(alignment, A)
+    R, C = aln.shape
+    n_cols, n_rows = ids.shape
+    if _is_model_grid(model):
+        if n_cols not in (1, C) or n_rows != R:
+            raise ValueError(f'model grid must be (1, {R}) or ({C}, {R}), got ({n_cols}, {n_rows})')
+    elif n_cols != C:
+        raise ValueError(f'model list has {n_cols} models for {C} columns')
+    parent, dist = eng.tree_arrays(tree, R)
+    dt, tdt = _PREC[precision]
+    mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
+    dev = eng.device
+    out = {}
+    if 'counts' in want:
+        out['counts'] = torch.empty((A, A, C), dtype=tdt, device=dev)
+    if 'root' in want:
+        out['root'] = torch.empty((A, C), dtype=tdt, device=dev)
+    if 'll' in want:
+        out['ll'] = torch.empty(C, dtype=torch.float64, device=dev)
+    if 'bcounts' in want:
+        out['bcounts'] = torch.empty((R, A, A, C), dtype=tdt, device=dev)
+    ptr = lambda k: out[k].data_ptr() if k in out else None  # noqa: E731
+    if 'bcounts' in want:
+        _lib.check(eng.lib.sb_branch_counts_multi(
+            eng.ctx, n_cols, n_rows, ids.ctypes.data, aln.data_ptr(), parent.ctypes.data,
+            dist.ctypes.data, R, C, dt, 1 if f81 else 0, mode, mptr, ptr('bcounts'), ptr('ll'),
+            _stream_ptr(dev)))
+    else:
+        _lib.check(eng.lib.sb_counts_multi(
+            eng.ctx, n_cols, n_rows, ids.ctypes.data, aln.data_ptr(), parent.ctypes.data,
+            dist.ctypes.data, R, C, dt, 1 if f81 else 0, mode, mptr, ptr('counts'), ptr('root'),
+            ptr('ll'), _stream_ptr(dev)))
+    return out
+
+
+def _stack_call(alignment, tree, stack, want, precision, engine, f81=False, branch_mask='auto'):
+    """(*H, ...) outputs of H stacked models over one alignment: one persistent
+    launch with one block per model, all reading the same device alignment."""
+    from .batch import Block
+    H, models = stack
+    eng = engine or default_engine()
+    A = len(ensure_diag(models[0]).pi)
+    aln = eng.alignment(alignment, A)
+    blocks = [Block(aln, tree, m, want=want, f81_fast=f81, branch_mask=branch_mask) for m in models]
+    eng.launcher().submit(blocks, precision=precision)
+    return {w: torch.stack([getattr(b, w) for b in blocks]).reshape(H + getattr(blocks[0], w).shape)
+            for w in want}
+
+
 def LogLike(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
             engine: Engine | None = None) -> torch.Tensor:
-    """Per-column log-likelihoods, (C,) float64 (subby/jax/__init__.py:207-291)."""
+    """Per-column log-likelihoods, (C,) float64 (subby/jax/__init__.py:207-291).
+    ``model``: one model, a list of C per-column models, a model grid
+    ([[m_0 .. m_{R-1}]] per branch, or C such rows), or models stacked along
+    leading axes (result (*H, C))."""
+    if _is_model_list(model):
+        return _multi(alignment, tree, model, {'ll'}, precision, engine)['ll']
+    st = _stacked(model)
+    if st is not None:
+        return _stack_call(alignment, tree, st, ('loglike',), precision, engine)['loglike']
     eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
     dt, _ = _PREC[precision]
     out = torch.empty(C, dtype=torch.float64, device=eng.device)
@@ -157,7 +279,13 @@ def LogLike(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'f
 
 def RootProb(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
              engine: Engine | None = None) -> torch.Tensor:
-    """Posterior root distribution, (A, C) (subby/jax/__init__.py:543-650)."""
+    """Posterior root distribution, (A, C) (subby/jax/__init__.py:543-650); model
+    lists, grids and stacked models as in LogLike."""
+    if _is_model_list(model):
+        return _multi(alignment, tree, model, {'root'}, precision, engine)['root']
+    st = _stacked(model)
+    if st is not None:
+        return _stack_call(alignment, tree, st, ('root',), precision, engine)['root']
     eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
     dt, tdt = _PREC[precision]
     out = torch.empty((A, C), dtype=tdt, device=eng.device)
@@ -194,7 +322,19 @@ def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool 
     the ungapped leaves), None, or an (R, C) bool array.  For an IrrevDiagModel
     (alphabets of at most 8 states) ``irrev_convention`` is "reference" (the
     reference's formulas, eigensub.py:271-434) or "exact" (expected counts
-    under Q itself)."""
+    under Q itself).  Model lists, grids and stacked models as in LogLike."""
+    if _is_model_list(model):
+        want = {'counts'} | ({'ll'} if return_loglike else set()) | ({'root'} if return_root else set())
+        out = _multi(alignment, tree, model, want, precision, engine, f81_fast_flag, branch_mask,
+                     irrev_convention)
+        res = [out['counts']] + ([out['ll']] if return_loglike else []) + ([out['root']] if return_root else [])
+        return tuple(res) if len(res) > 1 else res[0]
+    st = _stacked(model)
+    if st is not None:
+        want = ('counts',) + (('loglike',) if return_loglike else ()) + (('root',) if return_root else ())
+        out = _stack_call(alignment, tree, st, want, precision, engine, f81_fast_flag, branch_mask)
+        res = [out[w] for w in want]
+        return tuple(res) if len(res) > 1 else res[0]
     eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
     dt, tdt = _PREC[precision]
     mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
@@ -251,7 +391,10 @@ def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag:
     """Per-branch counts and dwell times, (R, A, A, C), branch 0 zeros
     (subby/jax/__init__.py:450-540).  subby.jax applies no branch mask here; the
     default ``branch_mask=None`` does the same ("auto" or an (R, C) array as in
-    Counts).  Alphabets of at most 8 states."""
+    Counts).  Alphabets of at most 8 states.  Model lists and grids as in LogLike."""
+    if _is_model_list(model):
+        return _multi(alignment, tree, model, {'bcounts'}, precision, engine, f81_fast_flag,
+                      branch_mask)['bcounts']
     return _inside_outside(alignment, tree, model, {'bcounts'}, f81_fast_flag, branch_mask,
                            precision, engine)['bcounts']
 
@@ -268,6 +411,9 @@ class InsideOutside:
     def __init__(self, alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
                  engine: Engine | None = None):
         self._eng = engine or default_engine()
+        if _is_model_list(model) or _stacked(model) is not None:
+            raise NotImplementedError('InsideOutside takes one model; use LogLike, Counts, BranchCounts '
+                                      'and RootProb for model lists, grids and stacked models')
         m = ensure_diag(model)
         self._args = (self._eng.alignment(alignment, len(m.pi)), tree, m)
         self._precision = precision
@@ -310,6 +456,35 @@ class InsideOutside:
         (> 0), or (R, A, A, C) for all branches (branch 0 zeros)."""
         self._posteriors()
         return self._bpost if node is None else self._bpost[node]
+
+
+def MixturePosterior(alignment, tree, models, log_weights, maxChunkSize: int = 128,
+                     precision: str = 'fp32', engine: Engine | None = None,
+                     return_loglikes: bool = False):
+    """(K, C) float64 posterior over mixture components per column,
+    softmax_k(LogLike_k + log_weights[k]) (subby/jax/__init__.py:653-678,
+    mixture.py:6-30).  ``models``: a list of K models or models stacked along
+    one leading axis (e.g. scale_model(base, gamma rates)); K <= 64."""
+    eng = engine or default_engine()
+    st = None if _is_model_list(models) else _stacked(models)
+    ms = st[1] if st is not None else list(models)
+    K = len(ms)
+    lw = np.ascontiguousarray(np.asarray(log_weights, dtype=np.float64).reshape(-1))
+    if lw.shape != (K,):
+        raise ValueError(f'log_weights must have shape ({K},), got {lw.shape}')
+    ids = np.ascontiguousarray([eng.model_id(m) for m in ms], dtype=np.int32)
+    A = len(ensure_diag(ms[0]).pi)
+    aln = eng.alignment(alignment, A)
+    R, C = aln.shape
+    parent, dist = eng.tree_arrays(tree, R)
+    dt, _ = _PREC[precision]
+    post = torch.empty((K, C), dtype=torch.float64, device=eng.device)
+    ll = torch.empty((K, C), dtype=torch.float64, device=eng.device) if return_loglikes else None
+    _lib.check(eng.lib.sb_mixture_posterior(
+        eng.ctx, K, ids.ctypes.data, aln.data_ptr(), parent.ctypes.data, dist.ctypes.data, R, C, dt,
+        lw.ctypes.data, post.data_ptr(), ll.data_ptr() if ll is not None else None,
+        _stream_ptr(eng.device)))
+    return (post, ll) if return_loglikes else post
 
 
 def loglike_grad(alignment, tree, model, g=None, rate_weights=None, precision: str = 'fp32',

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libsubby_amd.so')
-SOURCES = ['sb_host.cpp', 'sb_serve.cpp', 'sb_fused.hip', 'sb_large.hip', 'sb_f81w.hip', 'sb_gen2.hip', 'sb_post.hip', 'sb_plain.hip']
+SOURCES = ['sb_host.cpp', 'sb_serve.cpp', 'sb_fused.hip', 'sb_large.hip', 'sb_f81w.hip', 'sb_gen2.hip', 'sb_post.hip', 'sb_plain.hip', 'sb_multi.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [

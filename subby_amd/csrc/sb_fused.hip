@@ -380,7 +380,7 @@ __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *L->counter = 0;
 }
 
-template <typename T, int AP, bool F81, bool GRAD, bool IRR>
+template <typename T, int AP, bool F81, bool GRAD, bool IRR, bool ROWS>
 __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
   typedef Tile<T, AP> TL;
   const SbLaunch& args = *Lp;
@@ -659,6 +659,52 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
             const double v = wave_sum(gc * double(s) * double(sc_raw));
             if (lane == 0) gpart[n] = v;
           }
+        } else if constexpr (ROWS) {
+          // Per-branch models (_branch_counts_per_row, subby/jax/__init__.py:102-144):
+          // branch n is projected, accumulated and back-transformed in the
+          // eigenbasis of its own model, straight into natural-basis counts.
+          const SB_CONST T* mn = B->row_models
+              ? cst<T>((const void*)(cst<unsigned long long>(B->row_models)[n])) : md;
+          const SB_CONST T* Vsn = mn + sb_mr_Vs(AP);
+          const SB_CONST T* Vin = mn + sb_mr_Vi(AP);
+          const SB_CONST T* Vn = mn + sb_mr_V(AP);
+          const SB_CONST T* Sn = mn + sb_mr_S(AP);
+          T Ut[AP], Dt[AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            T su = T(0), sd = T(0);
+#pragma unroll
+            for (int b = 0; b < AP; ++b) {
+              su = fma(u[b], Vsn[b * AP + k], su);
+              sd = fma(d[b], Vin[b * AP + k], sd);
+            }
+            Ut[k] = su;
+            Dt[k] = sd * sc;
+          }
+          const SB_CONST T* J = bn + sb_bt_J(AP);
+          T Ce[AP * AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k)
+#pragma unroll
+            for (int q = 0; q < AP; ++q) Ce[k * AP + q] = Dt[k] * J[k * AP + q] * Ut[q];
+#pragma unroll
+          for (int i = 0; i < AP; ++i) {
+            T tmp[AP];
+#pragma unroll
+            for (int q = 0; q < AP; ++q) {
+              T v = T(0);
+#pragma unroll
+              for (int k = 0; k < AP; ++k) v = fma(Vn[i * AP + k], Ce[k * AP + q], v);
+              tmp[q] = v;
+            }
+#pragma unroll
+            for (int j = 0; j < AP; ++j) {
+              T v = T(0);
+#pragma unroll
+              for (int q = 0; q < AP; ++q) v = fma(tmp[q], Vn[j * AP + q], v);
+              acc.c[i * AP + j] = (i == j) ? acc.c[i * AP + j] + v : fma(Sn[i * AP + j], v, acc.c[i * AP + j]);
+            }
+          }
         } else {
           T Ut[AP], Dt[AP];
 #pragma unroll
@@ -694,7 +740,10 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
 
     // ---------------- epilogue: natural-basis counts ----------------
     T cnt[AP * AP];
-    if constexpr (F81) {
+    if constexpr (ROWS) {
+#pragma unroll
+      for (int i = 0; i < AP * AP; ++i) cnt[i] = acc.c[i];
+    } else if constexpr (F81) {
       const T fmu = md[sb_mr_misc(AP)];
 #pragma unroll
       for (int i = 0; i < AP; ++i)
@@ -811,14 +860,15 @@ __global__ void sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
 }
 
 // Explicit instantiations and host-visible launch helpers.
-#define SB_INST(T, AP, F81, GRAD, IRR) \
-  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD, IRR>(const SbLaunch*);
-#define SB_INST_T(T, AP)                \
-  SB_INST(T, AP, false, false, false)   \
-  SB_INST(T, AP, true, false, false)    \
-  SB_INST(T, AP, false, true, false)    \
-  SB_INST(T, AP, false, false, true)    \
-  SB_INST(T, AP, false, true, true)
+#define SB_INST(T, AP, F81, GRAD, IRR, ROWS) \
+  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD, IRR, ROWS>(const SbLaunch*);
+#define SB_INST_T(T, AP)                       \
+  SB_INST(T, AP, false, false, false, false)   \
+  SB_INST(T, AP, true, false, false, false)    \
+  SB_INST(T, AP, false, true, false, false)    \
+  SB_INST(T, AP, false, false, true, false)    \
+  SB_INST(T, AP, false, true, true, false)     \
+  SB_INST(T, AP, false, false, false, true)
 SB_INST_T(float, 4)
 SB_INST_T(float, 8)
 SB_INST_T(double, 4)
@@ -828,16 +878,18 @@ template __global__ void sb_branch_params_kernel<double>(const SbLaunch*);
 template __global__ void sb_branch_params_irrev_kernel<float>(const SbLaunch*);
 template __global__ void sb_branch_params_irrev_kernel<double>(const SbLaunch*);
 
-static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr))(const SbLaunch*) {
+static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr,
+                         bool rows))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
-  if (irr && f81) return nullptr;
-#define SB_PICK(T, APV)                                                               \
-  if (AP == APV) {                                                                    \
-    if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true>              \
-                      : sb_fused_small_kernel<T, APV, false, false, true>;            \
-    else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false>;             \
-    else if (f81) k = sb_fused_small_kernel<T, APV, true, false, false>;              \
-    else k = sb_fused_small_kernel<T, APV, false, false, false>;                      \
+  if ((irr && f81) || (rows && (irr || f81 || grad))) return nullptr;
+#define SB_PICK(T, APV)                                                                   \
+  if (AP == APV) {                                                                        \
+    if (rows) k = sb_fused_small_kernel<T, APV, false, false, false, true>;               \
+    else if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true, false>      \
+                           : sb_fused_small_kernel<T, APV, false, false, true, false>;    \
+    else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false, false>;          \
+    else if (f81) k = sb_fused_small_kernel<T, APV, true, false, false, false>;           \
+    else k = sb_fused_small_kernel<T, APV, false, false, false, false>;                   \
   }
   if (dtype == 0) { SB_PICK(float, 4) SB_PICK(float, 8) }
   else { SB_PICK(double, 4) SB_PICK(double, 8) }
@@ -847,8 +899,8 @@ static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr))(cons
 
 // Resident workgroups per CU for a block of `threads` threads (0 on error).
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        int threads) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr);
+                                        bool rows, int threads) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
   if (!k) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess) return 0;
@@ -856,9 +908,9 @@ extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, 
 }
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            dim3 grid, dim3 block, const SbLaunch* L,
+                                            bool rows, dim3 grid, dim3 block, const SbLaunch* L,
                                             hipStream_t s) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr);
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, grid, block, 0, s, L);
   return hipGetLastError();

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -22,14 +23,23 @@
 #include "sb_plan.h"
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            dim3 grid, dim3 block, const SbLaunch* L,
+                                            bool rows, dim3 grid, dim3 block, const SbLaunch* L,
                                             hipStream_t s);
+extern "C" hipError_t sb_launch_gather_cols(const void* src, void* dst, int esz, int rows, int C,
+                                            const int* const* cols, hipStream_t s);
+extern "C" hipError_t sb_launch_scatter_cols(const void* src, void* dst, int esz, int rows, int C,
+                                             const int* const* cols, hipStream_t s);
+extern "C" hipError_t sb_launch_prior_swap(double* ll, void* root, int dtype, int A, int C,
+                                           const int* const* cols, const double* w, const int* swap,
+                                           hipStream_t s);
+extern "C" hipError_t sb_launch_mixture_softmax(const double* ll, double* post, int K, int C,
+                                                const double* logw, hipStream_t s);
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81);
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        int threads);
+                                        bool rows, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
@@ -140,10 +150,10 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg, irr;
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows;
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows;
   }
 };
 
@@ -152,6 +162,12 @@ struct sb_batch {
   hipStream_t stream = nullptr;
   SbLaunch* header = nullptr;       // start of the plan buffer (fixed until it grows)
   DevBuf plan, bt, scratch, part, stage;
+  // calls with several models: gathered alignment / mask, staged outputs,
+  // column maps (sb_counts_multi, sb_inside_outside_multi, sb_mixture_posterior)
+  DevBuf xaln, xmask, xout, xcols;
+  std::vector<char> xhost;          // host side of xcols; rewritten after xev
+  hipEvent_t xev = nullptr;
+  bool xpending = false;
   int* counter = nullptr;
   char* host = nullptr;             // pinned staging: header + plan
   size_t host_cap = 0;
@@ -334,6 +350,7 @@ struct BlockIn {
   void* bcounts;              // per-branch outputs (sb_inside_outside)
   void* npost;
   void* bpost;
+  const Model* const* rows;   // [R] per-branch models (row 0: the root distribution) or null
 };
 
 int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
@@ -365,8 +382,8 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
                            b->header, s));
   else
-    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, grid, block,
-                                 b->header, s));
+    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0, grid,
+                                 block, b->header, s));
   if (b->timing) {
     SB_HIP(hipEventRecord(ev.second, s));
     b->tev.push_back(ev);
@@ -423,7 +440,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   std::vector<TreePlan> plans(nb);
   std::vector<const Model*> mods(nb);
   int nint_max = 0, R_max = 1;
-  size_t nsched = 0, njobs = 0, ntiles = 0, nrate = 0;
+  size_t nsched = 0, njobs = 0, ntiles = 0, nrate = 0, nrowm = 0;
+  bool any_rows = false;
   for (int i = 0; i < nb; ++i) {
     const sb_block& k = in[i].blk;
     const Model* m = get_model(ctx, k.model_id);
@@ -450,6 +468,17 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
       return fail(rc, "block %d: %s", i, msg.c_str());
     }
     mods[i] = m;
+    if (in[i].rows) {
+      for (int n = 0; n < k.R; ++n) {
+        const Model* r = in[i].rows[n];
+        if (!r) return fail(SB_ERR_MODEL, "block %d: missing model of branch %d", i, n);
+        if (r->AP != AP || r->irr != m->irr)
+          return fail(SB_ERR_UNSUPPORTED, "block %d: per-branch models must share the alphabet and "
+                      "reversibility", i);
+      }
+      nrowm += k.R;
+      any_rows = true;
+    }
     nint_max = std::max(nint_max, plans[i].nint);
     R_max = std::max(R_max, k.R);
     nsched += plans[i].recs.size();
@@ -480,6 +509,17 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     return fail(SB_ERR_UNSUPPORTED, "irreversible models need an alphabet of at most 8 states");
   if (irr && post)
     return fail(SB_ERR_UNSUPPORTED, "per-branch outputs are not implemented for irreversible models");
+  // per-branch models: counts are back-transformed branch by branch (fused ROWS variant)
+  bool rowsk = false;
+  if (any_rows) {
+    if (large || grad || f81)
+      return fail(SB_ERR_UNSUPPORTED, "per-branch models need an alphabet of at most 8, no gradient and "
+                  "no F81 path");
+    for (int i = 0; i < nb; ++i)
+      if (in[i].flags & SB_F_COUNTS) rowsk = true;
+    if (irr && (rowsk || post))
+      return fail(SB_ERR_UNSUPPORTED, "counts with per-branch irreversible models are not implemented");
+  }
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
   bool f81w = large && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
@@ -505,6 +545,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   off = align_up(off + sizeof(SbBranchJob) * njobs, 256);
   const size_t o_rate = off;
   off = align_up(off + sizeof(double) * nrate, 256);
+  const size_t o_rowm = off;
+  off = align_up(off + sizeof(void*) * nrowm, 256);
   const size_t total = off;
 
   int rc = ensure_host(b, total);
@@ -532,7 +574,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     const int occ = post ? sb_post_occupancy(dtype, AP, f81)
                     : f81w ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad)
-                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, 64 * wpg);
+                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
   }
   int nwaves = ctx->num_cus * wpc;
@@ -583,8 +625,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   int2* ht = reinterpret_cast<int2*>(h + o_tiles);
   SbBranchJob* hj = reinterpret_cast<SbBranchJob*>(h + o_jobs);
   double* hr = reinterpret_cast<double*>(h + o_rate);
+  const void** hm = reinterpret_cast<const void**>(h + o_rowm);
 
-  size_t ps = 0, pj = 0, pr = 0, tb = 0;
+  size_t ps = 0, pj = 0, pr = 0, tb = 0, pm = 0;
   std::vector<std::pair<int, int>> order;  // (nint, block) for tile ordering
   for (int i = 0; i < nb; ++i) {
     const sb_block& k = in[i].blk;
@@ -621,10 +664,15 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     std::memcpy(hs + ps, plans[i].recs.data(), plans[i].recs.size() * sizeof(int));
     ps += plans[i].recs.size();
     for (int n = 0; n < k.R; ++n) {
-      hj[pj + n].model64 = m->d64;
+      hj[pj + n].model64 = in[i].rows ? in[i].rows[n]->d64 : m->d64;
       hj[pj + n].t = k.distances[n];
     }
     pj += k.R;
+    if (in[i].rows) {
+      d.row_models = reinterpret_cast<const void* const*>(dev(o_rowm) + pm * sizeof(void*));
+      for (int n = 0; n < k.R; ++n) hm[pm + n] = in[i].rows[n]->rec[dtype];
+      pm += k.R;
+    }
     for (int q = 0; q < in[i].n_rate; ++q)
       for (int a = 0; a < AP; ++a)
         for (int c = 0; c < AP; ++c)
@@ -673,7 +721,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   b->staged_pending = true;
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg, irr ? 1 : 0};
+               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -715,8 +763,10 @@ void batch_free(sb_batch* b) {
       (void)hipEventDestroy(e.first);
       (void)hipEventDestroy(e.second);
     }
-  for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage})
+  for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask, &b->xout,
+                    &b->xcols})
     if (d->p) (void)hipFree(d->p);
+  if (b->xev) (void)hipEventDestroy(b->xev);
   if (b->counter) (void)hipFree(b->counter);
   if (b->host) (void)hipHostFree(b->host);
   if (b->staged) (void)hipEventDestroy(b->staged);
@@ -755,6 +805,220 @@ BlockIn make_block(int model_id, const int8_t* aln, const int32_t* parent, const
   bi.blk.parent_index = parent;
   bi.blk.distances = dist;
   return bi;
+}
+
+// ---- calls with several models (model lists, per-branch models, grids, mixtures)
+
+struct MultiOut {
+  void* counts = nullptr;
+  void* root = nullptr;
+  double* ll = nullptr;
+  void* bcounts = nullptr;
+};
+
+// Launcher of a context for a call on `stream` (the context lock is held).
+int context_launcher(sb_context* ctx, void* stream, sb_batch** out) {
+  SB_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!ctx->internal) {
+    int rc = batch_create(ctx, s, &ctx->internal);
+    if (rc) return rc;
+  }
+  sb_batch* b = ctx->internal;
+  if (b->stream != s) {
+    SB_HIP(hipStreamWaitEvent(s, b->done, 0));
+    b->stream = s;
+  }
+  *out = b;
+  return SB_OK;
+}
+
+// Host side of the column maps and per-group tables, uploaded to b->xcols.
+int upload_cols(sb_batch* b, const std::vector<int>& ints, const std::vector<double>& dbl,
+                size_t* dbl_off) {
+  if (b->xpending) {
+    SB_HIP(hipEventSynchronize(b->xev));
+    b->xpending = false;
+  }
+  const size_t ib = align_up(ints.size() * sizeof(int), 256);
+  const size_t bytes = ib + dbl.size() * sizeof(double);
+  int rc = ensure(b, b->xcols, std::max<size_t>(bytes, 256));
+  if (rc) return rc;
+  b->xhost.resize(bytes);
+  std::memcpy(b->xhost.data(), ints.data(), ints.size() * sizeof(int));
+  if (!dbl.empty()) std::memcpy(b->xhost.data() + ib, dbl.data(), dbl.size() * sizeof(double));
+  SB_HIP(hipMemcpyAsync(b->xcols.p, b->xhost.data(), bytes, hipMemcpyHostToDevice, b->stream));
+  if (!b->xev) SB_HIP(hipEventCreateWithFlags(&b->xev, hipEventDisableTiming));
+  SB_HIP(hipEventRecord(b->xev, b->stream));
+  b->xpending = true;
+  *dbl_off = ib;
+  return SB_OK;
+}
+
+// Model table ids[n_cols][n_rows] (n_cols in {1, C}, n_rows in {1, R}):
+//   (1, 1) one model; (C, 1) one model per column (model list);
+//   (1, R) one model per branch (row 0: root distribution); (C, R) both.
+// Columns with the same model row form one block; with more than one group the
+// columns are gathered into staging blocks and the outputs scattered back.
+int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t* aln,
+          const int32_t* parent, const double* dist, int R, int C, int dtype, int f81, int mask_mode,
+          const uint8_t* d_mask, const MultiOut& o, void* stream) {
+  if (!ctx || !ids || !parent || !dist) return fail(SB_ERR_ARG, "null argument");
+  if (R < 1 || C < 0) return fail(SB_ERR_ARG, "bad sizes R=%d C=%d", R, C);
+  if (!(n_cols == 1 || n_cols == C) || !(n_rows == 1 || n_rows == R))
+    return fail(SB_ERR_ARG, "model table must be (1 or C) x (1 or R), got %d x %d", n_cols, n_rows);
+  if (dtype != SB_F32 && dtype != SB_F64) return fail(SB_ERR_ARG, "dtype must be SB_F32 or SB_F64");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  std::vector<const Model*> tab(size_t(n_cols) * n_rows);
+  for (size_t i = 0; i < tab.size(); ++i) {
+    tab[i] = get_model(ctx, ids[i]);
+    if (!tab[i]) return fail(SB_ERR_MODEL, "unknown model id %d (table entry %zu)", ids[i], i);
+    if (tab[i]->A != tab[0]->A || tab[i]->irr != tab[0]->irr)
+      return fail(SB_ERR_UNSUPPORTED, "all models of one call must share the alphabet and reversibility");
+  }
+  const int A = tab[0]->A;
+  sb_batch* b = nullptr;
+  int rc = context_launcher(ctx, stream, &b);
+  if (rc) return rc;
+  const int flags = (o.counts ? SB_F_COUNTS : 0) | (o.root ? SB_F_ROOT : 0) | (o.ll ? SB_F_LL : 0) |
+                    (o.bcounts ? SB_F_BCOUNTS : 0);
+  if (!flags && C > 0) return fail(SB_ERR_ARG, "no output requested");
+  const bool per_branch = n_rows == R && R > 1;
+
+  // ---- group columns by model row (first appearance order)
+  std::vector<int> grp_of(C, 0), first_col;
+  if (n_cols == 1 || C == 0) {
+    first_col.push_back(0);
+  } else {
+    std::map<std::vector<int>, int> key;
+    for (int c = 0; c < C; ++c) {
+      std::vector<int> k(ids + size_t(c) * n_rows, ids + size_t(c + 1) * n_rows);
+      auto it = key.find(k);
+      if (it == key.end()) {
+        it = key.emplace(k, int(first_col.size())).first;
+        first_col.push_back(c);
+      }
+      grp_of[c] = it->second;
+    }
+  }
+  const int G = int(first_col.size());
+  auto row_of = [&](int g) { return &tab[size_t(n_cols == 1 ? 0 : first_col[g]) * n_rows]; };
+  auto id_of = [&](int g) { return ids[size_t(n_cols == 1 ? 0 : first_col[g]) * n_rows]; };
+  auto block_for = [&](int g) {
+    BlockIn bi = make_block(id_of(g), aln, parent, dist, R, C);
+    bi.flags = flags;
+    bi.blk.f81_fast = f81 ? 1 : 0;
+    bi.blk.mask_mode = mask_mode;
+    bi.blk.d_mask = d_mask;
+    bi.rows = per_branch ? row_of(g) : nullptr;
+    return bi;
+  };
+  if (G == 1) {   // one model row for every column: straight on the caller's arrays
+    BlockIn bi = block_for(0);
+    bi.blk.d_counts = o.counts;
+    bi.blk.d_root = o.root;
+    bi.blk.d_loglike = o.ll;
+    bi.bcounts = o.bcounts;
+    std::vector<BlockIn> v(1, bi);
+    return submit(b, v, dtype, 0);
+  }
+
+  // ---- model lists: LogLike / RootProb take the root distribution of the first
+  // column's model (the reference stacks the models and passes stacked.pi[0])
+  std::vector<int> swap(G, 0);
+  std::vector<double> w(size_t(G) * A, 1.0);
+  bool any_swap = false;
+  if (!per_branch && (o.ll || o.root)) {
+    const Model* m0 = tab[0];
+    for (int g = 0; g < G; ++g) {
+      const Model* mg = row_of(g)[0];
+      for (int a = 0; a < A; ++a) {
+        w[size_t(g) * A + a] = m0->pi[a] / mg->pi[a];
+        if (m0->pi[a] != mg->pi[a]) swap[g] = 1;
+      }
+      any_swap = any_swap || swap[g];
+    }
+  }
+
+  // ---- column maps: permuted order = groups one after another
+  std::vector<int> gsize(G, 0), gstart(G, 0);
+  for (int c = 0; c < C; ++c) ++gsize[grp_of[c]];
+  for (int g = 1; g < G; ++g) gstart[g] = gstart[g - 1] + gsize[g - 1];
+  std::vector<int> ints(size_t(2) * C + 3 * G);
+  int* perm = ints.data();
+  int* grp = perm + C;
+  {
+    std::vector<int> fill(gstart);
+    for (int c = 0; c < C; ++c) {
+      const int j = fill[grp_of[c]]++;
+      perm[j] = c;
+      grp[j] = grp_of[c];
+    }
+  }
+  std::memcpy(grp + C, gstart.data(), G * sizeof(int));
+  std::memcpy(grp + C + G, gsize.data(), G * sizeof(int));
+  std::memcpy(grp + C + 2 * G, swap.data(), G * sizeof(int));
+  size_t w_off = 0;
+  rc = upload_cols(b, ints, any_swap ? w : std::vector<double>(), &w_off);
+  if (rc) return rc;
+  const int* dcols = static_cast<const int*>(b->xcols.p);
+  const int* cols[4] = {dcols, dcols + C, dcols + 2 * C, dcols + 2 * C + G};
+  const int* dswap = dcols + 2 * C + 2 * G;
+  const double* dw = reinterpret_cast<const double*>(static_cast<char*>(b->xcols.p) + w_off);
+
+  // ---- gather the alignment (and an explicit mask) into per-group blocks
+  hipStream_t s = b->stream;
+  rc = ensure(b, b->xaln, std::max<size_t>(size_t(R) * C, 256));
+  if (rc) return rc;
+  SB_HIP(sb_launch_gather_cols(aln, b->xaln.p, 1, R, C, cols, s));
+  const bool emask = mask_mode == SB_MASK_EXPLICIT;
+  if (emask) {
+    if (!d_mask) return fail(SB_ERR_ARG, "explicit mask without mask buffer");
+    rc = ensure(b, b->xmask, std::max<size_t>(size_t(R) * C, 256));
+    if (rc) return rc;
+    SB_HIP(sb_launch_gather_cols(d_mask, b->xmask.p, 1, R, C, cols, s));
+  }
+  // ---- staged outputs
+  const size_t esz = dtype == SB_F32 ? 4 : 8;
+  const bool want_root = o.root || any_swap;
+  size_t off = 0;
+  auto carve = [&](bool on, size_t bytes) {
+    const size_t at = off;
+    if (on) off = align_up(off + bytes, 256);
+    return at;
+  };
+  const size_t x_cnt = carve(o.counts, size_t(A) * A * C * esz);
+  const size_t x_root = carve(want_root, size_t(A) * C * esz);
+  const size_t x_ll = carve(o.ll, size_t(C) * 8);
+  const size_t x_bc = carve(o.bcounts, size_t(R) * A * A * C * esz);
+  rc = ensure(b, b->xout, std::max<size_t>(off, 256));
+  if (rc) return rc;
+  char* X = static_cast<char*>(b->xout.p);
+  std::vector<BlockIn> blocks(G);
+  for (int g = 0; g < G; ++g) {
+    BlockIn bi = block_for(g);
+    const size_t st = size_t(gstart[g]);
+    bi.flags |= any_swap && !o.root ? SB_F_ROOT : 0;
+    bi.blk.C = gsize[g];
+    bi.blk.d_alignment = static_cast<const int8_t*>(b->xaln.p) + size_t(R) * st;
+    bi.blk.d_mask = emask ? static_cast<const uint8_t*>(b->xmask.p) + size_t(R) * st : nullptr;
+    bi.blk.d_counts = o.counts ? X + x_cnt + size_t(A) * A * st * esz : nullptr;
+    bi.blk.d_root = want_root ? X + x_root + size_t(A) * st * esz : nullptr;
+    bi.blk.d_loglike = o.ll ? reinterpret_cast<double*>(X + x_ll) + st : nullptr;
+    bi.bcounts = o.bcounts ? X + x_bc + size_t(R) * A * A * st * esz : nullptr;
+    blocks[g] = bi;
+  }
+  rc = submit(b, blocks, dtype, 0);
+  if (rc) return rc;
+  if (any_swap)
+    SB_HIP(sb_launch_prior_swap(o.ll ? reinterpret_cast<double*>(X + x_ll) : nullptr, X + x_root, dtype, A, C,
+                                cols, dw, dswap, s));
+  if (o.counts) SB_HIP(sb_launch_scatter_cols(X + x_cnt, o.counts, int(esz), A * A, C, cols, s));
+  if (o.root) SB_HIP(sb_launch_scatter_cols(X + x_root, o.root, int(esz), A, C, cols, s));
+  if (o.ll) SB_HIP(sb_launch_scatter_cols(X + x_ll, o.ll, 8, 1, C, cols, s));
+  if (o.bcounts) SB_HIP(sb_launch_scatter_cols(X + x_bc, o.bcounts, int(esz), R * A * A, C, cols, s));
+  SB_HIP(hipEventRecord(b->done, s));
+  return SB_OK;
 }
 
 }  // namespace
@@ -1022,6 +1286,7 @@ int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues, const 
   m.A = A;
   m.AP = AP;
   m.irr = true;
+  m.pi.assign(pi, pi + A);
   const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
                bd = align_up(rec.size() * 8, 256);
   std::vector<char> blob(b64 + bf + bd, 0);
@@ -1135,6 +1400,73 @@ int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment, co
   bi.n_rate = n_rate;
   bi.rate_w_host = rate_weights;
   return single(ctx, bi, dtype, stream);
+}
+
+int sb_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* model_ids,
+                    const int8_t* d_alignment, const int32_t* parent_index, const double* distances,
+                    int R, int C, int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
+                    void* d_counts, void* d_root, double* d_loglike, void* stream) {
+  MultiOut o;
+  o.counts = d_counts;
+  o.root = d_root;
+  o.ll = d_loglike;
+  return multi(ctx, n_cols, n_rows, model_ids, d_alignment, parent_index, distances, R, C, dtype,
+               f81_fast, mask_mode, d_mask, o, stream);
+}
+
+int sb_branch_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* model_ids,
+                           const int8_t* d_alignment, const int32_t* parent_index,
+                           const double* distances, int R, int C, int dtype, int f81_fast,
+                           int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
+                           double* d_loglike, void* stream) {
+  if (!d_branch_counts && C > 0) return fail(SB_ERR_ARG, "null branch counts output");
+  MultiOut o;
+  o.bcounts = d_branch_counts;
+  o.ll = d_loglike;
+  return multi(ctx, n_cols, n_rows, model_ids, d_alignment, parent_index, distances, R, C, dtype,
+               f81_fast, mask_mode, d_mask, o, stream);
+}
+
+int sb_mixture_posterior(sb_context* ctx, int n_models, const int* model_ids,
+                         const int8_t* d_alignment, const int32_t* parent_index,
+                         const double* distances, int R, int C, int dtype, const double* log_weights,
+                         double* d_post, double* d_loglikes, void* stream) {
+  if (!ctx || !model_ids || !log_weights || !parent_index || !distances)
+    return fail(SB_ERR_ARG, "null argument");
+  if (n_models < 1 || n_models > SB_MAX_MIX)
+    return fail(SB_ERR_ARG, "n_models must be 1..%d, got %d", SB_MAX_MIX, n_models);
+  if (!d_post && C > 0) return fail(SB_ERR_ARG, "null posterior output");
+  if (R < 1 || C < 0) return fail(SB_ERR_ARG, "bad sizes R=%d C=%d", R, C);
+  for (int k = 0; k < n_models; ++k)
+    if (!std::isfinite(log_weights[k]) && !(log_weights[k] < 0))
+      return fail(SB_ERR_ARG, "log_weights[%d] is not a number or +inf", k);
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  sb_batch* b = nullptr;
+  int rc = context_launcher(ctx, stream, &b);
+  if (rc) return rc;
+  if (C == 0) return SB_OK;
+  double* ll = d_loglikes;
+  if (!ll) {
+    rc = ensure(b, b->xout, size_t(n_models) * C * sizeof(double));
+    if (rc) return rc;
+    ll = static_cast<double*>(b->xout.p);
+  }
+  std::vector<BlockIn> blocks(n_models);
+  for (int k = 0; k < n_models; ++k) {
+    blocks[k] = make_block(model_ids[k], d_alignment, parent_index, distances, R, C);
+    blocks[k].flags = SB_F_LL;
+    blocks[k].blk.d_loglike = ll + size_t(k) * C;
+  }
+  for (int k = 1; k < n_models; ++k) {
+    const Model* a = get_model(ctx, model_ids[0]);
+    const Model* m = get_model(ctx, model_ids[k]);
+    if (a && m && a->A != m->A) return fail(SB_ERR_UNSUPPORTED, "mixture components must share the alphabet");
+  }
+  rc = submit(b, blocks, dtype, 0);
+  if (rc) return rc;
+  SB_HIP(sb_launch_mixture_softmax(ll, d_post, n_models, C, log_weights, b->stream));
+  SB_HIP(hipEventRecord(b->done, b->stream));
+  return SB_OK;
 }
 
 int sb_batch_create(sb_context* ctx, void* stream, sb_batch** out) {

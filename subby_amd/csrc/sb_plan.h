@@ -13,6 +13,7 @@
 
 #define SB_WAVE 64
 #define SB_MAX_RATE 8
+#define SB_MAX_MIX 64   // mixture components of one MixturePosterior call
 #define SB_LARGE_TC 4   // columns per wave tile on the large-alphabet path
 
 // Per-branch parameter record, element type T (float or double), padded
@@ -132,6 +133,7 @@ struct SbBlockDesc {
   void* bcounts;           // [R][A][A][C] of T or null (per-branch outputs)
   void* npost;             // [R][A][C] of T or null
   void* bpost;             // [R][A][A][C] of T or null
+  const void* const* row_models;  // [R] model records (T) of per-branch models, or null
   int R, C, nint, flags;
   int A, tile_base, ntiles, n_rate;
 };

@@ -325,10 +325,14 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
             }
           }
         } else {
-          const SB_CONST T* Vs = md + sb_mr_Vs(AP);
-          const SB_CONST T* Vi = md + sb_mr_Vi(AP);
-          const SB_CONST T* V = md + sb_mr_V(AP);
-          const SB_CONST T* S = md + sb_mr_S(AP);
+          // per-branch models: branch n in the eigenbasis of its own model
+          // (_branch_counts_per_row, subby/jax/__init__.py:102-144)
+          const SB_CONST T* mn = B->row_models
+              ? cst<T>((const void*)(cst<unsigned long long>(B->row_models)[n])) : md;
+          const SB_CONST T* Vs = mn + sb_mr_Vs(AP);
+          const SB_CONST T* Vi = mn + sb_mr_Vi(AP);
+          const SB_CONST T* V = mn + sb_mr_V(AP);
+          const SB_CONST T* S = mn + sb_mr_S(AP);
           const SB_CONST T* J = bt + (long long)n * BTS + sb_bt_J(AP);
           T Ut[AP], Dt[AP];
           vecmat<T, AP>(kn.u, Vs, Ut);   // Ut_q = sum_a Vs[a][q] u_a

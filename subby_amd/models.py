@@ -102,6 +102,42 @@ def f81_model(pi) -> DiagModel:
     return DiagModel(ev, Q[:, :A], pi)
 
 
+def gamma_rate_categories(alpha, K):
+    """Yang (1994) discretised gamma rate categories (models.py:126-158): the
+    medians of K equal-probability bins of Gamma(alpha, 1/alpha), found by 64
+    bisection steps on the regularised incomplete gamma function, normalised to
+    mean 1; uniform weights.  Returns (rates (K,), weights (K,))."""
+    from scipy.special import gammainc
+    p = (np.arange(K) + 0.5) / K
+    lo = np.zeros(K)
+    hi = np.full(K, max(50.0, 10.0 / float(alpha)))
+    for _ in range(64):
+        mid = 0.5 * (lo + hi)
+        below = gammainc(alpha, mid * alpha) < p
+        lo = np.where(below, mid, lo)
+        hi = np.where(below, hi, mid)
+    r = 0.5 * (lo + hi)
+    return r * K / r.sum(), np.full(K, 1.0 / K)
+
+
+def scale_model(model, rate_multiplier):
+    """Eigenvalues times a rate multiplier (models.py:161-186).  A (K,) multiplier
+    gives a model stacked along a leading K axis, which LogLike / Counts /
+    RootProb evaluate as K models over the same alignment."""
+    r = np.asarray(rate_multiplier, dtype=np.float64)
+    if isinstance(model, IrrevDiagModel):
+        if r.ndim:
+            raise ValueError('irreversible models take a scalar rate multiplier')
+        return model._replace(eigenvalues=np.asarray(model.eigenvalues) * r)
+    m = ensure_diag(model)
+    if r.ndim == 0:
+        return DiagModel(m.eigenvalues * r, m.eigenvectors, m.pi)
+    K = r.shape[0]
+    return DiagModel(m.eigenvalues[None, ...] * r.reshape((K,) + (1,) * m.eigenvalues.ndim),
+                     np.broadcast_to(m.eigenvectors[None], (K,) + m.eigenvectors.shape),
+                     np.broadcast_to(m.pi[None], (K,) + m.pi.shape))
+
+
 def diagonalize_rate_matrix(subRate, rootProb) -> DiagModel:
     """eigh of the symmetrised S_ij = Q_ij sqrt(pi_i/pi_j) (diagonalize.py:6-26)."""
     Q = np.asarray(subRate, dtype=np.float64)

@@ -1,0 +1,218 @@
+"""GPU parity for mixtures, rate categories, stacked models, per-column model
+lists and per-branch model grids against the FP64 oracle.
+
+The oracle's functions are pinned in tests/test_oracle_models.py (the
+reference's mixture golden, scipy gamma quantiles, enumeration, quadrature).
+Tolerances, per output:
+  FP32: logLike max_c |d|/max(|ref|,1) <= 1e-5; counts normwise <= 1e-4;
+        root and mixture posteriors max |d| <= 1e-5.
+  FP64: logLike <= 1e-12; root <= 1e-11 and counts <= 1e-10: slow gamma
+        categories (rate 0.03) scale the eigenvalues down, so M(t) = I + O(1e-3)
+        comes out of a cancelling eigen-sum in both implementations (root
+        differences near 2e-12), and the eigenvalue gaps near 1e-3 cost the
+        oracle's J, the reference's difference quotient
+        (E_k - E_l) / (mu_k - mu_l), digits that the kernels' expm1 form keeps.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import subby_oracle as O
+from tests.golden_cases import load
+from tests.test_gpu_parity import err_counts, err_ll, err_root, otree, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = {'fp32': dict(ll=1e-5, counts=1e-4, root=1e-5),
+       'fp64': dict(ll=1e-12, counts=1e-10, root=1e-11)}
+
+
+@pytest.fixture(scope='module')
+def sb():
+    import subby_amd
+    assert torch.cuda.is_available(), 'GPU tests need a ROCm GPU'
+    return subby_amd
+
+
+@pytest.fixture(scope='module')
+def eng(sb):
+    return sb.Engine(0)
+
+
+def om(m):
+    return O.DiagModel(*(np.asarray(x, dtype=np.float64) for x in m))
+
+
+def err_bc(got, ref):
+    got = got.detach().cpu().double().numpy()
+    num = np.abs(got - ref).max(axis=(1, 2))
+    den = np.maximum(np.abs(ref).max(axis=(1, 2)), 1e-300)
+    return float(np.max(num[1:] / den[1:])) if ref.shape[0] > 1 else 0.0
+
+
+def test_mixture_posterior_golden(sb, eng):
+    say('mixture golden')
+    g = load('7node_mixture_jc4')
+    tree = sb.Tree(g['inputs.parentIndex'], g['inputs.distanceToParent'])
+    base = sb.DiagModel(g['inputs.base_model.eigenvalues'], g['inputs.base_model.eigenvectors'],
+                        g['inputs.base_model.pi'])
+    aln = g['inputs.alignment']
+    for prec in ('fp64', 'fp32'):
+        for models in ([sb.scale_model(base, r) for r in g['inputs.rates']],
+                       sb.scale_model(base, g['inputs.rates'])):
+            post, ll = sb.MixturePosterior(aln, tree, models, g['inputs.log_weights'], precision=prec,
+                                           engine=eng, return_loglikes=True)
+            e_ll = float(np.max(np.abs(ll.cpu().numpy() - g['outputs.log_likes'])
+                                / np.maximum(np.abs(g['outputs.log_likes']), 1)))
+            e_p = float(np.abs(post.cpu().numpy() - g['outputs.posteriors']).max())
+            say('   ', prec, f'loglikes {e_ll:.2e} posteriors {e_p:.2e}')
+            assert e_ll <= TOL[prec]['ll'] and e_p <= TOL[prec]['root']
+        # the stacked model through LogLike gives (K, C)
+        ll2 = sb.LogLike(aln, tree, sb.scale_model(base, g['inputs.rates']), precision=prec, engine=eng)
+        assert ll2.shape == (3, 8)
+        assert err_ll(ll2.reshape(-1), g['outputs.log_likes'].reshape(-1)) <= TOL[prec]['ll']
+
+
+def _hky_case(sb, seed, leaves=30, C=900):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(np.full(4, 5.0))
+    m = sb.hky85_diag(2.0, pi)
+    t = sb.synthetic.random_tree(leaves, rng, 0.1)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.05, ambig_frac=0.02)
+    return rng, m, t, aln
+
+
+def test_gamma_mixture_and_stacked_models(sb, eng):
+    say('gamma rate categories, stacked models')
+    rng, m, t, aln = _hky_case(sb, 71)
+    rates, w = sb.gamma_rate_categories(0.5, 4)
+    ref_r, _ = O.gamma_rate_categories(0.5, 4)
+    np.testing.assert_allclose(rates, ref_r, rtol=1e-14)
+    stack = sb.scale_model(m, rates)
+    oms = [O.scale_model(om(m), r) for r in rates]
+    ref_post = O.MixturePosterior(aln, otree(t), oms, np.log(w))
+    for prec in ('fp64', 'fp32'):
+        post = sb.MixturePosterior(aln, t, stack, np.log(w), precision=prec, engine=eng)
+        assert float(np.abs(post.cpu().numpy() - ref_post).max()) <= TOL[prec]['root']
+        cnt, ll, root = sb.Counts(aln, t, stack, precision=prec, engine=eng, return_loglike=True,
+                                  return_root=True)
+        assert cnt.shape == (4, 4, 4, aln.shape[1]) and ll.shape == (4, aln.shape[1])
+        for k in range(4):
+            rc, rl, rr = O.counts_loglike_root(aln, otree(t), oms[k])
+            assert err_counts(cnt[k], rc) <= TOL[prec]['counts']
+            assert err_ll(ll[k], rl) <= TOL[prec]['ll'] and err_root(root[k], rr) <= TOL[prec]['root']
+        rp = sb.RootProb(aln, t, stack, precision=prec, engine=eng)
+        assert err_root(rp[2], O.RootProb(aln, otree(t), oms[2])) <= TOL[prec]['root']
+
+
+def test_model_list(sb, eng):
+    """Per-column models from three distinct models (different pi): counts per
+    column under its own model; loglike / root with the first column's root
+    distribution (the reference's stacked-model convention)."""
+    say('per-column model list')
+    rng, m, t, aln = _hky_case(sb, 81, C=700)
+    C = aln.shape[1]
+    base = [sb.hky85_diag(k, rng.dirichlet(np.full(4, 4.0))) for k in (1.5, 2.5, 4.0)]
+    which = rng.integers(0, 3, C)
+    models = [base[i] for i in which]
+    oms = [om(x) for x in models]
+    mk = rng.random(aln.shape) < 0.7
+    ref_ll = O.LogLike_columns(aln, otree(t), oms)
+    ref_root = O.RootProb_columns(aln, otree(t), oms)
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        ll = sb.LogLike(aln, t, models, precision=prec, engine=eng)
+        rp = sb.RootProb(aln, t, models, precision=prec, engine=eng)
+        e = dict(ll=err_ll(ll, ref_ll), root=err_root(rp, ref_root))
+        for mask in ('auto', None, mk):
+            cnt, ll2, rp2 = sb.Counts(aln, t, models, branch_mask=mask, precision=prec, engine=eng,
+                                      return_loglike=True, return_root=True)
+            ref_c = O.Counts_columns(aln, otree(t), oms, branch_mask=mask)
+            e['counts'] = max(e.get('counts', 0.0), err_counts(cnt, ref_c))
+            e['ll'] = max(e['ll'], err_ll(ll2, ref_ll))
+            e['root'] = max(e['root'], err_root(rp2, ref_root))
+        bc = sb.BranchCounts(aln, t, models, precision=prec, engine=eng)
+        e['bcounts'] = err_bc(bc, O.BranchCounts_columns(aln, otree(t), oms))
+        f = sb.Counts(aln, t, [sb.f81_model(x.pi) for x in models], f81_fast_flag=True, precision=prec,
+                      engine=eng)
+        e['f81'] = err_counts(f, O.Counts_columns(aln, otree(t), [om(sb.f81_model(x.pi)) for x in models],
+                                                  f81_fast=True))
+        say('   ', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
+        assert max(e['counts'], e['bcounts'], e['f81']) <= tol['counts']
+    # [m] * C is the single model, bitwise
+    same = sb.Counts(aln, t, [m] * C, engine=eng)
+    assert torch.equal(same, sb.Counts(aln, t, m, engine=eng))
+
+
+def test_per_branch_models(sb, eng):
+    """Model grid (1, R): branch n under its own model, row 0 the root distribution."""
+    say('per-branch models')
+    rng, m, t, aln = _hky_case(sb, 91, leaves=25, C=500)
+    R = len(t.parentIndex)
+    rows = [sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 4.0))) for _ in range(R)]
+    orows = [om(x) for x in rows]
+    mk = rng.random(aln.shape) < 0.6
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        e = dict(ll=err_ll(sb.LogLike(aln, t, [rows], precision=prec, engine=eng),
+                           O.LogLike_rows(aln, otree(t), orows)),
+                 root=err_root(sb.RootProb(aln, t, [rows], precision=prec, engine=eng),
+                               O.RootProb_rows(aln, otree(t), orows)))
+        for mask in ('auto', None, mk):
+            cnt = sb.Counts(aln, t, [rows], branch_mask=mask, precision=prec, engine=eng)
+            e['counts'] = max(e.get('counts', 0.0), err_counts(cnt, O.Counts_rows(aln, otree(t), orows, mask)))
+        e['bcounts'] = err_bc(sb.BranchCounts(aln, t, [rows], precision=prec, engine=eng),
+                              O.BranchCounts_rows(aln, otree(t), orows))
+        say('   ', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
+        assert max(e['counts'], e['bcounts']) <= tol['counts']
+    # all branches on one model: the single-model result
+    got = sb.Counts(aln, t, [[m] * R], precision='fp64', engine=eng)
+    ref = O.Counts(aln, otree(t), om(m))
+    assert err_counts(got, ref) <= 1e-11
+    # irreversible per-branch models: log-likelihood and root (counts are not implemented)
+    Q = rng.uniform(0.05, 1.0, (4, 4))
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    irows = [sb.diagonalize_irreversible(Q * float(rng.uniform(0.5, 2)), rows[0].pi) for _ in range(R)]
+    oir = [O.IrrevModel(*(np.asarray(x) for x in r), False) for r in irows]
+    assert err_ll(sb.LogLike(aln, t, [irows], precision='fp64', engine=eng),
+                  O.LogLike_rows(aln, otree(t), oir)) <= 1e-12
+    with pytest.raises(sb.SubbyError, match='not implemented'):
+        sb.Counts(aln, t, [irows], engine=eng)
+
+
+def test_model_grid(sb, eng):
+    """Model grid (C, R): a few distinct rows spread over the columns."""
+    say('model grid (C, R)')
+    rng, m, t, aln = _hky_case(sb, 101, leaves=12, C=300)
+    R, C = aln.shape
+    rowsets = [[sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 4.0))) for _ in range(R)]
+               for _ in range(3)]
+    grid = [rowsets[i] for i in rng.integers(0, 3, C)]
+    ogrid = [[om(x) for x in row] for row in grid]
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        cnt, ll, rp = sb.Counts(aln, t, grid, precision=prec, engine=eng, return_loglike=True, return_root=True)
+        e = dict(counts=err_counts(cnt, O.Counts_grid(aln, otree(t), ogrid)),
+                 ll=err_ll(ll, O.LogLike_grid(aln, otree(t), ogrid)),
+                 root=err_root(rp, O.RootProb_grid(aln, otree(t), ogrid)))
+        say('   ', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root'] and e['counts'] <= tol['counts']
+
+
+def test_model_table_errors(sb, eng):
+    say('model table errors')
+    rng, m, t, aln = _hky_case(sb, 111, leaves=6, C=40)
+    R, C = aln.shape
+    with pytest.raises(ValueError):
+        sb.LogLike(aln, t, [m] * (C - 1), engine=eng)
+    with pytest.raises(sb.SubbyError, match='F81'):
+        sb.Counts(aln, t, [[sb.f81_model(m.pi)] * R], f81_fast_flag=True, engine=eng)
+    with pytest.raises(NotImplementedError):
+        sb.InsideOutside(aln, t, [m] * C, engine=eng)
+    with pytest.raises(sb.SubbyError, match='alphabet'):
+        sb.LogLike(aln, t, [m] + [sb.jukes_cantor_model(5)] * (C - 1), engine=eng)
+    # the engine still works after errors
+    assert err_ll(sb.LogLike(aln, t, [m] * C, engine=eng), O.LogLike(aln, otree(t), om(m))) <= 1e-5
