@@ -237,6 +237,58 @@ def other_workload(args, log):
                             "kernel_ms": kavg},
                "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "root_abs": err_r,
                           "columns_checked": len(cols)}}
+    elif w in ('mixture', 'rows'):
+        # 'mixture': MixturePosterior over 4 gamma rate categories of HKY85 (K blocks in one
+        #            launch over the shared alignment, then the softmax kernel);
+        # 'rows':    Counts with one HKY85 model per branch (model grid (1, R)): branch-by-branch
+        #            back-transform in each branch's eigenbasis.
+        rng = np.random.default_rng(7000)
+        pi = rng.dirichlet(np.full(4, 5.0))
+        m = sb.hky85_diag(2.0, pi)
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        C = 200_000
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        R = len(t.parentIndex)
+        A = 4
+        aln = torch.from_numpy(aln_np).to(dev)
+        cols = np.arange(0, C, max(1, C // 1000))[:1000]
+        sub = aln_np[:, cols]
+        if w == 'mixture':
+            K = 4
+            rates, wts = sb.gamma_rate_categories(0.5, K)
+            stack = sb.scale_model(m, rates)
+            lw = np.log(wts)
+            fn = lambda: sb.MixturePosterior(aln, t, stack, lw, engine=eng)  # noqa: E731
+            units = C * (R - 1) * K
+            metric = ("MixturePosterior column*branches/sec (FP32, HKY85 x 4 gamma categories, R=127; "
+                      "units = columns x branches x categories)")
+            kname = "sb_fused_small_kernel<float,4,false,false,false,false> (LogLike only) + sb_mixture_softmax_kernel"
+        else:
+            rows = [sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0))) for _ in range(R)]
+            fn = lambda: sb.Counts(aln, t, [rows], engine=eng)  # noqa: E731
+            units = C * (R - 1)
+            metric = "Counts column*branches/sec (FP32, one HKY85 model per branch, R=127, mask auto)"
+            kname = "sb_fused_small_kernel<float,4,false,false,false,true>"
+        out_t = fn()
+        torch.cuda.synchronize()
+        steps = max(5, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out_t = fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        ot = O.Tree(*t)
+        if w == 'mixture':
+            ref = O.MixturePosterior(sub, ot, [O.scale_model(O.DiagModel(*m), r) for r in rates], lw)
+            par = {"posterior_abs": float(np.abs(out_t[:, cols].cpu().numpy() - ref).max())}
+        else:
+            ref = O.Counts_rows(sub, ot, [O.DiagModel(*x) for x in rows])
+            g = out_t[:, :, cols].double().cpu().numpy()
+            par = {"counts_normwise": float(np.max(np.abs(g - ref).max((0, 1)) / np.abs(ref).max((0, 1))))}
+        par["columns_checked"] = len(cols)
+        out = {"metric": metric, "value": units / wall, "unit": "column*branches/s", "n_gpus": 1,
+               "steps": steps, "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": w, "A": A, "R": R, "C": C}, "kernel": kname, "parity": par}
     elif w == 'io':
         # per-branch outputs on the headline block: BranchCounts, then node + branch posteriors
         rng = np.random.default_rng(2000)
@@ -372,7 +424,7 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev', 'mixture', 'rows'],
                     help='c2 (default, the headline line) or one of the other configurations: '
                          'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
                          'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
