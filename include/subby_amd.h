@@ -74,8 +74,8 @@ int sb_model_drop(sb_context* ctx, int model_id);
  * of Q^T); exact != 0 gives the expected counts under Q itself (projections
  * V^{-T} / V, back-transform V^{-T} C V^T).  LogLike, RootProb and the
  * gradient do not depend on it.  Such models run on alphabets of at most 8
- * states; the F81 fast path and the per-branch outputs reject them
- * (SB_ERR_UNSUPPORTED). */
+ * states, including the per-branch outputs of sb_inside_outside; the F81 fast
+ * path rejects them (SB_ERR_UNSUPPORTED). */
 int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues,
                        const double* eigenvectors, const double* eigenvectors_inv,
                        const double* pi, int exact, int* model_id);

@@ -363,7 +363,7 @@ def _exact(irrev_convention):
 
 
 def _inside_outside(alignment, tree, model, want, f81_fast=False, branch_mask=None, precision='fp32',
-                    engine=None):
+                    engine=None, irrev_convention='reference'):
     """One sb_inside_outside call; ``want`` is a set of 'bcounts', 'npost', 'bpost', 'll'."""
     eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
     dt, tdt = _PREC[precision]
@@ -380,23 +380,24 @@ def _inside_outside(alignment, tree, model, want, f81_fast=False, branch_mask=No
         out['ll'] = torch.empty(C, dtype=torch.float64, device=dev)
     ptr = lambda k: out[k].data_ptr() if k in out else None  # noqa: E731
     _lib.check(eng.lib.sb_inside_outside(
-        eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data, dist.ctypes.data, R, C, dt,
+        eng.ctx, eng.model_id(m, _exact(irrev_convention)), aln.data_ptr(), parent.ctypes.data, dist.ctypes.data, R, C, dt,
         1 if f81_fast else 0, mode, mptr, ptr('bcounts'), ptr('npost'), ptr('bpost'), ptr('ll'),
         _stream_ptr(dev)))
     return out
 
 
 def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool = False,
-                 branch_mask=None, precision: str = 'fp32', engine: Engine | None = None):
+                 branch_mask=None, precision: str = 'fp32', engine: Engine | None = None,
+                 irrev_convention: str = 'reference'):
     """Per-branch counts and dwell times, (R, A, A, C), branch 0 zeros
     (subby/jax/__init__.py:450-540).  subby.jax applies no branch mask here; the
     default ``branch_mask=None`` does the same ("auto" or an (R, C) array as in
     Counts).  Alphabets of at most 8 states.  Model lists and grids as in LogLike."""
     if _is_model_list(model):
         return _multi(alignment, tree, model, {'bcounts'}, precision, engine, f81_fast_flag,
-                      branch_mask)['bcounts']
+                      branch_mask, irrev_convention)['bcounts']
     return _inside_outside(alignment, tree, model, {'bcounts'}, f81_fast_flag, branch_mask,
-                           precision, engine)['bcounts']
+                           precision, engine, irrev_convention)['bcounts']
 
 
 class InsideOutside:

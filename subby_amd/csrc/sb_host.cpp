@@ -46,9 +46,9 @@ extern "C" int sb_f81w_waves_per_group();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
 extern "C" int sb_f81w_occupancy(int AP);
 extern "C" hipError_t sb_launch_f81w(int AP, int nwaves, const SbLaunch* L, hipStream_t s);
-extern "C" int sb_post_occupancy(int dtype, int AP, bool f81);
-extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, int nwaves, const SbLaunch* L,
-                                     hipStream_t s);
+extern "C" int sb_post_occupancy(int dtype, int AP, bool f81, bool irr);
+extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, bool irr, int nwaves,
+                                     const SbLaunch* L, hipStream_t s);
 extern "C" int sb_gen2_tile_cols();
 extern "C" long long sb_gen2_tile_bytes(int AP, int rmax);
 extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaunch* L, hipStream_t s);
@@ -372,7 +372,7 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     SB_HIP(hipEventRecord(ev.first, s));
   }
   if (key.large == 4) {
-    SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, nwaves, b->header, s));
+    SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, key.irr != 0, nwaves, b->header, s));
   } else if (key.large == 3) {
     for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
       SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
@@ -507,8 +507,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   if (irr && f81) return fail(SB_ERR_UNSUPPORTED, "F81 fast path is reversible-only");
   if (irr && large)
     return fail(SB_ERR_UNSUPPORTED, "irreversible models need an alphabet of at most 8 states");
-  if (irr && post)
-    return fail(SB_ERR_UNSUPPORTED, "per-branch outputs are not implemented for irreversible models");
   // per-branch models: counts are back-transformed branch by branch (fused ROWS variant)
   bool rowsk = false;
   if (any_rows) {
@@ -571,7 +569,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   const int wpg = (large || post) ? 1 : b->waves_per_group;
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
-    const int occ = post ? sb_post_occupancy(dtype, AP, f81)
+    const int occ = post ? sb_post_occupancy(dtype, AP, f81, irr)
                     : f81w ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad)
                             : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);

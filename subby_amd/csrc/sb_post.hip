@@ -85,7 +85,7 @@ __device__ inline void out_store(SB_GLOBAL T* p, T v) { __builtin_nontemporal_st
 
 }  // namespace
 
-template <typename T, int AP, bool F81>
+template <typename T, int AP, bool F81, bool IRR>
 __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_post_kernel(const SbLaunch* __restrict__ Lp) {
   const SbLaunch& L = *Lp;
   const int lane = threadIdx.x;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
   SB_GLOBAL T* Dreg = Ureg + (long long)NI * AP * SB_WAVE;   // [slot][AP][64]
   SB_GLOBAL int* Uw = (SB_GLOBAL int*)(Dreg + (long long)NI * AP * SB_WAVE);
   SB_GLOBAL int* Dw = Uw + (long long)NI * SB_WAVE;
-  const int BTS = sb_bt_stride(AP);
+  const int BTS = sb_bt_stride_k(AP, IRR);
 
   for (;;) {
     int t = 0;
@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
     const SB_CONST T* bt = cst<T>(B->bt);
     const SB_CONST T* md = cst<T>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
-    const SB_CONST T* pi = md + sb_mr_pi(AP);
-    const SB_CONST T* ones = md + sb_mr_ones(AP);
+    const SB_CONST T* pi = md + sb_mr_pi_k(AP, IRR);
+    const SB_CONST T* ones = md + sb_mr_ones_k(AP, IRR);
     auto Mat = [&](int n) SB_INLINE { return bt + (long long)n * BTS + sb_bt_M(AP); };
     auto vstore = [&](SB_GLOBAL T* base, int slot, const T* v) SB_INLINE {
 #pragma unroll
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
       rt = nrt;
       pt = npt;
     }
-    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
     double s0 = 0.0;
 #pragma unroll
     for (int a = 0; a < AP; ++a)
@@ -322,6 +322,74 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
               const T I = al * Ds[a] * kn.u[j] + be * (Ds[a] * piU + pi[a] * Dsum * kn.u[j]) +
                           ga * pi[a] * Dsum * piU;
               out_store(out + (long long)(a * A + j) * C, a == j ? I : fmu * pi[j] * I);
+            }
+          }
+        } else if constexpr (IRR) {
+          // complex eigenbasis of an irreversible model, per branch
+          // (accumulate_C_complex_per_branch / back_transform_irrev_per_branch,
+          // subby/jax/eigensub.py:175-231, 388-434): counts convention set 0
+          const SB_CONST T* Pu = md + sb_mri_set(AP, 0) + sb_mri_Pu(AP);
+          const SB_CONST T* Pd = md + sb_mri_set(AP, 0) + sb_mri_Pd(AP);
+          const SB_CONST T* Lr = md + sb_mri_set(AP, 0) + sb_mri_L(AP);
+          const SB_CONST T* Rr = md + sb_mri_set(AP, 0) + sb_mri_Rt(AP);
+          const SB_CONST T* Wr = md + sb_mri_W(AP);
+          const SB_CONST T* Jr = bt + (long long)n * BTS + sb_bti_Jre(AP);
+          const SB_CONST T* Ji = bt + (long long)n * BTS + sb_bti_Jim(AP);
+          constexpr int Q2 = AP * AP;
+          T Ur[AP], Ui[AP], Dr[AP], Di[AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k) {
+            T ur = T(0), ui = T(0), dr = T(0), di = T(0);
+#pragma unroll
+            for (int b = 0; b < AP; ++b) {
+              ur = fma(kn.u[b], Pu[b * AP + k], ur);
+              ui = fma(kn.u[b], Pu[Q2 + b * AP + k], ui);
+              dr = fma(d[b], Pd[b * AP + k], dr);
+              di = fma(d[b], Pd[Q2 + b * AP + k], di);
+            }
+            Ur[k] = ur;
+            Ui[k] = ui;
+            Dr[k] = dr * sc;
+            Di[k] = di * sc;
+          }
+          T Cr[AP * AP], Ci[AP * AP];
+#pragma unroll
+          for (int k = 0; k < AP; ++k)
+#pragma unroll
+            for (int q = 0; q < AP; ++q) {
+              const T jr = Jr[k * AP + q], ji = Ji[k * AP + q];
+              const T xr = fma(Dr[k], jr, -Di[k] * ji), xi = fma(Dr[k], ji, Di[k] * jr);
+              Cr[k * AP + q] = fma(xr, Ur[q], -xi * Ui[q]);
+              Ci[k * AP + q] = fma(xr, Ui[q], xi * Ur[q]);
+            }
+#pragma unroll
+          for (int a = 0; a < AP; ++a) {
+            if (a >= A) continue;
+            T tr[AP], ti[AP];
+#pragma unroll
+            for (int q = 0; q < AP; ++q) {
+              T sr = T(0), si = T(0);
+#pragma unroll
+              for (int k = 0; k < AP; ++k) {
+                const T lr = Lr[a * AP + k], li = Lr[Q2 + a * AP + k];
+                sr = fma(lr, Cr[k * AP + q], fma(-li, Ci[k * AP + q], sr));
+                si = fma(lr, Ci[k * AP + q], fma(li, Cr[k * AP + q], si));
+              }
+              tr[q] = sr;
+              ti[q] = si;
+            }
+#pragma unroll
+            for (int j = 0; j < AP; ++j) {
+              if (j >= A) continue;
+              T vr = T(0), vi = T(0);
+#pragma unroll
+              for (int q = 0; q < AP; ++q) {
+                const T rr = Rr[q * AP + j], ri = Rr[Q2 + q * AP + j];
+                vr = fma(tr[q], rr, fma(-ti[q], ri, vr));
+                vi = fma(tr[q], ri, fma(ti[q], rr, vi));
+              }
+              const T wv = Wr[a * AP + j], wi = Wr[Q2 + a * AP + j];
+              out_store(out + (long long)(a * A + j) * C, a == j ? vr : T(fma(wv, vr, -wi * vi)));
             }
           }
         } else {
@@ -443,33 +511,36 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
   }
 }
 
-#define SB_PINST(T, AP)                                                   \
-  template __global__ void sb_post_kernel<T, AP, false>(const SbLaunch*); \
-  template __global__ void sb_post_kernel<T, AP, true>(const SbLaunch*);
+#define SB_PINST(T, AP)                                                          \
+  template __global__ void sb_post_kernel<T, AP, false, false>(const SbLaunch*); \
+  template __global__ void sb_post_kernel<T, AP, true, false>(const SbLaunch*);  \
+  template __global__ void sb_post_kernel<T, AP, false, true>(const SbLaunch*);
 SB_PINST(float, 4)
 SB_PINST(float, 8)
 SB_PINST(double, 4)
 SB_PINST(double, 8)
 
-static void (*pick_post(int dtype, int AP, bool f81))(const SbLaunch*) {
-#define SB_PP(T, APV) \
-  if (AP == APV) return f81 ? sb_post_kernel<T, APV, true> : sb_post_kernel<T, APV, false>;
+static void (*pick_post(int dtype, int AP, bool f81, bool irr))(const SbLaunch*) {
+  if (f81 && irr) return nullptr;
+#define SB_PP(T, APV)                                                                  \
+  if (AP == APV) return irr ? sb_post_kernel<T, APV, false, true>                      \
+                            : f81 ? sb_post_kernel<T, APV, true, false> : sb_post_kernel<T, APV, false, false>;
   if (dtype == 0) { SB_PP(float, 4) SB_PP(float, 8) }
   else { SB_PP(double, 4) SB_PP(double, 8) }
 #undef SB_PP
   return nullptr;
 }
 
-extern "C" int sb_post_occupancy(int dtype, int AP, bool f81) {
-  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81);
+extern "C" int sb_post_occupancy(int dtype, int AP, bool f81, bool irr) {
+  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81, irr);
   int n = 0;
   if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
   return n;
 }
 
-extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, int nwaves, const SbLaunch* L,
-                                     hipStream_t s) {
-  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81);
+extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, bool irr, int nwaves,
+                                     const SbLaunch* L, hipStream_t s) {
+  void (*k)(const SbLaunch*) = pick_post(dtype, AP, f81, irr);
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
   return hipGetLastError();

@@ -170,6 +170,34 @@ def test_irrev_batch_and_graph_equal_single(sb, eng):
         assert err_counts(b.counts, ref_cnt) <= 1e-4 and err_ll(b.loglike, ref_ll) <= 1e-5
 
 
+@pytest.mark.parametrize('A', [4, 6])
+def test_irrev_per_branch_outputs(sb, eng, A):
+    """BranchCounts (both conventions, all masks) and node / branch posteriors of
+    an irreversible model against the oracle (pinned by enumeration and quadrature)."""
+    say('irreversible per-branch outputs A =', A)
+    Q, pi, m, t, aln = _case(sb, A, 700 + A, leaves=16, C=300)
+    mk = np.random.default_rng(A).random(aln.shape) < 0.7
+    for precision in ('fp64', 'fp32'):
+        tb = 1e-11 if precision == 'fp64' else 1e-4
+        tp = 1e-12 if precision == 'fp64' else 1e-5
+        for exact in (False, True):
+            conv = 'exact' if exact else 'reference'
+            o = oirr(m, exact)
+            for mask in (None, 'auto', mk):
+                ref = O.BranchCounts(aln, otree(t), o, branch_mask=mask)
+                got = sb.BranchCounts(aln, t, m, branch_mask=mask, precision=precision, engine=eng,
+                                      irrev_convention=conv).detach().cpu().double().numpy()
+                num = np.abs(got - ref).max(axis=(1, 2))
+                den = np.maximum(np.abs(ref).max(axis=(1, 2)), 1e-300)
+                e = float(np.max(num[1:] / den[1:]))
+                assert np.all(got[0] == 0) and e <= tb, (precision, conv, e)
+        io = sb.InsideOutside(aln, t, m, precision=precision, engine=eng)
+        e_np = float(np.abs(io.node_posterior().cpu().double().numpy() - O.node_posterior(aln, otree(t), oirr(m, False))).max())
+        e_bp = float(np.abs(io.branch_posterior().cpu().double().numpy() - O.branch_posterior(aln, otree(t), oirr(m, False))).max())
+        say('   ', precision, f'node {e_np:.2e} branch {e_bp:.2e}')
+        assert e_np <= tp and e_bp <= tp
+
+
 def test_irrev_errors(sb, eng):
     say('irreversible errors')
     rng = np.random.default_rng(61)
@@ -179,8 +207,6 @@ def test_irrev_errors(sb, eng):
     aln = sb.synthetic.simulate(t, m, 50, rng)
     with pytest.raises(sb.SubbyError, match='reversible-only'):
         sb.Counts(aln, t, m, f81_fast_flag=True, engine=eng)
-    with pytest.raises(sb.SubbyError, match='irreversible'):
-        sb.BranchCounts(aln, t, m, engine=eng)
     with pytest.raises(sb.SubbyError, match='at most 8'):
         Q9 = _irrev_Q(9, rng)
         m9 = sb.diagonalize_irreversible(Q9, np.full(9, 1 / 9))
