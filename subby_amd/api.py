@@ -392,7 +392,8 @@ def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag:
     """Per-branch counts and dwell times, (R, A, A, C), branch 0 zeros
     (subby/jax/__init__.py:450-540).  subby.jax applies no branch mask here; the
     default ``branch_mask=None`` does the same ("auto" or an (R, C) array as in
-    Counts).  Alphabets of at most 8 states.  Model lists and grids as in LogLike."""
+    Counts).  Any alphabet (above 8 states: one branch-selecting block per branch on
+    the large-alphabet kernels).  Model lists and grids as in LogLike."""
     if _is_model_list(model):
         return _multi(alignment, tree, model, {'bcounts'}, precision, engine, f81_fast_flag,
                       branch_mask, irrev_convention)['bcounts']

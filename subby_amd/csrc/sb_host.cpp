@@ -351,6 +351,7 @@ struct BlockIn {
   void* npost;
   void* bpost;
   const Model* const* rows;   // [R] per-branch models (row 0: the root distribution) or null
+  int only_branch;            // > 0: counts of this branch only; 0: all branches
 };
 
 int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
@@ -427,12 +428,60 @@ int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_par
 }
 
 // Plan, upload and launch a list of blocks on the launcher's stream.
-int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph) {
+int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_graph) {
   sb_context* ctx = b->ctx;
   if (dtype != SB_F32 && dtype != SB_F64) return fail(SB_ERR_ARG, "dtype must be SB_F32 or SB_F64");
   const size_t esz = dtype == SB_F32 ? 4 : 8;
+
+  // Per-branch counts above A = 8 (no dedicated kernel there): one block per
+  // branch over the same columns, each accumulating its branch only, with the
+  // counts written straight into that branch's [A][A][C] slice; branch 0's
+  // slice is cleared.  (R - 1 inside/outside passes per block.)
+  std::vector<BlockIn> expanded;
+  std::vector<std::pair<void*, size_t>> zero;
+  const std::vector<BlockIn>* inp = &in_;
+  {
+    auto wide_bc = [&](const BlockIn& bi) {
+      const Model* m = get_model(ctx, bi.blk.model_id);
+      return m && !is_small(m->AP) && (bi.flags & SB_F_BCOUNTS) && !(bi.flags & (SB_F_NPOST | SB_F_BPOST));
+    };
+    bool need = false;
+    for (const BlockIn& bi : in_) need = need || wide_bc(bi);
+    if (need) {
+      for (const BlockIn& bi : in_) {
+        if (!wide_bc(bi)) {
+          expanded.push_back(bi);
+          continue;
+        }
+        if (!bi.bcounts && bi.blk.C > 0) return fail(SB_ERR_ARG, "null branch counts output");
+        const int A = get_model(ctx, bi.blk.model_id)->A;
+        const size_t slice = size_t(A) * A * size_t(std::max(bi.blk.C, 0)) * esz;
+        if (slice) zero.push_back({bi.bcounts, slice});
+        for (int n = 1; n < bi.blk.R; ++n) {
+          BlockIn e = bi;
+          e.flags = (bi.flags & ~SB_F_BCOUNTS) | SB_F_COUNTS;
+          if (n > 1) e.flags &= ~(SB_F_LL | SB_F_ROOT);
+          e.bcounts = nullptr;
+          e.blk.d_counts = static_cast<char*>(bi.bcounts) + size_t(n) * slice;
+          e.only_branch = n;
+          expanded.push_back(e);
+        }
+        if (bi.blk.R == 1 && (bi.flags & SB_F_LL)) {
+          BlockIn e = bi;
+          e.flags = SB_F_LL;
+          e.bcounts = nullptr;
+          expanded.push_back(e);
+        }
+      }
+      inp = &expanded;
+    }
+  }
+  const std::vector<BlockIn>& in = *inp;
   const int nb = int(in.size());
-  if (nb == 0) return SB_OK;
+  if (nb == 0) {
+    for (auto& z : zero) SB_HIP(hipMemsetAsync(z.first, 0, z.second, b->stream));
+    return SB_OK;
+  }
 
   // ---- validate and plan every block
   int AP = -1;
@@ -648,6 +697,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
     d.bcounts = in[i].bcounts;
     d.npost = in[i].npost;
     d.bpost = in[i].bpost;
+    d.only_branch = in[i].only_branch;
     d.rate_w = reinterpret_cast<const double*>(dev(o_rate) + pr * sizeof(double));
     d.R = k.R;
     d.C = k.C;
@@ -714,6 +764,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in, int dtype, int use_graph
   hipStream_t s = b->stream;
   rc = flush_models(ctx, s);
   if (rc) return rc;
+  for (auto& z : zero) SB_HIP(hipMemsetAsync(z.first, 0, z.second, s));   // branch 0 (eager calls only)
   SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;

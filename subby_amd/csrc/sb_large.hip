@@ -464,6 +464,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           if (GRAD || mask_mode == SB_MASK_NONE) act[c] = true;
           else if (mask_mode == SB_MASK_AUTO) act[c] = st && stp[c];
           else act[c] = emask[(long long)n * C + cc[c]] != 0;
+          if (B->only_branch > 0 && n != B->only_branch) act[c] = false;
           scr[c] = ldexp_(inv_s0[c], ed[c] + fn.e[c] - e0[c]);
           sc[c] = act[c] ? scr[c] : T(0);
         }

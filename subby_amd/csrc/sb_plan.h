@@ -136,6 +136,9 @@ struct SbBlockDesc {
   const void* const* row_models;  // [R] model records (T) of per-branch models, or null
   int R, C, nint, flags;
   int A, tile_base, ntiles, n_rate;
+  int only_branch;         // > 0: accumulate the counts of this branch only (per-branch counts,
+                           // A > 8); 0: all branches (branch 0 never holds counts)
+  int pad_;
 };
 
 struct SbBranchJob {       // one per (block, node): M(t), J(t) ... in FP64

@@ -111,7 +111,33 @@ def test_branch_counts_sum_to_golden_counts(sb, eng, case):
     assert np.max(num / den) <= 1e-12
 
 
-def test_per_branch_outputs_reject_large_alphabets(sb, eng):
-    t, m, aln, pi = _block(sb, 12, 40, seed=70)
+@pytest.mark.parametrize('A,C', [(12, 70), (20, 5), (20, 131), (64, 40)])
+def test_branch_counts_large_alphabets(sb, eng, A, C):
+    """Per-branch counts above A = 8 (one branch-selecting block per branch on the
+    large-alphabet kernels): general model FP64 / FP32, all mask modes, the F81
+    closed form, and sum over branches = Counts (the reference's
+    test_large_alphabet, tests/test_api.py:1406-1419, at A = 20)."""
+    say('branch counts, A =', A, 'C =', C)
+    t, m, aln, pi = _block(sb, A, C, seed=80 + A + C, leaves=7)
+    ot, om = otree(t), omodel(m)
+    mk = np.random.default_rng(A).random((len(t.parentIndex), C)) < 0.6
+    for precision in ('fp64', 'fp32'):
+        tol = 1e-10 if precision == 'fp64' else 1e-4
+        for mask in (None, 'auto', mk):
+            ref = O.BranchCounts(aln, ot, om, branch_mask=mask)
+            got = sb.BranchCounts(aln, t, m, branch_mask=mask, precision=precision, engine=eng)
+            e = per_branch_err(got, ref)
+            assert float(got[0].abs().max()) == 0.0
+            assert e <= tol, (precision, mask if isinstance(mask, (str, type(None))) else 'explicit', e)
+        jc = sb.jukes_cantor_model(A)
+        bc = sb.BranchCounts(aln, t, jc, precision=precision, engine=eng)
+        c = sb.Counts(aln, t, jc, branch_mask=None, precision=precision, engine=eng)
+        assert float((bc.sum(0) - c).abs().max()) <= (1e-10 if precision == 'fp64' else 1e-4) * float(c.abs().max())
+        f = sb.f81_model(pi)
+        ref = O.BranchCounts(aln, ot, omodel(f), f81_fast=True)
+        got = sb.BranchCounts(aln, t, f, f81_fast_flag=True, precision=precision, engine=eng)
+        e = per_branch_err(got, ref)
+        say('   ', precision, 'f81', f'{e:.2e}')
+        assert e <= tol
     with pytest.raises(sb.SubbyError, match='at most 8'):
-        sb.BranchCounts(aln, t, m, engine=eng)
+        sb.InsideOutside(aln, t, m, engine=eng).node_posterior()
