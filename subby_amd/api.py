@@ -25,6 +25,7 @@ import torch
 
 from . import _lib
 from .models import Tree, DiagModel, IrrevDiagModel, RateModel, ensure_diag
+from .padding import strip_dummy_leaves
 
 _PREC = {'fp32': (_lib.SB_F32, torch.float32), 'fp64': (_lib.SB_F64, torch.float64)}
 
@@ -150,13 +151,20 @@ def default_engine() -> Engine:
         return _default
 
 
-def _prep(alignment, tree, model, engine):
+def _prep(alignment, tree, model, engine, strip=False):
+    """Device alignment, host tree arrays and the model.  ``strip``: drop the
+    trailing dummy leaves of a padded tree (pad_tree), which leave LogLike,
+    RootProb and Counts unchanged."""
     eng = engine or default_engine()
     m = ensure_diag(model)
     A = len(m.pi)
     aln = eng.alignment(alignment, A)
     R, C = aln.shape
     parent, dist = eng.tree_arrays(tree, R)
+    if strip:
+        keep = strip_dummy_leaves(parent, dist, aln)
+        if keep < R:
+            aln, parent, dist, R = aln[:keep], parent[:keep].copy(), dist[:keep].copy(), keep
     return eng, m, A, aln, R, C, parent, dist
 
 
@@ -269,7 +277,7 @@ def LogLike(alignment, tree, model, maxChunkSize: int = 128, precision: str = 'f
     st = _stacked(model)
     if st is not None:
         return _stack_call(alignment, tree, st, ('loglike',), precision, engine)['loglike']
-    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine, strip=True)
     dt, _ = _PREC[precision]
     out = torch.empty(C, dtype=torch.float64, device=eng.device)
     _lib.check(eng.lib.sb_loglike(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
@@ -286,7 +294,7 @@ def RootProb(alignment, tree, model, maxChunkSize: int = 128, precision: str = '
     st = _stacked(model)
     if st is not None:
         return _stack_call(alignment, tree, st, ('root',), precision, engine)['root']
-    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine, strip=True)
     dt, tdt = _PREC[precision]
     out = torch.empty((A, C), dtype=tdt, device=eng.device)
     _lib.check(eng.lib.sb_root_prob(eng.ctx, eng.model_id(m), aln.data_ptr(), parent.ctypes.data,
@@ -335,7 +343,7 @@ def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool 
         out = _stack_call(alignment, tree, st, want, precision, engine, f81_fast_flag, branch_mask)
         res = [out[w] for w in want]
         return tuple(res) if len(res) > 1 else res[0]
-    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine, strip=True)
     dt, tdt = _PREC[precision]
     mode, mk, mptr = _mask_args(branch_mask, eng, R, C)
     counts = torch.empty((A, A, C), dtype=tdt, device=eng.device)
@@ -496,7 +504,8 @@ def loglike_grad(alignment, tree, model, g=None, rate_weights=None, precision: s
     derivative w.r.t. that rate parameter (subby/jax/vjp.py:62-133 + HR2002).
 
     Returns (dist_grad (R,) f64, rate_grad (n_rate,) f64, logLike (C,) f64)."""
-    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine)
+    R_full = len(np.asarray(_to_np(tree.parentIndex)))
+    eng, m, A, aln, R, C, parent, dist = _prep(alignment, tree, model, engine, strip=True)
     dt, _ = _PREC[precision]
     dev = eng.device
     gt = None
@@ -517,4 +526,6 @@ def loglike_grad(alignment, tree, model, g=None, rate_weights=None, precision: s
         gt.data_ptr() if gt is not None else None, dgrad.data_ptr(), nr,
         W.ctypes.data if nr else None, rgrad.data_ptr() if nr else None, ll.data_ptr(),
         _stream_ptr(dev)))
+    if R < R_full:   # dummy leaves of a padded tree: M(t) 1 = 1, derivative 0
+        dgrad = torch.cat([dgrad, torch.zeros(R_full - R, dtype=dgrad.dtype, device=dev)])
     return dgrad, rgrad[:nr], ll

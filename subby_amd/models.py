@@ -138,6 +138,71 @@ def scale_model(model, rate_multiplier):
                      np.broadcast_to(m.pi[None], (K,) + m.pi.shape))
 
 
+_BASES = 'ACGT'
+# Standard genetic code (NCBI translation table 1), codon -> amino acid ('*' = stop),
+# written per first base in ACGT order of the second and third bases.
+_CODE = {
+    'A': 'KNKNTTTTRSRSIIMI',
+    'C': 'QHQHPPPPRRRRLLLL',
+    'G': 'EDEDAAAAGGGGVVVV',
+    'T': '*Y*YSSSS*CWCLFLF',
+}
+
+
+def genetic_code():
+    """The 64 codons in ACGT product order (index 16*b1 + 4*b2 + b3), their amino
+    acids under the standard code, and the 61 sense codons (formats.py:879-919):
+    {'codons', 'amino_acids', 'sense_mask', 'sense_indices', 'codon_to_sense',
+     'sense_codons'}."""
+    codons = [a + b + c for a in _BASES for b in _BASES for c in _BASES]
+    aas = [_CODE[c[0]][4 * _BASES.index(c[1]) + _BASES.index(c[2])] for c in codons]
+    sense = np.array([a != '*' for a in aas])
+    idx = np.nonzero(sense)[0]
+    to_sense = np.full(64, -1, dtype=np.int64)
+    to_sense[idx] = np.arange(len(idx))
+    return {'codons': codons, 'amino_acids': aas, 'sense_mask': sense, 'sense_indices': idx,
+            'codon_to_sense': to_sense, 'sense_codons': [codons[i] for i in idx]}
+
+
+def codon_to_sense(alignment, A=64):
+    """Remap 64-codon tokens to 61 sense-codon tokens: stop codons and gaps become
+    the gap token -1, the unobserved token A becomes 61 (formats.py:922-960).
+    Returns (alignment, sense codon strings)."""
+    gc = genetic_code()
+    a = np.asarray(alignment)
+    out = np.full(a.shape, -1, dtype=np.int64)
+    obs = (a >= 0) & (a < A)
+    out[obs] = gc['codon_to_sense'][a[obs]]
+    out[a == A] = 61
+    return out, gc['sense_codons']
+
+
+def gy94_rate_matrix(omega, kappa, pi=None) -> np.ndarray:
+    """Goldman-Yang (1994) codon rates on the 61 sense codons (models.py:189-232):
+    Q_ij = pi_j kappa^[transition] omega^[nonsynonymous] for codons one nucleotide
+    apart, 0 otherwise; rows sum to 0; -sum_i pi_i Q_ii = 1."""
+    gc = genetic_code()
+    sc = gc['sense_codons']
+    aa = [gc['amino_acids'][i] for i in gc['sense_indices']]
+    pi = np.full(61, 1.0 / 61) if pi is None else np.asarray(pi, dtype=np.float64)
+    ts = {('A', 'G'), ('G', 'A'), ('C', 'T'), ('T', 'C')}
+    Q = np.zeros((61, 61))
+    for i, ci in enumerate(sc):
+        for j, cj in enumerate(sc):
+            d = [(x, y) for x, y in zip(ci, cj) if x != y]
+            if len(d) != 1:
+                continue
+            Q[i, j] = pi[j] * (kappa if d[0] in ts else 1.0) * (omega if aa[i] != aa[j] else 1.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    return Q / -(pi @ np.diag(Q))
+
+
+def gy94_model(omega, kappa, pi=None) -> DiagModel:
+    """GY94 codon model as a DiagModel with A = 61 (models.py:189-232)."""
+    pi = np.full(61, 1.0 / 61) if pi is None else np.asarray(pi, dtype=np.float64)
+    return diagonalize_rate_matrix(gy94_rate_matrix(omega, kappa, pi), pi)
+
+
 def diagonalize_rate_matrix(subRate, rootProb) -> DiagModel:
     """eigh of the symmetrised S_ij = Q_ij sqrt(pi_i/pi_j) (diagonalize.py:6-26)."""
     Q = np.asarray(subRate, dtype=np.float64)

@@ -216,3 +216,61 @@ def test_model_table_errors(sb, eng):
         sb.LogLike(aln, t, [m] + [sb.jukes_cantor_model(5)] * (C - 1), engine=eng)
     # the engine still works after errors
     assert err_ll(sb.LogLike(aln, t, [m] * C, engine=eng), O.LogLike(aln, otree(t), om(m))) <= 1e-5
+
+
+def test_padding_round_trips(sb, eng):
+    """Gap-padded columns and dummy leaves change nothing (the reference's
+    TestPadAlignment, tests/test_api.py:758-969): round trips, padded columns
+    with logL = 0, root posterior = pi and zero masked counts."""
+    say('padding round trips')
+    rng, m, t, aln = _hky_case(sb, 121, leaves=9, C=37)
+    pt, pa, R0, C0 = sb.pad_tree_and_alignment(t, aln)
+    assert pa.shape[0] > R0 and pa.shape[1] > C0
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        ll = sb.LogLike(aln, t, m, precision=prec, engine=eng)
+        llp = sb.LogLike(pa, pt, m, precision=prec, engine=eng)
+        assert float((sb.unpad_columns(llp, C0) - ll).abs().max()) <= tol['ll'] * max(1.0, float(ll.abs().max()))
+        # an all-gap column has logL = 0 exactly in FP64; FP32 rows of M sum to 1 only to rounding
+        assert float(llp[C0:].abs().max()) <= (1e-12 if prec == 'fp64' else tol['ll'])
+        rp = sb.RootProb(pa, pt, m, precision=prec, engine=eng)
+        assert float((sb.unpad_columns(rp, C0) - sb.RootProb(aln, t, m, precision=prec, engine=eng)).abs().max()) <= tol['root']
+        pi = torch.as_tensor(np.asarray(m.pi), dtype=rp.dtype, device=rp.device)
+        assert float((rp[:, C0:] - pi[:, None]).abs().max()) <= tol['root']
+        cp = sb.Counts(pa, pt, m, precision=prec, engine=eng)
+        c = sb.Counts(aln, t, m, precision=prec, engine=eng)
+        assert err_counts(sb.unpad_columns(cp, C0), c.double().cpu().numpy()) <= tol['counts']
+        assert float(cp[..., C0:].abs().max()) == 0.0
+        d, _, _ = sb.loglike_grad(pa, pt, m, precision=prec, engine=eng)
+        d0, _, _ = sb.loglike_grad(aln, t, m, precision=prec, engine=eng)
+        assert d.shape[0] == pa.shape[0] and float(d[R0:].abs().max()) == 0.0
+        assert float((d[:R0] - d0).abs().max()) <= (1e-10 if prec == 'fp64' else 1e-4) * float(d0.abs().max())
+    padded, C1 = sb.pad_alignment(aln, bin_size=16)
+    post = sb.MixturePosterior(padded, t, [sb.scale_model(m, r) for r in (0.5, 1.0, 2.0)], np.log(np.ones(3) / 3),
+                               engine=eng, precision='fp64')
+    ref = sb.MixturePosterior(aln, t, [sb.scale_model(m, r) for r in (0.5, 1.0, 2.0)], np.log(np.ones(3) / 3),
+                              engine=eng, precision='fp64')
+    assert float((sb.unpad_columns(post, C1) - ref).abs().max()) <= 1e-12
+
+
+def test_gy94_codon_model(sb, eng):
+    """GY94 on the 61 sense codons through the large-alphabet paths.  FP32 counts
+    are held to 1e-3 here, not 1e-4: the dwell times of this sparse 61-state
+    model come out of the eigenbasis with heavy cancellation, and the same
+    algorithm run in FP32 on the CPU (FP32 partials and projections) is off by
+    3e-4 on this block (measured on the GPU: 1.2e-4).  FP64 is the precision for
+    codon models."""
+    say('GY94 codon model')
+    rng = np.random.default_rng(131)
+    pi = rng.dirichlet(np.full(61, 10.0))
+    m = sb.gy94_model(0.4, 2.2, pi)
+    t = sb.synthetic.random_tree(10, rng, 0.2)
+    aln = sb.synthetic.simulate(t, m, 200, rng, gap_frac=0.05, ambig_frac=0.02)
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        rc, rl, rr = O.counts_loglike_root(aln, otree(t), om(m))
+        cnt, ll, root = sb.Counts(aln, t, m, precision=prec, engine=eng, return_loglike=True, return_root=True)
+        e = dict(counts=err_counts(cnt, rc), ll=err_ll(ll, rl), root=err_root(root, rr))
+        say('   ', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['counts'] <= (tol['counts'] if prec == 'fp64' else 1e-3)
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
