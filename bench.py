@@ -289,6 +289,76 @@ def other_workload(args, log):
         out = {"metric": metric, "value": units / wall, "unit": "column*branches/s", "n_gpus": 1,
                "steps": steps, "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
                "config": {"workload": w, "A": A, "R": R, "C": C}, "kernel": kname, "parity": par}
+    elif w == 'maf':
+        # MAF ingestion end to end: a synthetic MAF of 5,000 blocks on a 100-species guide
+        # tree (species subsets of 3-100, lengths LogNormal mean 500 clipped to [20, 2048]),
+        # parsed, pruned and expanded per block, all blocks in persistent launches, 20-feature
+        # output formed on the GPU and downloaded (files not written in the timed run).
+        import tempfile
+        from subby_amd import ingest
+        rng = np.random.default_rng(8000)
+        names = [f'sp{i}' for i in range(100)]
+        gt = sb.synthetic.random_tree(100, rng, 0.05)
+        nwk_names = {}
+        par = np.asarray(gt.parentIndex)
+        is_par = np.zeros(len(par), dtype=bool)
+        is_par[par[1:]] = True
+        leaf_ids = [n for n in range(len(par)) if not is_par[n]]
+        guide = {'parentIndex': par, 'distanceToParent': np.asarray(gt.distanceToParent),
+                 'node_names': [None] * len(par), 'leaf_names': names}
+        for k, n in enumerate(leaf_ids):
+            guide['node_names'][n] = names[k]
+            nwk_names[n] = names[k]
+        m = sb.hky85_diag(2.0, rng.dirichlet(np.full(4, 5.0)))
+        nblk = 5000
+        sigma = 1.0
+        lens = np.clip(np.round(rng.lognormal(np.log(500.0) - 0.5 * sigma ** 2, sigma, nblk)), 20, 2048).astype(int)
+        full = sb.synthetic.simulate(gt, m, int(lens.sum()), rng, gap_frac=0.03, ambig_frac=0.01)
+        letters = np.frombuffer(b'ACGTN-', dtype=np.uint8)
+        tmpd = tempfile.mkdtemp(prefix='sbmaf')
+        path = os.path.join(tmpd, 'synthetic.maf')
+        log('bench maf: writing a synthetic MAF of 5,000 blocks')
+        with open(path, 'w') as f:
+            f.write('##maf version=1\n')
+            off = 0
+            for b in range(nblk):
+                L = int(lens[b])
+                k = int(rng.integers(3, 101))
+                sub = rng.choice(len(leaf_ids), k, replace=False)
+                f.write(f'a score={b}\n')
+                for q in sub:
+                    row = full[leaf_ids[q], off:off + L].astype(np.int64)
+                    seq = letters[np.where(row < 0, 5, np.minimum(row, 4))].tobytes().decode()
+                    f.write(f's {names[q]}.chr1 {off} {L} + 10000000 {seq}\n')
+                f.write('\n')
+                off += L
+        ingest.precompute_maf(path, None, m, guide, engine=eng, write=False, max_cols=2048)   # warm-up
+        torch.cuda.synchronize()
+        st = {}
+        t0 = time.perf_counter()
+        manifest, feats = ingest.precompute_maf(path, None, m, guide, engine=eng, write=False, max_cols=2048,
+                                                stats=st)
+        wall = time.perf_counter() - t0
+        # parity on a sample of blocks
+        om = O.DiagModel(*m)
+        blocks = [bk for bk in ingest.iter_maf_blocks(path) if len(bk['species']) >= 3]
+        worst = 0.0
+        for bk, f in list(zip(blocks, feats))[::250]:
+            pr = ingest.prune_tree(guide, set(bk['species']))
+            aln = ingest.expand_alignment_to_tree(ingest.seqs_to_alignment(bk['seqs']), bk['species'], pr)
+            rf = ingest.counts_to_features(O.Counts(aln, O.Tree(pr['parentIndex'], pr['distanceToParent']), om))
+            num = np.abs(f[:, :16] - rf[:, :16]).max(1)
+            worst = max(worst, float((num / np.abs(rf[:, :16]).max(1)).max()))
+        os.remove(path)
+        out = {"metric": "MAF to features column*branches/sec (FP32 Counts, HKY85, per-block pruned guide tree, "
+                         "mask auto; parse + prune + expand + GPU + feature download)",
+               "value": st['column_branches'] / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": 1,
+               "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": "maf", "blocks": st['blocks'], "columns": st['columns'],
+                          "guide_tree_leaves": 100},
+               "stages_s": {"scan_prune_expand": st['scan_s'], "gpu_and_download": st['gpu_s']},
+               "gpu_stage_column_branches_per_s": st['column_branches'] / st['gpu_s'],
+               "parity": {"counts_features_normwise": worst, "blocks_checked": len(list(zip(blocks, feats))[::250])}}
     elif w == 'io':
         # per-branch outputs on the headline block: BranchCounts, then node + branch posteriors
         rng = np.random.default_rng(2000)
@@ -424,7 +494,7 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev', 'mixture', 'rows'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev', 'mixture', 'rows', 'maf'],
                     help='c2 (default, the headline line) or one of the other configurations: '
                          'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
                          'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
