@@ -572,11 +572,15 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         // Steiner membership (oracle.py:1398-1431)
         const bool st = (ch[s].leaf ? ch[s].has : ch[s].branch) || (stp && ch[s].has);
         const int dword = (ed << 1) | (st ? 1 : 0);
-        if (sn >= 0) {
+        // an internal child's outside vector is read once, by its own record:
+        // when that record is the next one it is forwarded in registers and
+        // never stored
+        const bool fw = (s == 0) ? fwd0 : fwd1;
+        if (sn >= 0 && !fw) {
           TL::store_vec(Dreg, sn, lane, d);
           Dw[sn * SB_WAVE + lane] = dword;
         }
-        if ((s == 0 && fwd0) || (s == 1 && fwd1)) {
+        if (fw) {
 #pragma unroll
           for (int a = 0; a < AP; ++a) qd.u[a] = d[a];
           qd.w = dword;
