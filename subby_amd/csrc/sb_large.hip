@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           ed[c] = fs.e[c] + ex[c] + e[c];
           const bool st = (fn.leaf ? fn.has[c] : fn.branch[c]) || (stp[c] && fn.has[c]);
           const int dword = (ed[c] << 1) | (st ? 1 : 0);
-          if (sn >= 0) {
+          if (sn >= 0 && !fwd) {     // a forwarded outside vector is never read back
             Dreg[((long long)sn * TC + c) * SB_WAVE + lane] = d.v[c];
             if (lane == 0) Dw[sn * TC + c] = dword;
           }

@@ -482,8 +482,10 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
       matvec<T, AP>(Mat(k1.n), k1.u, w1);
       T d0[AP], d1[AP];
       int word0, word1;
-      branch(k0, w0, k1, w1, rec[4], x, ex, stp, d0, word0);
-      branch(k1, w1, k0, w0, rec[5], x, ex, stp, d1, word1);
+      // a child whose record is the next one gets its outside vector forwarded
+      // below, so it is not stored
+      branch(k0, w0, k1, w1, rec[4] == i + 1 ? -1 : rec[4], x, ex, stp, d0, word0);
+      branch(k1, w1, k0, w0, rec[5] == i + 1 ? -1 : rec[5], x, ex, stp, d1, word1);
       // the next record's load of slot i + 1 went out before this record stored it
       if (rec[4] == i + 1) {
 #pragma unroll
