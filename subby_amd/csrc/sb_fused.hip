@@ -421,6 +421,8 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     }
     const int BTS = sb_bt_stride_k(AP, IRR);
 
+    const bool outside = (flags & (SB_F_COUNTS | SB_F_GRAD)) != 0;
+
     // ---------------- inside pass ----------------
     // Internal nodes in postorder.  The next node's children are fetched one
     // iteration ahead; a child that is the node being computed now (always the
@@ -465,8 +467,12 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
       const bool br = cl.has && cr.has;
       const int word = (ep << 2) | (br ? 2 : 0) | (has ? 1 : 0);
       if (i > 0) {
-        TL::store_vec(Ureg, i, lane, up);
-        Uw[i * SB_WAVE + lane] = word;
+        // the outside pass reads every internal U; without it, a U forwarded
+        // to the next record is never read back
+        if (outside || !(fwd_l || fwd_r)) {
+          TL::store_vec(Ureg, i, lane, up);
+          Uw[i * SB_WAVE + lane] = word;
+        }
         if (fwd_l) {
 #pragma unroll
           for (int a = 0; a < AP; ++a) nl.u[a] = up[a];
@@ -499,7 +505,7 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
       for (int a = 0; a < AP; ++a)
         if (a < A) root[(long long)a * C + c] = pi[a] * u0[a] * inv_s0;
     }
-    if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
+    if (!outside) continue;
 
     // ---------------- outside pass + accumulation ----------------
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
