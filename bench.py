@@ -414,6 +414,54 @@ def other_workload(args, log):
                               "output_GB_per_s": out_post / t_post / 1e9},
                "parity": {"branch_counts_per_branch_normwise": err_bc, "node_posterior_abs": err_np,
                           "branch_posterior_abs": err_bp, "columns_checked": len(cols)}}
+    elif w == 'io20':
+        # node + branch posteriors above A = 8: general A = 20 model, 100-leaf tree
+        A = 20
+        rng = np.random.default_rng(2020)
+        pi = rng.dirichlet(np.full(A, 2.0))
+        Q = sb.reversible_rate_matrix(rng.lognormal(0.0, 1.0, (A, A)), pi)
+        m = sb.diagonalize_rate_matrix(Q, pi)
+        t = sb.synthetic.random_tree(100, rng, 0.1)
+        C = 20_000
+        log(f'bench io20: simulating A={A}, R={len(t.parentIndex)}, C={C}')
+        aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        aln = torch.from_numpy(aln_np).to(dev)
+        R = len(t.parentIndex)
+        io = sb.InsideOutside(aln, t, m, engine=eng)
+
+        def post():
+            io._npost = io._bpost = None
+            io.node_posterior()
+        post()
+        torch.cuda.synchronize()
+        reps = max(5, args.steps // 5)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record()
+        for _ in range(reps):
+            post()
+        ev1.record()
+        torch.cuda.synchronize()
+        t_post = (time.perf_counter() - t0) / reps
+        ev_ms = ev0.elapsed_time(ev1) / reps
+        cols = np.arange(0, C, max(1, C // 200))[:200]
+        ot, om = O.Tree(*t), O.DiagModel(*m)
+        sub = aln_np[:, cols]
+        err_np = float(np.abs(io.node_posterior()[:, :, cols].double().cpu().numpy() - O.node_posterior(sub, ot, om)).max())
+        err_bp = float(np.abs(io.branch_posterior()[:, :, :, cols].double().cpu().numpy() - O.branch_posterior(sub, ot, om)).max())
+        units = C * (R - 1)
+        out_b = R * (A + A * A) * C * 4
+        out = {"metric": "InsideOutside node + branch posteriors column*branches/sec (FP32, general A=20, R=199)",
+               "value": units / t_post, "unit": "column*branches/s", "n_gpus": 1, "steps": reps,
+               "ms_per_step": t_post * 1e3, "higher_is_better": True, "dtype": "f32",
+               "config": {"workload": "io20", "A": A, "R": R, "C": C},
+               "roofline": {"bound": "hbm", "note": "write-dominated: (R,A,C) + (R,A,A,C) output bytes / stream time",
+                            "achieved": out_b / (ev_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": out_b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                            "output_bytes_per_call": out_b, "stream_ms_per_call": ev_ms,
+                            "kernel": "sb_large_kernel<float, 32, false, false, true>"},
+               "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
+                          "columns_checked": len(cols)}}
     else:  # c5 gradient
         rng = np.random.default_rng(5000)
         pi = rng.dirichlet(np.full(4, 5.0))
@@ -494,11 +542,12 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'irrev', 'mixture', 'rows', 'maf'],
+    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'io20', 'irrev', 'mixture', 'rows', 'maf'],
                     help='c2 (default, the headline line) or one of the other configurations: '
                          'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
                          'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
-                         'node and branch posteriors) HKY85 R=127 C=2e5')
+                         'node and branch posteriors) HKY85 R=127 C=2e5; io20 node and branch '
+                         'posteriors, general A=20 R=199 C=2e4')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     args = ap.parse_args()

@@ -126,9 +126,10 @@ int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment,
  *   d_node_post      T   [R][A][C]     node state posteriors (row 0: the root)
  *   d_branch_post    T   [R][A][A][C]  joint (parent = i, node = j) posteriors (branch 0 zeros)
  *   d_loglike        f64 [C]
- * Node and branch posteriors need alphabets of at most 8 states
- * (SB_ERR_UNSUPPORTED otherwise); branch counts take any alphabet (above 8:
- * one branch-selecting block per branch, R - 1 passes). */
+ * Every output takes any alphabet.  Above 8 states the posteriors run in the
+ * large-alphabet kernel's posterior variant (one pass), and branch counts as
+ * one branch-selecting block per branch (R - 1 passes); asking for both
+ * launches the two in turn. */
 int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
                       const int32_t* parent_index, const double* distances, int R, int C, int dtype,
                       int f81_fast, int mask_mode, const uint8_t* d_mask, void* d_branch_counts,

@@ -414,8 +414,8 @@ class InsideOutside:
 
     The reference stores U and D once and answers queries from them; here each
     query runs the fused GPU passes again (the tables never leave the chip),
-    and results are torch tensors on the GPU.  Alphabets of at most 8 states
-    for the per-branch queries.
+    and results are torch tensors on the GPU.  Every query takes any
+    alphabet up to 64 states.
     """
 
     def __init__(self, alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',

@@ -41,6 +41,8 @@ extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
                                         bool rows, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
+extern "C" int sb_large_post_occupancy(int dtype, int AP);
+extern "C" hipError_t sb_launch_large_post(int dtype, int AP, int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
@@ -372,7 +374,9 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     }
     SB_HIP(hipEventRecord(ev.first, s));
   }
-  if (key.large == 4) {
+  if (key.large == 5) {
+    SB_HIP(sb_launch_large_post(key.dtype, key.AP, nwaves, b->header, s));
+  } else if (key.large == 4) {
     SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, key.irr != 0, nwaves, b->header, s));
   } else if (key.large == 3) {
     for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
@@ -543,12 +547,24 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
-  // per-branch outputs (branch counts, posteriors): their own small-alphabet kernel
-  bool post = false;
+  // per-branch outputs (branch counts, posteriors): their own small-alphabet
+  // kernel; above A = 8 posteriors run in the large-alphabet kernel's
+  // posterior variant (branch counts there were split into per-branch
+  // counts blocks above)
+  bool post = false, lpost = false;
   for (int i = 0; i < nb; ++i)
     if (in[i].flags & (SB_F_BCOUNTS | SB_F_NPOST | SB_F_BPOST)) post = true;
-  if (post && (large || grad))
-    return fail(SB_ERR_UNSUPPORTED, "per-branch outputs need an alphabet of at most 8 and no gradient");
+  if (post && grad) return fail(SB_ERR_UNSUPPORTED, "per-branch outputs cannot be combined with the gradient");
+  if (post && large) {
+    for (int i = 0; i < nb; ++i) {
+      const int fl = in[i].flags;
+      if (!(fl & (SB_F_NPOST | SB_F_BPOST)) || (fl & (SB_F_COUNTS | SB_F_BCOUNTS)))
+        return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, posteriors are launched apart from counts");
+    }
+    lpost = true;
+    post = false;
+    f81 = false;   // posteriors do not depend on the counts path
+  }
   const bool irr = mods[0]->irr;
   for (int i = 1; i < nb; ++i)
     if (mods[i]->irr != irr)
@@ -571,9 +587,10 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   bool f81w = large && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  const bool gen2 = large && !f81 && dtype == SB_F32 && !grad;
-  const int tile_cols = gen2   ? sb_gen2_tile_cols()
-                        : f81w ? sb_f81w_tile_cols()
+  const bool gen2 = large && !lpost && !f81 && dtype == SB_F32 && !grad;
+  const int tile_cols = gen2    ? sb_gen2_tile_cols()
+                        : lpost ? sb_large_tile_cols(dtype, AP, false)
+                        : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81)
                                 : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
@@ -618,8 +635,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   const int wpg = (large || post) ? 1 : b->waves_per_group;
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
-    const int occ = post ? sb_post_occupancy(dtype, AP, f81, irr)
-                    : f81w ? sb_f81w_occupancy(AP)
+    const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
+                    : lpost ? sb_large_post_occupancy(dtype, AP)
+                    : f81w  ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad)
                             : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
@@ -769,7 +787,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
-  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0, post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
+  GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
+               lpost ? 5 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
                nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
@@ -1420,6 +1439,15 @@ int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
                       void* d_node_post, void* d_branch_post, double* d_loglike, void* stream) {
   if (!d_branch_counts && !d_node_post && !d_branch_post && C > 0)
     return fail(SB_ERR_ARG, "no per-branch output requested");
+  const Model* m = get_model(ctx, model_id);
+  if (m && !is_small(m->AP) && d_branch_counts && (d_node_post || d_branch_post)) {
+    // above A = 8 branch counts and posteriors run in different kernels
+    int rc = sb_inside_outside(ctx, model_id, d_alignment, parent_index, distances, R, C, dtype, f81_fast,
+                               mask_mode, d_mask, d_branch_counts, nullptr, nullptr, d_loglike, stream);
+    if (rc) return rc;
+    return sb_inside_outside(ctx, model_id, d_alignment, parent_index, distances, R, C, dtype, f81_fast,
+                             mask_mode, d_mask, nullptr, d_node_post, d_branch_post, nullptr, stream);
+  }
   BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
   bi.flags = (d_branch_counts ? SB_F_BCOUNTS : 0) | (d_node_post ? SB_F_NPOST : 0) |
              (d_branch_post ? SB_F_BPOST : 0) | (d_loglike ? SB_F_LL : 0);

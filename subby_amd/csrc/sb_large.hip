@@ -173,7 +173,11 @@ __device__ inline void rescale(Vec<T, TC>& w, int* e) {
 
 }  // namespace
 
-template <typename T, int AW, bool F81, bool GRAD>
+// POST: node and branch posteriors instead of counts (InsideOutside queries,
+// subby/jax/__init__.py:800-1040): per branch n, with outside vector d,
+// inside vector u and M = M(t_n), node posterior (d^T M) * u / z and branch
+// joint posterior d_i M_ij u_j / z, z = d . (M u) (the scales cancel).
+template <typename T, int AW, bool F81, bool GRAD, bool POST = false>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
   constexpr int TC = TCof<T, AW, F81>::v;
   typedef Vec<T, TC> V_;
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   const SbLaunch& L = *Lp;
   __shared__ T bc[TC * SB_WAVE];
   __shared__ T sm[64 * 64];      // staged AP x AP matrix; also the epilogue accumulator
-  __shared__ T sv[F81 ? 1 : 64 * 64];   // V^T for the back-transform
+  __shared__ T sv[(F81 || POST) ? 1 : 64 * 64];   // V^T for the back-transform
   T* accb = sm;
   const int lane = threadIdx.x;
   const int gw = blockIdx.x;
@@ -363,7 +367,21 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       if (valid[c] && lane < A && (flags & SB_F_ROOT) && B->root)
         glb<T>(B->root)[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
     }
-    if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
+    SB_GLOBAL T* npost = (POST && (flags & SB_F_NPOST)) ? glb<T>(B->npost) : nullptr;
+    SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
+    if constexpr (POST) {
+      // row 0: the root posterior; branch 0 (above the root) is zero
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        if (!valid[c] || lane >= A) continue;
+        if (npost) npost[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
+        if (bpost)
+          for (int jj = 0; jj < A; ++jj) bpost[((long long)lane * A + jj) * C + cc[c]] = T(0);
+      }
+      if (!(npost || bpost)) continue;
+    } else {
+      if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
+    }
 
     // ---------------- outside pass + accumulation (preorder, one node ahead) ----------------
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
@@ -469,7 +487,37 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           sc[c] = act[c] ? scr[c] : T(0);
         }
         const SB_GLOBAL T* bn = bt + (long long)n * BTS;
-        if constexpr (F81) {
+        if constexpr (POST) {
+          V_ wn;
+          apply_M(n, u, wn);                       // M_n u_n
+          T iz[TC];
+#pragma unroll
+          for (int c = 0; c < TC; ++c) iz[c] = T(1) / wsum(d.v[c] * wn.v[c]);
+          if (npost) {
+            V_ y;                                  // d^T M_n
+            if (f81m) f81_vecmat<T, TC>(bn[sb_bt_E(AP) + kstar], pia, d, y);
+            else contract<T, TC>(bn + sb_bt_M(AP), AP, d, y, bc, sm, lane);
+#pragma unroll
+            for (int c = 0; c < TC; ++c)
+              if (valid[c] && lane < A) npost[((long long)n * A + lane) * C + cc[c]] = y.v[c] * u.v[c] * iz[c];
+          }
+          if (bpost) {
+            __syncthreads();
+            if (!f81m) stage<T>(bn + sb_bt_Mt(AP), AP, sm, lane);   // sm[j*64 + lane] = M[lane][j]
+#pragma unroll
+            for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
+            __syncthreads();
+            const T ef = f81m ? bn[sb_bt_E(AP) + kstar] : T(0);
+            for (int j = 0; j < A; ++j) {
+              const T mij = f81m ? ((lane == j ? ef : T(0)) + (T(1) - ef) * md[sb_mr_pi(AP) + j])
+                                 : sm[j * SB_WAVE + lane];
+#pragma unroll
+              for (int c = 0; c < TC; ++c)
+                if (valid[c] && lane < A)
+                  bpost[(((long long)n * A + lane) * A + j) * C + cc[c]] = d.v[c] * mij * bc[c * SB_WAVE + j] * iz[c];
+            }
+          }
+        } else if constexpr (F81) {
           const T al = bn[sb_bt_coef(AP) + 0], be = bn[sb_bt_coef(AP) + 1], ga = bn[sb_bt_coef(AP) + 2];
           __syncthreads();
 #pragma unroll
@@ -523,6 +571,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         }
       }
     }
+
+    if constexpr (POST) continue;
 
     // ---------------- epilogue: natural-basis counts per column ----------------
     const SB_CONST T* Vc = mdc + sb_mr_V(AP);
@@ -598,10 +648,11 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   }
 }
 
-#define SB_LINST(T, AW)                                                        \
-  template __global__ void sb_large_kernel<T, AW, false, false>(const SbLaunch*); \
-  template __global__ void sb_large_kernel<T, AW, true, false>(const SbLaunch*);  \
-  template __global__ void sb_large_kernel<T, AW, false, true>(const SbLaunch*);
+#define SB_LINST(T, AW)                                                               \
+  template __global__ void sb_large_kernel<T, AW, false, false>(const SbLaunch*);        \
+  template __global__ void sb_large_kernel<T, AW, true, false>(const SbLaunch*);         \
+  template __global__ void sb_large_kernel<T, AW, false, true>(const SbLaunch*);         \
+  template __global__ void sb_large_kernel<T, AW, false, false, true>(const SbLaunch*);
 SB_LINST(float, 32)
 SB_LINST(float, 64)
 SB_LINST(double, 32)
@@ -638,6 +689,25 @@ static void (*pick_large(int dtype, int AP, bool f81, bool grad))(const SbLaunch
   }
 #undef SB_LPICK
   return k;
+}
+
+static void (*pick_large_post(int dtype, int AP))(const SbLaunch*) {
+  if (dtype == 0) return AP <= 32 ? sb_large_kernel<float, 32, false, false, true>
+                                  : sb_large_kernel<float, 64, false, false, true>;
+  return AP <= 32 ? sb_large_kernel<double, 32, false, false, true>
+                  : sb_large_kernel<double, 64, false, false, true>;
+}
+
+extern "C" int sb_large_post_occupancy(int dtype, int AP) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick_large_post(dtype, AP), 64, 0) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hipError_t sb_launch_large_post(int dtype, int AP, int nwaves, const SbLaunch* L, hipStream_t s) {
+  if (AP > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pick_large_post(dtype, AP), dim3(nwaves), dim3(64), 0, s, L);
+  return hipGetLastError();
 }
 
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad) {

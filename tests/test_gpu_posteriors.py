@@ -139,5 +139,36 @@ def test_branch_counts_large_alphabets(sb, eng, A, C):
         e = per_branch_err(got, ref)
         say('   ', precision, 'f81', f'{e:.2e}')
         assert e <= tol
-    with pytest.raises(sb.SubbyError, match='at most 8'):
-        sb.InsideOutside(aln, t, m, engine=eng).node_posterior()
+
+
+@pytest.mark.parametrize('A,C', [(12, 70), (20, 131), (33, 45), (64, 40)])
+def test_posteriors_large_alphabets(sb, eng, A, C):
+    """Node and branch posteriors above A = 8 (the large-alphabet kernel's
+    posterior variant): general model and Jukes-Cantor (F81-structured M,
+    applied in O(A)), FP64 / FP32, vs the oracle; row 0 of the node
+    posteriors is RootProb; every posterior sums to 1; branch counts and
+    posteriors asked for together give what each gives alone."""
+    say('posteriors, A =', A, 'C =', C)
+    t, m, aln, pi = _block(sb, A, C, seed=90 + A + C, leaves=7)
+    ot = otree(t)
+    for name, mod in (('general', m), ('jc', sb.jukes_cantor_model(A))):
+        om = omodel(mod)
+        rn, rb = O.node_posterior(aln, ot, om), O.branch_posterior(aln, ot, om)
+        for precision in ('fp64', 'fp32'):
+            tol = 1e-12 if precision == 'fp64' else 1e-5
+            io = sb.InsideOutside(aln, t, mod, precision=precision, engine=eng)
+            npost, bpost = io.node_posterior(), io.branch_posterior()
+            en, eb = abs_err(npost, rn), abs_err(bpost, rb)
+            say('   ', name, precision, f'node {en:.2e} branch {eb:.2e}')
+            assert en <= tol and eb <= tol, (name, precision, en, eb)
+            assert float(bpost[0].abs().max()) == 0.0
+            assert abs_err(io.node_posterior(0), O.RootProb(aln, ot, om)) <= tol
+            assert float((npost.sum(1) - 1).abs().max()) <= 10 * tol
+            assert float((bpost[1:].sum((1, 2)) - 1).abs().max()) <= 10 * tol
+    from subby_amd.api import _inside_outside
+    both = _inside_outside(aln, t, m, {'bcounts', 'npost', 'bpost', 'll'}, precision='fp64', engine=eng)
+    assert per_branch_err(both['bcounts'], O.BranchCounts(aln, ot, omodel(m))) <= 1e-10
+    assert abs_err(both['npost'], O.node_posterior(aln, ot, omodel(m))) <= 1e-12
+    assert abs_err(both['bpost'], O.branch_posterior(aln, ot, omodel(m))) <= 1e-12
+    ll = O.LogLike(aln, ot, omodel(m))
+    assert np.max(np.abs(both['ll'].cpu().numpy() - ll) / np.maximum(np.abs(ll), 1)) <= 1e-12
