@@ -449,6 +449,27 @@ def other_workload(args, log):
         sub = aln_np[:, cols]
         err_np = float(np.abs(io.node_posterior()[:, :, cols].double().cpu().numpy() - O.node_posterior(sub, ot, om)).max())
         err_bp = float(np.abs(io.branch_posterior()[:, :, :, cols].double().cpu().numpy() - O.branch_posterior(sub, ot, om)).max())
+        io._npost = io._bpost = None
+        # per-branch counts on the same block (per-branch epilogue of the large-alphabet kernel)
+        bc_holder = {}
+
+        def bcounts():
+            bc_holder['bc'] = None
+            bc_holder['bc'] = sb.BranchCounts(aln, t, m, engine=eng)
+        bcounts()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            bcounts()
+        ev1.record()
+        torch.cuda.synchronize()
+        bc_ms = ev0.elapsed_time(ev1) / reps
+        rbc = O.BranchCounts(sub, ot, om)
+        gbc = bc_holder['bc'][:, :, :, cols].double().cpu().numpy()
+        num = np.abs(gbc - rbc).max(axis=(1, 2))
+        den = np.abs(rbc).max(axis=(1, 2))
+        err_bc = float(np.max(num[1:] / den[1:]))
+        bc_holder.clear()
         units = C * (R - 1)
         out_b = R * (A + A * A) * C * 4
         out = {"metric": "InsideOutside node + branch posteriors column*branches/sec (FP32, general A=20, R=199)",
@@ -460,8 +481,11 @@ def other_workload(args, log):
                             "frac": out_b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                             "output_bytes_per_call": out_b, "stream_ms_per_call": ev_ms,
                             "kernel": "sb_large_kernel<float, 32, false, false, true>"},
+               "branch_counts": {"stream_ms_per_call": bc_ms, "column*branches/s": units / (bc_ms * 1e-3),
+                                 "output_GB_per_s": R * A * A * C * 4 / (bc_ms * 1e-3) / 1e9,
+                                 "kernel": "sb_large_kernel<float, 32, false, false, LOUT_BRANCH>"},
                "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
-                          "columns_checked": len(cols)}}
+                          "branch_counts_per_branch_normwise": err_bc, "columns_checked": len(cols)}}
     else:  # c5 gradient
         rng = np.random.default_rng(5000)
         pi = rng.dirichlet(np.full(4, 5.0))

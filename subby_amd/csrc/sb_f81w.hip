@@ -458,7 +458,6 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
           if (mask_mode == SB_MASK_NONE) act = true;
           else if (mask_mode == SB_MASK_AUTO) act = st && stp;
           else act = oc.mk[s] != 0;
-          if (B->only_branch > 0 && n != B->only_branch) act = false;
           const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + fn.e - e0) : 0.f;
           const float dsum = sd * sc;
           const float al = oc.al[s], be = oc.be[s];

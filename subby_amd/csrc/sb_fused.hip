@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) sb_branch_params_kernel(const SbLaunch* _
   __shared__ double vt[64 * 64];          // vt[q * AP + j] = V[j][q]
   __shared__ double mu[64], E[64], sq[64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (blockIdx.x == 0 && tid == 0) *L->counter = 0;  // tile counter of the launch that follows
+  if (blockIdx.x == 0 && tid < SB_NCOUNTER) L->counter[tid] = 0;  // tile counters of the launch that follows
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
     const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
     const SB_CONST double* m = cst<double>(job->model64);
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLau
   SB_GLOBAL T* out = glb<T>(L->bt);
   __shared__ double er[64], ei[64], mr[64], mi[64];
   const int tid = threadIdx.x;
-  if (blockIdx.x == 0 && tid == 0) *L->counter = 0;  // tile counter of the launch that follows
+  if (blockIdx.x == 0 && tid < SB_NCOUNTER) L->counter[tid] = 0;  // tile counters of the launch that follows
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
     const SB_CONST SbBranchJob* job = cst<SbBranchJob>(L->jobs) + jb;
     const SB_CONST double* m = cst<double>(job->model64);
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLau
 }
 
 __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *L->counter = 0;
+  if (threadIdx.x < SB_NCOUNTER && blockIdx.x == 0) L->counter[threadIdx.x] = 0;
 }
 
 template <typename T, int AP, bool F81, bool GRAD, bool IRR, bool ROWS>

@@ -438,7 +438,6 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
         if (mask_mode == SB_MASK_NONE) act = true;
         else if (mask_mode == SB_MASK_AUTO) act = stn && stp;
         else act = (s == 0 ? cur.mkl : cur.mkr) != 0;
-        if (B->only_branch > 0 && kn_.n != B->only_branch) act = false;
         const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + kn_.e - e0) : 0.f;
         float t[NV];
         gemm<AW>(sVi, d, t, lane);

@@ -34,15 +34,13 @@ extern "C" hipError_t sb_launch_prior_swap(double* ll, void* root, int dtype, in
                                            hipStream_t s);
 extern "C" hipError_t sb_launch_mixture_softmax(const double* ll, double* post, int K, int C,
                                                 const double* logw, hipStream_t s);
-extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int waves_per_group,
+extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int out,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81);
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
                                         bool rows, int threads);
-extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad);
-extern "C" int sb_large_post_occupancy(int dtype, int AP);
-extern "C" hipError_t sb_launch_large_post(int dtype, int AP, int nwaves, const SbLaunch* L, hipStream_t s);
+extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad, int out);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max);
@@ -353,7 +351,6 @@ struct BlockIn {
   void* npost;
   void* bpost;
   const Model* const* rows;   // [R] per-branch models (row 0: the root distribution) or null
-  int only_branch;            // > 0: counts of this branch only; 0: all branches
 };
 
 int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
@@ -374,18 +371,16 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
     }
     SB_HIP(hipEventRecord(ev.first, s));
   }
-  if (key.large == 5) {
-    SB_HIP(sb_launch_large_post(key.dtype, key.AP, nwaves, b->header, s));
-  } else if (key.large == 4) {
+  if (key.large == 4) {
     SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, key.irr != 0, nwaves, b->header, s));
   } else if (key.large == 3) {
     for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
       SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
   } else if (key.large == 2)
     SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
-  else if (key.large)
-    SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad, b->waves_per_group, nwaves,
-                           b->header, s));
+  else if (key.large)   // 1 counts, 5 posteriors, 6 per-branch counts
+    SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad,
+                           key.large == 5 ? 1 : key.large == 6 ? 2 : 0, nwaves, b->header, s));
   else
     SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0, grid,
                                  block, b->header, s));
@@ -437,55 +432,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   if (dtype != SB_F32 && dtype != SB_F64) return fail(SB_ERR_ARG, "dtype must be SB_F32 or SB_F64");
   const size_t esz = dtype == SB_F32 ? 4 : 8;
 
-  // Per-branch counts above A = 8 (no dedicated kernel there): one block per
-  // branch over the same columns, each accumulating its branch only, with the
-  // counts written straight into that branch's [A][A][C] slice; branch 0's
-  // slice is cleared.  (R - 1 inside/outside passes per block.)
-  std::vector<BlockIn> expanded;
-  std::vector<std::pair<void*, size_t>> zero;
-  const std::vector<BlockIn>* inp = &in_;
-  {
-    auto wide_bc = [&](const BlockIn& bi) {
-      const Model* m = get_model(ctx, bi.blk.model_id);
-      return m && !is_small(m->AP) && (bi.flags & SB_F_BCOUNTS) && !(bi.flags & (SB_F_NPOST | SB_F_BPOST));
-    };
-    bool need = false;
-    for (const BlockIn& bi : in_) need = need || wide_bc(bi);
-    if (need) {
-      for (const BlockIn& bi : in_) {
-        if (!wide_bc(bi)) {
-          expanded.push_back(bi);
-          continue;
-        }
-        if (!bi.bcounts && bi.blk.C > 0) return fail(SB_ERR_ARG, "null branch counts output");
-        const int A = get_model(ctx, bi.blk.model_id)->A;
-        const size_t slice = size_t(A) * A * size_t(std::max(bi.blk.C, 0)) * esz;
-        if (slice) zero.push_back({bi.bcounts, slice});
-        for (int n = 1; n < bi.blk.R; ++n) {
-          BlockIn e = bi;
-          e.flags = (bi.flags & ~SB_F_BCOUNTS) | SB_F_COUNTS;
-          if (n > 1) e.flags &= ~(SB_F_LL | SB_F_ROOT);
-          e.bcounts = nullptr;
-          e.blk.d_counts = static_cast<char*>(bi.bcounts) + size_t(n) * slice;
-          e.only_branch = n;
-          expanded.push_back(e);
-        }
-        if (bi.blk.R == 1 && (bi.flags & SB_F_LL)) {
-          BlockIn e = bi;
-          e.flags = SB_F_LL;
-          e.bcounts = nullptr;
-          expanded.push_back(e);
-        }
-      }
-      inp = &expanded;
-    }
-  }
-  const std::vector<BlockIn>& in = *inp;
+  const std::vector<BlockIn>& in = in_;
   const int nb = int(in.size());
-  if (nb == 0) {
-    for (auto& z : zero) SB_HIP(hipMemsetAsync(z.first, 0, z.second, b->stream));
-    return SB_OK;
-  }
+  if (nb == 0) return SB_OK;
 
   // ---- validate and plan every block
   int AP = -1;
@@ -548,22 +497,26 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   if (grad && f81) return fail(SB_ERR_UNSUPPORTED, "gradient uses the eigensub path; f81_fast must be 0");
   const bool large = !is_small(AP);
   // per-branch outputs (branch counts, posteriors): their own small-alphabet
-  // kernel; above A = 8 posteriors run in the large-alphabet kernel's
-  // posterior variant (branch counts there were split into per-branch
-  // counts blocks above)
-  bool post = false, lpost = false;
+  // kernel; above A = 8 the large-alphabet kernel's posterior or per-branch
+  // counts variant, one kind of output per launch
+  bool post = false, lpost = false, lbc = false;
   for (int i = 0; i < nb; ++i)
     if (in[i].flags & (SB_F_BCOUNTS | SB_F_NPOST | SB_F_BPOST)) post = true;
   if (post && grad) return fail(SB_ERR_UNSUPPORTED, "per-branch outputs cannot be combined with the gradient");
   if (post && large) {
     for (int i = 0; i < nb; ++i) {
       const int fl = in[i].flags;
-      if (!(fl & (SB_F_NPOST | SB_F_BPOST)) || (fl & (SB_F_COUNTS | SB_F_BCOUNTS)))
-        return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, posteriors are launched apart from counts");
+      const bool p = (fl & (SB_F_NPOST | SB_F_BPOST)) != 0, bcn = (fl & SB_F_BCOUNTS) != 0;
+      if ((fl & SB_F_COUNTS) || (p && bcn))
+        return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, counts, branch counts and posteriors are "
+                    "launched apart");
+      lpost = lpost || p;
+      lbc = lbc || bcn;
     }
-    lpost = true;
+    if (lpost && lbc)
+      return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, branch counts and posteriors are launched apart");
     post = false;
-    f81 = false;   // posteriors do not depend on the counts path
+    if (lpost) f81 = false;   // posteriors do not depend on the counts path
   }
   const bool irr = mods[0]->irr;
   for (int i = 1; i < nb; ++i)
@@ -584,12 +537,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
       return fail(SB_ERR_UNSUPPORTED, "counts with per-branch irreversible models are not implemented");
   }
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
-  bool f81w = large && f81 && dtype == SB_F32 && !grad;
+  bool f81w = large && !lbc && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  const bool gen2 = large && !lpost && !f81 && dtype == SB_F32 && !grad;
+  const bool gen2 = large && !lpost && !lbc && !f81 && dtype == SB_F32 && !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
-                        : lpost ? sb_large_tile_cols(dtype, AP, false)
                         : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81)
                                 : SB_WAVE;
@@ -636,9 +588,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
     const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
-                    : lpost ? sb_large_post_occupancy(dtype, AP)
                     : f81w  ? sb_f81w_occupancy(AP)
-                    : large ? sb_large_occupancy(dtype, AP, f81, grad)
+                    : large ? sb_large_occupancy(dtype, AP, f81, grad, lpost ? 1 : lbc ? 2 : 0)
                             : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
   }
@@ -715,7 +666,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     d.bcounts = in[i].bcounts;
     d.npost = in[i].npost;
     d.bpost = in[i].bpost;
-    d.only_branch = in[i].only_branch;
     d.rate_w = reinterpret_cast<const double*>(dev(o_rate) + pr * sizeof(double));
     d.R = k.R;
     d.C = k.C;
@@ -782,13 +732,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   hipStream_t s = b->stream;
   rc = flush_models(ctx, s);
   if (rc) return rc;
-  for (auto& z : zero) SB_HIP(hipMemsetAsync(z.first, 0, z.second, s));   // branch 0 (eager calls only)
   SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
   SB_HIP(hipEventRecord(b->staged, s));
   b->staged_pending = true;
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
-               lpost ? 5 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
+               lpost ? 5 : lbc ? 6 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
                nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);

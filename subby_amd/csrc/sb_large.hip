@@ -171,21 +171,50 @@ __device__ inline void rescale(Vec<T, TC>& w, int* e) {
   }
 }
 
+// Next column tile of a wave.  The tile list is cut into SB_NXCD contiguous
+// ranges with a queue each; a wave starts on the range of its own XCD
+// (workgroups are dispatched to the XCDs round-robin by id), so the waves
+// that run together on one XCD take neighbouring tiles and the few columns
+// each writes into an output row [..][C] fill whole cache lines in that XCD's
+// L2 before write-back.  An exhausted range sends the wave on to the next.
+// `k` (initially 0) is the wave's number of exhausted ranges; -1 = no tiles.
+__device__ inline int claim_tile(int* counter, int ntiles, int lane, int& k) {
+  while (k < SB_NXCD) {
+    const int r = (int(blockIdx.x) + k) % SB_NXCD;
+    const int lo = int((long long)ntiles * r / SB_NXCD), hi = int((long long)ntiles * (r + 1) / SB_NXCD);
+    int t = 0;
+    if (lane == 0) t = atomicAdd(counter + r * SB_XCD_GAP, 1);
+    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if (lo + t < hi) return lo + t;
+    ++k;
+  }
+  return -1;
+}
+
 }  // namespace
 
-// POST: node and branch posteriors instead of counts (InsideOutside queries,
-// subby/jax/__init__.py:800-1040): per branch n, with outside vector d,
-// inside vector u and M = M(t_n), node posterior (d^T M) * u / z and branch
-// joint posterior d_i M_ij u_j / z, z = d . (M u) (the scales cancel).
-template <typename T, int AW, bool F81, bool GRAD, bool POST = false>
+// OUT selects the outputs beside logLike and root:
+//  LOUT_COUNTS  counts summed over branches (and the gradient when GRAD);
+//  LOUT_POST    node and branch posteriors (InsideOutside queries,
+//               subby/jax/__init__.py:800-1040): per branch n, with outside
+//               vector d, inside vector u and M = M(t_n), node posterior
+//               (d^T M) * u / z and branch joint posterior d_i M_ij u_j / z,
+//               z = d . (M u) (the scales cancel);
+//  LOUT_BRANCH  per-branch counts (BranchCounts, subby/jax/__init__.py:450-540):
+//               the epilogue runs after every branch into that branch's slice.
+enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2 };
+
+template <typename T, int AW, bool F81, bool GRAD, int OUT = LOUT_COUNTS>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
+  constexpr bool POST = OUT == LOUT_POST, BRC = OUT == LOUT_BRANCH;
+  static_assert(!(GRAD && OUT != LOUT_COUNTS) && !(F81 && POST), "unsupported variant");
   constexpr int TC = TCof<T, AW, F81>::v;
   typedef Vec<T, TC> V_;
   typedef Raw<T, TC> R_;
   typedef Flags<TC> F_;
   const SbLaunch& L = *Lp;
   __shared__ T bc[TC * SB_WAVE];
-  __shared__ T sm[64 * 64];      // staged AP x AP matrix; also the epilogue accumulator
+  __shared__ T sm[(POST ? AW : 64) * 64];   // staged AP x AP matrix; also the epilogue accumulator
   __shared__ T sv[(F81 || POST) ? 1 : 64 * 64];   // V^T for the back-transform
   T* accb = sm;
   const int lane = threadIdx.x;
@@ -200,11 +229,10 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   SB_GLOBAL int* Dw = Uw + NI * TC;
   const int BTS = sb_bt_stride(AP);
 
+  int kx = 0;
   for (;;) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(L.counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
-    if (t >= L.ntiles) break;
+    const int t = claim_tile(L.counter, L.ntiles, lane, kx);
+    if (t < 0) break;
     const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * t;
     const int bidx = tbp[0], tidx = tbp[1];
     const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(L.blocks) + bidx;
@@ -368,6 +396,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         glb<T>(B->root)[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
     }
     SB_GLOBAL T* npost = (POST && (flags & SB_F_NPOST)) ? glb<T>(B->npost) : nullptr;
+    SB_GLOBAL T* bcnt = (BRC && (flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
     SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
     if constexpr (POST) {
       // row 0: the root posterior; branch 0 (above the root) is zero
@@ -379,6 +408,13 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           for (int jj = 0; jj < A; ++jj) bpost[((long long)lane * A + jj) * C + cc[c]] = T(0);
       }
       if (!(npost || bpost)) continue;
+    } else if constexpr (BRC) {
+      if (!bcnt) continue;
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {      // branch 0 (above the root) holds no counts
+        if (!valid[c] || lane >= A) continue;
+        for (int jj = 0; jj < A; ++jj) bcnt[((long long)lane * A + jj) * C + cc[c]] = T(0);
+      }
     } else {
       if (!(flags & (SB_F_COUNTS | SB_F_GRAD))) continue;
     }
@@ -403,6 +439,73 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     for (int c = 0; c < TC; ++c) { t2a[c] = T(0); t2b[c] = T(0); t3[c] = T(0); }
     const SB_GLOBAL T* Vs = md + sb_mr_Vs(AP);
     const SB_GLOBAL T* Vi = md + sb_mr_Vi(AP);
+
+    // ---- epilogue: natural-basis counts per column from acc (and t2a/t2b/t3)
+    // into out[A][A][C] (null: none); GRAD adds the rate-parameter partials
+    const SB_CONST T* Vc = mdc + sb_mr_V(AP);
+    const SB_GLOBAL T* S = md + sb_mr_S(AP);
+    const T fmu = mdc[sb_mr_misc(AP)];
+    if constexpr (!F81 && !POST) {
+      __syncthreads();
+      // sv[k*64 + lane] = V[lane][k]
+      const SB_GLOBAL T* Vg = md + sb_mr_V(AP);
+      for (int k = 0; k < AP; ++k) sv[k * SB_WAVE + lane] = on ? Vg[lane * AP + k] : T(0);
+    }
+    double rsum[SB_MAX_RATE];
+#pragma unroll
+    for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
+    auto epilogue = [&](SB_GLOBAL T* out) {
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        // natural-basis counts of column c into accb[j*64 + i] (i = lane = row)
+        if constexpr (F81) {
+          __syncthreads();
+          bc[lane] = t2b[c];
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < AW; ++j) {
+            const T I = acc[c][j] + t2a[c] + pia * (bc[j] + t3[c]);
+            const T pj = md[sb_mr_pi(AP) + (j < AP ? j : 0)];
+            accb[j * SB_WAVE + lane] = (j == lane) ? I : fmu * pj * I;
+          }
+        } else {
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < AW; ++q) accb[lane * 64 + q] = acc[c][q];
+          __syncthreads();
+          T tmp[AW];
+#pragma unroll
+          for (int q = 0; q < AW; ++q) tmp[q] = T(0);
+          for (int k = 0; k < AP; ++k) {
+            const T vik = sv[k * SB_WAVE + lane];
+#pragma unroll
+            for (int q = 0; q < AW; ++q) tmp[q] = fma(vik, accb[k * 64 + q], tmp[q]);
+          }
+          __syncthreads();
+          for (int jj = 0; jj < AP; ++jj) {
+            T sacc = T(0);
+#pragma unroll
+            for (int q = 0; q < AW; ++q)
+              if (q < AP) sacc = fma(tmp[q], Vc[jj * AP + q], sacc);
+            accb[jj * SB_WAVE + lane] = (jj == lane) ? sacc : (on ? S[lane * AP + jj] : T(0)) * sacc;
+          }
+        }
+        if (valid[c] && out && lane < A)
+          for (int jj = 0; jj < A; ++jj) out[((long long)lane * A + jj) * C + cc[c]] = accb[jj * SB_WAVE + lane];
+        if constexpr (GRAD) {
+          for (int q = 0; q < B->n_rate; ++q) {
+            const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
+            double sq = 0.0;
+            if (on)
+              for (int jj = 0; jj < AP; ++jj) sq = fma(W[lane * AP + jj], double(accb[jj * SB_WAVE + lane]), sq);
+            const double v = gcol[c] * wsum(sq);
+#pragma unroll
+            for (int qq = 0; qq < SB_MAX_RATE; ++qq)
+              if (qq == q) rsum[qq] += v;
+          }
+        }
+      }
+    };
 
     R_ ql, qr, qd;
     int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
@@ -482,7 +585,6 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           if (GRAD || mask_mode == SB_MASK_NONE) act[c] = true;
           else if (mask_mode == SB_MASK_AUTO) act[c] = st && stp[c];
           else act[c] = emask[(long long)n * C + cc[c]] != 0;
-          if (B->only_branch > 0 && n != B->only_branch) act[c] = false;
           scr[c] = ldexp_(inv_s0[c], ed[c] + fn.e[c] - e0[c]);
           sc[c] = act[c] ? scr[c] : T(0);
         }
@@ -569,76 +671,20 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
             if (lane == 0) gpart[n] = v;
           }
         }
+        if constexpr (BRC) {
+          epilogue(bcnt + (long long)n * A * A * C);
+#pragma unroll
+          for (int c = 0; c < TC; ++c) {
+#pragma unroll
+            for (int q = 0; q < AW; ++q) acc[c][q] = T(0);
+            t2a[c] = T(0); t2b[c] = T(0); t3[c] = T(0);
+          }
+        }
       }
     }
 
-    if constexpr (POST) continue;
-
-    // ---------------- epilogue: natural-basis counts per column ----------------
-    const SB_CONST T* Vc = mdc + sb_mr_V(AP);
-    const SB_GLOBAL T* S = md + sb_mr_S(AP);
-    const T fmu = mdc[sb_mr_misc(AP)];
-    if constexpr (!F81) {
-      __syncthreads();
-      // sv[k*64 + lane] = V[lane][k]
-      const SB_GLOBAL T* Vg = md + sb_mr_V(AP);
-      for (int k = 0; k < AP; ++k) sv[k * SB_WAVE + lane] = on ? Vg[lane * AP + k] : T(0);
-    }
-    double rsum[SB_MAX_RATE];
-#pragma unroll
-    for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
-#pragma unroll
-    for (int c = 0; c < TC; ++c) {
-      // natural-basis counts of column c into accb[j*64 + i] (i = lane = row)
-      if constexpr (F81) {
-        __syncthreads();
-        bc[lane] = t2b[c];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < AW; ++j) {
-          const T I = acc[c][j] + t2a[c] + pia * (bc[j] + t3[c]);
-          const T pj = md[sb_mr_pi(AP) + (j < AP ? j : 0)];
-          accb[j * SB_WAVE + lane] = (j == lane) ? I : fmu * pj * I;
-        }
-      } else {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < AW; ++q) accb[lane * 64 + q] = acc[c][q];
-        __syncthreads();
-        T tmp[AW];
-#pragma unroll
-        for (int q = 0; q < AW; ++q) tmp[q] = T(0);
-        for (int k = 0; k < AP; ++k) {
-          const T vik = sv[k * SB_WAVE + lane];
-#pragma unroll
-          for (int q = 0; q < AW; ++q) tmp[q] = fma(vik, accb[k * 64 + q], tmp[q]);
-        }
-        __syncthreads();
-        for (int jj = 0; jj < AP; ++jj) {
-          T sacc = T(0);
-#pragma unroll
-          for (int q = 0; q < AW; ++q)
-            if (q < AP) sacc = fma(tmp[q], Vc[jj * AP + q], sacc);
-          accb[jj * SB_WAVE + lane] = (jj == lane) ? sacc : (on ? S[lane * AP + jj] : T(0)) * sacc;
-        }
-      }
-      if (valid[c] && (flags & SB_F_COUNTS) && B->counts && lane < A) {
-        SB_GLOBAL T* out = glb<T>(B->counts);
-        for (int jj = 0; jj < A; ++jj) out[((long long)lane * A + jj) * C + cc[c]] = accb[jj * SB_WAVE + lane];
-      }
-      if constexpr (GRAD) {
-        for (int q = 0; q < B->n_rate; ++q) {
-          const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
-          double sq = 0.0;
-          if (on)
-            for (int jj = 0; jj < AP; ++jj) sq = fma(W[lane * AP + jj], double(accb[jj * SB_WAVE + lane]), sq);
-          const double v = gcol[c] * wsum(sq);
-#pragma unroll
-          for (int qq = 0; qq < SB_MAX_RATE; ++qq)
-            if (qq == q) rsum[qq] += v;
-        }
-      }
-    }
+    if constexpr (POST || BRC) continue;
+    epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr);
     if constexpr (GRAD) {
       SB_GLOBAL double* rpart = glb<double>(L.rate_part) + (long long)(B->tile_base + tidx) * SB_MAX_RATE;
 #pragma unroll
@@ -648,11 +694,13 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   }
 }
 
-#define SB_LINST(T, AW)                                                               \
-  template __global__ void sb_large_kernel<T, AW, false, false>(const SbLaunch*);        \
-  template __global__ void sb_large_kernel<T, AW, true, false>(const SbLaunch*);         \
-  template __global__ void sb_large_kernel<T, AW, false, true>(const SbLaunch*);         \
-  template __global__ void sb_large_kernel<T, AW, false, false, true>(const SbLaunch*);
+#define SB_LINST(T, AW)                                                                      \
+  template __global__ void sb_large_kernel<T, AW, false, false>(const SbLaunch*);               \
+  template __global__ void sb_large_kernel<T, AW, true, false>(const SbLaunch*);                \
+  template __global__ void sb_large_kernel<T, AW, false, true>(const SbLaunch*);                \
+  template __global__ void sb_large_kernel<T, AW, false, false, LOUT_POST>(const SbLaunch*);    \
+  template __global__ void sb_large_kernel<T, AW, false, false, LOUT_BRANCH>(const SbLaunch*);  \
+  template __global__ void sb_large_kernel<T, AW, true, false, LOUT_BRANCH>(const SbLaunch*);
 SB_LINST(float, 32)
 SB_LINST(float, 64)
 SB_LINST(double, 32)
@@ -674,13 +722,18 @@ extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool 
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
 
-static void (*pick_large(int dtype, int AP, bool f81, bool grad))(const SbLaunch*) {
+// out: 0 counts (and gradient), 1 posteriors, 2 per-branch counts
+static void (*pick_large(int dtype, int AP, bool f81, bool grad, int out))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
-#define SB_LPICK(T, AWV)                                             \
-  {                                                                  \
-    if (grad) k = sb_large_kernel<T, AWV, false, true>;              \
-    else if (f81) k = sb_large_kernel<T, AWV, true, false>;          \
-    else k = sb_large_kernel<T, AWV, false, false>;                  \
+#define SB_LPICK(T, AWV)                                                          \
+  {                                                                               \
+    if (out == LOUT_POST) k = sb_large_kernel<T, AWV, false, false, LOUT_POST>;  \
+    else if (out == LOUT_BRANCH)                                                  \
+      k = f81 ? sb_large_kernel<T, AWV, true, false, LOUT_BRANCH>                 \
+              : sb_large_kernel<T, AWV, false, false, LOUT_BRANCH>;               \
+    else if (grad) k = sb_large_kernel<T, AWV, false, true>;                      \
+    else if (f81) k = sb_large_kernel<T, AWV, true, false>;                       \
+    else k = sb_large_kernel<T, AWV, false, false>;                               \
   }
   if (dtype == 0) {
     if (AP <= 32) SB_LPICK(float, 32) else SB_LPICK(float, 64)
@@ -691,35 +744,16 @@ static void (*pick_large(int dtype, int AP, bool f81, bool grad))(const SbLaunch
   return k;
 }
 
-static void (*pick_large_post(int dtype, int AP))(const SbLaunch*) {
-  if (dtype == 0) return AP <= 32 ? sb_large_kernel<float, 32, false, false, true>
-                                  : sb_large_kernel<float, 64, false, false, true>;
-  return AP <= 32 ? sb_large_kernel<double, 32, false, false, true>
-                  : sb_large_kernel<double, 64, false, false, true>;
-}
-
-extern "C" int sb_large_post_occupancy(int dtype, int AP) {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick_large_post(dtype, AP), 64, 0) != hipSuccess) return 0;
-  return n;
-}
-
-extern "C" hipError_t sb_launch_large_post(int dtype, int AP, int nwaves, const SbLaunch* L, hipStream_t s) {
-  if (AP > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pick_large_post(dtype, AP), dim3(nwaves), dim3(64), 0, s, L);
-  return hipGetLastError();
-}
-
-extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad) {
-  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad);
+extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad, int out) {
+  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad, out);
   int n = 0;
   if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 64, 0) != hipSuccess) return 0;
   return n;
 }
 
-extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int, int nwaves,
+extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int out, int nwaves,
                                       const SbLaunch* L, hipStream_t s) {
-  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad);
+  void (*k)(const SbLaunch*) = pick_large(dtype, AP, f81, grad, out);
   if (!k || AP > 64) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(nwaves), dim3(64), 0, s, L);
   return hipGetLastError();

@@ -13,7 +13,12 @@
 
 #define SB_WAVE 64
 #define SB_MAX_RATE 8
-#define SB_MAX_MIX 64   // mixture components of one MixturePosterior call
+#define SB_MAX_MIX 64
+// Tile counters: counter[0] for kernels with one tile queue; the
+// large-alphabet kernel keeps one queue per XCD at counter[x * SB_XCD_GAP].
+#define SB_NCOUNTER 64
+#define SB_NXCD 8
+#define SB_XCD_GAP 8   // mixture components of one MixturePosterior call
 #define SB_LARGE_TC 4   // columns per wave tile on the large-alphabet path
 
 // Per-branch parameter record, element type T (float or double), padded
@@ -136,9 +141,6 @@ struct SbBlockDesc {
   const void* const* row_models;  // [R] model records (T) of per-branch models, or null
   int R, C, nint, flags;
   int A, tile_base, ntiles, n_rate;
-  int only_branch;         // > 0: accumulate the counts of this branch only (per-branch counts,
-                           // A > 8); 0: all branches (branch 0 never holds counts)
-  int pad_;
 };
 
 struct SbBranchJob {       // one per (block, node): M(t), J(t) ... in FP64
@@ -154,7 +156,7 @@ struct SbLaunch {
   const int2* tiles;       // {block, tile-in-block}, longest trees first
   const SbBranchJob* jobs; // one per (block, node)
   void* bt;                // branch records, filled by the params kernel
-  int* counter;            // dynamic tile counter (zeroed by sb_reset_kernel)
+  int* counter;            // SB_NCOUNTER dynamic tile counters (zeroed by the params / reset kernels)
   char* scratch;           // nwaves * wave_stride bytes
   double* grad_part;       // [tile][grad_stride]: per-tile branch-length partials
   double* rate_part;       // [tile][SB_MAX_RATE]
