@@ -163,6 +163,18 @@ int sb_branch_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* m
                            int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
                            double* d_loglike, void* stream);
 
+/* InsideOutside over a per-column model list (subby/jax/__init__.py:719-780,
+ * node_posterior :870-960, branch_posterior :962-1040): model_ids is a host
+ * table [n_cols] (n_cols = 1 or C) of resident models sharing the alphabet and
+ * reversibility.  As in the reference, the inside and outside passes of
+ * column c run under model c's matrices with the root distribution of
+ * model_ids[0], and the root row of the node posteriors uses model c's own.
+ *   d_node_post T [R][A][C], d_branch_post T [R][A][A][C] (branch 0 zeros),
+ *   d_loglike f64 [C] (LogLike of the list); any may be NULL, not all. */
+int sb_inside_outside_multi(sb_context* ctx, int n_cols, const int* model_ids, const int8_t* d_alignment,
+                            const int32_t* parent_index, const double* distances, int R, int C, int dtype,
+                            void* d_node_post, void* d_branch_post, double* d_loglike, void* stream);
+
 /* MixturePosterior (subby/jax/__init__.py:653-678, mixture.py:6-30): the
  * log-likelihoods of n_models (1..64) resident models over one alignment in
  * one persistent launch, then d_post[k][c] = softmax_k(ll[k][c] + log_weights[k]).

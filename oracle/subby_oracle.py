@@ -517,11 +517,13 @@ def _resolve_mask(alignment, tree, model, branch_mask):
     return branch_mask
 
 
-def inside_outside(alignment, tree, model):
-    """All intermediates of one Counts call, for tests."""
+def inside_outside(alignment, tree, model, root_pi=None):
+    """All intermediates of one Counts call, for tests.  ``root_pi``: root
+    distribution of both passes when not the model's own (model lists)."""
+    pi = model.pi if root_pi is None else root_pi
     M = compute_sub_matrices(model, tree.distanceToParent)
-    U, lnU, ll = upward_pass(alignment, tree, M, model.pi)
-    D, lnD = downward_pass(U, lnU, tree, M, model.pi, alignment)
+    U, lnU, ll = upward_pass(alignment, tree, M, pi)
+    D, lnD = downward_pass(U, lnU, tree, M, pi, alignment)
     return M, U, lnU, ll, D, lnD
 
 
@@ -640,13 +642,15 @@ def _with_root_outside(D, lnD, pi):
     return D, lnD
 
 
-def node_posterior(alignment, tree, model):
+def node_posterior(alignment, tree, model, root_pi=None):
     """(R,A,C) posterior state distribution of every node (oracle.py:1755-1832).
 
     Root: pi_a U_0a normalised.  Non-root n: (sum_a D[n]_a M_n[a,j]) U_nj normalised,
     D[n] being the outside vector of branch n (indexed by the parent's state).
+    ``root_pi``: root distribution of the passes if not the model's own; the
+    root row keeps the model's pi (D[0] = model.pi, __init__.py:768-780).
     """
-    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
+    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model, root_pi)
     D, lnD = _with_root_outside(D, lnD, model.pi)
     with np.errstate(invalid='ignore', over='ignore'):
         sc = np.exp(lnU + lnD - ll[None, :])
@@ -657,10 +661,11 @@ def node_posterior(alignment, tree, model):
     return np.moveaxis(post, -1, -2)
 
 
-def branch_posterior(alignment, tree, model):
+def branch_posterior(alignment, tree, model, root_pi=None):
     """(R,A,A,C) joint posterior P(x_parent(n)=i, x_n=j | column), branch 0 zeros
-    (oracle.py:1833-1902): D[n]_i M_n[i,j] U[n]_j normalised over (i, j)."""
-    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
+    (oracle.py:1833-1902): D[n]_i M_n[i,j] U[n]_j normalised over (i, j).
+    ``root_pi`` as in node_posterior."""
+    M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model, root_pi)
     D, lnD = _with_root_outside(D, lnD, model.pi)
     with np.errstate(invalid='ignore', over='ignore'):
         sc = np.exp(lnU + lnD - ll[None, :])
@@ -797,6 +802,26 @@ def RootProb_columns(alignment, tree, models):
         U, lnU, ll = upward_pass(aln[:, c:c + 1], tree, compute_sub_matrices(m, tree.distanceToParent), pi0)
         out[:, c] = m.pi * U[0, 0] * np.exp(lnU[0, 0] - ll[0])
     return out
+
+
+def node_posterior_columns(alignment, tree, models):
+    """InsideOutside(models list).node_posterior() (subby/jax/__init__.py:719-780,
+    900-960): column c under models[c]'s matrices with the root distribution of
+    models[0] in both passes; the root row with models[c].pi (D_root is the
+    stacked per-column pi there)."""
+    aln = np.asarray(alignment)
+    pi0 = models[0].pi
+    return np.concatenate([node_posterior(aln[:, c:c + 1], tree, m, root_pi=pi0)
+                           for c, m in enumerate(models)], axis=-1)
+
+
+def branch_posterior_columns(alignment, tree, models):
+    """InsideOutside(models list).branch_posterior() (__init__.py:1008-1040):
+    column c under models[c] with the root distribution of models[0]."""
+    aln = np.asarray(alignment)
+    pi0 = models[0].pi
+    return np.concatenate([branch_posterior(aln[:, c:c + 1], tree, m, root_pi=pi0)
+                           for c, m in enumerate(models)], axis=-1)
 
 
 def _per_column_mask(branch_mask, c):

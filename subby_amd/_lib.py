@@ -22,7 +22,7 @@ EXPORTS = [
     'sb_batch_sync', 'sb_batch_set_timing', 'sb_batch_kernel_ms', 'sb_counts_plain',
     'sb_server_create', 'sb_server_destroy', 'sb_server_submit', 'sb_server_poll',
     'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
-    'sb_counts_multi', 'sb_branch_counts_multi', 'sb_mixture_posterior',
+    'sb_counts_multi', 'sb_branch_counts_multi', 'sb_inside_outside_multi', 'sb_mixture_posterior',
 ]
 
 
@@ -111,6 +111,7 @@ def load(path=LIB_PATH):
     lib.sb_inside_outside.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp, vp]
     lib.sb_counts_multi.argtypes = [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]
     lib.sb_branch_counts_multi.argtypes = [vp, i, i, vp, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp]
+    lib.sb_inside_outside_multi.argtypes = [vp, i, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp]
     lib.sb_mixture_posterior.argtypes = [vp, i, vp, vp, vp, vp, i, i, i, vp, vp, vp, vp]
     ll = ctypes.c_longlong
     lib.sb_server_create.argtypes = [vp, i, ll, i, ll, P(vp)]

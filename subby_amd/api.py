@@ -198,8 +198,9 @@ def _stacked(model):
 
 def _multi(alignment, tree, model, want, precision, engine, f81=False, branch_mask='auto',
            irrev_convention='reference'):
-    """One sb_counts_multi / sb_branch_counts_multi call for a model list (C, 1)
-    or a model grid (1 or C, R).  ``want``: subset of {'counts', 'root', 'll', 'bcounts'}."""
+    """One sb_counts_multi / sb_branch_counts_multi / sb_inside_outside_multi call
+    for a model list (C, 1) or a model grid (1 or C, R).  ``want``: subset of
+    {'counts', 'root', 'll', 'bcounts'} or of {'npost', 'bpost', 'll'}."""
     eng = engine or default_engine()
     exact = _exact(irrev_convention)
     if _is_model_grid(model):
@@ -238,8 +239,19 @@ def _multi(alignment, tree, model, want, precision, engine, f81=False, branch_ma
         out['ll'] = torch.empty(C, dtype=torch.float64, device=dev)
     if 'bcounts' in want:
         out['bcounts'] = torch.empty((R, A, A, C), dtype=tdt, device=dev)
+    if 'npost' in want:
+        out['npost'] = torch.empty((R, A, C), dtype=tdt, device=dev)
+    if 'bpost' in want:
+        out['bpost'] = torch.empty((R, A, A, C), dtype=tdt, device=dev)
     ptr = lambda k: out[k].data_ptr() if k in out else None  # noqa: E731
-    if 'bcounts' in want:
+    if 'npost' in want or 'bpost' in want:
+        if n_rows != 1:
+            raise NotImplementedError('InsideOutside does not support model grids (2D model arrays); '
+                                      'use LogLike, Counts, BranchCounts and RootProb directly')
+        _lib.check(eng.lib.sb_inside_outside_multi(
+            eng.ctx, n_cols, ids.ctypes.data, aln.data_ptr(), parent.ctypes.data, dist.ctypes.data,
+            R, C, dt, ptr('npost'), ptr('bpost'), ptr('ll'), _stream_ptr(dev)))
+    elif 'bcounts' in want:
         _lib.check(eng.lib.sb_branch_counts_multi(
             eng.ctx, n_cols, n_rows, ids.ctypes.data, aln.data_ptr(), parent.ctypes.data,
             dist.ctypes.data, R, C, dt, 1 if f81 else 0, mode, mptr, ptr('bcounts'), ptr('ll'),
@@ -415,17 +427,30 @@ class InsideOutside:
     The reference stores U and D once and answers queries from them; here each
     query runs the fused GPU passes again (the tables never leave the chip),
     and results are torch tensors on the GPU.  Every query takes any
-    alphabet up to 64 states.
+    alphabet up to 64 states.  ``model`` is one model or a per-column model
+    list (C models); as in the reference, a list's passes run under each
+    column's matrices with the root distribution of the first model, the root
+    row of ``node_posterior`` uses each column's own, and ``counts`` /
+    ``branch_counts`` are those of Counts / BranchCounts of the list.  Model
+    grids raise NotImplementedError, as in the reference.
     """
 
     def __init__(self, alignment, tree, model, maxChunkSize: int = 128, precision: str = 'fp32',
                  engine: Engine | None = None):
         self._eng = engine or default_engine()
-        if _is_model_list(model) or _stacked(model) is not None:
-            raise NotImplementedError('InsideOutside takes one model; use LogLike, Counts, BranchCounts '
-                                      'and RootProb for model lists, grids and stacked models')
-        m = ensure_diag(model)
-        self._args = (self._eng.alignment(alignment, len(m.pi)), tree, m)
+        if _is_model_grid(model):
+            raise NotImplementedError('InsideOutside does not support model grids (2D model arrays); '
+                                      'use LogLike, Counts, BranchCounts and RootProb directly')
+        if _stacked(model) is not None:
+            raise NotImplementedError('InsideOutside takes one model or a model list; use LogLike, '
+                                      'Counts, BranchCounts and RootProb for stacked models')
+        self._list = _is_model_list(model)
+        if self._list:
+            A = len(ensure_diag(model[0]).pi)
+            self._args = (self._eng.alignment(alignment, A), tree, list(model))
+        else:
+            m = ensure_diag(model)
+            self._args = (self._eng.alignment(alignment, len(m.pi)), tree, m)
         self._precision = precision
         self._ll = None
         self._npost = None
@@ -452,8 +477,11 @@ class InsideOutside:
 
     def _posteriors(self):
         if self._npost is None:
-            out = _inside_outside(*self._args, {'npost', 'bpost'}, precision=self._precision,
-                                  engine=self._eng)
+            if self._list:
+                out = _multi(*self._args, {'npost', 'bpost'}, self._precision, self._eng)
+            else:
+                out = _inside_outside(*self._args, {'npost', 'bpost'}, precision=self._precision,
+                                      engine=self._eng)
             self._npost, self._bpost = out['npost'], out['bpost']
 
     def node_posterior(self, node: int | None = None):

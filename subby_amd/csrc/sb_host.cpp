@@ -351,6 +351,7 @@ struct BlockIn {
   void* npost;
   void* bpost;
   const Model* const* rows;   // [R] per-branch models (row 0: the root distribution) or null
+  const Model* root_model;    // root distribution of the passes if not the model's own (posteriors)
 };
 
 int get_graph(sb_batch* b, const GraphKey& key, hipGraphExec_t* out, int nwg_params, dim3 grid,
@@ -480,6 +481,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
       }
       nrowm += k.R;
       any_rows = true;
+    }
+    if (const Model* rm = in[i].root_model) {
+      if (rm->AP != AP || rm->irr != m->irr)
+        return fail(SB_ERR_UNSUPPORTED, "block %d: the root model must share the alphabet and reversibility", i);
+      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS | SB_F_GRAD | SB_F_ROOT) || in[i].rows)
+        return fail(SB_ERR_UNSUPPORTED, "block %d: a separate root distribution is for posteriors only", i);
     }
     nint_max = std::max(nint_max, plans[i].nint);
     R_max = std::max(R_max, k.R);
@@ -666,6 +673,10 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     d.bcounts = in[i].bcounts;
     d.npost = in[i].npost;
     d.bpost = in[i].bpost;
+    if (in[i].root_model) {
+      d.root_model = in[i].root_model->rec[dtype];
+      d.root_model64 = in[i].root_model->d64;
+    }
     d.rate_w = reinterpret_cast<const double*>(dev(o_rate) + pr * sizeof(double));
     d.R = k.R;
     d.C = k.C;
@@ -831,6 +842,8 @@ struct MultiOut {
   void* root = nullptr;
   double* ll = nullptr;
   void* bcounts = nullptr;
+  void* npost = nullptr;   // posteriors: passes under the first column model's root distribution
+  void* bpost = nullptr;
 };
 
 // Launcher of a context for a call on `stream` (the context lock is held).
@@ -898,9 +911,15 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
   int rc = context_launcher(ctx, stream, &b);
   if (rc) return rc;
   const int flags = (o.counts ? SB_F_COUNTS : 0) | (o.root ? SB_F_ROOT : 0) | (o.ll ? SB_F_LL : 0) |
-                    (o.bcounts ? SB_F_BCOUNTS : 0);
+                    (o.bcounts ? SB_F_BCOUNTS : 0) | (o.npost ? SB_F_NPOST : 0) | (o.bpost ? SB_F_BPOST : 0);
   if (!flags && C > 0) return fail(SB_ERR_ARG, "no output requested");
   const bool per_branch = n_rows == R && R > 1;
+  const bool posts = o.npost || o.bpost;
+  if (posts && (o.counts || o.root || o.bcounts))
+    return fail(SB_ERR_ARG, "posteriors of model lists are requested apart from counts and RootProb");
+  if (posts && n_rows != 1)
+    return fail(SB_ERR_UNSUPPORTED, "posteriors take one model per column (model grids are not supported, "
+                "as in the reference)");
 
   // ---- group columns by model row (first appearance order)
   std::vector<int> grp_of(C, 0), first_col;
@@ -928,6 +947,9 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
     bi.blk.mask_mode = mask_mode;
     bi.blk.d_mask = d_mask;
     bi.rows = per_branch ? row_of(g) : nullptr;
+    // posteriors of a model list: passes under the first column model's root
+    // distribution (the reference's InsideOutside over a list)
+    bi.root_model = posts && row_of(g)[0] != tab[0] ? tab[0] : nullptr;
     return bi;
   };
   if (G == 1) {   // one model row for every column: straight on the caller's arrays
@@ -936,6 +958,8 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
     bi.blk.d_root = o.root;
     bi.blk.d_loglike = o.ll;
     bi.bcounts = o.bcounts;
+    bi.npost = o.npost;
+    bi.bpost = o.bpost;
     std::vector<BlockIn> v(1, bi);
     return submit(b, v, dtype, 0);
   }
@@ -945,7 +969,7 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
   std::vector<int> swap(G, 0);
   std::vector<double> w(size_t(G) * A, 1.0);
   bool any_swap = false;
-  if (!per_branch && (o.ll || o.root)) {
+  if (!per_branch && !posts && (o.ll || o.root)) {
     const Model* m0 = tab[0];
     for (int g = 0; g < G; ++g) {
       const Model* mg = row_of(g)[0];
@@ -1008,6 +1032,8 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
   const size_t x_root = carve(want_root, size_t(A) * C * esz);
   const size_t x_ll = carve(o.ll, size_t(C) * 8);
   const size_t x_bc = carve(o.bcounts, size_t(R) * A * A * C * esz);
+  const size_t x_np = carve(o.npost, size_t(R) * A * C * esz);
+  const size_t x_bp = carve(o.bpost, size_t(R) * A * A * C * esz);
   rc = ensure(b, b->xout, std::max<size_t>(off, 256));
   if (rc) return rc;
   char* X = static_cast<char*>(b->xout.p);
@@ -1023,6 +1049,8 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
     bi.blk.d_root = want_root ? X + x_root + size_t(A) * st * esz : nullptr;
     bi.blk.d_loglike = o.ll ? reinterpret_cast<double*>(X + x_ll) + st : nullptr;
     bi.bcounts = o.bcounts ? X + x_bc + size_t(R) * A * A * st * esz : nullptr;
+    bi.npost = o.npost ? X + x_np + size_t(R) * A * st * esz : nullptr;
+    bi.bpost = o.bpost ? X + x_bp + size_t(R) * A * A * st * esz : nullptr;
     blocks[g] = bi;
   }
   rc = submit(b, blocks, dtype, 0);
@@ -1034,6 +1062,8 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
   if (o.root) SB_HIP(sb_launch_scatter_cols(X + x_root, o.root, int(esz), A, C, cols, s));
   if (o.ll) SB_HIP(sb_launch_scatter_cols(X + x_ll, o.ll, 8, 1, C, cols, s));
   if (o.bcounts) SB_HIP(sb_launch_scatter_cols(X + x_bc, o.bcounts, int(esz), R * A * A, C, cols, s));
+  if (o.npost) SB_HIP(sb_launch_scatter_cols(X + x_np, o.npost, int(esz), R * A, C, cols, s));
+  if (o.bpost) SB_HIP(sb_launch_scatter_cols(X + x_bp, o.bpost, int(esz), R * A * A, C, cols, s));
   SB_HIP(hipEventRecord(b->done, s));
   return SB_OK;
 }
@@ -1451,6 +1481,18 @@ int sb_branch_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* m
   o.ll = d_loglike;
   return multi(ctx, n_cols, n_rows, model_ids, d_alignment, parent_index, distances, R, C, dtype,
                f81_fast, mask_mode, d_mask, o, stream);
+}
+
+int sb_inside_outside_multi(sb_context* ctx, int n_cols, const int* model_ids, const int8_t* d_alignment,
+                            const int32_t* parent_index, const double* distances, int R, int C, int dtype,
+                            void* d_node_post, void* d_branch_post, double* d_loglike, void* stream) {
+  if (!d_node_post && !d_branch_post && C > 0) return fail(SB_ERR_ARG, "no posterior output requested");
+  MultiOut o;
+  o.npost = d_node_post;
+  o.bpost = d_branch_post;
+  o.ll = d_loglike;
+  return multi(ctx, n_cols, 1, model_ids, d_alignment, parent_index, distances, R, C, dtype, 0,
+               SB_MASK_NONE, nullptr, o, stream);
 }
 
 int sb_mixture_posterior(sb_context* ctx, int n_models, const int* model_ids,

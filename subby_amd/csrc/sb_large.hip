@@ -256,6 +256,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const bool on = lane < AP;
     const T onea = on ? md[sb_mr_ones(AP) + lane] : T(0);
     const T pia = on ? md[sb_mr_pi(AP) + lane] : T(0);
+    const T pra = (on && B->root_model) ? glb<T>(B->root_model)[sb_mr_pi(AP) + lane] : pia;   // root prior
     const bool f81m = mdc[sb_mr_misc(AP) + 1] != T(0);    // F81-structured M
     const int kstar = int(mdc[sb_mr_misc(AP) + 2]);
 
@@ -382,7 +383,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       }
     }
     // log-likelihood, root posterior
-    const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+    const SB_CONST double* pi64 =
+        B->root_model64 ? cst<double>(B->root_model64) + sb_m64_pi(AP) : cst<double>(B->model64) + sb_m64_pi(AP);
     T inv_s0[TC];
     double s0s[TC];
 #pragma unroll
@@ -399,11 +401,15 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     SB_GLOBAL T* bcnt = (BRC && (flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
     SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
     if constexpr (POST) {
-      // row 0: the root posterior; branch 0 (above the root) is zero
+      // row 0: the root posterior (under the model's own root distribution);
+      // branch 0 (above the root) is zero
+      T inv_own[TC];
+#pragma unroll
+      for (int c = 0; c < TC; ++c) inv_own[c] = B->root_model ? T(1) / wsum(pia * u0.v[c]) : inv_s0[c];
 #pragma unroll
       for (int c = 0; c < TC; ++c) {
         if (!valid[c] || lane >= A) continue;
-        if (npost) npost[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
+        if (npost) npost[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_own[c];
         if (bpost)
           for (int jj = 0; jj < A; ++jj) bpost[((long long)lane * A + jj) * C + cc[c]] = T(0);
       }
@@ -537,7 +543,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         token_vec(tokp, o);
         if (i == 0) {
 #pragma unroll
-          for (int c = 0; c < TC; ++c) { x.v[c] = o.v[c] * pia; ex[c] = 0; stp[c] = rbr[c]; }
+          for (int c = 0; c < TC; ++c) { x.v[c] = o.v[c] * pra; ex[c] = 0; stp[c] = rbr[c]; }
         } else {
           V_ dp;
 #pragma unroll

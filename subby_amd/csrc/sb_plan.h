@@ -139,6 +139,12 @@ struct SbBlockDesc {
   void* npost;             // [R][A][C] of T or null
   void* bpost;             // [R][A][A][C] of T or null
   const void* const* row_models;  // [R] model records (T) of per-branch models, or null
+  // Root distribution of the inside/outside passes when it is not the
+  // model's own (model lists: that of the first column's model): its model
+  // record (T) and FP64 master, or null.  The root row of the node
+  // posteriors keeps the model's own distribution.
+  const void* root_model;
+  const double* root_model64;
   int R, C, nint, flags;
   int A, tile_base, ntiles, n_rate;
 };

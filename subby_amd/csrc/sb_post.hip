@@ -117,6 +117,7 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
     const SB_CONST int* sched = cst<int>(B->sched);
     const SB_CONST T* pi = md + sb_mr_pi_k(AP, IRR);
     const SB_CONST T* ones = md + sb_mr_ones_k(AP, IRR);
+    const SB_CONST T* pr = B->root_model ? cst<T>(B->root_model) + sb_mr_pi_k(AP, IRR) : pi;
     auto Mat = [&](int n) SB_INLINE { return bt + (long long)n * BTS + sb_bt_M(AP); };
     auto vstore = [&](SB_GLOBAL T* base, int slot, const T* v) SB_INLINE {
 #pragma unroll
@@ -228,12 +229,19 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
       rt = nrt;
       pt = npt;
     }
+    // root distribution of the passes: the model's own or (model lists) the
+    // first column model's; the root row of the node posteriors keeps the own
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
-    double s0 = 0.0;
+    const SB_CONST double* pr64 = B->root_model64 ? cst<double>(B->root_model64) + sb_m64_pi_k(AP, IRR) : pi64;
+    double s0 = 0.0, s0own = 0.0;
 #pragma unroll
     for (int a = 0; a < AP; ++a)
-      if (a < A) s0 = fma(pi64[a], double(u0[a]), s0);
+      if (a < A) {
+        s0 = fma(pr64[a], double(u0[a]), s0);
+        s0own = fma(pi64[a], double(u0[a]), s0own);
+      }
     const T inv_s0 = T(1.0 / s0);
+    const T inv_own = T(1.0 / s0own);
     SB_GLOBAL T* bcnt = (flags & SB_F_BCOUNTS) ? glb<T>(B->bcounts) : nullptr;
     SB_GLOBAL T* npost = (flags & SB_F_NPOST) ? glb<T>(B->npost) : nullptr;
     SB_GLOBAL T* bpost = (flags & SB_F_BPOST) ? glb<T>(B->bpost) : nullptr;
@@ -243,7 +251,7 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
       if (npost) {
 #pragma unroll
         for (int a = 0; a < AP; ++a)
-          if (a < A) out_store(npost + (long long)a * C + cc, pi[a] * u0[a] * inv_s0);
+          if (a < A) out_store(npost + (long long)a * C + cc, pi[a] * u0[a] * inv_own);
       }
       for (int ij = 0; ij < A * A; ++ij) {     // branch 0 (above the root) is zero
         if (bcnt) out_store(bcnt + (long long)ij * C + cc, T(0));
@@ -463,7 +471,7 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
         token_vec(xt, o);
         if (i == 0) {
 #pragma unroll
-          for (int a = 0; a < AP; ++a) x[a] = o[a] * pi[a];
+          for (int a = 0; a < AP; ++a) x[a] = o[a] * pr[a];
           ex = 0;
           stp = rbr;
         } else {

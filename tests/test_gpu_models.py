@@ -145,6 +145,82 @@ def test_model_list(sb, eng):
     assert torch.equal(same, sb.Counts(aln, t, m, engine=eng))
 
 
+@pytest.mark.parametrize('A', [4, 12])
+def test_inside_outside_model_list(sb, eng, A):
+    """InsideOutside over a per-column model list: node and branch posteriors
+    with the passes under the first model's root distribution and the root row
+    under each column's own (the reference's convention), log-likelihood of
+    the list, counts and branch counts of the list; A = 4 (per-branch kernel)
+    and A = 12 (large-alphabet posterior variant)."""
+    say('InsideOutside over a model list, A =', A)
+    rng = np.random.default_rng(95 + A)
+    t = sb.synthetic.random_tree(9, rng, 0.15)
+    C = 300 if A == 4 else 90
+    base = []
+    for k in range(3):
+        pi = rng.dirichlet(np.full(A, 3.0))
+        Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi)
+        base.append(sb.diagonalize_rate_matrix(Q, pi))
+    aln = sb.synthetic.simulate(t, base[0], C, rng, gap_frac=0.05, ambig_frac=0.02)
+    which = rng.integers(0, 3, C)
+    which[0] = 1                                   # the first column's model is not base[0]
+    models = [base[i] for i in which]
+    oms = [om(x) for x in models]
+    ot = otree(t)
+    rn = O.node_posterior_columns(aln, ot, oms)
+    rb = O.branch_posterior_columns(aln, ot, oms)
+    rl = O.LogLike_columns(aln, ot, oms)
+    for prec in ('fp64', 'fp32'):
+        tp = 1e-12 if prec == 'fp64' else 1e-5
+        io = sb.InsideOutside(aln, t, models, precision=prec, engine=eng)
+        en = float(np.abs(io.node_posterior().double().cpu().numpy() - rn).max())
+        eb = float(np.abs(io.branch_posterior().double().cpu().numpy() - rb).max())
+        el = err_ll(io.log_likelihood, rl)
+        say('   ', prec, f'node {en:.2e} branch {eb:.2e} ll {el:.2e}')
+        assert en <= tp and eb <= tp and el <= TOL[prec]['ll']
+        assert float(io.branch_posterior(0).abs().max()) == 0.0
+        assert err_counts(io.counts(branch_mask=None), O.Counts_columns(aln, ot, oms, branch_mask=None)) \
+            <= TOL[prec]['counts']
+        assert err_bc(io.branch_counts(), O.BranchCounts_columns(aln, ot, oms)) <= TOL[prec]['counts']
+    # a list of one repeated model is that model, bitwise
+    io1 = sb.InsideOutside(aln, t, [base[0]] * C, precision='fp64', engine=eng)
+    io2 = sb.InsideOutside(aln, t, base[0], precision='fp64', engine=eng)
+    assert torch.equal(io1.node_posterior(), io2.node_posterior())
+    assert torch.equal(io1.branch_posterior(), io2.branch_posterior())
+
+
+def test_inside_outside_irreversible_model_list(sb, eng):
+    """The same over irreversible models (A = 4, complex eigensystems)."""
+    say('InsideOutside over an irreversible model list')
+    rng = np.random.default_rng(97)
+    t = sb.synthetic.random_tree(8, rng, 0.15)
+    C = 200
+    base = []
+    for k in range(2):
+        Q = rng.uniform(0.05, 1.0, (4, 4))
+        for a in range(4):
+            Q[a, (a + 1) % 4] += 1.5                # a rotational part: complex eigenvalues
+        np.fill_diagonal(Q, 0.0)
+        np.fill_diagonal(Q, -Q.sum(1))
+        base.append(sb.diagonalize_irreversible(Q, rng.dirichlet(np.full(4, 3.0))))
+    aln = sb.synthetic.simulate(t, base[0], C, rng, gap_frac=0.05, ambig_frac=0.02)
+    which = rng.integers(0, 2, C)
+    which[0] = 1
+    models = [base[i] for i in which]
+    oms = [O.IrrevModel(np.asarray(m.eigenvalues), np.asarray(m.eigenvectors), np.asarray(m.eigenvectors_inv),
+                        np.asarray(m.pi), False) for m in models]
+    ot = otree(t)
+    rn = O.node_posterior_columns(aln, ot, oms)
+    rb = O.branch_posterior_columns(aln, ot, oms)
+    for prec in ('fp64', 'fp32'):
+        tp = 1e-11 if prec == 'fp64' else 1e-5
+        io = sb.InsideOutside(aln, t, models, precision=prec, engine=eng)
+        en = float(np.abs(io.node_posterior().double().cpu().numpy() - rn).max())
+        eb = float(np.abs(io.branch_posterior().double().cpu().numpy() - rb).max())
+        say('   ', prec, f'node {en:.2e} branch {eb:.2e}')
+        assert en <= tp and eb <= tp
+
+
 def test_per_branch_models(sb, eng):
     """Model grid (1, R): branch n under its own model, row 0 the root distribution."""
     say('per-branch models')
@@ -211,7 +287,7 @@ def test_model_table_errors(sb, eng):
     with pytest.raises(sb.SubbyError, match='F81'):
         sb.Counts(aln, t, [[sb.f81_model(m.pi)] * R], f81_fast_flag=True, engine=eng)
     with pytest.raises(NotImplementedError):
-        sb.InsideOutside(aln, t, [m] * C, engine=eng)
+        sb.InsideOutside(aln, t, [[m] * R], engine=eng)
     with pytest.raises(sb.SubbyError, match='alphabet'):
         sb.LogLike(aln, t, [m] + [sb.jukes_cantor_model(5)] * (C - 1), engine=eng)
     # the engine still works after errors

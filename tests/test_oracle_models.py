@@ -112,6 +112,29 @@ def test_model_list_root_distribution_is_the_first_models():
                                    atol=1e-13)
 
 
+def test_model_list_posteriors_by_enumeration():
+    """InsideOutside over a model list: node and branch posteriors of the
+    non-root nodes of column c are those of exhaustive enumeration under Q_c
+    with the root distribution of models[0]; the root row is the root
+    marginal of the enumeration under models[c]'s own pi."""
+    rng, tree, aln = _setup(6, R=5, C=3)
+    pis = [rng.dirichlet(np.ones(4) * 2) for _ in range(3)]
+    ms = [O.diagonalize_rate_matrix(_rev_Q(rng.lognormal(0, 1, (4, 4)), p), p) for p in pis]
+    npost = O.node_posterior_columns(aln, tree, ms)
+    bpost = O.branch_posterior_columns(aln, tree, ms)
+    assert np.all(bpost[0] == 0)
+    for c in range(3):
+        Q = O.reconstruct_rate_matrix(ms[c])
+        P, root, pair, _ = _enumerate_rows(aln[:, c:c + 1], tree, [Q] * 5, pis[0])
+        np.testing.assert_allclose(bpost[1:, :, :, c], pair[1:, :, :, 0], atol=1e-11)
+        for n in range(1, 5):
+            np.testing.assert_allclose(npost[n, :, c], pair[n, :, :, 0].sum(0), atol=1e-11)
+        _, root_c, _, _ = _enumerate_rows(aln[:, c:c + 1], tree, [Q] * 5, pis[c])
+        np.testing.assert_allclose(npost[0, :, c], root_c[:, 0], atol=1e-11)
+    # the first column's own root distribution: the plain posteriors
+    np.testing.assert_allclose(npost[..., 0], O.node_posterior(aln[:, :1], tree, ms[0])[..., 0], atol=1e-13)
+
+
 @pytest.mark.parametrize('seed', [0, 1])
 def test_per_branch_models_by_enumeration_and_quadrature(seed):
     rng, tree, aln = _setup(10 + seed, R=5, C=4, A=3)
