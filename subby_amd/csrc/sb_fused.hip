@@ -347,6 +347,7 @@ __global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLau
         s += xr * Wr[q * AP + j] - xi * Wi[q * AP + j];
       }
       o[i * AP + j] = T(s);
+      if (AP > 8) o[sb_bti_Mt(AP) + j * AP + i] = T(s);   // the large-alphabet kernels' transposed copy
       // J_ij (k = i, l = j)
       const double dr = mr[i] - mr[j], di = mi[i] - mi[j];
       double jr, ji;

@@ -531,7 +531,10 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
       return fail(SB_ERR_UNSUPPORTED, "irreversible and reversible models cannot share a batch");
   if (irr && f81) return fail(SB_ERR_UNSUPPORTED, "F81 fast path is reversible-only");
   if (irr && large)
-    return fail(SB_ERR_UNSUPPORTED, "irreversible models need an alphabet of at most 8 states");
+    for (int i = 0; i < nb; ++i)
+      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS | SB_F_GRAD))
+        return fail(SB_ERR_UNSUPPORTED, "irreversible models above an alphabet of 8 run the log-likelihood, "
+                    "root distribution and posteriors only (counts and gradients need at most 8 states)");
   // per-branch models: counts are back-transformed branch by branch (fused ROWS variant)
   bool rowsk = false;
   if (any_rows) {
@@ -547,7 +550,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   bool f81w = large && !lbc && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  const bool gen2 = large && !lpost && !lbc && !f81 && dtype == SB_F32 && !grad;
+  const bool gen2 = large && !lpost && !lbc && !irr && !f81 && dtype == SB_F32 && !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81)
@@ -683,6 +686,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     d.nint = plans[i].nint;
     int fl = in[i].flags;
     if (k.f81_fast) fl |= SB_F_F81;
+    if (m->irr) fl |= SB_F_IRR;
     fl |= (k.mask_mode & 3) << SB_F_MASK_SHIFT;
     d.flags = fl;
     d.A = m->A;

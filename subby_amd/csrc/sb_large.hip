@@ -227,7 +227,6 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   SB_GLOBAL T* Dreg = (SB_GLOBAL T*)(wscr + vbytes);
   SB_GLOBAL int* Uw = (SB_GLOBAL int*)(wscr + 2 * vbytes);
   SB_GLOBAL int* Dw = Uw + NI * TC;
-  const int BTS = sb_bt_stride(AP);
 
   int kx = 0;
   for (;;) {
@@ -254,16 +253,22 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const SB_CONST T* mdc = cst<T>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
     const bool on = lane < AP;
-    const T onea = on ? md[sb_mr_ones(AP) + lane] : T(0);
-    const T pia = on ? md[sb_mr_pi(AP) + lane] : T(0);
-    const T pra = (on && B->root_model) ? glb<T>(B->root_model)[sb_mr_pi(AP) + lane] : pia;   // root prior
-    const bool f81m = mdc[sb_mr_misc(AP) + 1] != T(0);    // F81-structured M
-    const int kstar = int(mdc[sb_mr_misc(AP) + 2]);
+    // irreversible models (log-likelihood, root and posteriors only): their
+    // record layouts; M and its transpose are real
+    const bool irr = (flags & SB_F_IRR) != 0;
+    const int BTS = sb_bt_stride_k(AP, irr);
+    const int o_Mt = irr ? sb_bti_Mt(AP) : sb_bt_Mt(AP);
+    const int o_pi = sb_mr_pi_k(AP, irr), o_misc = irr ? sb_mri_misc(AP) : sb_mr_misc(AP);
+    const T onea = on ? md[sb_mr_ones_k(AP, irr) + lane] : T(0);
+    const T pia = on ? md[o_pi + lane] : T(0);
+    const T pra = (on && B->root_model) ? glb<T>(B->root_model)[o_pi + lane] : pia;   // root prior
+    const bool f81m = mdc[o_misc + 1] != T(0);    // F81-structured M (never for irreversible models)
+    const int kstar = int(mdc[o_misc + 2]);
 
     auto apply_M = [&](int n, const V_& u, V_& w) {
       const SB_GLOBAL T* bn = bt + (long long)n * BTS;
       if (f81m) f81_matvec<T, TC>(bn[sb_bt_E(AP) + kstar], pia, onea, u, w);
-      else contract<T, TC>(bn + sb_bt_Mt(AP), AP, u, w, bc, sm, lane);
+      else contract<T, TC>(bn + o_Mt, AP, u, w, bc, sm, lane);
     };
     auto tokens = [&](int n, int* tok) {
 #pragma unroll
@@ -384,7 +389,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     }
     // log-likelihood, root posterior
     const SB_CONST double* pi64 =
-        B->root_model64 ? cst<double>(B->root_model64) + sb_m64_pi(AP) : cst<double>(B->model64) + sb_m64_pi(AP);
+        B->root_model64 ? cst<double>(B->root_model64) + sb_m64_pi_k(AP, irr)
+                        : cst<double>(B->model64) + sb_m64_pi_k(AP, irr);
     T inv_s0[TC];
     double s0s[TC];
 #pragma unroll
@@ -611,13 +617,13 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
           if (bpost) {
             __syncthreads();
-            if (!f81m) stage<T>(bn + sb_bt_Mt(AP), AP, sm, lane);   // sm[j*64 + lane] = M[lane][j]
+            if (!f81m) stage<T>(bn + o_Mt, AP, sm, lane);   // sm[j*64 + lane] = M[lane][j]
 #pragma unroll
             for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
             __syncthreads();
             const T ef = f81m ? bn[sb_bt_E(AP) + kstar] : T(0);
             for (int j = 0; j < A; ++j) {
-              const T mij = f81m ? ((lane == j ? ef : T(0)) + (T(1) - ef) * md[sb_mr_pi(AP) + j])
+              const T mij = f81m ? ((lane == j ? ef : T(0)) + (T(1) - ef) * md[o_pi + j])
                                  : sm[j * SB_WAVE + lane];
 #pragma unroll
               for (int c = 0; c < TC; ++c)

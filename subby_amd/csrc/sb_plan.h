@@ -13,12 +13,12 @@
 
 #define SB_WAVE 64
 #define SB_MAX_RATE 8
-#define SB_MAX_MIX 64
+#define SB_MAX_MIX 64   // mixture components of one MixturePosterior call
 // Tile counters: counter[0] for kernels with one tile queue; the
 // large-alphabet kernel keeps one queue per XCD at counter[x * SB_XCD_GAP].
 #define SB_NCOUNTER 64
 #define SB_NXCD 8
-#define SB_XCD_GAP 8   // mixture components of one MixturePosterior call
+#define SB_XCD_GAP 8
 #define SB_LARGE_TC 4   // columns per wave tile on the large-alphabet path
 
 // Per-branch parameter record, element type T (float or double), padded
@@ -66,8 +66,10 @@ __host__ __device__ inline int sb_m64_size(int AP) { return 2 * AP + AP * AP + 4
 // followed by an imaginary plane.
 //
 // Branch record (element T):  M[AP*AP] (real) | Jre[AP*AP] | Jim[AP*AP] |
-//   coef[4] = {0, 0, 0, t} | dEre[AP] | dEim[AP]   (dE_k = mu_k e^{mu_k t})
-__host__ __device__ inline int sb_bti_stride(int AP) { return 3 * AP * AP + 4 + 2 * AP; }
+//   coef[4] = {0, 0, 0, t} | dEre[AP] | dEim[AP]   (dE_k = mu_k e^{mu_k t}) |
+//   (AP > 8 only) Mt[AP*AP] = M transposed
+__host__ __device__ inline int sb_bti_stride(int AP) { return (AP > 8 ? 4 : 3) * AP * AP + 4 + 2 * AP; }
+__host__ __device__ inline int sb_bti_Mt(int AP) { return 3 * AP * AP + 4 + 2 * AP; }
 __host__ __device__ inline int sb_bti_Jre(int AP) { return AP * AP; }
 __host__ __device__ inline int sb_bti_Jim(int AP) { return 2 * AP * AP; }
 __host__ __device__ inline int sb_bti_coef(int AP) { return 3 * AP * AP; }
@@ -119,6 +121,7 @@ enum {
   SB_F_BCOUNTS = 128,      // per-branch counts   [R][A][A][C]
   SB_F_NPOST = 256,        // node posteriors     [R][A][C]
   SB_F_BPOST = 512,        // branch joint posteriors [R][A][A][C]
+  SB_F_IRR = 1024,         // irreversible model records (set by the host)
 };
 
 struct SbBlockDesc {

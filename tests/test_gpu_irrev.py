@@ -198,6 +198,33 @@ def test_irrev_per_branch_outputs(sb, eng, A):
         assert e_np <= tp and e_bp <= tp
 
 
+@pytest.mark.parametrize('A,C', [(9, 60), (20, 150), (64, 40)])
+def test_irrev_large_alphabets(sb, eng, A, C):
+    """Irreversible models above A = 8 on the large-alphabet kernel (real M
+    from the complex eigensystem, the irreversible record layouts): LogLike,
+    RootProb and the InsideOutside posteriors vs the oracle, FP64 and FP32;
+    counts and gradients stay at A <= 8."""
+    say('irreversible, A =', A, 'C =', C)
+    Q, pi, m, t, aln = _case(sb, A, 300 + A, leaves=9, C=C)
+    ot, om = otree(t), oirr(m, False)
+    ref_ll, ref_root = O.LogLike(aln, ot, om), O.RootProb(aln, ot, om)
+    rn, rb = O.node_posterior(aln, ot, om), O.branch_posterior(aln, ot, om)
+    for precision in ('fp64', 'fp32'):
+        tol = TOL[precision]
+        tp = 1e-11 if precision == 'fp64' else 1e-5
+        ll = sb.LogLike(aln, t, m, precision=precision, engine=eng)
+        root = sb.RootProb(aln, t, m, precision=precision, engine=eng)
+        io = sb.InsideOutside(aln, t, m, precision=precision, engine=eng)
+        e = dict(ll=err_ll(ll, ref_ll), root=err_root(root, ref_root),
+                 node=float(np.abs(io.node_posterior().cpu().double().numpy() - rn).max()),
+                 branch=float(np.abs(io.branch_posterior().cpu().double().numpy() - rb).max()))
+        say('   ', precision, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
+        assert e['node'] <= tp and e['branch'] <= tp
+    with pytest.raises(sb.SubbyError, match='at most 8'):
+        sb.Counts(aln, t, m, engine=eng)
+
+
 def test_irrev_errors(sb, eng):
     say('irreversible errors')
     rng = np.random.default_rng(61)
@@ -210,7 +237,7 @@ def test_irrev_errors(sb, eng):
     with pytest.raises(sb.SubbyError, match='at most 8'):
         Q9 = _irrev_Q(9, rng)
         m9 = sb.diagonalize_irreversible(Q9, np.full(9, 1 / 9))
-        sb.LogLike(np.zeros((len(t.parentIndex), 4), np.int8), t, m9, engine=eng)
+        sb.Counts(np.zeros((len(t.parentIndex), 4), np.int8), t, m9, engine=eng)
     bad = sb.IrrevDiagModel(m.eigenvalues, m.eigenvectors, 2.0 * np.asarray(m.eigenvectors_inv), m.pi)
     with pytest.raises(sb.SubbyError, match='inverse'):
         eng.model_id(bad)
