@@ -506,15 +506,20 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   // per-branch outputs (branch counts, posteriors): their own small-alphabet
   // kernel; above A = 8 the large-alphabet kernel's posterior or per-branch
   // counts variant, one kind of output per launch
-  bool post = false, lpost = false, lbc = false;
-  for (int i = 0; i < nb; ++i)
+  // (above 8, Counts with per-branch models also run the per-branch counts
+  // variant, summing the branches' back-transformed counts in the output)
+  bool post = false, lpost = false, lbc = false, rows_cnt = false;
+  for (int i = 0; i < nb; ++i) {
     if (in[i].flags & (SB_F_BCOUNTS | SB_F_NPOST | SB_F_BPOST)) post = true;
+    if (in[i].rows && (in[i].flags & SB_F_COUNTS)) rows_cnt = true;
+  }
   if (post && grad) return fail(SB_ERR_UNSUPPORTED, "per-branch outputs cannot be combined with the gradient");
-  if (post && large) {
+  if ((post || rows_cnt) && large) {
     for (int i = 0; i < nb; ++i) {
       const int fl = in[i].flags;
-      const bool p = (fl & (SB_F_NPOST | SB_F_BPOST)) != 0, bcn = (fl & SB_F_BCOUNTS) != 0;
-      if ((fl & SB_F_COUNTS) || (p && bcn))
+      const bool cnt = (fl & SB_F_COUNTS) != 0;
+      const bool p = (fl & (SB_F_NPOST | SB_F_BPOST)) != 0, bcn = (fl & SB_F_BCOUNTS) != 0 || (cnt && in[i].rows);
+      if ((cnt && !in[i].rows) || (p && bcn) || (cnt && (fl & SB_F_BCOUNTS)))
         return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, counts, branch counts and posteriors are "
                     "launched apart");
       lpost = lpost || p;
@@ -535,22 +540,21 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
       if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS | SB_F_GRAD))
         return fail(SB_ERR_UNSUPPORTED, "irreversible models above an alphabet of 8 run the log-likelihood, "
                     "root distribution and posteriors only (counts and gradients need at most 8 states)");
-  // per-branch models: counts are back-transformed branch by branch (fused ROWS variant)
+  // per-branch models: counts are back-transformed branch by branch (fused
+  // ROWS variant; above 8 the large-alphabet per-branch counts variant)
   bool rowsk = false;
   if (any_rows) {
-    if (large || grad || f81)
-      return fail(SB_ERR_UNSUPPORTED, "per-branch models need an alphabet of at most 8, no gradient and "
-                  "no F81 path");
-    for (int i = 0; i < nb; ++i)
-      if (in[i].flags & SB_F_COUNTS) rowsk = true;
-    if (irr && (rowsk || post))
+    if (grad || f81)
+      return fail(SB_ERR_UNSUPPORTED, "per-branch models need no gradient and no F81 path");
+    if (irr && (rows_cnt || post || lbc))
       return fail(SB_ERR_UNSUPPORTED, "counts with per-branch irreversible models are not implemented");
+    rowsk = rows_cnt && !large;
   }
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel
   bool f81w = large && !lbc && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  const bool gen2 = large && !lpost && !lbc && !irr && !f81 && dtype == SB_F32 && !grad;
+  const bool gen2 = large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81)

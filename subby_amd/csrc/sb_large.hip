@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     const T onea = on ? md[sb_mr_ones_k(AP, irr) + lane] : T(0);
     const T pia = on ? md[o_pi + lane] : T(0);
     const T pra = (on && B->root_model) ? glb<T>(B->root_model)[o_pi + lane] : pia;   // root prior
-    const bool f81m = mdc[o_misc + 1] != T(0);    // F81-structured M (never for irreversible models)
+    // F81-structured M (never for irreversible or per-branch models)
+    const bool f81m = mdc[o_misc + 1] != T(0) && B->row_models == nullptr;
     const int kstar = int(mdc[o_misc + 2]);
 
     auto apply_M = [&](int n, const V_& u, V_& w) {
@@ -404,7 +405,13 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         glb<T>(B->root)[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
     }
     SB_GLOBAL T* npost = (POST && (flags & SB_F_NPOST)) ? glb<T>(B->npost) : nullptr;
-    SB_GLOBAL T* bcnt = (BRC && (flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
+    // BRC: per-branch slices of B->bcounts, or (per-branch models, Counts) the
+    // sum over branches accumulated in B->counts
+    const bool rows = B->row_models != nullptr;
+    const bool bsum = BRC && rows && (flags & SB_F_COUNTS) && B->counts;
+    SB_GLOBAL T* bcnt = !BRC ? nullptr
+                        : bsum ? glb<T>(B->counts)
+                        : ((flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
     SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
     if constexpr (POST) {
       // row 0: the root posterior (under the model's own root distribution);
@@ -423,7 +430,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     } else if constexpr (BRC) {
       if (!bcnt) continue;
 #pragma unroll
-      for (int c = 0; c < TC; ++c) {      // branch 0 (above the root) holds no counts
+      for (int c = 0; c < TC; ++c) {      // branch 0 (above the root) holds no counts; a sum starts at 0
         if (!valid[c] || lane >= A) continue;
         for (int jj = 0; jj < A; ++jj) bcnt[((long long)lane * A + jj) * C + cc[c]] = T(0);
       }
@@ -449,24 +456,26 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     T t2a[TC], t2b[TC], t3[TC];
 #pragma unroll
     for (int c = 0; c < TC; ++c) { t2a[c] = T(0); t2b[c] = T(0); t3[c] = T(0); }
-    const SB_GLOBAL T* Vs = md + sb_mr_Vs(AP);
-    const SB_GLOBAL T* Vi = md + sb_mr_Vi(AP);
+    // model record of branch n (per-branch models: its own)
+    auto model_of = [&](int n) -> const SB_GLOBAL T* { return rows ? glb<T>(B->row_models[n]) : md; };
 
     // ---- epilogue: natural-basis counts per column from acc (and t2a/t2b/t3)
     // into out[A][A][C] (null: none); GRAD adds the rate-parameter partials
-    const SB_CONST T* Vc = mdc + sb_mr_V(AP);
-    const SB_GLOBAL T* S = md + sb_mr_S(AP);
     const T fmu = mdc[sb_mr_misc(AP)];
-    if constexpr (!F81 && !POST) {
+    // sv[k*64 + lane] = V[lane][k] of model record mv
+    auto stage_V = [&](const SB_GLOBAL T* mv) {
       __syncthreads();
-      // sv[k*64 + lane] = V[lane][k]
-      const SB_GLOBAL T* Vg = md + sb_mr_V(AP);
+      const SB_GLOBAL T* Vg = mv + sb_mr_V(AP);
       for (int k = 0; k < AP; ++k) sv[k * SB_WAVE + lane] = on ? Vg[lane * AP + k] : T(0);
-    }
+    };
+    if constexpr (!F81 && !POST) stage_V(md);
     double rsum[SB_MAX_RATE];
 #pragma unroll
     for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
-    auto epilogue = [&](SB_GLOBAL T* out) {
+    // out: written, or added to when `add` (the wave owns these columns)
+    auto epilogue = [&](SB_GLOBAL T* out, const SB_GLOBAL T* me, bool add) {
+      const SB_GLOBAL T* Vc = me + sb_mr_V(AP);
+      const SB_GLOBAL T* S = me + sb_mr_S(AP);
 #pragma unroll
       for (int c = 0; c < TC; ++c) {
         // natural-basis counts of column c into accb[j*64 + i] (i = lane = row)
@@ -503,7 +512,10 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
         }
         if (valid[c] && out && lane < A)
-          for (int jj = 0; jj < A; ++jj) out[((long long)lane * A + jj) * C + cc[c]] = accb[jj * SB_WAVE + lane];
+          for (int jj = 0; jj < A; ++jj) {
+            SB_GLOBAL T* o = out + ((long long)lane * A + jj) * C + cc[c];
+            *o = add ? *o + accb[jj * SB_WAVE + lane] : accb[jj * SB_WAVE + lane];
+          }
         if constexpr (GRAD) {
           for (int q = 0; q < B->n_rate; ++q) {
             const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
@@ -656,8 +668,9 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
         } else {
           V_ Ut, Dt;
-          contract<T, TC>(Vs, AP, u, Ut, bc, sm, lane);
-          contract<T, TC>(Vi, AP, d, Dt, bc, sm, lane);
+          const SB_GLOBAL T* mn = model_of(n);
+          contract<T, TC>(mn + sb_mr_Vs(AP), AP, u, Ut, bc, sm, lane);
+          contract<T, TC>(mn + sb_mr_Vi(AP), AP, d, Dt, bc, sm, lane);
           __syncthreads();
           stage<T>(bn + sb_bt_J(AP), AP, sm, lane);   // J is symmetric: row q across lanes
 #pragma unroll
@@ -684,7 +697,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
           }
         }
         if constexpr (BRC) {
-          epilogue(bcnt + (long long)n * A * A * C);
+          if (rows) stage_V(model_of(n));
+          epilogue(bsum ? bcnt : bcnt + (long long)n * A * A * C, model_of(n), bsum);
 #pragma unroll
           for (int c = 0; c < TC; ++c) {
 #pragma unroll
@@ -696,7 +710,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     }
 
     if constexpr (POST || BRC) continue;
-    epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr);
+    epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr, md, false);
     if constexpr (GRAD) {
       SB_GLOBAL double* rpart = glb<double>(L.rate_part) + (long long)(B->tile_base + tidx) * SB_MAX_RATE;
 #pragma unroll

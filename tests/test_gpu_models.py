@@ -259,6 +259,53 @@ def test_per_branch_models(sb, eng):
         sb.Counts(aln, t, [irows], engine=eng)
 
 
+@pytest.mark.parametrize('A', [12, 20])
+def test_per_branch_models_large_alphabets(sb, eng, A):
+    """Model grid (1, R) above A = 8: LogLike and RootProb through the large-
+    alphabet kernel with per-branch M; Counts (every mask) and BranchCounts
+    through its per-branch counts variant with each branch's own eigenbasis
+    and back-transform; irreversible per-branch LogLike; a grid of one model is
+    that model."""
+    say('per-branch models, A =', A)
+    rng = np.random.default_rng(190 + A)
+    t = sb.synthetic.random_tree(8, rng, 0.15)
+    R = len(t.parentIndex)
+    C = 120
+
+    def rev(scale):
+        pi = rng.dirichlet(np.full(A, 3.0))
+        Q = sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi) * scale
+        return sb.diagonalize_rate_matrix(Q, pi)
+    rows = [rev(float(rng.uniform(0.5, 2.0))) for _ in range(R)]
+    aln = sb.synthetic.simulate(t, rows[0], C, rng, gap_frac=0.05, ambig_frac=0.02)
+    orows = [om(x) for x in rows]
+    ot = otree(t)
+    mk = rng.random(aln.shape) < 0.6
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        e = dict(ll=err_ll(sb.LogLike(aln, t, [rows], precision=prec, engine=eng), O.LogLike_rows(aln, ot, orows)),
+                 root=err_root(sb.RootProb(aln, t, [rows], precision=prec, engine=eng),
+                               O.RootProb_rows(aln, ot, orows)))
+        for mask in ('auto', None, mk):
+            cnt = sb.Counts(aln, t, [rows], branch_mask=mask, precision=prec, engine=eng)
+            e['counts'] = max(e.get('counts', 0.0), err_counts(cnt, O.Counts_rows(aln, ot, orows, mask)))
+        e['bcounts'] = err_bc(sb.BranchCounts(aln, t, [rows], precision=prec, engine=eng),
+                              O.BranchCounts_rows(aln, ot, orows))
+        say('   ', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
+        assert max(e['counts'], e['bcounts']) <= tol['counts']
+    got = sb.Counts(aln, t, [[rows[3]] * R], branch_mask=None, precision='fp64', engine=eng)
+    assert err_counts(got, O.Counts(aln, ot, om(rows[3]), branch_mask=None)) <= 1e-10
+    Q = rng.uniform(0.05, 1.0, (A, A))
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    irows = [sb.diagonalize_irreversible(Q * float(rng.uniform(0.5, 2)), rows[0].pi) for _ in range(R)]
+    oir = [O.IrrevModel(*(np.asarray(x) for x in r), False) for r in irows]
+    assert err_ll(sb.LogLike(aln, t, [irows], precision='fp64', engine=eng), O.LogLike_rows(aln, ot, oir)) <= 1e-12
+    with pytest.raises(sb.SubbyError, match='at most 8'):
+        sb.Counts(aln, t, [irows], engine=eng)
+
+
 def test_model_grid(sb, eng):
     """Model grid (C, R): a few distinct rows spread over the columns."""
     say('model grid (C, R)')
