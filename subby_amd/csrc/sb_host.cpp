@@ -36,8 +36,8 @@ extern "C" hipError_t sb_launch_mixture_softmax(const double* ll, double* post, 
                                                 const double* logw, hipStream_t s);
 extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, int out,
                                       int nwaves, const SbLaunch* L, hipStream_t s);
-extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81);
-extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81);
+extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81, int out);
+extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81, int out);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
                                         bool rows, int threads);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad, int out);
@@ -557,7 +557,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   const bool gen2 = large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
-                        : large ? sb_large_tile_cols(dtype, AP, f81)
+                        : large ? sb_large_tile_cols(dtype, AP, f81, lpost ? 1 : lbc ? 2 : 0)
                                 : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
@@ -623,7 +623,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
   long long wave_stride;
   if (f81w) wave_stride = sb_f81w_wave_stride(AP, nint_max);
-  else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81);
+  else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81, lpost ? 1 : lbc ? 2 : 0);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
   if (!gen2) {

@@ -80,6 +80,7 @@ struct TCof {
   static constexpr int v = (sizeof(T) == 8 || (AW == 64 && !F81)) ? 2 : 4;
 };
 
+
 // Per-lane view of TC column vectors: v[c] = component `lane` of column c.
 template <typename T, int TC>
 struct Vec {
@@ -216,6 +217,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   __shared__ T bc[TC * SB_WAVE];
   __shared__ T sm[(POST ? AW : 64) * 64];   // staged AP x AP matrix; also the epilogue accumulator
   __shared__ T sv[(F81 || POST) ? 1 : 64 * 64];   // V^T for the back-transform
+  __shared__ T px[POST ? TC * SB_WAVE : 1];        // posteriors: d_i of each column
+  __shared__ T pnode[POST ? TC * SB_WAVE : 1];     // posteriors: the node row
   T* accb = sm;
   const int lane = threadIdx.x;
   const int gw = blockIdx.x;
@@ -405,6 +408,28 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         glb<T>(B->root)[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_s0[c];
     }
     SB_GLOBAL T* npost = (POST && (flags & SB_F_NPOST)) ? glb<T>(B->npost) : nullptr;
+    SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
+    // POST outputs of node / branch n from LDS: pnode[i*TC + c] (node row n),
+    // px[i*TC + c] = d_i, bc[j*TC + c] = u_j / z and mij(i, j) (branch n > 0).
+    // Lanes run over (row, column) so that each store writes TC consecutive
+    // columns of 64 / TC rows [..][C].
+    const long long col0 = (long long)tidx * TC;
+    const int ncol = int(C - col0 < TC ? C - col0 : TC);
+    const float invA = 1.0f / float(A);
+    auto post_store = [&](int n, auto mij) {
+      if (npost)
+        for (int f = lane; f < A * TC; f += SB_WAVE) {
+          const int i = f / TC, c = f % TC;
+          if (c < ncol) npost[((long long)n * A + i) * C + col0 + c] = pnode[f];
+        }
+      if (bpost)
+        for (int f = lane; f < A * A * TC; f += SB_WAVE) {
+          const int r = f / TC, c = f % TC;
+          const int i = int((float(r) + 0.5f) * invA), j = r - i * A;   // exact: r < 4096, A <= 64
+          if (c < ncol)
+            bpost[((long long)n * A * A + r) * C + col0 + c] = n == 0 ? T(0) : px[i * TC + c] * mij(i, j) * bc[j * TC + c];
+        }
+    };
     // BRC: per-branch slices of B->bcounts, or (per-branch models, Counts) the
     // sum over branches accumulated in B->counts
     const bool rows = B->row_models != nullptr;
@@ -412,20 +437,16 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     SB_GLOBAL T* bcnt = !BRC ? nullptr
                         : bsum ? glb<T>(B->counts)
                         : ((flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
-    SB_GLOBAL T* bpost = (POST && (flags & SB_F_BPOST)) ? glb<T>(B->bpost) : nullptr;
     if constexpr (POST) {
       // row 0: the root posterior (under the model's own root distribution);
       // branch 0 (above the root) is zero
-      T inv_own[TC];
-#pragma unroll
-      for (int c = 0; c < TC; ++c) inv_own[c] = B->root_model ? T(1) / wsum(pia * u0.v[c]) : inv_s0[c];
 #pragma unroll
       for (int c = 0; c < TC; ++c) {
-        if (!valid[c] || lane >= A) continue;
-        if (npost) npost[(long long)lane * C + cc[c]] = pia * u0.v[c] * inv_own[c];
-        if (bpost)
-          for (int jj = 0; jj < A; ++jj) bpost[((long long)lane * A + jj) * C + cc[c]] = T(0);
+        const T inv_own = B->root_model ? T(1) / wsum(pia * u0.v[c]) : inv_s0[c];
+        pnode[lane * TC + c] = pia * u0.v[c] * inv_own;
       }
+      __syncthreads();
+      post_store(0, [&](int, int) { return T(0); });
       if (!(npost || bpost)) continue;
     } else if constexpr (BRC) {
       if (!bcnt) continue;
@@ -473,9 +494,9 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
 #pragma unroll
     for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
     // out: written, or added to when `add` (the wave owns these columns)
-    auto epilogue = [&](SB_GLOBAL T* out, const SB_GLOBAL T* me, bool add) {
-      const SB_GLOBAL T* Vc = me + sb_mr_V(AP);
-      const SB_GLOBAL T* S = me + sb_mr_S(AP);
+    auto epilogue = [&](SB_GLOBAL T* out, const void* me, bool add) {
+      const SB_CONST T* Vc = cst<T>(me) + sb_mr_V(AP);   // uniform reads: scalar loads
+      const SB_GLOBAL T* S = glb<T>(me) + sb_mr_S(AP);
 #pragma unroll
       for (int c = 0; c < TC; ++c) {
         // natural-basis counts of column c into accb[j*64 + i] (i = lane = row)
@@ -614,35 +635,27 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         }
         const SB_GLOBAL T* bn = bt + (long long)n * BTS;
         if constexpr (POST) {
-          V_ wn;
+          V_ wn, y;
           apply_M(n, u, wn);                       // M_n u_n
           T iz[TC];
 #pragma unroll
           for (int c = 0; c < TC; ++c) iz[c] = T(1) / wsum(d.v[c] * wn.v[c]);
-          if (npost) {
-            V_ y;                                  // d^T M_n
-            if (f81m) f81_vecmat<T, TC>(bn[sb_bt_E(AP) + kstar], pia, d, y);
-            else contract<T, TC>(bn + sb_bt_M(AP), AP, d, y, bc, sm, lane);
+          // y = d^T M_n; the contraction leaves M_n staged as sm[i*64 + j] = M[i][j]
+          if (f81m) f81_vecmat<T, TC>(bn[sb_bt_E(AP) + kstar], pia, d, y);
+          else contract<T, TC>(bn + sb_bt_M(AP), AP, d, y, bc, sm, lane);
+          __syncthreads();
 #pragma unroll
-            for (int c = 0; c < TC; ++c)
-              if (valid[c] && lane < A) npost[((long long)n * A + lane) * C + cc[c]] = y.v[c] * u.v[c] * iz[c];
+          for (int c = 0; c < TC; ++c) {
+            px[lane * TC + c] = d.v[c];
+            bc[lane * TC + c] = u.v[c] * iz[c];
+            pnode[lane * TC + c] = y.v[c] * u.v[c] * iz[c];
           }
-          if (bpost) {
-            __syncthreads();
-            if (!f81m) stage<T>(bn + o_Mt, AP, sm, lane);   // sm[j*64 + lane] = M[lane][j]
-#pragma unroll
-            for (int c = 0; c < TC; ++c) bc[c * SB_WAVE + lane] = u.v[c];
-            __syncthreads();
-            const T ef = f81m ? bn[sb_bt_E(AP) + kstar] : T(0);
-            for (int j = 0; j < A; ++j) {
-              const T mij = f81m ? ((lane == j ? ef : T(0)) + (T(1) - ef) * md[o_pi + j])
-                                 : sm[j * SB_WAVE + lane];
-#pragma unroll
-              for (int c = 0; c < TC; ++c)
-                if (valid[c] && lane < A)
-                  bpost[(((long long)n * A + lane) * A + j) * C + cc[c]] = d.v[c] * mij * bc[c * SB_WAVE + j] * iz[c];
-            }
-          }
+          __syncthreads();
+          const T ef = f81m ? bn[sb_bt_E(AP) + kstar] : T(0);
+          auto mij = [&](int i, int j) -> T {
+            return f81m ? ((i == j ? ef : T(0)) + (T(1) - ef) * md[o_pi + j]) : sm[i * SB_WAVE + j];
+          };
+          post_store(n, mij);
         } else if constexpr (F81) {
           const T al = bn[sb_bt_coef(AP) + 0], be = bn[sb_bt_coef(AP) + 1], ga = bn[sb_bt_coef(AP) + 2];
           __syncthreads();
@@ -698,7 +711,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         }
         if constexpr (BRC) {
           if (rows) stage_V(model_of(n));
-          epilogue(bsum ? bcnt : bcnt + (long long)n * A * A * C, model_of(n), bsum);
+          epilogue(bsum ? bcnt : bcnt + (long long)n * A * A * C, rows ? B->row_models[n] : B->model, bsum);
 #pragma unroll
           for (int c = 0; c < TC; ++c) {
 #pragma unroll
@@ -710,7 +723,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     }
 
     if constexpr (POST || BRC) continue;
-    epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr, md, false);
+    epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr, B->model, false);
     if constexpr (GRAD) {
       SB_GLOBAL double* rpart = glb<double>(L.rate_part) + (long long)(B->tile_base + tidx) * SB_MAX_RATE;
 #pragma unroll
@@ -732,7 +745,7 @@ SB_LINST(float, 64)
 SB_LINST(double, 32)
 SB_LINST(double, 64)
 
-static int large_tc(int dtype, int AP, bool f81) {
+static int large_tc(int dtype, int AP, bool f81, int) {
   if (dtype == 0) {
     if (AP <= 32) return TCof<float, 32, false>::v;
     return f81 ? TCof<float, 64, true>::v : TCof<float, 64, false>::v;
@@ -740,11 +753,11 @@ static int large_tc(int dtype, int AP, bool f81) {
   return TCof<double, 64, false>::v;
 }
 
-extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81) { return large_tc(dtype, AP, f81); }
+extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81, int out) { return large_tc(dtype, AP, f81, out); }
 
-extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81) {
+extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81, int out) {
   const long long esz = dtype == 0 ? 4 : 8;
-  const int TC = large_tc(dtype, AP, f81);
+  const int TC = large_tc(dtype, AP, f81, out);
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
 
