@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     SB_GLOBAL double* gpart =
         GRAD ? glb<double>(args.grad_part) + (long long)(B->tile_base + tb.y) * args.grad_stride : nullptr;
 
-    Acc<T, AP, F81, IRR> acc;
+    Acc<T, AP, F81, IRR && !ROWS> acc;   // IRR + ROWS: natural-basis real counts
     acc.zero();
 
     // Internal nodes in preorder.  The next node's children (read-only U
@@ -622,8 +622,11 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         } else if constexpr (IRR) {
           // Complex eigenbasis (eigensub.py:271-385): U~ = u Pu, D~ = d Pd,
           // C[k][q] += D~_k sc J_kq U~_q.  Gradients use the exact set, whose
-          // projections are those of vjp.py:80-88.
-          const SB_CONST T* P = md + sb_mri_set(AP, GRAD ? 1 : 0);
+          // projections are those of vjp.py:80-88.  Per-branch models (ROWS):
+          // branch n in its own model's eigenbasis, back-transformed at once.
+          const SB_CONST T* mI = (ROWS && B->row_models)
+              ? cst<T>((const void*)(cst<unsigned long long>(B->row_models)[n])) : md;
+          const SB_CONST T* P = mI + sb_mri_set(AP, GRAD ? 1 : 0);
           const SB_CONST T* Pu = P + sb_mri_Pu(AP);
           const SB_CONST T* Pd = P + sb_mri_Pd(AP);
           T Ur[AP], Ui[AP], Dr[AP], Di[AP];
@@ -644,6 +647,50 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
           }
           const SB_CONST T* Jr = bn + sb_bti_Jre(AP);
           const SB_CONST T* Ji = bn + sb_bti_Jim(AP);
+          if constexpr (ROWS) {
+            // counts_ij += Re(W_ij (L C Rt)_ij) with C_kq = D~_k sc J_kq U~_q:
+            // row i of L C is t_q = U~_q sum_k (L_ik D~_k sc) J_kq
+            const SB_CONST T* Lr = P + sb_mri_L(AP);
+            const SB_CONST T* Li = Lr + AP * AP;
+            const SB_CONST T* Rr = P + sb_mri_Rt(AP);
+            const SB_CONST T* Ri = Rr + AP * AP;
+            const SB_CONST T* Wr = mI + sb_mri_W(AP);
+            const SB_CONST T* Wi = Wr + AP * AP;
+#pragma unroll
+            for (int i = 0; i < AP; ++i) {
+              T yr[AP], yi[AP];
+#pragma unroll
+              for (int k = 0; k < AP; ++k) {
+                const T lr = Lr[i * AP + k], li = Li[i * AP + k], dr = Dr[k] * sc, di = Di[k] * sc;
+                yr[k] = fma(lr, dr, -li * di);
+                yi[k] = fma(lr, di, li * dr);
+              }
+              T tr[AP], ti[AP];
+#pragma unroll
+              for (int q = 0; q < AP; ++q) {
+                T sr = T(0), si = T(0);
+#pragma unroll
+                for (int k = 0; k < AP; ++k) {
+                  const T jr = Jr[k * AP + q], ji = Ji[k * AP + q];
+                  sr = fma(yr[k], jr, fma(-yi[k], ji, sr));
+                  si = fma(yr[k], ji, fma(yi[k], jr, si));
+                }
+                tr[q] = fma(sr, Ur[q], -si * Ui[q]);
+                ti[q] = fma(sr, Ui[q], si * Ur[q]);
+              }
+#pragma unroll
+              for (int j = 0; j < AP; ++j) {
+                T vr = T(0), vi = T(0);
+#pragma unroll
+                for (int q = 0; q < AP; ++q) {
+                  const T rr = Rr[q * AP + j], ri = Ri[q * AP + j];
+                  vr = fma(tr[q], rr, fma(-ti[q], ri, vr));
+                  vi = fma(tr[q], ri, fma(ti[q], rr, vi));
+                }
+                acc.c[i * AP + j] += fma(Wr[i * AP + j], vr, -Wi[i * AP + j] * vi);
+              }
+            }
+          } else {
 #pragma unroll
           for (int k = 0; k < AP; ++k) {
             const T dr = Dr[k] * sc, di = Di[k] * sc;
@@ -656,6 +703,7 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
               cr = fma(xr, Ur[q], fma(-xi, Ui[q], cr));
               ci = fma(xr, Ui[q], fma(xi, Ur[q], ci));
             }
+          }
           }
           if constexpr (GRAD) {
             // d logL / d t_n = scale * Re sum_k D~_k mu_k e^{mu_k t} U~_k  (vjp.py:92-125)
@@ -879,7 +927,8 @@ __global__ void sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
   SB_INST(T, AP, false, true, false, false)    \
   SB_INST(T, AP, false, false, true, false)    \
   SB_INST(T, AP, false, true, true, false)     \
-  SB_INST(T, AP, false, false, false, true)
+  SB_INST(T, AP, false, false, false, true)    \
+  SB_INST(T, AP, false, false, true, true)
 SB_INST_T(float, 4)
 SB_INST_T(float, 8)
 SB_INST_T(double, 4)
@@ -892,10 +941,11 @@ template __global__ void sb_branch_params_irrev_kernel<double>(const SbLaunch*);
 static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr,
                          bool rows))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
-  if ((irr && f81) || (rows && (irr || f81 || grad))) return nullptr;
+  if ((irr && f81) || (rows && (f81 || grad))) return nullptr;
 #define SB_PICK(T, APV)                                                                   \
   if (AP == APV) {                                                                        \
-    if (rows) k = sb_fused_small_kernel<T, APV, false, false, false, true>;               \
+    if (rows) k = irr ? sb_fused_small_kernel<T, APV, false, false, true, true>           \
+                      : sb_fused_small_kernel<T, APV, false, false, false, true>;         \
     else if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true, false>      \
                            : sb_fused_small_kernel<T, APV, false, false, true, false>;    \
     else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false, false>;          \

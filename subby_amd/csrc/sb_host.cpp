@@ -546,8 +546,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   if (any_rows) {
     if (grad || f81)
       return fail(SB_ERR_UNSUPPORTED, "per-branch models need no gradient and no F81 path");
-    if (irr && (rows_cnt || post || lbc))
-      return fail(SB_ERR_UNSUPPORTED, "counts with per-branch irreversible models are not implemented");
     rowsk = rows_cnt && !large;
   }
   // F81 fast path in FP32 on F81-structured models: MFMA-accumulating kernel

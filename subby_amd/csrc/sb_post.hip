@@ -335,12 +335,15 @@ __global__ void __launch_bounds__(64, (sizeof(T) == 4 && AP == 4) ? 4 : 1) sb_po
         } else if constexpr (IRR) {
           // complex eigenbasis of an irreversible model, per branch
           // (accumulate_C_complex_per_branch / back_transform_irrev_per_branch,
-          // subby/jax/eigensub.py:175-231, 388-434): counts convention set 0
-          const SB_CONST T* Pu = md + sb_mri_set(AP, 0) + sb_mri_Pu(AP);
-          const SB_CONST T* Pd = md + sb_mri_set(AP, 0) + sb_mri_Pd(AP);
-          const SB_CONST T* Lr = md + sb_mri_set(AP, 0) + sb_mri_L(AP);
-          const SB_CONST T* Rr = md + sb_mri_set(AP, 0) + sb_mri_Rt(AP);
-          const SB_CONST T* Wr = md + sb_mri_W(AP);
+          // subby/jax/eigensub.py:175-231, 388-434): counts convention set 0;
+          // per-branch models: branch n's own
+          const SB_CONST T* mI = B->row_models
+              ? cst<T>((const void*)(cst<unsigned long long>(B->row_models)[n])) : md;
+          const SB_CONST T* Pu = mI + sb_mri_set(AP, 0) + sb_mri_Pu(AP);
+          const SB_CONST T* Pd = mI + sb_mri_set(AP, 0) + sb_mri_Pd(AP);
+          const SB_CONST T* Lr = mI + sb_mri_set(AP, 0) + sb_mri_L(AP);
+          const SB_CONST T* Rr = mI + sb_mri_set(AP, 0) + sb_mri_Rt(AP);
+          const SB_CONST T* Wr = mI + sb_mri_W(AP);
           const SB_CONST T* Jr = bt + (long long)n * BTS + sb_bti_Jre(AP);
           const SB_CONST T* Ji = bt + (long long)n * BTS + sb_bti_Jim(AP);
           constexpr int Q2 = AP * AP;

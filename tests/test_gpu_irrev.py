@@ -198,6 +198,28 @@ def test_irrev_per_branch_outputs(sb, eng, A):
         assert e_np <= tp and e_bp <= tp
 
 
+def test_irrev_per_branch_models_a6(sb, eng):
+    """Per-branch irreversible models at A = 6 (the AP = 8 kernels): Counts and
+    BranchCounts with each branch in its own complex eigenbasis vs the oracle."""
+    say('irreversible per-branch models, A = 6')
+    Q, pi, m, t, aln = _case(sb, 6, 71, leaves=10, C=300)
+    R = len(t.parentIndex)
+    rng = np.random.default_rng(72)
+    rows = [sb.diagonalize_irreversible(_irrev_Q(6, rng), pi) for _ in range(R)]
+    orows = [oirr(r, False) for r in rows]
+    for precision in ('fp64', 'fp32'):
+        tol = TOL[precision]
+        e = dict(ll=err_ll(sb.LogLike(aln, t, [rows], precision=precision, engine=eng),
+                           O.LogLike_rows(aln, otree(t), orows)),
+                 counts=err_counts(sb.Counts(aln, t, [rows], precision=precision, engine=eng),
+                                   O.Counts_rows(aln, otree(t), orows, 'auto')))
+        bc = sb.BranchCounts(aln, t, [rows], precision=precision, engine=eng).cpu().double().numpy()
+        ref = O.BranchCounts_rows(aln, otree(t), orows)
+        e['bcounts'] = float(np.max(np.abs(bc - ref).max(axis=(1, 2))[1:] / np.abs(ref).max(axis=(1, 2))[1:]))
+        say('   ', precision, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and max(e['counts'], e['bcounts']) <= tol['counts']
+
+
 @pytest.mark.parametrize('A,C', [(9, 60), (20, 150), (64, 40)])
 def test_irrev_large_alphabets(sb, eng, A, C):
     """Irreversible models above A = 8 on the large-alphabet kernel (real M

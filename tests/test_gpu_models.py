@@ -247,16 +247,28 @@ def test_per_branch_models(sb, eng):
     got = sb.Counts(aln, t, [[m] * R], precision='fp64', engine=eng)
     ref = O.Counts(aln, otree(t), om(m))
     assert err_counts(got, ref) <= 1e-11
-    # irreversible per-branch models: log-likelihood and root (counts are not implemented)
+    # irreversible per-branch models: log-likelihood, root, counts (each branch
+    # in its own complex eigenbasis, the reference's convention) and branch counts
     Q = rng.uniform(0.05, 1.0, (4, 4))
+    for a in range(4):
+        Q[a, (a + 1) % 4] += 1.5                     # complex eigenvalues
     np.fill_diagonal(Q, 0.0)
     np.fill_diagonal(Q, -Q.sum(1))
     irows = [sb.diagonalize_irreversible(Q * float(rng.uniform(0.5, 2)), rows[0].pi) for _ in range(R)]
     oir = [O.IrrevModel(*(np.asarray(x) for x in r), False) for r in irows]
-    assert err_ll(sb.LogLike(aln, t, [irows], precision='fp64', engine=eng),
-                  O.LogLike_rows(aln, otree(t), oir)) <= 1e-12
-    with pytest.raises(sb.SubbyError, match='not implemented'):
-        sb.Counts(aln, t, [irows], engine=eng)
+    for prec in ('fp64', 'fp32'):
+        tol = TOL[prec]
+        e = dict(ll=err_ll(sb.LogLike(aln, t, [irows], precision=prec, engine=eng), O.LogLike_rows(aln, otree(t), oir)),
+                 root=err_root(sb.RootProb(aln, t, [irows], precision=prec, engine=eng),
+                               O.RootProb_rows(aln, otree(t), oir)))
+        for mask in ('auto', None, mk):
+            cnt = sb.Counts(aln, t, [irows], branch_mask=mask, precision=prec, engine=eng)
+            e['counts'] = max(e.get('counts', 0.0), err_counts(cnt, O.Counts_rows(aln, otree(t), oir, mask)))
+        e['bcounts'] = err_bc(sb.BranchCounts(aln, t, [irows], precision=prec, engine=eng),
+                              O.BranchCounts_rows(aln, otree(t), oir))
+        say('    irreversible', prec, {k: f'{v:.2e}' for k, v in e.items()})
+        assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
+        assert max(e['counts'], e['bcounts']) <= tol['counts']
 
 
 @pytest.mark.parametrize('A', [12, 20])
