@@ -73,10 +73,8 @@ int sb_model_drop(sb_context* ctx, int model_id);
  * projections V / V^{-T}, back-transform V C V^{-1}, whose integrand is that
  * of Q^T); exact != 0 gives the expected counts under Q itself (projections
  * V^{-T} / V, back-transform V^{-T} C V^T).  LogLike, RootProb and the
- * gradient do not depend on it.  Counts, branch counts and the gradient of
- * such models run on alphabets of at most 8 states; the log-likelihood, root
- * distribution and node/branch posteriors on any alphabet up to 64 (their
- * passes need only the real M(t)).  The F81 fast path rejects them
+ * gradient do not depend on it.  Every output takes alphabets up to 64
+ * except the gradient (at most 8 states).  The F81 fast path rejects them
  * (SB_ERR_UNSUPPORTED). */
 int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues,
                        const double* eigenvectors, const double* eigenvectors_inv,

@@ -340,7 +340,7 @@ def Counts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag: bool 
     """Expected substitution counts (off-diagonal) and dwell times (diagonal), (A, A, C)
     (subby/jax/__init__.py:338-447).  ``branch_mask``: "auto" (Steiner mask of
     the ungapped leaves), None, or an (R, C) bool array.  For an IrrevDiagModel
-    (counts: alphabets of at most 8 states) ``irrev_convention`` is "reference" (the
+    ``irrev_convention`` is "reference" (the
     reference's formulas, eigensub.py:271-434) or "exact" (expected counts
     under Q itself).  Model lists, grids and stacked models as in LogLike."""
     if _is_model_list(model):

@@ -379,9 +379,10 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
       SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
   } else if (key.large == 2)
     SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
-  else if (key.large)   // 1 counts, 5 posteriors, 6 per-branch counts
+  else if (key.large)   // 1 counts, 5 posteriors, 6 per-branch counts, 7 irreversible counts
     SB_HIP(sb_launch_large(key.dtype, key.AP, key.f81, key.grad,
-                           key.large == 5 ? 1 : key.large == 6 ? 2 : 0, nwaves, b->header, s));
+                           key.large == 5 ? 1 : key.large == 6 ? 2 : key.large == 7 ? 3 : 0, nwaves,
+                           b->header, s));
   else
     SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0, grid,
                                  block, b->header, s));
@@ -535,11 +536,19 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     if (mods[i]->irr != irr)
       return fail(SB_ERR_UNSUPPORTED, "irreversible and reversible models cannot share a batch");
   if (irr && f81) return fail(SB_ERR_UNSUPPORTED, "F81 fast path is reversible-only");
-  if (irr && large)
-    for (int i = 0; i < nb; ++i)
-      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS | SB_F_GRAD))
-        return fail(SB_ERR_UNSUPPORTED, "irreversible models above an alphabet of 8 run the log-likelihood, "
-                    "root distribution and posteriors only (counts and gradients need at most 8 states)");
+  // irreversible models above 8: counts and branch counts in the large-alphabet
+  // kernel's complex variant; no gradient
+  bool lirr = false;
+  if (irr && large) {
+    for (int i = 0; i < nb; ++i) {
+      if (in[i].flags & SB_F_GRAD)
+        return fail(SB_ERR_UNSUPPORTED, "gradients of irreversible models need an alphabet of at most 8 states");
+      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS)) lirr = true;
+    }
+    if (lirr && lpost)
+      return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, counts and posteriors are launched apart");
+    lbc = false;
+  }
   // per-branch models: counts are back-transformed branch by branch (fused
   // ROWS variant; above 8 the large-alphabet per-branch counts variant)
   bool rowsk = false;
@@ -555,7 +564,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   const bool gen2 = large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
-                        : large ? sb_large_tile_cols(dtype, AP, f81, lpost ? 1 : lbc ? 2 : 0)
+                        : large ? sb_large_tile_cols(dtype, AP, f81, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
                                 : SB_WAVE;
   for (int i = 0; i < nb; ++i) ntiles += (size_t(in[i].blk.C) + tile_cols - 1) / tile_cols;
 
@@ -601,7 +610,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   if (wpc <= 0) {
     const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
                     : f81w  ? sb_f81w_occupancy(AP)
-                    : large ? sb_large_occupancy(dtype, AP, f81, grad, lpost ? 1 : lbc ? 2 : 0)
+                    : large ? sb_large_occupancy(dtype, AP, f81, grad, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
                             : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);
     wpc = occ > 0 ? occ * wpg : 8;
   }
@@ -621,7 +630,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
   long long wave_stride;
   if (f81w) wave_stride = sb_f81w_wave_stride(AP, nint_max);
-  else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81, lpost ? 1 : lbc ? 2 : 0);
+  else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
   if (!gen2) {
@@ -754,7 +763,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   b->staged_pending = true;
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
-               lpost ? 5 : lbc ? 6 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
+               lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
                nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);

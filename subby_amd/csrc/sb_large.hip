@@ -202,14 +202,20 @@ __device__ inline int claim_tile(int* counter, int ntiles, int lane, int& k) {
 //               (d^T M) * u / z and branch joint posterior d_i M_ij u_j / z,
 //               z = d . (M u) (the scales cancel);
 //  LOUT_BRANCH  per-branch counts (BranchCounts, subby/jax/__init__.py:450-540):
-//               the epilogue runs after every branch into that branch's slice.
-enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2 };
+//               the epilogue runs after every branch into that branch's slice;
+//  LOUT_IRR     counts or per-branch counts of irreversible models (complex
+//               eigensystem, eigensub.py:234-434): each branch's
+//               Re(W o (L C_n Rt)) is added to natural-basis accumulators
+//               (lane = column j of the counts, acc[c][i] = row i).
+enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2, LOUT_IRR = 3 };
 
 template <typename T, int AW, bool F81, bool GRAD, int OUT = LOUT_COUNTS>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
-  constexpr bool POST = OUT == LOUT_POST, BRC = OUT == LOUT_BRANCH;
-  static_assert(!(GRAD && OUT != LOUT_COUNTS) && !(F81 && POST), "unsupported variant");
-  constexpr int TC = TCof<T, AW, F81>::v;
+  constexpr bool POST = OUT == LOUT_POST, BRC = OUT == LOUT_BRANCH, IRC = OUT == LOUT_IRR;
+  static_assert(!(GRAD && OUT != LOUT_COUNTS) && !(F81 && (POST || IRC)), "unsupported variant");
+  // irreversible counts hold their accumulators beside complex temporaries:
+  // one column per wave where 2 would not fit the registers
+  constexpr int TC = (IRC && (sizeof(T) == 8 || AW == 64)) ? 1 : TCof<T, AW, F81>::v;
   typedef Vec<T, TC> V_;
   typedef Raw<T, TC> R_;
   typedef Flags<TC> F_;
@@ -219,6 +225,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   __shared__ T sv[(F81 || POST) ? 1 : 64 * 64];   // V^T for the back-transform
   __shared__ T px[POST ? TC * SB_WAVE : 1];        // posteriors: d_i of each column
   __shared__ T pnode[POST ? TC * SB_WAVE : 1];     // posteriors: the node row
+  __shared__ T dim_[IRC ? TC * SB_WAVE : 1];       // irreversible: Im of the projected outside vectors
+  __shared__ T yrow[IRC ? 2 * SB_WAVE : 1];         // irreversible: one row of L C_n (re | im)
   T* accb = sm;
   const int lane = threadIdx.x;
   const int gw = blockIdx.x;
@@ -437,6 +445,11 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     SB_GLOBAL T* bcnt = !BRC ? nullptr
                         : bsum ? glb<T>(B->counts)
                         : ((flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts) : nullptr;
+    // IRC: per-branch slices (BranchCounts) or the sum (Counts)
+    SB_GLOBAL T* icnt = !IRC ? nullptr
+                        : ((flags & SB_F_BCOUNTS) && B->bcounts) ? glb<T>(B->bcounts)
+                        : ((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr;
+    const bool islice = IRC && (flags & SB_F_BCOUNTS);
     if constexpr (POST) {
       // row 0: the root posterior (under the model's own root distribution);
       // branch 0 (above the root) is zero
@@ -448,6 +461,14 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       __syncthreads();
       post_store(0, [&](int, int) { return T(0); });
       if (!(npost || bpost)) continue;
+    } else if constexpr (IRC) {
+      if (!icnt) continue;
+      if (islice)
+#pragma unroll
+        for (int c = 0; c < TC; ++c) {    // branch 0 (above the root) holds no counts
+          if (!valid[c] || lane >= A) continue;
+          for (int jj = 0; jj < A; ++jj) icnt[((long long)lane * A + jj) * C + cc[c]] = T(0);
+        }
     } else if constexpr (BRC) {
       if (!bcnt) continue;
 #pragma unroll
@@ -477,6 +498,14 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     T t2a[TC], t2b[TC], t3[TC];
 #pragma unroll
     for (int c = 0; c < TC; ++c) { t2a[c] = T(0); t2b[c] = T(0); t3[c] = T(0); }
+    // acc (lane = column j, acc[c][i] = row i) into out[i][j][C]
+    auto irc_store = [&](SB_GLOBAL T* out) {
+#pragma unroll
+      for (int c = 0; c < TC; ++c)
+#pragma unroll
+        for (int i = 0; i < AW; ++i)
+          if (i < A && valid[c] && lane < A) out[((long long)i * A + lane) * C + cc[c]] = acc[c][i];
+    };
     // model record of branch n (per-branch models: its own)
     auto model_of = [&](int n) -> const SB_GLOBAL T* { return rows ? glb<T>(B->row_models[n]) : md; };
 
@@ -489,7 +518,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       const SB_GLOBAL T* Vg = mv + sb_mr_V(AP);
       for (int k = 0; k < AP; ++k) sv[k * SB_WAVE + lane] = on ? Vg[lane * AP + k] : T(0);
     };
-    if constexpr (!F81 && !POST) stage_V(md);
+    if constexpr (!F81 && !POST && !IRC) stage_V(md);
     double rsum[SB_MAX_RATE];
 #pragma unroll
     for (int q = 0; q < SB_MAX_RATE; ++q) rsum[q] = 0.0;
@@ -679,6 +708,68 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
             t2b[c] = fma(be * Dsum[c], u.v[c], t2b[c]);
             t3[c] = fma(ga * Dsum[c], piU[c], t3[c]);
           }
+        } else if constexpr (IRC) {
+          // U~ = u Pu, D~ = d Pd (complex; the model's counts convention, set 0)
+          const SB_GLOBAL T* mI = model_of(n);
+          const SB_GLOBAL T* P0 = mI + sb_mri_set(AP, 0);
+          V_ Ur, Ui, Dr, Di;
+          contract<T, TC>(P0 + sb_mri_Pu(AP), AP, u, Ur, bc, sm, lane);
+          contract<T, TC>(P0 + sb_mri_Pu(AP) + AP * AP, AP, u, Ui, bc, sm, lane);
+          contract<T, TC>(P0 + sb_mri_Pd(AP), AP, d, Dr, bc, sm, lane);
+          contract<T, TC>(P0 + sb_mri_Pd(AP) + AP * AP, AP, d, Di, bc, sm, lane);
+          __syncthreads();
+          stage<T>(bn + sb_bti_Jre(AP), AP, sm, lane);   // sm[k*64 + q] = Re J_kq
+          stage<T>(bn + sb_bti_Jim(AP), AP, sv, lane);   // sv[k*64 + q] = Im J_kq
+#pragma unroll
+          for (int c = 0; c < TC; ++c) {
+            bc[c * SB_WAVE + lane] = Dr.v[c] * sc[c];
+            dim_[c * SB_WAVE + lane] = Di.v[c] * sc[c];
+          }
+          __syncthreads();
+          const SB_CONST T* Lr = cst<T>(rows ? B->row_models[n] : B->model) + sb_mri_set(AP, 0) + sb_mri_L(AP);
+          const SB_CONST T* Li = Lr + AP * AP;
+          const SB_GLOBAL T* Rr = P0 + sb_mri_Rt(AP);
+          const SB_GLOBAL T* Ri = Rr + AP * AP;
+          const SB_GLOBAL T* Wr = mI + sb_mri_W(AP);
+          const SB_GLOBAL T* Wi = Wr + AP * AP;
+#pragma unroll
+          for (int c = 0; c < TC; ++c) {
+#pragma unroll
+            for (int i = 0; i < AW; ++i) {
+              if (i >= AP) continue;
+              // (L C_n)[i][q = lane] = U~_q sum_k (L_ik D~_k sc) J_kq
+              T sr = T(0), si = T(0);
+              for (int k = 0; k < AP; ++k) {
+                const T lr = Lr[i * AP + k], li = Li[i * AP + k];
+                const T dr = bc[c * SB_WAVE + k], di = dim_[c * SB_WAVE + k];
+                const T zr = fma(lr, dr, -li * di), zi = fma(lr, di, li * dr);
+                const T jr = sm[k * SB_WAVE + lane], ji = sv[k * SB_WAVE + lane];
+                sr = fma(zr, jr, fma(-zi, ji, sr));
+                si = fma(zr, ji, fma(zi, jr, si));
+              }
+              __syncthreads();
+              yrow[lane] = fma(sr, Ur.v[c], -si * Ui.v[c]);
+              yrow[SB_WAVE + lane] = fma(sr, Ui.v[c], si * Ur.v[c]);
+              __syncthreads();
+              // (L C_n Rt)[i][j = lane], then counts_ij += Re(W_ij (L C_n Rt)_ij)
+              T vr = T(0), vi = T(0);
+              for (int q = 0; q < AP; ++q) {
+                const T rr = on ? Rr[q * AP + lane] : T(0), ri = on ? Ri[q * AP + lane] : T(0);
+                const T yr = yrow[q], yi = yrow[SB_WAVE + q];
+                vr = fma(yr, rr, fma(-yi, ri, vr));
+                vi = fma(yr, ri, fma(yi, rr, vi));
+              }
+              const T wr = on ? Wr[i * AP + lane] : T(0), wi = on ? Wi[i * AP + lane] : T(0);
+              acc[c][i] += fma(wr, vr, -wi * vi);
+            }
+          }
+          if (islice) {
+            irc_store(icnt + (long long)n * A * A * C);
+#pragma unroll
+            for (int c = 0; c < TC; ++c)
+#pragma unroll
+              for (int q = 0; q < AW; ++q) acc[c][q] = T(0);
+          }
         } else {
           V_ Ut, Dt;
           const SB_GLOBAL T* mn = model_of(n);
@@ -722,6 +813,10 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       }
     }
 
+    if constexpr (IRC) {
+      if (!islice) irc_store(icnt);
+      continue;
+    }
     if constexpr (POST || BRC) continue;
     epilogue(((flags & SB_F_COUNTS) && B->counts) ? glb<T>(B->counts) : nullptr, B->model, false);
     if constexpr (GRAD) {
@@ -739,13 +834,15 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   template __global__ void sb_large_kernel<T, AW, false, true>(const SbLaunch*);                \
   template __global__ void sb_large_kernel<T, AW, false, false, LOUT_POST>(const SbLaunch*);    \
   template __global__ void sb_large_kernel<T, AW, false, false, LOUT_BRANCH>(const SbLaunch*);  \
-  template __global__ void sb_large_kernel<T, AW, true, false, LOUT_BRANCH>(const SbLaunch*);
+  template __global__ void sb_large_kernel<T, AW, true, false, LOUT_BRANCH>(const SbLaunch*);    \
+  template __global__ void sb_large_kernel<T, AW, false, false, LOUT_IRR>(const SbLaunch*);
 SB_LINST(float, 32)
 SB_LINST(float, 64)
 SB_LINST(double, 32)
 SB_LINST(double, 64)
 
-static int large_tc(int dtype, int AP, bool f81, int) {
+static int large_tc(int dtype, int AP, bool f81, int out) {
+  if (out == LOUT_IRR && (dtype != 0 || AP > 32)) return 1;
   if (dtype == 0) {
     if (AP <= 32) return TCof<float, 32, false>::v;
     return f81 ? TCof<float, 64, true>::v : TCof<float, 64, false>::v;
@@ -761,12 +858,13 @@ extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool 
   return (long long)nint_max * (2 * TC * SB_WAVE * esz + 2 * TC * 4);
 }
 
-// out: 0 counts (and gradient), 1 posteriors, 2 per-branch counts
+// out: 0 counts (and gradient), 1 posteriors, 2 per-branch counts, 3 irreversible counts
 static void (*pick_large(int dtype, int AP, bool f81, bool grad, int out))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
 #define SB_LPICK(T, AWV)                                                          \
   {                                                                               \
     if (out == LOUT_POST) k = sb_large_kernel<T, AWV, false, false, LOUT_POST>;  \
+    else if (out == LOUT_IRR) k = sb_large_kernel<T, AWV, false, false, LOUT_IRR>; \
     else if (out == LOUT_BRANCH)                                                  \
       k = f81 ? sb_large_kernel<T, AWV, true, false, LOUT_BRANCH>                 \
               : sb_large_kernel<T, AWV, false, false, LOUT_BRANCH>;               \

@@ -225,7 +225,8 @@ def test_irrev_large_alphabets(sb, eng, A, C):
     """Irreversible models above A = 8 on the large-alphabet kernel (real M
     from the complex eigensystem, the irreversible record layouts): LogLike,
     RootProb and the InsideOutside posteriors vs the oracle, FP64 and FP32;
-    counts and gradients stay at A <= 8."""
+    Counts (both conventions, masks auto/none) and BranchCounts through the
+    complex counts variant; gradients stay at A <= 8."""
     say('irreversible, A =', A, 'C =', C)
     Q, pi, m, t, aln = _case(sb, A, 300 + A, leaves=9, C=C)
     ot, om = otree(t), oirr(m, False)
@@ -243,8 +244,22 @@ def test_irrev_large_alphabets(sb, eng, A, C):
         say('   ', precision, {k: f'{v:.2e}' for k, v in e.items()})
         assert e['ll'] <= tol['ll'] and e['root'] <= tol['root']
         assert e['node'] <= tp and e['branch'] <= tp
+        # counts (both conventions) and branch counts: the complex variant
+        tc = 1e-10 if precision == 'fp64' else tol['counts']
+        for exact in (False, True):
+            conv = 'exact' if exact else 'reference'
+            for mask in ('auto', None):
+                cnt = sb.Counts(aln, t, m, branch_mask=mask, precision=precision, engine=eng, irrev_convention=conv)
+                rc, _, _ = O.counts_loglike_root(np.asarray(aln), ot, oirr(m, exact), branch_mask=mask)
+                ec = err_counts(cnt, rc)
+                assert ec <= tc, (precision, conv, mask, ec)
+        bc = sb.BranchCounts(aln, t, m, precision=precision, engine=eng).cpu().double().numpy()
+        ref = O.BranchCounts(np.asarray(aln), ot, om)
+        eb = float(np.max(np.abs(bc - ref).max(axis=(1, 2))[1:] / np.abs(ref).max(axis=(1, 2))[1:]))
+        say('   ', precision, f'counts {ec:.2e} branch counts {eb:.2e}')
+        assert eb <= tc and float(np.abs(bc[0]).max()) == 0.0
     with pytest.raises(sb.SubbyError, match='at most 8'):
-        sb.Counts(aln, t, m, engine=eng)
+        sb.loglike_grad(aln, t, m, engine=eng)
 
 
 def test_irrev_errors(sb, eng):
@@ -259,7 +274,7 @@ def test_irrev_errors(sb, eng):
     with pytest.raises(sb.SubbyError, match='at most 8'):
         Q9 = _irrev_Q(9, rng)
         m9 = sb.diagonalize_irreversible(Q9, np.full(9, 1 / 9))
-        sb.Counts(np.zeros((len(t.parentIndex), 4), np.int8), t, m9, engine=eng)
+        sb.loglike_grad(np.zeros((len(t.parentIndex), 4), np.int8), t, m9, engine=eng)
     bad = sb.IrrevDiagModel(m.eigenvalues, m.eigenvectors, 2.0 * np.asarray(m.eigenvectors_inv), m.pi)
     with pytest.raises(sb.SubbyError, match='inverse'):
         eng.model_id(bad)

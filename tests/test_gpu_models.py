@@ -314,8 +314,13 @@ def test_per_branch_models_large_alphabets(sb, eng, A):
     irows = [sb.diagonalize_irreversible(Q * float(rng.uniform(0.5, 2)), rows[0].pi) for _ in range(R)]
     oir = [O.IrrevModel(*(np.asarray(x) for x in r), False) for r in irows]
     assert err_ll(sb.LogLike(aln, t, [irows], precision='fp64', engine=eng), O.LogLike_rows(aln, ot, oir)) <= 1e-12
-    with pytest.raises(sb.SubbyError, match='at most 8'):
-        sb.Counts(aln, t, [irows], engine=eng)
+    for prec in ('fp64', 'fp32'):
+        tc = 1e-10 if prec == 'fp64' else TOL[prec]['counts']
+        ec = err_counts(sb.Counts(aln, t, [irows], branch_mask=mk, precision=prec, engine=eng),
+                        O.Counts_rows(aln, ot, oir, mk))
+        eb = err_bc(sb.BranchCounts(aln, t, [irows], precision=prec, engine=eng), O.BranchCounts_rows(aln, ot, oir))
+        say('    irreversible', prec, f'counts {ec:.2e} branch counts {eb:.2e}')
+        assert ec <= tc and eb <= tc
 
 
 def test_model_grid(sb, eng):
