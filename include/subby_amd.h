@@ -73,9 +73,8 @@ int sb_model_drop(sb_context* ctx, int model_id);
  * projections V / V^{-T}, back-transform V C V^{-1}, whose integrand is that
  * of Q^T); exact != 0 gives the expected counts under Q itself (projections
  * V^{-T} / V, back-transform V^{-T} C V^T).  LogLike, RootProb and the
- * gradient do not depend on it.  Every output takes alphabets up to 64
- * except the gradient (at most 8 states).  The F81 fast path rejects them
- * (SB_ERR_UNSUPPORTED). */
+ * gradient do not depend on it.  Every output, the gradient included, takes
+ * alphabets up to 64.  The F81 fast path rejects them (SB_ERR_UNSUPPORTED). */
 int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues,
                        const double* eigenvectors, const double* eigenvectors_inv,
                        const double* pi, int exact, int* model_id);

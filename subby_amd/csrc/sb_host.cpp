@@ -536,15 +536,12 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     if (mods[i]->irr != irr)
       return fail(SB_ERR_UNSUPPORTED, "irreversible and reversible models cannot share a batch");
   if (irr && f81) return fail(SB_ERR_UNSUPPORTED, "F81 fast path is reversible-only");
-  // irreversible models above 8: counts and branch counts in the large-alphabet
-  // kernel's complex variant; no gradient
+  // irreversible models above 8: counts, branch counts and gradients in the
+  // large-alphabet kernel's complex variant
   bool lirr = false;
   if (irr && large) {
-    for (int i = 0; i < nb; ++i) {
-      if (in[i].flags & SB_F_GRAD)
-        return fail(SB_ERR_UNSUPPORTED, "gradients of irreversible models need an alphabet of at most 8 states");
-      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS)) lirr = true;
-    }
+    for (int i = 0; i < nb; ++i)
+      if (in[i].flags & (SB_F_COUNTS | SB_F_BCOUNTS | SB_F_GRAD)) lirr = true;
     if (lirr && lpost)
       return fail(SB_ERR_UNSUPPORTED, "above an alphabet of 8, counts and posteriors are launched apart");
     lbc = false;

@@ -212,7 +212,7 @@ enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2, LOUT_IRR = 3 };
 template <typename T, int AW, bool F81, bool GRAD, int OUT = LOUT_COUNTS>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
   constexpr bool POST = OUT == LOUT_POST, BRC = OUT == LOUT_BRANCH, IRC = OUT == LOUT_IRR;
-  static_assert(!(GRAD && OUT != LOUT_COUNTS) && !(F81 && (POST || IRC)), "unsupported variant");
+  static_assert(!(GRAD && OUT != LOUT_COUNTS && OUT != LOUT_IRR) && !(F81 && (POST || IRC)), "unsupported variant");
   // irreversible counts hold their accumulators beside complex temporaries:
   // one column per wave where 2 would not fit the registers
   constexpr int TC = (IRC && (sizeof(T) == 8 || AW == 64)) ? 1 : TCof<T, AW, F81>::v;
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
       post_store(0, [&](int, int) { return T(0); });
       if (!(npost || bpost)) continue;
     } else if constexpr (IRC) {
-      if (!icnt) continue;
+      if (!icnt && !GRAD) continue;
       if (islice)
 #pragma unroll
         for (int c = 0; c < TC; ++c) {    // branch 0 (above the root) holds no counts
@@ -709,14 +709,27 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
             t3[c] = fma(ga * Dsum[c], piU[c], t3[c]);
           }
         } else if constexpr (IRC) {
-          // U~ = u Pu, D~ = d Pd (complex; the model's counts convention, set 0)
+          // U~ = u Pu, D~ = d Pd (complex; the model's counts convention, set 0;
+          // gradients: the exact set 1, vjp.py:80-131)
           const SB_GLOBAL T* mI = model_of(n);
-          const SB_GLOBAL T* P0 = mI + sb_mri_set(AP, 0);
+          const SB_GLOBAL T* P0 = mI + sb_mri_set(AP, GRAD ? 1 : 0);
           V_ Ur, Ui, Dr, Di;
           contract<T, TC>(P0 + sb_mri_Pu(AP), AP, u, Ur, bc, sm, lane);
           contract<T, TC>(P0 + sb_mri_Pu(AP) + AP * AP, AP, u, Ui, bc, sm, lane);
           contract<T, TC>(P0 + sb_mri_Pd(AP), AP, d, Dr, bc, sm, lane);
           contract<T, TC>(P0 + sb_mri_Pd(AP) + AP * AP, AP, d, Di, bc, sm, lane);
+          if constexpr (GRAD) {
+            // d logL / d t_n = scale * Re sum_k D~_k mu_k e^{mu_k t} U~_k  (vjp.py:92-125)
+            const T er = on ? bn[sb_bti_dEre(AP) + lane] : T(0), ei = on ? bn[sb_bti_dEim(AP) + lane] : T(0);
+            double v = 0.0;
+#pragma unroll
+            for (int c = 0; c < TC; ++c) {
+              const T yr = fma(Dr.v[c], er, -Di.v[c] * ei), yi = fma(Dr.v[c], ei, Di.v[c] * er);
+              v += gcol[c] * double(scr[c]) * double(wsum(fma(yr, Ur.v[c], -yi * Ui.v[c])));
+            }
+            if (lane == 0) gpart[n] = v;
+            if (B->n_rate == 0) continue;   // no counts needed
+          }
           __syncthreads();
           stage<T>(bn + sb_bti_Jre(AP), AP, sm, lane);   // sm[k*64 + q] = Re J_kq
           stage<T>(bn + sb_bti_Jim(AP), AP, sv, lane);   // sv[k*64 + q] = Im J_kq
@@ -726,7 +739,7 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
             dim_[c * SB_WAVE + lane] = Di.v[c] * sc[c];
           }
           __syncthreads();
-          const SB_CONST T* Lr = cst<T>(rows ? B->row_models[n] : B->model) + sb_mri_set(AP, 0) + sb_mri_L(AP);
+          const SB_CONST T* Lr = cst<T>(rows ? B->row_models[n] : B->model) + sb_mri_set(AP, GRAD ? 1 : 0) + sb_mri_L(AP);
           const SB_CONST T* Li = Lr + AP * AP;
           const SB_GLOBAL T* Rr = P0 + sb_mri_Rt(AP);
           const SB_GLOBAL T* Ri = Rr + AP * AP;
@@ -814,7 +827,24 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
     }
 
     if constexpr (IRC) {
-      if (!islice) irc_store(icnt);
+      if (!islice && icnt) irc_store(icnt);
+      if constexpr (GRAD) {
+        // rate parameters: sum_ij W_q[i][j] counts_ij (exact counts), lane = j
+        SB_GLOBAL double* rpart = glb<double>(L.rate_part) + (long long)(B->tile_base + tidx) * SB_MAX_RATE;
+        for (int q = 0; q < B->n_rate; ++q) {
+          const SB_CONST double* W = cst<double>(B->rate_w) + (long long)q * AP * AP;
+          double v = 0.0;
+#pragma unroll
+          for (int c = 0; c < TC; ++c) {
+            double sq = 0.0;
+#pragma unroll
+            for (int i = 0; i < AW; ++i)
+              if (i < AP && on) sq = fma(W[i * AP + lane], double(acc[c][i]), sq);
+            v += gcol[c] * wsum(sq);
+          }
+          if (lane == 0) rpart[q] = v;
+        }
+      }
       continue;
     }
     if constexpr (POST || BRC) continue;
@@ -835,7 +865,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   template __global__ void sb_large_kernel<T, AW, false, false, LOUT_POST>(const SbLaunch*);    \
   template __global__ void sb_large_kernel<T, AW, false, false, LOUT_BRANCH>(const SbLaunch*);  \
   template __global__ void sb_large_kernel<T, AW, true, false, LOUT_BRANCH>(const SbLaunch*);    \
-  template __global__ void sb_large_kernel<T, AW, false, false, LOUT_IRR>(const SbLaunch*);
+  template __global__ void sb_large_kernel<T, AW, false, false, LOUT_IRR>(const SbLaunch*);      \
+  template __global__ void sb_large_kernel<T, AW, false, true, LOUT_IRR>(const SbLaunch*);
 SB_LINST(float, 32)
 SB_LINST(float, 64)
 SB_LINST(double, 32)
@@ -864,7 +895,9 @@ static void (*pick_large(int dtype, int AP, bool f81, bool grad, int out))(const
 #define SB_LPICK(T, AWV)                                                          \
   {                                                                               \
     if (out == LOUT_POST) k = sb_large_kernel<T, AWV, false, false, LOUT_POST>;  \
-    else if (out == LOUT_IRR) k = sb_large_kernel<T, AWV, false, false, LOUT_IRR>; \
+    else if (out == LOUT_IRR)                                                     \
+      k = grad ? sb_large_kernel<T, AWV, false, true, LOUT_IRR>                   \
+               : sb_large_kernel<T, AWV, false, false, LOUT_IRR>;                 \
     else if (out == LOUT_BRANCH)                                                  \
       k = f81 ? sb_large_kernel<T, AWV, true, false, LOUT_BRANCH>                 \
               : sb_large_kernel<T, AWV, false, false, LOUT_BRANCH>;               \
