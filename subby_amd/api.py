@@ -412,11 +412,18 @@ def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag:
     """Per-branch counts and dwell times, (R, A, A, C), branch 0 zeros
     (subby/jax/__init__.py:450-540).  subby.jax applies no branch mask here; the
     default ``branch_mask=None`` does the same ("auto" or an (R, C) array as in
-    Counts).  Any alphabet (above 8 states: one branch-selecting block per branch on
-    the large-alphabet kernels).  Model lists and grids as in LogLike."""
+    Counts).  Any alphabet up to 64 (above 8 states: the large-alphabet kernel's
+    per-branch counts variant).  Model lists and grids as in LogLike; stacked
+    models give (*H, R, A, A, C)."""
     if _is_model_list(model):
         return _multi(alignment, tree, model, {'bcounts'}, precision, engine, f81_fast_flag,
                       branch_mask, irrev_convention)['bcounts']
+    st = _stacked(model)
+    if st is not None:
+        H, flat = st
+        outs = [_inside_outside(alignment, tree, m, {'bcounts'}, f81_fast_flag, branch_mask, precision,
+                                engine, irrev_convention)['bcounts'] for m in flat]
+        return torch.stack(outs).reshape(tuple(H) + tuple(outs[0].shape))
     return _inside_outside(alignment, tree, model, {'bcounts'}, f81_fast_flag, branch_mask,
                            precision, engine, irrev_convention)['bcounts']
 
@@ -431,7 +438,8 @@ class InsideOutside:
     list (C models); as in the reference, a list's passes run under each
     column's matrices with the root distribution of the first model, the root
     row of ``node_posterior`` uses each column's own, and ``counts`` /
-    ``branch_counts`` are those of Counts / BranchCounts of the list.  Model
+    ``branch_counts`` are those of Counts / BranchCounts of the list.  Stacked
+    models (leading *H axes) give every query with those leading axes.  Model
     grids raise NotImplementedError, as in the reference.
     """
 
@@ -441,11 +449,12 @@ class InsideOutside:
         if _is_model_grid(model):
             raise NotImplementedError('InsideOutside does not support model grids (2D model arrays); '
                                       'use LogLike, Counts, BranchCounts and RootProb directly')
-        if _stacked(model) is not None:
-            raise NotImplementedError('InsideOutside takes one model or a model list; use LogLike, '
-                                      'Counts, BranchCounts and RootProb for stacked models')
         self._list = _is_model_list(model)
-        if self._list:
+        self._stack = _stacked(model)
+        if self._stack is not None:
+            A = len(self._stack[1][0].pi)
+            self._args = (self._eng.alignment(alignment, A), tree, model)
+        elif self._list:
             A = len(ensure_diag(model[0]).pi)
             self._args = (self._eng.alignment(alignment, A), tree, list(model))
         else:
@@ -476,6 +485,12 @@ class InsideOutside:
                             precision=self._precision, engine=self._eng)
 
     def _posteriors(self):
+        if self._npost is None and self._stack is not None:
+            H, flat = self._stack
+            outs = [_inside_outside(self._args[0], self._args[1], m, {'npost', 'bpost'},
+                                    precision=self._precision, engine=self._eng) for m in flat]
+            self._npost = torch.stack([o['npost'] for o in outs]).reshape(tuple(H) + tuple(outs[0]['npost'].shape))
+            self._bpost = torch.stack([o['bpost'] for o in outs]).reshape(tuple(H) + tuple(outs[0]['bpost'].shape))
         if self._npost is None:
             if self._list:
                 out = _multi(*self._args, {'npost', 'bpost'}, self._precision, self._eng)
@@ -487,13 +502,13 @@ class InsideOutside:
     def node_posterior(self, node: int | None = None):
         """(A, C) posterior of one node, or (R, A, C) for all nodes (row 0: root)."""
         self._posteriors()
-        return self._npost if node is None else self._npost[node]
+        return self._npost if node is None else self._npost[..., node, :, :]
 
     def branch_posterior(self, node: int | None = None):
         """(A, A, C) joint posterior P(x_parent = i, x_node = j) of branch ``node``
         (> 0), or (R, A, A, C) for all branches (branch 0 zeros)."""
         self._posteriors()
-        return self._bpost if node is None else self._bpost[node]
+        return self._bpost if node is None else self._bpost[..., node, :, :, :]
 
 
 def MixturePosterior(alignment, tree, models, log_weights, maxChunkSize: int = 128,

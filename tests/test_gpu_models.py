@@ -103,6 +103,20 @@ def test_gamma_mixture_and_stacked_models(sb, eng):
             assert err_ll(ll[k], rl) <= TOL[prec]['ll'] and err_root(root[k], rr) <= TOL[prec]['root']
         rp = sb.RootProb(aln, t, stack, precision=prec, engine=eng)
         assert err_root(rp[2], O.RootProb(aln, otree(t), oms[2])) <= TOL[prec]['root']
+    # InsideOutside and BranchCounts over the stack: leading K axis, each slice the
+    # single category's result (bitwise) and the oracle's
+    io = sb.InsideOutside(aln, t, stack, precision='fp64', engine=eng)
+    npost, bpost = io.node_posterior(), io.branch_posterior()
+    bc = sb.BranchCounts(aln, t, stack, precision='fp64', engine=eng)
+    R, C = aln.shape
+    assert npost.shape == (4, R, 4, C) and bpost.shape == (4, R, 4, 4, C) and bc.shape == (4, R, 4, 4, C)
+    assert io.node_posterior(3).shape == (4, 4, C)
+    for k in (0, 3):
+        one = sb.InsideOutside(aln, t, sb.scale_model(m, rates[k]), precision='fp64', engine=eng)
+        assert torch.equal(npost[k], one.node_posterior()) and torch.equal(bpost[k], one.branch_posterior())
+        assert float(np.abs(npost[k].cpu().numpy() - O.node_posterior(aln, otree(t), oms[k])).max()) \
+            <= TOL['fp64']['root']
+        assert torch.equal(bc[k], sb.BranchCounts(aln, t, sb.scale_model(m, rates[k]), precision='fp64', engine=eng))
 
 
 def test_model_list(sb, eng):
