@@ -1177,11 +1177,11 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
       rec[sb_mr_Vi(AP) + a * AP + k] = V[a * AP + k] / sp;
     }
   }
-  for (int i = 0; i < AP; ++i)
-    for (int j = 0; j < AP; ++j) {
+  for (int i = 0; i < AP; ++i)   // S is symmetric; stored exactly so (kernels read S[j][i] for S[i][j])
+    for (int j = i; j < AP; ++j) {
       double s = 0.0;
       for (int k = 0; k < AP; ++k) s += V[i * AP + k] * m64[sb_m64_mu(AP) + k] * V[j * AP + k];
-      rec[sb_mr_S(AP) + i * AP + j] = (i < A && j < A) ? s : 0.0;
+      rec[sb_mr_S(AP) + i * AP + j] = rec[sb_mr_S(AP) + j * AP + i] = (i < A && j < A) ? s : 0.0;
     }
   for (int a = 0; a < AP; ++a) {
     rec[sb_mr_pi(AP) + a] = a < A ? pi[a] : 0.0;

@@ -553,12 +553,15 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
             for (int q = 0; q < AW; ++q) tmp[q] = fma(vik, accb[k * 64 + q], tmp[q]);
           }
           __syncthreads();
+          // tmp[q] = 0 for q >= AP (acc holds zeros there), so the row of V is
+          // read unguarded up to AW (into the record's next matrix, finite
+          // values): contiguous uniform reads the compiler merges into wide
+          // scalar loads
           for (int jj = 0; jj < AP; ++jj) {
             T sacc = T(0);
 #pragma unroll
-            for (int q = 0; q < AW; ++q)
-              if (q < AP) sacc = fma(tmp[q], Vc[jj * AP + q], sacc);
-            accb[jj * SB_WAVE + lane] = (jj == lane) ? sacc : (on ? S[lane * AP + jj] : T(0)) * sacc;
+            for (int q = 0; q < AW; ++q) sacc = fma(tmp[q], Vc[jj * AP + q], sacc);
+            accb[jj * SB_WAVE + lane] = (jj == lane) ? sacc : (on ? S[jj * AP + lane] : T(0)) * sacc;   // S symmetric
           }
         }
         if (valid[c] && out && lane < A)
