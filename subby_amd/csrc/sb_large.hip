@@ -215,14 +215,18 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
   static_assert(!(GRAD && OUT != LOUT_COUNTS && OUT != LOUT_IRR) && !(F81 && (POST || IRC)), "unsupported variant");
   // irreversible counts hold their accumulators beside complex temporaries:
   // one column per wave where 2 would not fit the registers
-  constexpr int TC = (IRC && (sizeof(T) == 8 || AW == 64)) ? 1 : TCof<T, AW, F81>::v;
+  constexpr int TC = (IRC && (sizeof(T) == 8 || AW == 64)) ? 1
+                     : (BRC && !F81 && sizeof(T) == 4 && AW == 32) ? 2
+                     : TCof<T, AW, F81>::v;
   typedef Vec<T, TC> V_;
   typedef Raw<T, TC> R_;
   typedef Flags<TC> F_;
   const SbLaunch& L = *Lp;
   __shared__ T bc[TC * SB_WAVE];
-  __shared__ T sm[(POST ? AW : 64) * 64];   // staged AP x AP matrix; also the epilogue accumulator
-  __shared__ T sv[(F81 || POST) ? 1 : 64 * 64];   // V^T for the back-transform
+  // rows j < AP <= AW of 64 floats: the staged AP x AP matrix, and the epilogue's
+  // transposition buffer (rows of lanes < AW)
+  __shared__ T sm[AW * 64];
+  __shared__ T sv[(F81 || POST) ? 1 : AW * 64];   // V^T for the back-transform
   __shared__ T px[POST ? TC * SB_WAVE : 1];        // posteriors: d_i of each column
   __shared__ T pnode[POST ? TC * SB_WAVE : 1];     // posteriors: the node row
   __shared__ T dim_[IRC ? TC * SB_WAVE : 1];       // irreversible: Im of the projected outside vectors
@@ -542,7 +546,8 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         } else {
           __syncthreads();
 #pragma unroll
-          for (int q = 0; q < AW; ++q) accb[lane * 64 + q] = acc[c][q];
+          for (int q = 0; q < AW; ++q)
+            if (lane < AW) accb[lane * 64 + q] = acc[c][q];   // lanes >= AW hold zeros
           __syncthreads();
           T tmp[AW];
 #pragma unroll
@@ -877,6 +882,7 @@ SB_LINST(double, 64)
 
 static int large_tc(int dtype, int AP, bool f81, int out) {
   if (out == LOUT_IRR && (dtype != 0 || AP > 32)) return 1;
+  if (out == LOUT_BRANCH && !f81 && dtype == 0 && AP <= 32) return 2;
   if (dtype == 0) {
     if (AP <= 32) return TCof<float, 32, false>::v;
     return f81 ? TCof<float, 64, true>::v : TCof<float, 64, false>::v;
