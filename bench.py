@@ -123,12 +123,12 @@ def other_workload(args, log):
     eng = sb.Engine(0)
     dev = eng.device
     w = args.workload
-    if w in ('c3a', 'c3b'):
+    if w in ('f81-a64', 'general-a64'):
         A = 64
-        rng = np.random.default_rng(3000 if w == 'c3a' else 3001)
+        rng = np.random.default_rng(3000 if w == 'f81-a64' else 3001)
         t = sb.synthetic.random_tree(100, rng, 0.1)
         pi = rng.dirichlet(np.full(A, 2.0))
-        if w == 'c3a':
+        if w == 'f81-a64':
             m, C, f81 = sb.f81_model(pi), 1_000_000, True
         else:
             Q = sb.reversible_rate_matrix(rng.lognormal(0.0, 1.0, (A, A)), pi)
@@ -159,7 +159,7 @@ def other_workload(args, log):
         err_c = float(np.max(np.abs(gc - rc).max((0, 1)) / np.abs(rc).max((0, 1))))
         err_l = float(np.max(np.abs(blk.loglike[cols].cpu().numpy() - rl) / np.maximum(np.abs(rl), 1)))
         units = C * (R - 1)
-        if w == 'c3a':
+        if w == 'f81-a64':
             balg = b_alg_counts(R, A, C)
             roof = {"bound": "hbm", "achieved": balg / (kavg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": balg / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
@@ -359,7 +359,7 @@ def other_workload(args, log):
                "stages_s": {"scan_prune_expand": st['scan_s'], "gpu_and_download": st['gpu_s']},
                "gpu_stage_column_branches_per_s": st['column_branches'] / st['gpu_s'],
                "parity": {"counts_features_normwise": worst, "blocks_checked": len(list(zip(blocks, feats))[::250])}}
-    elif w == 'io':
+    elif w == 'branch-hky85':
         # per-branch outputs on the headline block: BranchCounts, then node + branch posteriors
         rng = np.random.default_rng(2000)
         pi = rng.dirichlet(np.full(4, 5.0))
@@ -405,7 +405,7 @@ def other_workload(args, log):
         out = {"metric": "BranchCounts column*branches/sec (FP32, HKY85 R=127, unmasked, (R,A,A,C) output)",
                "value": units / t_bc, "unit": "column*branches/s", "n_gpus": 1, "steps": reps,
                "ms_per_step": t_bc * 1e3, "higher_is_better": True, "dtype": "f32",
-               "config": {"workload": "io", "A": A, "R": R, "C": C},
+               "config": {"workload": "branch-hky85", "A": A, "R": R, "C": C},
                "roofline": {"bound": "hbm", "note": "write-dominated: output bytes / call time",
                             "achieved": out_bc / t_bc / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": out_bc / t_bc / 1e9 / HBM_PEAK_GBS, "traffic": None,
@@ -414,7 +414,7 @@ def other_workload(args, log):
                               "output_GB_per_s": out_post / t_post / 1e9},
                "parity": {"branch_counts_per_branch_normwise": err_bc, "node_posterior_abs": err_np,
                           "branch_posterior_abs": err_bp, "columns_checked": len(cols)}}
-    elif w == 'io20':
+    elif w == 'post-a20':
         # node + branch posteriors above A = 8: general A = 20 model, 100-leaf tree
         A = 20
         rng = np.random.default_rng(2020)
@@ -423,7 +423,7 @@ def other_workload(args, log):
         m = sb.diagonalize_rate_matrix(Q, pi)
         t = sb.synthetic.random_tree(100, rng, 0.1)
         C = 20_000
-        log(f'bench io20: simulating A={A}, R={len(t.parentIndex)}, C={C}')
+        log(f'bench post-a20: simulating A={A}, R={len(t.parentIndex)}, C={C}')
         aln_np = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
         aln = torch.from_numpy(aln_np).to(dev)
         R = len(t.parentIndex)
@@ -475,7 +475,7 @@ def other_workload(args, log):
         out = {"metric": "InsideOutside node + branch posteriors column*branches/sec (FP32, general A=20, R=199)",
                "value": units / t_post, "unit": "column*branches/s", "n_gpus": 1, "steps": reps,
                "ms_per_step": t_post * 1e3, "higher_is_better": True, "dtype": "f32",
-               "config": {"workload": "io20", "A": A, "R": R, "C": C},
+               "config": {"workload": "post-a20", "A": A, "R": R, "C": C},
                "roofline": {"bound": "hbm", "note": "write-dominated: (R,A,C) + (R,A,A,C) output bytes / stream time",
                             "achieved": out_b / (ev_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": out_b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
@@ -486,7 +486,7 @@ def other_workload(args, log):
                                  "kernel": "sb_large_kernel<float, 32, false, false, LOUT_BRANCH>"},
                "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
                           "branch_counts_per_branch_normwise": err_bc, "columns_checked": len(cols)}}
-    else:  # c5 gradient
+    else:  # grad-hky85: the gradient
         rng = np.random.default_rng(5000)
         pi = rng.dirichlet(np.full(4, 5.0))
         kappa = 2.0
@@ -549,7 +549,7 @@ def other_workload(args, log):
                          "d/d branch lengths and d/d log kappa)",
                "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
-               "config": {"workload": "c5", "A": 4, "R": R, "C": C},
+               "config": {"workload": "grad-hky85", "A": 4, "R": R, "C": C},
                "parity": par}
     print(json.dumps(out), flush=True)
 
@@ -566,11 +566,13 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=20000)
     ap.add_argument('--serve-pool', type=int, default=500,
                     help='distinct simulated (tree, alignment) pairs the request stream draws from')
-    ap.add_argument('--workload', default='c2', choices=['c2', 'c3a', 'c3b', 'c5', 'io', 'io20', 'irrev', 'mixture', 'rows', 'maf'],
-                    help='c2 (default, the headline line) or one of the other configurations: '
-                         'c3a F81 A=64 R=199 C=1e6; c3b general A=64 R=199 C=2e4; '
-                         'c5 HKY85 gradient R=127 C=5e4; io per-branch outputs (BranchCounts, '
-                         'node and branch posteriors) HKY85 R=127 C=2e5; io20 node and branch '
+    ap.add_argument('--workload', default='hky85',
+                    choices=['hky85', 'f81-a64', 'general-a64', 'grad-hky85', 'branch-hky85', 'post-a20',
+                             'irrev', 'mixture', 'rows', 'maf'],
+                    help='hky85 (default, the headline line) or one of the other configurations: '
+                         'f81-a64 F81 A=64 R=199 C=1e6; general-a64 general A=64 R=199 C=2e4; '
+                         'grad-hky85 HKY85 gradient R=127 C=5e4; branch-hky85 per-branch outputs (BranchCounts, '
+                         'node and branch posteriors) HKY85 R=127 C=2e5; post-a20 node and branch '
                          'posteriors, general A=20 R=199 C=2e4')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
@@ -587,7 +589,7 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
-    if args.workload != 'c2':
+    if args.workload != 'hky85':
         return other_workload(args, log)
 
     if args.copy_calibration:
