@@ -672,14 +672,14 @@ __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict
         }
         const SB_GLOBAL T* bn = bt + (long long)n * BTS;
         if constexpr (POST) {
-          V_ wn, y;
-          apply_M(n, u, wn);                       // M_n u_n
-          T iz[TC];
-#pragma unroll
-          for (int c = 0; c < TC; ++c) iz[c] = T(1) / wsum(d.v[c] * wn.v[c]);
-          // y = d^T M_n; the contraction leaves M_n staged as sm[i*64 + j] = M[i][j]
+          // y = d^T M_n; the contraction leaves M_n staged as sm[i*64 + j] = M[i][j].
+          // z = d . (M_n u) = y . u
+          V_ y;
           if (f81m) f81_vecmat<T, TC>(bn[sb_bt_E(AP) + kstar], pia, d, y);
           else contract<T, TC>(bn + sb_bt_M(AP), AP, d, y, bc, sm, lane);
+          T iz[TC];
+#pragma unroll
+          for (int c = 0; c < TC; ++c) iz[c] = T(1) / wsum(y.v[c] * u.v[c]);
           __syncthreads();
 #pragma unroll
           for (int c = 0; c < TC; ++c) {
