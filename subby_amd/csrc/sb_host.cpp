@@ -1093,6 +1093,9 @@ int sb_internal_error(int code, const char* msg) {
   return code;
 }
 int sb_internal_device(sb_context* ctx) { return ctx ? ctx->device : -1; }
+// Padded alphabet of an alphabet size (-1 outside 2..64): requests of one
+// padded alphabet (and reversibility) share a launch.
+int sb_internal_pad_alphabet(int A) { return A < 2 ? -1 : pad_alphabet(A); }
 // 1 for an irreversible model, 0 for a reversible one, -1 for an unknown id.
 int sb_internal_model_irr(sb_context* ctx, int model_id) {
   if (!ctx) return -1;

@@ -36,6 +36,7 @@
 extern "C" int sb_internal_error(int code, const char* msg);
 extern "C" int sb_internal_device(sb_context* ctx);
 extern "C" int sb_internal_model_irr(sb_context* ctx, int model_id);
+extern "C" int sb_internal_pad_alphabet(int A);
 
 namespace {
 
@@ -119,6 +120,7 @@ class Pool {
 struct Item {
   sb_request r;
   int A = 0;
+  int kind = 0;                     // padded alphabet: one kind per launch
   size_t in_bytes = 0, out_bytes = 0;
   size_t in_off = 0, out_off = 0;   // within the lane buffers
   double* done_slot = nullptr;      // replay: completion time (seconds since server start)
@@ -326,6 +328,7 @@ void run_lane(sb_server* s, Lane& L) {
         long long cols = 0;
         while (!s->queue.empty() && int(take.size()) < s->max_blocks) {
           const Item& it = s->queue.front();
+          if (!take.empty() && it.kind != take[0].kind) break;   // one launch runs one padded alphabet
           const size_t b = it.in_bytes + it.out_bytes;
           if (!take.empty() && (cols + it.r.C > s->max_cols || s->in_flight + bytes + b > s->budget))
             break;
@@ -399,6 +402,7 @@ int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
                     "reversible models)", it.r.tag, it.r.model_id);
       return sb_internal_error(SB_ERR_UNSUPPORTED, buf);
     }
+    it.kind = sb_internal_pad_alphabet(it.A);
     it.in_bytes = align256(size_t(it.r.R) * size_t(it.r.C));
     size_t oc, ol, orr;
     it.out_bytes = out_layout(it, &oc, &ol, &orr);

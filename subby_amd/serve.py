@@ -30,7 +30,7 @@ import torch
 
 from . import _lib
 from .api import Engine, default_engine, _PREC
-from .models import ensure_diag
+from .models import IrrevDiagModel, ensure_diag
 
 _BLOCK_DTYPE = np.dtype({
     'names': ['model_id', 'R', 'C', 'aln', 'parent', 'dist', 'want', 'f81', 'mask_mode',
@@ -206,6 +206,8 @@ class Server:
                     r = pending[head]
                     if realtime and r.arrival > now:
                         break
+                    if take and _kind(r) != _kind(take[0]):   # one launch runs one kind
+                        break
                     bi, bo = self._bytes(r)
                     if take and (cols + r.alignment.shape[1] > self.max_cols or
                                  self.in_flight + nbytes + bi + bo > self.budget):
@@ -322,6 +324,17 @@ class NativeServer:
                  'mean_ms': float(lat.mean() * 1e3)}
         results.sort(key=lambda r: r.rid)
         return results, stats
+
+
+_PADDED = (4, 8, 16, 20, 32, 64)
+
+
+def _kind(r):
+    """Requests that can share a persistent launch: same padded alphabet and
+    reversibility (the kernels are specialised on both)."""
+    m = ensure_diag(r.model)
+    A = len(m.pi)
+    return next(p for p in _PADDED if A <= p), isinstance(m, IrrevDiagModel)
 
 
 def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000),

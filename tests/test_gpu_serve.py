@@ -117,3 +117,69 @@ def test_native_tight_budget_burst_and_poll(eng):
             _lib.check(srv.lib.sb_server_submit(srv.h, 1, ctypes.cast(bad, ctypes.c_void_p)))
     finally:
         srv.close()
+
+
+def _mixed_stream(sb, n, seed, irreversible):
+    """HKY85 stream with every third request on a general A = 20 model and (if
+    asked) every fifth on an irreversible A = 4 model."""
+    from subby_amd.serve import make_stream
+    reqs = make_stream(n, qps=1e9, seed=seed, leaves=(4, 30), mean_cols=150)
+    rng = np.random.default_rng(seed + 1)
+    pi20 = rng.dirichlet(np.full(20, 3.0))
+    m20 = sb.diagonalize_rate_matrix(sb.reversible_rate_matrix(rng.lognormal(0, 1, (20, 20)), pi20), pi20)
+    Q = rng.uniform(0.05, 1.0, (4, 4))
+    for a in range(4):
+        Q[a, (a + 1) % 4] += 1.5
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    mirr = sb.diagonalize_irreversible(Q, rng.dirichlet(np.full(4, 3.0)))
+    for r in reqs:
+        if r.rid % 3 == 1:
+            r.model = m20
+        elif irreversible and r.rid % 5 == 2:
+            r.model = mirr
+        else:
+            continue
+        r.alignment = sb.synthetic.simulate(r.tree, r.model, r.alignment.shape[1], rng, gap_frac=0.03)
+    return reqs
+
+
+def _check_alone_any(sb, eng, reqs, results):
+    for r, res in zip(reqs, results):
+        assert res.rid == r.rid
+        cnt, ll, root = sb.Counts(r.alignment, r.tree, r.model, engine=eng, return_loglike=True, return_root=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(cnt.cpu().numpy(), res.counts), r.rid
+        assert np.array_equal(ll.cpu().numpy(), res.loglike), r.rid
+        assert np.array_equal(root.cpu().numpy(), res.root), r.rid
+
+
+def test_served_mixed_alphabets_and_models(eng):
+    """Requests over A = 4 and A = 20 and reversible / irreversible models in
+    one stream: each launch takes one kind (padded alphabet, reversibility);
+    every result equals the request computed alone."""
+    import subby_amd as sb
+    from subby_amd.serve import Server
+    say('serve: mixed alphabets and irreversible models')
+    reqs = _mixed_stream(sb, 90, 21, irreversible=True)
+    srv = Server(eng, hbm_budget=1 << 30, lanes=2, max_blocks=16)
+    results, stats = srv.run(reqs, realtime=False)
+    say('   stats', stats)
+    assert stats['requests'] == 90
+    _check_alone_any(sb, eng, reqs, results)
+
+
+def test_native_served_mixed_alphabets(eng):
+    """The native loop with A = 4 and A = 20 requests interleaved."""
+    import subby_amd as sb
+    from subby_amd.serve import NativeServer
+    say('native serve: mixed alphabets')
+    reqs = _mixed_stream(sb, 90, 22, irreversible=False)
+    srv = NativeServer(eng, hbm_budget=1 << 30, lanes=2, max_blocks=16)
+    try:
+        results, stats = srv.run(reqs, realtime=False)
+    finally:
+        srv.close()
+    say('   stats', stats)
+    assert stats['requests'] == 90
+    _check_alone_any(sb, eng, reqs, results)
