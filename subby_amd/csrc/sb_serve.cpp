@@ -120,7 +120,7 @@ class Pool {
 struct Item {
   sb_request r;
   int A = 0;
-  int kind = 0;                     // padded alphabet: one kind per launch
+  int kind = 0;                     // 2 x padded alphabet + irreversible: one kind per launch
   size_t in_bytes = 0, out_bytes = 0;
   size_t in_off = 0, out_off = 0;   // within the lane buffers
   double* done_slot = nullptr;      // replay: completion time (seconds since server start)
@@ -328,7 +328,7 @@ void run_lane(sb_server* s, Lane& L) {
         long long cols = 0;
         while (!s->queue.empty() && int(take.size()) < s->max_blocks) {
           const Item& it = s->queue.front();
-          if (!take.empty() && it.kind != take[0].kind) break;   // one launch runs one padded alphabet
+          if (!take.empty() && it.kind != take[0].kind) break;   // one launch runs one kind
           const size_t b = it.in_bytes + it.out_bytes;
           if (!take.empty() && (cols + it.r.C > s->max_cols || s->in_flight + bytes + b > s->budget))
             break;
@@ -395,14 +395,16 @@ int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
     it.r = reqs[i];
     const int rc = validate(it.r, &it.A);
     if (rc) return rc;
-    if (it.r.model_id >= 0 && sb_internal_model_irr(s->ctx, it.r.model_id) != 0) {
-      char buf[160];
-      std::snprintf(buf, sizeof buf,
-                    "request tag %lld: unknown or irreversible model id %d (the server runs "
-                    "reversible models)", it.r.tag, it.r.model_id);
-      return sb_internal_error(SB_ERR_UNSUPPORTED, buf);
+    int irr = 0;   // models given inline are reversible
+    if (it.r.model_id >= 0) {
+      irr = sb_internal_model_irr(s->ctx, it.r.model_id);
+      if (irr < 0) {
+        char buf[128];
+        std::snprintf(buf, sizeof buf, "request tag %lld: unknown model id %d", it.r.tag, it.r.model_id);
+        return sb_internal_error(SB_ERR_MODEL, buf);
+      }
     }
-    it.kind = sb_internal_pad_alphabet(it.A);
+    it.kind = 2 * sb_internal_pad_alphabet(it.A) + irr;
     it.in_bytes = align256(size_t(it.r.R) * size_t(it.r.C));
     size_t oc, ol, orr;
     it.out_bytes = out_layout(it, &oc, &ol, &orr);

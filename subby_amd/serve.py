@@ -278,8 +278,9 @@ class NativeServer:
         keep, outs = [], []
         for k, r in enumerate(requests):
             m = ensure_diag(r.model)
-            ev = np.ascontiguousarray(m.eigenvalues, dtype=np.float64)
-            V = np.ascontiguousarray(m.eigenvectors, dtype=np.float64)
+            irr = isinstance(m, IrrevDiagModel)
+            ev = None if irr else np.ascontiguousarray(m.eigenvalues, dtype=np.float64)
+            V = None if irr else np.ascontiguousarray(m.eigenvectors, dtype=np.float64)
             pi = np.ascontiguousarray(m.pi, dtype=np.float64)
             aln = np.ascontiguousarray(r.alignment, dtype=np.int8)
             parent = np.ascontiguousarray(r.tree.parentIndex, dtype=np.int32)
@@ -292,9 +293,12 @@ class NativeServer:
             root = np.full((A, C), np.nan, dtype=np.float32)
             ll = np.full(C, np.nan, dtype=np.float64)
             q = arr[k]
-            q.model_id = -1
             q.A = A
-            q.eigenvalues, q.eigenvectors, q.pi = ev.ctypes.data, V.ctypes.data, pi.ctypes.data
+            if irr:   # resident model (inline models are reversible)
+                q.model_id = self.eng.model_id(m)
+            else:
+                q.model_id = -1
+                q.eigenvalues, q.eigenvectors, q.pi = ev.ctypes.data, V.ctypes.data, pi.ctypes.data
             q.R, q.C = R, C
             q.alignment, q.parent_index, q.distances = aln.ctypes.data, parent.ctypes.data, dist.ctypes.data
             q.mask_mode = _lib.SB_MASK_AUTO if r.branch_mask == 'auto' else _lib.SB_MASK_NONE

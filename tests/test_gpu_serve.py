@@ -170,11 +170,12 @@ def test_served_mixed_alphabets_and_models(eng):
 
 
 def test_native_served_mixed_alphabets(eng):
-    """The native loop with A = 4 and A = 20 requests interleaved."""
+    """The native loop with A = 4 and A = 20 requests interleaved, and
+    irreversible models as resident model ids."""
     import subby_amd as sb
     from subby_amd.serve import NativeServer
-    say('native serve: mixed alphabets')
-    reqs = _mixed_stream(sb, 90, 22, irreversible=False)
+    say('native serve: mixed alphabets and irreversible models')
+    reqs = _mixed_stream(sb, 90, 22, irreversible=True)
     srv = NativeServer(eng, hbm_budget=1 << 30, lanes=2, max_blocks=16)
     try:
         results, stats = srv.run(reqs, realtime=False)
