@@ -10,7 +10,8 @@ from .models import (Tree, DiagModel, IrrevDiagModel, RateModel, hky85_diag, hky
                      diagonalize_rate_matrix, reversible_rate_matrix, ensure_diag,
                      diagonalize_irreversible, irrev_model_from_rate_matrix,
                      model_from_rate_matrix, check_detailed_balance, gamma_rate_categories,
-                     scale_model, genetic_code, codon_to_sense, gy94_rate_matrix, gy94_model)
+                     scale_model, genetic_code, codon_to_sense, gy94_rate_matrix, gy94_model,
+                     diagonalize_rate_matrix_auto)
 from ._lib import SubbyError
 from .api import (Engine, default_engine, LogLike, Counts, RootProb, loglike_grad, BranchCounts,
                   InsideOutside, MixturePosterior)
@@ -24,8 +25,8 @@ __all__ = ['Tree', 'DiagModel', 'IrrevDiagModel', 'RateModel',
            'hky85_log_kappa_weights', 'jukes_cantor_model', 'f81_model',
            'diagonalize_rate_matrix', 'reversible_rate_matrix', 'ensure_diag', 'SubbyError',
            'Engine', 'default_engine', 'LogLike', 'Counts', 'RootProb', 'loglike_grad',
-           'BranchCounts', 'InsideOutside',
+           'BranchCounts', 'InsideOutside', 'GEOM_BINS', 'diagonalize_rate_matrix_auto',
            'LogLikeCustomGrad', 'hky85_loglike']
-from .padding import (pad_alignment, unpad_columns, pad_tree, pad_tree_and_alignment,  # noqa: E402
-                      pad_to_geom_bin)
+from .padding import (GEOM_BINS, pad_alignment, unpad_columns, pad_tree,  # noqa: E402
+                      pad_tree_and_alignment, pad_to_geom_bin)
 from . import synthetic, batch  # noqa: E402

@@ -37,9 +37,14 @@ class _LogLikeDist(torch.autograd.Function):
 
 def LogLikeCustomGrad(alignment, tree, model, maxChunkSize: int = 128, method: str = 'eigen',
                       precision: str = 'fp32', engine=None):
-    """LogLike differentiable in tree.distanceToParent (subby/jax/__init__.py:294-335)."""
-    if method != 'eigen':
-        raise ValueError("only method='eigen' is provided")
+    """LogLike differentiable in tree.distanceToParent (subby/jax/__init__.py:294-335).
+
+    ``method='pade'`` (the reference's expm forward / Q M(t) backward) gives the
+    same values and is served by the same eigenbasis adjoint kernels: the model
+    must be diagonalizable (every DiagModel, IrrevDiagModel or RateModel this
+    module builds is)."""
+    if method not in ('eigen', 'pade'):
+        raise ValueError("method must be 'eigen' or 'pade'")
     d = tree.distanceToParent
     if not isinstance(d, torch.Tensor):
         d = torch.as_tensor(np.asarray(d, dtype=np.float64))

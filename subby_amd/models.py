@@ -256,6 +256,13 @@ def model_from_rate_matrix(subRate, pi, reversible=None, tol=1e-10):
     return diagonalize_irreversible(subRate, pi)
 
 
+def diagonalize_rate_matrix_auto(subRate, rootProb, reversible=None, tol=1e-10):
+    """The reference's name for model_from_rate_matrix (diagonalize.py:89-120):
+    eigh when reversible (detected by detailed balance when ``reversible`` is
+    None), else the irreversible decomposition."""
+    return model_from_rate_matrix(subRate, rootProb, reversible=reversible, tol=tol)
+
+
 def ensure_diag(model):
     """RateModel -> DiagModel (subby/jax/__init__.py:44-50); IrrevDiagModel passes
     through with complex128 arrays."""

@@ -143,6 +143,11 @@ def test_irrev_gradient(sb, eng):
     ll = sb.LogLikeCustomGrad(aln, sb.Tree(t.parentIndex, dist), m, precision='fp64', engine=eng)
     (ll.cpu() * torch.as_tensor(g)).sum().backward()
     assert np.abs(dist.grad.numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
+    # method='pade' gives the same values (same adjoint)
+    dist2 = dist.detach().clone().requires_grad_(True)
+    ll2 = sb.LogLikeCustomGrad(aln, sb.Tree(t.parentIndex, dist2), m, precision='fp64', engine=eng, method='pade')
+    (ll2.cpu() * torch.as_tensor(g)).sum().backward()
+    assert torch.equal(ll2, ll) and torch.equal(dist2.grad, dist.grad)
 
 
 @pytest.mark.parametrize('A', [12, 40])
