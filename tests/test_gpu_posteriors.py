@@ -172,3 +172,39 @@ def test_posteriors_large_alphabets(sb, eng, A, C):
     assert abs_err(both['bpost'], O.branch_posterior(aln, ot, omodel(m))) <= 1e-12
     ll = O.LogLike(aln, ot, omodel(m))
     assert np.max(np.abs(both['ll'].cpu().numpy() - ll) / np.maximum(np.abs(ll), 1)) <= 1e-12
+
+
+def test_large_alphabet_per_branch_edge_shapes(sb, eng):
+    """Edge shapes above A = 8 for the posterior, per-branch counts and
+    irreversible counts variants: a single-node tree, one column, ragged
+    column tiles, all-gap and ambiguous columns; FP64 and FP32 vs the oracle."""
+    say('large-alphabet per-branch edge shapes')
+    rng = np.random.default_rng(77)
+    A = 12
+    pi = rng.dirichlet(np.full(A, 3.0))
+    m = sb.diagonalize_rate_matrix(sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi), pi)
+    Q = rng.uniform(0.05, 1.0, (A, A))
+    for a in range(A):
+        Q[a, (a + 1) % A] += 1.5
+    np.fill_diagonal(Q, 0.0)
+    np.fill_diagonal(Q, -Q.sum(1))
+    mi = sb.diagonalize_irreversible(Q, pi)
+    omi = O.IrrevModel(np.asarray(mi.eigenvalues), np.asarray(mi.eigenvectors),
+                       np.asarray(mi.eigenvectors_inv), np.asarray(mi.pi), False)
+    for R, C in [(1, 5), (3, 1), (3, 7), (9, 66)]:
+        t = sb.Tree(np.array([-1]), np.array([0.0])) if R == 1 else sb.synthetic.random_tree((R + 1) // 2, rng, 0.2)
+        aln = rng.integers(-1, A + 2, size=(R, C)).astype(np.int8)
+        aln[:, 0] = -1                                   # an all-gap column
+        ot = otree(t)
+        for precision in ('fp64', 'fp32'):
+            tol = 1e-10 if precision == 'fp64' else 1e-4
+            tp = 1e-11 if precision == 'fp64' else 1e-5
+            io = sb.InsideOutside(aln, t, m, precision=precision, engine=eng)
+            assert abs_err(io.node_posterior(), O.node_posterior(aln, ot, omodel(m))) <= tp, (R, C, precision)
+            assert abs_err(io.branch_posterior(), O.branch_posterior(aln, ot, omodel(m))) <= tp, (R, C, precision)
+            bc = sb.BranchCounts(aln, t, m, precision=precision, engine=eng)
+            rbc = O.BranchCounts(aln, ot, omodel(m))
+            assert abs_err(bc, rbc) <= tol * max(1.0, float(np.abs(rbc).max())), (R, C, precision)
+            ci = sb.Counts(aln, t, mi, precision=precision, engine=eng)
+            rci, _, _ = O.counts_loglike_root(aln, ot, omi)
+            assert abs_err(ci, rci) <= tol * max(1.0, float(np.abs(rci).max())), (R, C, precision)
