@@ -480,10 +480,10 @@ def other_workload(args, log):
                             "achieved": out_b / (ev_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": out_b / (ev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                             "output_bytes_per_call": out_b, "stream_ms_per_call": ev_ms,
-                            "kernel": "sb_large_kernel<float, 32, false, false, true>"},
+                            "kernel": "sb_large_kernel<float, 20, false, false, LOUT_POST>"},
                "branch_counts": {"stream_ms_per_call": bc_ms, "column*branches/s": units / (bc_ms * 1e-3),
                                  "output_GB_per_s": R * A * A * C * 4 / (bc_ms * 1e-3) / 1e9,
-                                 "kernel": "sb_large_kernel<float, 32, false, false, LOUT_BRANCH>"},
+                                 "kernel": "sb_large_kernel<float, 20, false, false, LOUT_BRANCH>"},
                "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
                           "branch_counts_per_branch_normwise": err_bc, "columns_checked": len(cols)}}
     else:  # grad-hky85: the gradient

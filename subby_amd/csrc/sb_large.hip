@@ -73,12 +73,23 @@ __device__ inline T wmax(T v) {
   return fmax(fmax(rl(v, 0), rl(v, 16)), fmax(rl(v, 32), rl(v, 48)));
 }
 
+// Output variants of the kernel (see sb_large_kernel).
+enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2, LOUT_IRR = 3 };
+
+// Alphabet tile width for a padded alphabet: 20, 32 or 64 states.
+inline constexpr int large_aw(int AP) { return AP <= 20 ? 20 : AP <= 32 ? 32 : 64; }
+
 // Columns per wave tile: 4, except where the A x A accumulator is widest
-// (FP64, or the general 64-state FP32 model) where 2 keeps it in registers.
-template <typename T, int AW, bool F81>
-struct TCof {
-  static constexpr int v = (sizeof(T) == 8 || (AW == 64 && !F81)) ? 2 : 4;
-};
+// (FP64, or the general 64-state FP32 model) where 2 keeps it in registers;
+// FP32 per-branch counts take 2 (three waves per SIMD instead of one); the
+// irreversible counts hold complex temporaries beside the accumulator (1
+// column where 2 would not fit).  Host and kernel share this rule.
+__host__ __device__ inline constexpr int large_tc_of(bool dbl, int AW, bool f81, int out) {
+  return (out == LOUT_IRR && (dbl || AW == 64)) ? 1
+         : (out == LOUT_BRANCH && !f81 && !dbl && AW <= 32) ? 2
+         : (dbl || (AW == 64 && !f81)) ? 2
+         : 4;
+}
 
 
 // Per-lane view of TC column vectors: v[c] = component `lane` of column c.
@@ -207,17 +218,13 @@ __device__ inline int claim_tile(int* counter, int ntiles, int lane, int& k) {
 //               eigensystem, eigensub.py:234-434): each branch's
 //               Re(W o (L C_n Rt)) is added to natural-basis accumulators
 //               (lane = column j of the counts, acc[c][i] = row i).
-enum { LOUT_COUNTS = 0, LOUT_POST = 1, LOUT_BRANCH = 2, LOUT_IRR = 3 };
-
 template <typename T, int AW, bool F81, bool GRAD, int OUT = LOUT_COUNTS>
 __global__ void __launch_bounds__(64) sb_large_kernel(const SbLaunch* __restrict__ Lp) {
   constexpr bool POST = OUT == LOUT_POST, BRC = OUT == LOUT_BRANCH, IRC = OUT == LOUT_IRR;
   static_assert(!(GRAD && OUT != LOUT_COUNTS && OUT != LOUT_IRR) && !(F81 && (POST || IRC)), "unsupported variant");
   // irreversible counts hold their accumulators beside complex temporaries:
   // one column per wave where 2 would not fit the registers
-  constexpr int TC = (IRC && (sizeof(T) == 8 || AW == 64)) ? 1
-                     : (BRC && !F81 && sizeof(T) == 4 && AW == 32) ? 2
-                     : TCof<T, AW, F81>::v;
+  constexpr int TC = large_tc_of(sizeof(T) == 8, AW, F81, OUT);
   typedef Vec<T, TC> V_;
   typedef Raw<T, TC> R_;
   typedef Flags<TC> F_;
@@ -879,15 +886,11 @@ SB_LINST(float, 32)
 SB_LINST(float, 64)
 SB_LINST(double, 32)
 SB_LINST(double, 64)
+SB_LINST(float, 20)
+SB_LINST(double, 20)
 
 static int large_tc(int dtype, int AP, bool f81, int out) {
-  if (out == LOUT_IRR && (dtype != 0 || AP > 32)) return 1;
-  if (out == LOUT_BRANCH && !f81 && dtype == 0 && AP <= 32) return 2;
-  if (dtype == 0) {
-    if (AP <= 32) return TCof<float, 32, false>::v;
-    return f81 ? TCof<float, 64, true>::v : TCof<float, 64, false>::v;
-  }
-  return TCof<double, 64, false>::v;
+  return large_tc_of(dtype != 0, large_aw(AP), f81, out);
 }
 
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81, int out) { return large_tc(dtype, AP, f81, out); }
@@ -915,9 +918,9 @@ static void (*pick_large(int dtype, int AP, bool f81, bool grad, int out))(const
     else k = sb_large_kernel<T, AWV, false, false>;                               \
   }
   if (dtype == 0) {
-    if (AP <= 32) SB_LPICK(float, 32) else SB_LPICK(float, 64)
+    if (AP <= 20) SB_LPICK(float, 20) else if (AP <= 32) SB_LPICK(float, 32) else SB_LPICK(float, 64)
   } else {
-    if (AP <= 32) SB_LPICK(double, 32) else SB_LPICK(double, 64)
+    if (AP <= 20) SB_LPICK(double, 20) else if (AP <= 32) SB_LPICK(double, 32) else SB_LPICK(double, 64)
   }
 #undef SB_LPICK
   return k;
