@@ -470,6 +470,20 @@ def other_workload(args, log):
         den = np.abs(rbc).max(axis=(1, 2))
         err_bc = float(np.max(num[1:] / den[1:]))
         bc_holder.clear()
+        # Counts (+ logLike, root) on the same block, for scale
+        cnt_holder = {}
+
+        def counts():
+            cnt_holder['c'] = sb.Counts(aln, t, m, engine=eng)
+        counts()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            counts()
+        ev1.record()
+        torch.cuda.synchronize()
+        counts_ms = ev0.elapsed_time(ev1) / reps
+        cnt_holder.clear()
         units = C * (R - 1)
         out_b = R * (A + A * A) * C * 4
         out = {"metric": "InsideOutside node + branch posteriors column*branches/sec (FP32, general A=20, R=199)",
@@ -484,6 +498,7 @@ def other_workload(args, log):
                "branch_counts": {"stream_ms_per_call": bc_ms, "column*branches/s": units / (bc_ms * 1e-3),
                                  "output_GB_per_s": R * A * A * C * 4 / (bc_ms * 1e-3) / 1e9,
                                  "kernel": "sb_large_kernel<float, 20, false, false, LOUT_BRANCH>"},
+               "counts": {"stream_ms_per_call": counts_ms, "column*branches/s": units / (counts_ms * 1e-3)},
                "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
                           "branch_counts_per_branch_normwise": err_bc, "columns_checked": len(cols)}}
     else:  # grad-hky85: the gradient
