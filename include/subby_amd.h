@@ -146,9 +146,9 @@ int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
  *          distribution; counts are back-transformed per branch in each
  *          branch's eigenbasis (:102-144);
  *   (C, R) both (model grid (C, R)).
- * All models share the alphabet and reversibility.  Per-branch models need
- * A <= 8, no F81 path, and reversible models for counts (SB_ERR_UNSUPPORTED
- * otherwise).  Outputs as sb_counts; any may be NULL, at least one given. */
+ * All models share the alphabet and reversibility.  Per-branch models take
+ * no F81 path (SB_ERR_UNSUPPORTED), and no gradient.  Outputs as sb_counts;
+ * any may be NULL, at least one given. */
 int sb_counts_multi(sb_context* ctx, int n_cols, int n_rows, const int* model_ids,
                     const int8_t* d_alignment, const int32_t* parent_index, const double* distances,
                     int R, int C, int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
