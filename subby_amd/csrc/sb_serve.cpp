@@ -326,17 +326,28 @@ void run_lane(sb_server* s, Lane& L) {
       std::unique_lock<std::mutex> g(s->mu);
       for (;;) {
         long long cols = 0;
-        while (!s->queue.empty() && int(take.size()) < s->max_blocks) {
-          const Item& it = s->queue.front();
-          if (!take.empty() && it.kind != take[0].kind) break;   // one launch runs one kind
+        // One launch runs one kind (padded alphabet, reversibility): the oldest
+        // request sets it, and requests of that kind are taken from a bounded
+        // window of the queue in order; the others keep their places (the
+        // oldest of them leads the next admission, so none waits forever).
+        auto q = s->queue.begin();
+        int scanned = 0;
+        const int window = 4 * s->max_blocks;
+        while (q != s->queue.end() && int(take.size()) < s->max_blocks && scanned < window) {
+          ++scanned;
+          const Item& it = *q;
+          if (!take.empty() && it.kind != take[0].kind) {
+            ++q;
+            continue;
+          }
           const size_t b = it.in_bytes + it.out_bytes;
           if (!take.empty() && (cols + it.r.C > s->max_cols || s->in_flight + bytes + b > s->budget))
             break;
           if (take.empty() && s->in_flight > 0 && s->in_flight + b > s->budget) break;
-          take.push_back(it);
-          s->queue.pop_front();
           bytes += b;
           cols += it.r.C;
+          take.push_back(it);
+          q = s->queue.erase(q);   // `it` is gone from here on
         }
         if (!take.empty()) break;
         if (s->stop && s->queue.empty()) return;
