@@ -185,8 +185,9 @@ class Server:
     def run(self, requests, realtime: bool = True):
         """Serve ``requests`` (sorted by arrival).  Returns (results, stats)."""
         pending = list(requests)
+        admitted = [False] * len(pending)
         results = []
-        head = 0
+        head = 0                        # first request not yet admitted
         t0 = time.perf_counter()
         batches = 0
         while head < len(pending) or any(l.busy for l in self.lanes):
@@ -201,13 +202,22 @@ class Server:
                     continue
                 if realtime and pending[head].arrival > now:
                     break
+                # one launch runs one kind: the oldest waiting request sets it, and
+                # requests of that kind are taken in order from a bounded window
                 take, cols, nbytes = [], 0, 0
-                while head < len(pending) and len(take) < self.max_blocks:
-                    r = pending[head]
+                kind0, j, scanned = None, head, 0
+                while j < len(pending) and len(take) < self.max_blocks and scanned < 4 * self.max_blocks:
+                    if admitted[j]:
+                        j += 1
+                        continue
+                    r = pending[j]
                     if realtime and r.arrival > now:
                         break
-                    if take and _kind(r) != _kind(take[0]):   # one launch runs one kind
-                        break
+                    scanned += 1
+                    k = _kind(r)
+                    if take and k != kind0:
+                        j += 1
+                        continue
                     bi, bo = self._bytes(r)
                     if take and (cols + r.alignment.shape[1] > self.max_cols or
                                  self.in_flight + nbytes + bi + bo > self.budget):
@@ -215,8 +225,12 @@ class Server:
                     if not take and self.in_flight > 0 and self.in_flight + bi + bo > self.budget:
                         break
                     take.append(r)
+                    kind0 = k
+                    admitted[j] = True
                     cols += r.alignment.shape[1]
                     nbytes += bi + bo
+                    j += 1
+                while head < len(pending) and admitted[head]:
                     head += 1
                 if take:
                     self._launch(lane, take)
