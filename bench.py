@@ -34,7 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE = os.path.join(ROOT, 'profiles', 'fused_small_hky85_c2.md')
+PROFILE = os.path.join(ROOT, 'profiles', 'fused_small_hky85_r127_c2e5.md')
 
 
 def profiled_traffic(path=PROFILE):
