@@ -385,9 +385,9 @@ def other_workload(args, log):
         t_bc = timed(lambda: bc_holder.__setitem__('bc', sb.BranchCounts(aln, t, m, engine=eng)), reps)
         io = sb.InsideOutside(aln, t, m, engine=eng)
 
-        def post():
-            io._npost = None
-            io.node_posterior()
+        def post():   # node and branch posteriors in one launch
+            io._npost = io._bpost = None
+            io._posteriors(('npost', 'bpost'))
         t_post = timed(post, reps)
         cols = np.arange(0, C, max(1, C // 300))[:300]
         ot, om = O.Tree(*t), O.DiagModel(*m)
@@ -429,9 +429,9 @@ def other_workload(args, log):
         R = len(t.parentIndex)
         io = sb.InsideOutside(aln, t, m, engine=eng)
 
-        def post():
+        def post():   # node and branch posteriors in one launch
             io._npost = io._bpost = None
-            io.node_posterior()
+            io._posteriors(('npost', 'bpost'))
         post()
         torch.cuda.synchronize()
         reps = max(5, args.steps // 5)

@@ -484,30 +484,35 @@ class InsideOutside:
         return BranchCounts(*self._args, f81_fast_flag=f81_fast_flag, branch_mask=None,
                             precision=self._precision, engine=self._eng)
 
-    def _posteriors(self):
-        if self._npost is None and self._stack is not None:
+    def _posteriors(self, want=('npost', 'bpost')):
+        """Compute the requested posteriors ('npost', 'bpost') that are not cached
+        yet, in one launch (each query asks only for its own output: the branch
+        joints are R A^2 C values)."""
+        want = {w for w in want if getattr(self, '_' + w) is None}
+        if not want:
+            return
+        if self._stack is not None:
             H, flat = self._stack
-            outs = [_inside_outside(self._args[0], self._args[1], m, {'npost', 'bpost'},
+            outs = [_inside_outside(self._args[0], self._args[1], m, want,
                                     precision=self._precision, engine=self._eng) for m in flat]
-            self._npost = torch.stack([o['npost'] for o in outs]).reshape(tuple(H) + tuple(outs[0]['npost'].shape))
-            self._bpost = torch.stack([o['bpost'] for o in outs]).reshape(tuple(H) + tuple(outs[0]['bpost'].shape))
-        if self._npost is None:
-            if self._list:
-                out = _multi(*self._args, {'npost', 'bpost'}, self._precision, self._eng)
-            else:
-                out = _inside_outside(*self._args, {'npost', 'bpost'}, precision=self._precision,
-                                      engine=self._eng)
-            self._npost, self._bpost = out['npost'], out['bpost']
+            out = {w: torch.stack([o[w] for o in outs]).reshape(tuple(H) + tuple(outs[0][w].shape))
+                   for w in want}
+        elif self._list:
+            out = _multi(*self._args, want, self._precision, self._eng)
+        else:
+            out = _inside_outside(*self._args, want, precision=self._precision, engine=self._eng)
+        for w in want:
+            setattr(self, '_' + w, out[w])
 
     def node_posterior(self, node: int | None = None):
         """(A, C) posterior of one node, or (R, A, C) for all nodes (row 0: root)."""
-        self._posteriors()
+        self._posteriors(('npost',))
         return self._npost if node is None else self._npost[..., node, :, :]
 
     def branch_posterior(self, node: int | None = None):
         """(A, A, C) joint posterior P(x_parent = i, x_node = j) of branch ``node``
         (> 0), or (R, A, A, C) for all branches (branch 0 zeros)."""
-        self._posteriors()
+        self._posteriors(('bpost',))
         return self._bpost if node is None else self._bpost[..., node, :, :, :]
 
 
