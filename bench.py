@@ -50,6 +50,13 @@ def profiled_traffic(path=PROFILE):
     return None
 
 
+def b_alg_grad(R, A, C):
+    """Algorithmic bytes of one gradient step: tokens, U and D written and read
+    (as b_alg_counts) and the FP64 logLike out; the per-branch partials are
+    R doubles per 64-column tile."""
+    return C * (R * 1 + 2 * R * (4 * A + 8) + 2 * (R - 1) * (4 * A + 8) + 8) + (C + 63) // 64 * R * 8
+
+
 def b_alg_counts(R, A, C):
     """Algorithmic bytes of one Counts call: tokens (1 B), U written+read for R
     nodes, D written+read for R-1 nodes (A FP32 + one FP64 log-scale each),
@@ -565,6 +572,13 @@ def other_workload(args, log):
                "value": units / wall, "unit": "column*branches/s", "n_gpus": 1, "steps": steps,
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
                "config": {"workload": "grad-hky85", "A": 4, "R": R, "C": C},
+               "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                            "achieved": b_alg_grad(R, 4, C) / wall / 1e9,
+                            "frac": b_alg_grad(R, 4, C) / wall / 1e9 / HBM_PEAK_GBS,
+                            "alg_bytes_per_step": b_alg_grad(R, 4, C),
+                            "note": "whole step (forward + adjoint kernel, per-tile reduction, host glue) "
+                                    "against the fused pass's algorithmic bytes; 782 column tiles "
+                                    "fill under a fifth of the wave slots"},
                "parity": par}
     print(json.dumps(out), flush=True)
 
