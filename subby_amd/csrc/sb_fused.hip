@@ -897,24 +897,53 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
 
 // Deterministic reduction of per-tile gradient partials, one thread per
 // (block, branch) / (block, rate parameter).
-__global__ void sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
+// Sum of the per-tile gradient partials.  A workgroup takes 16 nodes (or the
+// rate parameters) of one block; 16 threads per node add fixed tile strides
+// and a fixed tree adds their sums, so the result does not depend on timing.
+__global__ void __launch_bounds__(256) sb_grad_reduce_kernel(const SbLaunch* __restrict__ L) {
+  __shared__ double red[256];
   const SB_GLOBAL double* grad_part = glb<double>(L->grad_part);
   const SB_GLOBAL double* rate_part = glb<double>(L->rate_part);
-  const int grad_stride = L->grad_stride;
-  for (int b = blockIdx.x; b < L->nblocks; b += gridDim.x) {
-  const SbBlockDesc* B = L->blocks + b;
-  if (!(B->flags & SB_F_GRAD)) continue;
-  for (int n = threadIdx.x; n < B->R; n += blockDim.x) {
-    double s = 0.0;
-    if (n > 0)
-      for (int t = 0; t < B->ntiles; ++t) s += grad_part[(long long)(B->tile_base + t) * grad_stride + n];
-    if (B->dist_grad) B->dist_grad[n] = s;
-  }
-  for (int q = threadIdx.x; q < B->n_rate; q += blockDim.x) {
-    double s = 0.0;
-    for (int t = 0; t < B->ntiles; ++t) s += rate_part[(long long)(B->tile_base + t) * SB_MAX_RATE + q];
-    if (B->rate_grad) B->rate_grad[q] = s;
-  }
+  const long long grad_stride = L->grad_stride;
+  const int tid = threadIdx.x, nl = tid & 15, part = tid >> 4;
+  long long g = 0;
+  for (int b = 0; b < L->nblocks; ++b) {
+    const SbBlockDesc* B = L->blocks + b;
+    if (!(B->flags & SB_F_GRAD)) continue;
+    const int ngrp = (B->R + 15) / 16 + 1;     // node groups, then the rate parameters
+    for (int gr = 0; gr < ngrp; ++gr, ++g) {
+      if (g % gridDim.x != blockIdx.x) continue;
+      const bool rates = gr == ngrp - 1;
+      const int n = rates ? nl : gr * 16 + nl;
+      const int lim = rates ? B->n_rate : B->R;
+      const long long stride = rates ? SB_MAX_RATE : grad_stride;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      if (n < lim && (rates || n > 0)) {
+        const SB_GLOBAL double* base = (rates ? rate_part : grad_part) + (long long)B->tile_base * stride + n;
+        int t = part;
+        for (; t + 48 < B->ntiles; t += 64) {
+          s0 += base[(long long)t * stride];
+          s1 += base[(long long)(t + 16) * stride];
+          s2 += base[(long long)(t + 32) * stride];
+          s3 += base[(long long)(t + 48) * stride];
+        }
+        for (; t < B->ntiles; t += 16) s0 += base[(long long)t * stride];
+      }
+      red[tid] = (s0 + s1) + (s2 + s3);
+      __syncthreads();
+      for (int w = 8; w >= 1; w >>= 1) {
+        if (part < w) red[tid] += red[tid + 16 * w];
+        __syncthreads();
+      }
+      if (part == 0 && n < lim) {
+        if (rates) {
+          if (B->rate_grad) B->rate_grad[n] = red[tid];
+        } else if (B->dist_grad) {
+          B->dist_grad[n] = n > 0 ? red[tid] : 0.0;
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
