@@ -15,7 +15,8 @@ Extra keys on the JSON line:
                 int8 tokens, FP32 partials, FP64 log-scales) / kernel time
                 measured with HIP events on the launch stream.
   cpu_baseline  the FP64 NumPy oracle (subby.jax data flow, 128-column inside
-                chunks) on a bounded column sample, one thread.
+                chunks) on the whole block, one forked worker per core of the
+                CPU affinity (at most 16), one BLAS thread each; median of 5.
   plain         the plain one-block path (per-level launches, partials in HBM)
                 on the same block.
   batched_vs_plain  a stream of HKY85 127-node blocks (lengths LogNormal,
@@ -87,29 +88,52 @@ def make_stream(sb, n_blocks, seed=777):
     return blocks
 
 
-def cpu_baseline(t, m, aln, budget_s=12.0):
-    """FP64 oracle on a bounded sample of the same workload, one thread."""
+_CPU = {}
+
+
+def _cpu_chunk(bounds):
+    """One worker's share of the CPU baseline: Counts of columns [c0, c1)."""
     from threadpoolctl import threadpool_limits
     from oracle import subby_oracle as O
-    sample = 40000
-    sub = np.ascontiguousarray(aln[:, :sample])
-    ot, om = O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent)), O.DiagModel(*m)
-    R = len(t.parentIndex)
+    c0, c1 = bounds
     with threadpool_limits(limits=1):
-        O.counts_chunked(sub[:, :256], ot, om)           # warm-up
-        times = []
-        t0 = time.perf_counter()
-        while True:
+        O.counts_chunked(_CPU['aln'][:, c0:c1], _CPU['tree'], _CPU['model'], chunk=128)
+    return c1 - c0
+
+
+def cpu_baseline(t, m, aln, reps=5, max_workers=16):
+    """The FP64 oracle (subby.jax data flow: 128-column inside chunks, einsum
+    contractions, Counts with mask auto) over the whole block on every core of
+    this process's CPU affinity (at most ``max_workers``): one forked worker
+    process per core, each with one BLAS thread, on contiguous column ranges.
+    Must run before anything initialises the GPU (the workers are forked).
+    1 warm-up + ``reps`` timed reps; median and sd."""
+    import multiprocessing as mp
+    from oracle import subby_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    n = max(1, min(cores, max_workers))
+    R, C = aln.shape
+    _CPU.update(aln=np.ascontiguousarray(aln), tree=O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent)),
+                model=O.DiagModel(*m))
+    edges = np.linspace(0, C, 4 * n + 1).astype(int)
+    work = [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    times = []
+    with mp.get_context('fork').Pool(n) as pool:
+        pool.map(_cpu_chunk, work)                      # warm-up
+        for _ in range(reps):
             s = time.perf_counter()
-            O.counts_chunked(sub, ot, om, chunk=128)
+            done = sum(pool.map(_cpu_chunk, work))
             times.append(time.perf_counter() - s)
-            if time.perf_counter() - t0 > budget_s or len(times) >= 10:
-                break
-    med = float(np.median(times))
-    return {"value": sample * (R - 1) / med, "unit": "column*branches/s", "cores": 1, "kind": "port",
-            "sample": f"{sample} columns of the same HKY85 127-node block, Counts(mask auto) in FP64 "
-                      f"NumPy (subby.jax data flow, 128-column inside chunks), median of {len(times)} reps, "
-                      f"{med:.2f} s/rep, 1 BLAS thread"}
+    assert done == C
+    _CPU.clear()
+    med, sd = float(np.median(times)), float(np.std(times))
+    return {"value": C * (R - 1) / med, "unit": "column*branches/s", "cores": n, "kind": "port",
+            "affinity_cores": cores, "blas_threads_per_worker": 1, "reps": reps,
+            "seconds_median": med, "seconds_sd": sd,
+            "sample": f"the whole block ({C} columns, R = {R}): Counts (mask auto) in FP64 NumPy "
+                      f"(subby.jax data flow, 128-column inside chunks), {n} forked worker processes "
+                      f"x 1 BLAS thread over {len(work)} column ranges; 1 warm-up + {reps} reps, "
+                      f"median {med:.3f} s, sd {sd:.3f} s; reported baseline only"}
 
 
 FP32_PEAK_TFLOPS = 157.3
@@ -634,6 +658,11 @@ def main():
     log('bench: building the HKY85 127-node workload (C = 200,000)')
     t, m, aln_np = make_c2(sb)
     R, C, A = len(t.parentIndex), aln_np.shape[1], 4
+    cpu = None
+    if not args.no_cpu:   # first: its workers are forked before anything touches the GPU
+        log('bench: CPU baseline (FP64 oracle, whole block, all cores)')
+        cpu = cpu_baseline(t, m, aln_np)
+        log(f"bench: CPU baseline {cpu['value']:.3e} col*br/s on {cpu['cores']} cores")
     eng = sb.Engine(0)
     dev = eng.device
     aln = torch.from_numpy(aln_np).to(dev)
@@ -787,9 +816,8 @@ def main():
             "gpu_only_batch": gpu_only,
             "unit": "column*branches/s"}
 
-    if not args.no_cpu:
-        log('bench: CPU baseline (FP64 oracle, bounded sample)')
-        out["cpu_baseline"] = cpu_baseline(t, m, aln_np)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
 
     print(json.dumps(out), flush=True)
 

@@ -62,7 +62,15 @@ int sb_destroy(sb_context* ctx);
  * pi[A].  The model stays resident until sb_model_drop / sb_destroy. */
 int sb_model_put(sb_context* ctx, int A, const double* eigenvalues,
                  const double* eigenvectors, const double* pi, int* model_id);
+/* Retire a model id.  Its arena record is reused by a later sb_model_put of
+ * the same alphabet once every launch already submitted that reads it has
+ * finished (each launcher's submissions are fenced by events). */
 int sb_model_drop(sb_context* ctx, int model_id);
+/* Model arena diagnostics: bytes of records handed out so far (live models
+ * plus records waiting for reuse), live model count, retired records waiting
+ * for reuse.  Any output may be NULL.  (No reference counterpart.) */
+int sb_arena_stats(sb_context* ctx, long long* reserved_bytes, long long* live_models,
+                   int* retired_records);
 
 /* Upload an irreversible eigensystem (IrrevDiagModel, subby/jax/types.py:14-18;
  * built by diagonalize_irreversible, subby/jax/diagonalize.py:49-67):
@@ -126,9 +134,10 @@ int sb_loglike_grad(sb_context* ctx, int model_id, const int8_t* d_alignment,
  *   d_branch_post    T   [R][A][A][C]  joint (parent = i, node = j) posteriors (branch 0 zeros)
  *   d_loglike        f64 [C]
  * Every output takes any alphabet.  Above 8 states the posteriors run in the
- * large-alphabet kernel's posterior variant (one pass), and branch counts as
- * one branch-selecting block per branch (R - 1 passes); asking for both
- * launches the two in turn. */
+ * large-alphabet kernel's posterior variant and branch counts in its
+ * per-branch counts variant (the back-transform after every branch into that
+ * branch's slice), each one inside/outside pass; asking for both launches
+ * the two in turn. */
 int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,
                       const int32_t* parent_index, const double* distances, int R, int C, int dtype,
                       int f81_fast, int mask_mode, const uint8_t* d_mask, void* d_branch_counts,
@@ -216,6 +225,18 @@ int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype
  * the current value) and waves per CU (1..32; 0 = as many as the kernel's
  * occupancy allows, the default; negative keeps the current value). */
 int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
+/* Launch-shape knobs of a launcher (the autotuner's search space; no
+ * reference counterpart — the reference sizes its vmap batches by a memory
+ * budget only, scripts/batch_eigensub_maf.py:296-304):
+ *   SB_TUNE_WAVES_PER_GROUP  1, 2 or 4 (small-alphabet kernels)
+ *   SB_TUNE_WAVES_PER_CU     0..32, 0 = from the kernel's occupancy
+ *   SB_TUNE_STAGE_TOKENS     1 = the small-alphabet walk copies each tile's
+ *                            tokens to LDS first (default), 0 = reads them
+ *                            from HBM as it goes
+ * Results do not depend on any knob (bitwise). */
+enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_STAGE_TOKENS = 2 };
+int sb_batch_set_tune(sb_batch* b, int knob, int value);
+int sb_batch_get_tune(sb_batch* b, int knob, int* value);
 /* Block until the launcher's last submission has finished. */
 int sb_batch_sync(sb_batch* b);
 /* Kernel timing: when enabled, eager (non-graph) submissions bracket the
@@ -235,7 +256,10 @@ int sb_batch_kernel_ms(sb_batch* b, double* total_ms, int* launches);
  * copy, runs it as one persistent launch, and copies results straight into
  * the callers' host buffers.  Results are bitwise those of the same request
  * run alone.  All host pointers of a request must stay valid until it
- * completes.  Outputs are FP32 counts / root and FP64 logLike. */
+ * completes.  Outputs are FP32 counts / root and FP64 logLike.  Inline models
+ * are kept resident by content (requests with the same eigensystem share one
+ * record); beyond 4096 of them the least recently used idle one is dropped,
+ * so a stream of distinct models runs in a bounded model arena. */
 typedef struct sb_server sb_server;
 typedef struct sb_request {
   int model_id;                 /* resident model, or -1: the inline model below */

@@ -679,13 +679,11 @@ def branch_posterior(alignment, tree, model, root_pi=None):
 # Gradient  (subby/jax/vjp.py:62-133; rate parameters via HR2002 eqs. 3-4)
 # ---------------------------------------------------------------------------
 
-def loglike_branch_grad(alignment, tree, model, g=None):
-    """d(sum_c g_c logL_c)/d t_n, n = 0..R-1 (root entry 0) (vjp.py:62-133).
-
-    per_branch[n,c] = scale[n,c] sum_k D~_k mu_k e^{mu_k t_n} U~_k.
+def loglike_branch_grad_columns(alignment, tree, model):
+    """Per-column derivatives d logL_c / d t_n, (R, C) (root row 0), and logL
+    (vjp.py:62-133): per_branch[n,c] = scale[n,c] sum_k D~_k mu_k e^{mu_k t_n} U~_k.
     Irreversible models project D with V and U with V^{-1} and keep the real
-    part (vjp.py:80-88, 113-131).
-    """
+    part (vjp.py:80-88, 113-131)."""
     M, U, lnU, ll, D, lnD = inside_outside(alignment, tree, model)
     if is_irrev(model):
         Dt = D @ model.eigenvectors
@@ -698,7 +696,13 @@ def loglike_branch_grad(alignment, tree, model, g=None):
     sc = branch_scale(lnU, lnD, ll)
     per = (np.einsum('nck,nk,nck->nc', Dt, me, Ut) * sc).real
     per[0] = 0.0
-    C = U.shape[1]
+    return per, ll
+
+
+def loglike_branch_grad(alignment, tree, model, g=None):
+    """d(sum_c g_c logL_c)/d t_n, n = 0..R-1 (root entry 0) (vjp.py:62-133)."""
+    per, ll = loglike_branch_grad_columns(alignment, tree, model)
+    C = per.shape[1]
     g = np.ones(C) if g is None else np.asarray(g, dtype=np.float64)
     return per @ g, ll
 

@@ -13,6 +13,7 @@ SB_OK = 0
 SB_F32, SB_F64 = 0, 1
 SB_MASK_NONE, SB_MASK_AUTO, SB_MASK_EXPLICIT = 0, 1, 2
 SB_WANT_LOGLIKE, SB_WANT_ROOT, SB_WANT_COUNTS = 1, 2, 4
+SB_TUNE_WAVES_PER_GROUP, SB_TUNE_WAVES_PER_CU, SB_TUNE_STAGE_TOKENS = 0, 1, 2
 
 # Every exported symbol of include/subby_amd.h.
 EXPORTS = [
@@ -23,6 +24,7 @@ EXPORTS = [
     'sb_server_create', 'sb_server_destroy', 'sb_server_submit', 'sb_server_poll',
     'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
     'sb_counts_multi', 'sb_branch_counts_multi', 'sb_inside_outside_multi', 'sb_mixture_posterior',
+    'sb_arena_stats', 'sb_batch_set_tune', 'sb_batch_get_tune',
 ]
 
 
@@ -96,6 +98,7 @@ def load(path=LIB_PATH):
     lib.sb_model_put.argtypes = [vp, i, vp, vp, vp, P(i)]
     lib.sb_model_put_irrev.argtypes = [vp, i, vp, vp, vp, vp, i, P(i)]
     lib.sb_model_drop.argtypes = [vp, i]
+    lib.sb_arena_stats.argtypes = [vp, vp, vp, vp]
     lib.sb_loglike.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp]
     lib.sb_root_prob.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp]
     lib.sb_counts.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]
@@ -104,6 +107,8 @@ def load(path=LIB_PATH):
     lib.sb_batch_destroy.argtypes = [vp]
     lib.sb_batch_submit.argtypes = [vp, i, vp, i, i]
     lib.sb_batch_set_tiling.argtypes = [vp, i, i]
+    lib.sb_batch_set_tune.argtypes = [vp, i, i]
+    lib.sb_batch_get_tune.argtypes = [vp, i, P(i)]
     lib.sb_batch_sync.argtypes = [vp]
     lib.sb_batch_set_timing.argtypes = [vp, i]
     lib.sb_batch_kernel_ms.argtypes = [vp, P(d), P(i)]

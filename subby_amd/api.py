@@ -18,6 +18,7 @@ Python assertions (non-binary tree, F81 with an unsupported model) raise
 from __future__ import annotations
 
 import ctypes
+from collections import OrderedDict
 import threading
 
 import numpy as np
@@ -35,18 +36,38 @@ def _stream_ptr(device):
 
 
 class Engine:
-    """One sb_context on one GPU plus the resident-model cache."""
+    """One sb_context on one GPU plus the resident-model cache.
 
-    def __init__(self, device=0):
+    The cache holds up to ``max_models`` eigensystems by content (None: no
+    limit).  Beyond that, the least recently used model that no call since
+    the last ``new_call()`` has asked for is dropped; the library reuses its
+    device record once the launches that read it have finished.  Every API
+    call starts a new generation, so the ids a call resolves stay valid until
+    it has submitted its launch (one calling thread per Engine)."""
+
+    def __init__(self, device=0, max_models: int | None = 4096):
         if not torch.cuda.is_available():
             raise RuntimeError('subby_amd needs a ROCm GPU (torch.cuda.is_available() is False)')
         self.lib = _lib.load()
         self.device = torch.device('cuda', device)
         self.ctx = ctypes.c_void_p()
         _lib.check(self.lib.sb_create(ctypes.byref(self.ctx), device))
-        self._models = {}
+        self._models = OrderedDict()      # key -> [model id, generation of last use]
+        self._gen = 0
+        self.max_models = max_models
         self._launchers = {}
         self._lock = threading.Lock()
+
+    def new_call(self):
+        """Start a call: models used by earlier calls become evictable."""
+        self._gen += 1
+        return self
+
+    def arena_stats(self):
+        """(reserved model-arena bytes, live models, retired records)."""
+        r, n, q = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int()
+        _lib.check(self.lib.sb_arena_stats(self.ctx, ctypes.byref(r), ctypes.byref(n), ctypes.byref(q)))
+        return r.value, n.value, q.value
 
     def close(self):
         for L in getattr(self, '_launchers', {}).values():
@@ -80,8 +101,13 @@ class Engine:
         key = (A, irr, bool(exact) and irr, ev.tobytes(), V.tobytes(),
                Vi.tobytes() if irr else b'', pi.tobytes())
         with self._lock:
-            mid = self._models.get(key)
-            if mid is None:
+            ent = self._models.get(key)
+            if ent is not None:
+                ent[1] = self._gen
+                self._models.move_to_end(key)
+                return ent[0]
+            self._evict()
+            if True:
                 out = ctypes.c_int()
                 if irr:
                     _lib.check(self.lib.sb_model_put_irrev(self.ctx, A, ev.ctypes.data, V.ctypes.data,
@@ -91,8 +117,18 @@ class Engine:
                     _lib.check(self.lib.sb_model_put(self.ctx, A, ev.ctypes.data, V.ctypes.data,
                                                      pi.ctypes.data, ctypes.byref(out)))
                 mid = out.value
-                self._models[key] = mid
+                self._models[key] = [mid, self._gen]
         return mid
+
+    def _evict(self):
+        if self.max_models is None:
+            return
+        while len(self._models) >= self.max_models:
+            key, (mid, gen) = next(iter(self._models.items()))
+            if gen >= self._gen:        # everything left was asked for by the current call
+                return
+            del self._models[key]
+            _lib.check(self.lib.sb_model_drop(self.ctx, mid))
 
     def launcher(self):
         """A batch launcher on the current stream (created once per stream)."""
@@ -108,7 +144,7 @@ class Engine:
 
     def drop_models(self):
         with self._lock:
-            for mid in self._models.values():
+            for mid, _ in self._models.values():
                 self.lib.sb_model_drop(self.ctx, mid)
             self._models.clear()
 
@@ -155,7 +191,7 @@ def _prep(alignment, tree, model, engine, strip=False):
     """Device alignment, host tree arrays and the model.  ``strip``: drop the
     trailing dummy leaves of a padded tree (pad_tree), which leave LogLike,
     RootProb and Counts unchanged."""
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     m = ensure_diag(model)
     A = len(m.pi)
     aln = eng.alignment(alignment, A)
@@ -201,7 +237,7 @@ def _multi(alignment, tree, model, want, precision, engine, f81=False, branch_ma
     """One sb_counts_multi / sb_branch_counts_multi / sb_inside_outside_multi call
     for a model list (C, 1) or a model grid (1 or C, R).  ``want``: subset of
     {'counts', 'root', 'll', 'bcounts'} or of {'npost', 'bpost', 'll'}."""
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     exact = _exact(irrev_convention)
     if _is_model_grid(model):
         table = [list(row) for row in model]
@@ -264,15 +300,38 @@ def _multi(alignment, tree, model, want, precision, engine, f81=False, branch_ma
     return out
 
 
+def _stack_masks(branch_mask, H, R, C):
+    """Per-model branch masks for H stacked models: "auto" / None apply to every
+    model; an explicit mask is (R, C) (shared) or (*H, R, C) (one slice per model,
+    subby/jax/__init__.py:355-356), or any shape that broadcasts to (*H, R, C)."""
+    n = int(np.prod(H))
+    if branch_mask is None or isinstance(branch_mask, str):
+        return [branch_mask] * n
+    mk = branch_mask if isinstance(branch_mask, torch.Tensor) else torch.from_numpy(np.asarray(branch_mask))
+    full = tuple(H) + (R, C)
+    if mk.dim() < 2 or tuple(mk.shape[-2:]) != (R, C) or mk.dim() > len(full):
+        raise ValueError(f'branch_mask for {tuple(H)} stacked models must have shape ({R}, {C}) or '
+                         f'{full}, got {tuple(mk.shape)}')
+    try:
+        mk = torch.broadcast_to(mk, full)
+    except RuntimeError:
+        raise ValueError(f'branch_mask of shape {tuple(mk.shape)} does not broadcast to {full}') from None
+    mk = mk.reshape((n, R, C))
+    return [mk[h] for h in range(n)]
+
+
 def _stack_call(alignment, tree, stack, want, precision, engine, f81=False, branch_mask='auto'):
     """(*H, ...) outputs of H stacked models over one alignment: one persistent
-    launch with one block per model, all reading the same device alignment."""
+    launch with one block per model, all reading the same device alignment;
+    model h uses slice h of a (*H, R, C) explicit branch mask."""
     from .batch import Block
     H, models = stack
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     A = len(ensure_diag(models[0]).pi)
     aln = eng.alignment(alignment, A)
-    blocks = [Block(aln, tree, m, want=want, f81_fast=f81, branch_mask=branch_mask) for m in models]
+    R, C = aln.shape
+    masks = _stack_masks(branch_mask, H, R, C)
+    blocks = [Block(aln, tree, m, want=want, f81_fast=f81, branch_mask=mk) for m, mk in zip(models, masks)]
     eng.launcher().submit(blocks, precision=precision)
     return {w: torch.stack([getattr(b, w) for b in blocks]).reshape(H + getattr(blocks[0], w).shape)
             for w in want}
@@ -421,8 +480,10 @@ def BranchCounts(alignment, tree, model, maxChunkSize: int = 128, f81_fast_flag:
     st = _stacked(model)
     if st is not None:
         H, flat = st
-        outs = [_inside_outside(alignment, tree, m, {'bcounts'}, f81_fast_flag, branch_mask, precision,
-                                engine, irrev_convention)['bcounts'] for m in flat]
+        R, C = np.shape(alignment)
+        masks = _stack_masks(branch_mask, H, R, C)
+        outs = [_inside_outside(alignment, tree, m, {'bcounts'}, f81_fast_flag, mk, precision,
+                                engine, irrev_convention)['bcounts'] for m, mk in zip(flat, masks)]
         return torch.stack(outs).reshape(tuple(H) + tuple(outs[0].shape))
     return _inside_outside(alignment, tree, model, {'bcounts'}, f81_fast_flag, branch_mask,
                            precision, engine, irrev_convention)['bcounts']
@@ -523,7 +584,7 @@ def MixturePosterior(alignment, tree, models, log_weights, maxChunkSize: int = 1
     softmax_k(LogLike_k + log_weights[k]) (subby/jax/__init__.py:653-678,
     mixture.py:6-30).  ``models``: a list of K models or models stacked along
     one leading axis (e.g. scale_model(base, gamma rates)); K <= 64."""
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     st = None if _is_model_list(models) else _stacked(models)
     ms = st[1] if st is not None else list(models)
     K = len(ms)

@@ -82,7 +82,7 @@ class Launcher:
         """Build the sb_block array for ``blocks`` (outputs allocated here).  Host
         tree arrays stay referenced by the blocks for replays."""
         dt, tdt = _PREC[precision]
-        dev = self.eng.device
+        dev = self.eng.new_call().device
         arr = (_lib.SbBlock * len(blocks))()
         with torch.cuda.stream(self.stream):
             for i, b in enumerate(blocks):
@@ -102,11 +102,17 @@ class Launcher:
                     if 'root' in b.want and b.root is None:
                         b.root = torch.empty((A, C), dtype=tdt, device=dev)
                 if isinstance(b.branch_mask, str) or b.branch_mask is None:
+                    if b.branch_mask not in MASK:
+                        raise ValueError(f'block {i}: branch_mask must be "auto", None or an (R, C) array, '
+                                         f'got {b.branch_mask!r}')
                     mode, mptr = MASK[b.branch_mask], None
                 else:
                     mk = torch.as_tensor(np.asarray(b.branch_mask) if not isinstance(b.branch_mask, torch.Tensor)
-                                         else b.branch_mask).to(dev)
-                    mk = (mk != 0).to(torch.uint8).contiguous()
+                                         else b.branch_mask)
+                    if tuple(mk.shape) != (R, C):
+                        raise ValueError(f'block {i}: branch_mask must have shape ({R}, {C}), '
+                                         f'got {tuple(mk.shape)}')
+                    mk = (mk.to(dev) != 0).to(torch.uint8).contiguous()
                     b._keep.append(mk)
                     mode, mptr = _lib.SB_MASK_EXPLICIT, mk.data_ptr()
                 b._keep += [aln, parent, dist]
@@ -151,7 +157,7 @@ class Launcher:
 def counts_plain(alignment, tree, model, branch_mask='auto', engine: Engine | None = None):
     """Plain one-block path (A = 4, FP32): per-level launches, partials in HBM.
     Returns (counts (4,4,C) f32, logLike (C,) f64, root (4,C) f32); synchronous."""
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     m = ensure_diag(model)
     A = len(m.pi)
     aln = eng.alignment(alignment, A)

@@ -102,6 +102,25 @@ __device__ inline void token_vec_r(int tok, int A, const T* ones, T* u) {
   for (int a = 0; a < AP; ++a) u[a] = (a == tok) ? T(1) : fill * ones[a];
 }
 
+// Token of node n for this lane's column: from the wave's LDS stage when the
+// launch stages tokens (tl != null, uniform), else from the alignment.
+__device__ inline int tok_at(const int8_t* tl, const SB_GLOBAL int8_t* aln, long long C, int n, int lane) {
+  return tl ? int(tl[n * SB_WAVE + lane]) : int(aln[(long long)n * C]);
+}
+
+// Copy the tile's tokens (R rows x 64 columns, one byte each) into the wave's
+// LDS stage: 32 rows of loads in flight, then their LDS writes.
+__device__ inline void stage_tokens(int8_t* tl, const SB_GLOBAL int8_t* aln, long long C, int R, int lane) {
+  for (int r0 = 0; r0 < R; r0 += 32) {
+    int v[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = (r0 + k < R) ? int(aln[(long long)(r0 + k) * C]) : 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      if (r0 + k < R) tl[(r0 + k) * SB_WAVE + lane] = int8_t(v[k]);
+  }
+}
+
 // Child data as loaded (decoded when used): internal -> vector + word, leaf -> token in w.
 template <typename T, int AP>
 struct Raw {
@@ -111,9 +130,10 @@ struct Raw {
 
 template <typename T, int AP>
 __device__ inline void load_raw(Raw<T, AP>& r, int n, int slot, const SB_GLOBAL int8_t* aln, long long C,
-                                const SB_GLOBAL char* reg, const SB_GLOBAL int* words, int lane) {
+                                const SB_GLOBAL char* reg, const SB_GLOBAL int* words, int lane,
+                                const int8_t* tl) {
   if (slot < 0) {
-    r.w = aln[(long long)n * C];
+    r.w = tok_at(tl, aln, C, n, lane);
   } else {
     Tile<T, AP>::load_vec(reg, slot, lane, r.u);
     r.w = words[slot * SB_WAVE + lane];
@@ -377,6 +397,9 @@ __global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLau
   }
 }
 
+// LDS token stage of the small-alphabet walk: tok_rows x 64 bytes per wave.
+extern __shared__ int8_t sb_tok_lds[];
+
 __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
   if (threadIdx.x < SB_NCOUNTER && blockIdx.x == 0) L->counter[threadIdx.x] = 0;
 }
@@ -421,6 +444,12 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
       pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
     }
     const int BTS = sb_bt_stride_k(AP, IRR);
+    // the tile's tokens go to LDS once, so no token read of the walk waits on HBM
+    int8_t* tl = nullptr;
+    if (args.tok_rows > 0 && nint > 0) {
+      tl = sb_tok_lds + (threadIdx.x / SB_WAVE) * args.tok_rows * SB_WAVE;
+      stage_tokens(tl, aln, C, B->R, lane);
+    }
 
     const bool outside = (flags & (SB_F_COUNTS | SB_F_GRAD)) != 0;
 
@@ -437,21 +466,21 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     if (nint > 0) {
       const SB_CONST int* rec = sched + (nint - 1) * 8;
       np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
-      load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane);
-      load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane);
+      load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane, tl);
+      load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane, tl);
     }
     for (int i = nint - 1; i >= 0; --i) {
       const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
       const Raw<T, AP> rl = nl, rr = nr;
-      const int tokp = aln[(long long)p * C];
+      const int tokp = tok_at(tl, aln, C, p, lane);
       bool fwd_l = false, fwd_r = false;
       if (i > 0) {
         const SB_CONST int* rec = sched + (i - 1) * 8;
         np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
         fwd_l = nsl == i;
         fwd_r = nsr == i;
-        if (!fwd_l) load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane);
-        if (!fwd_r) load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane);
+        if (!fwd_l) load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane, tl);
+        if (!fwd_r) load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane, tl);
       }
       Child<T, AP> cl, cr;
       decode_child<T, AP>(rl, sl, A, ones, cl);
@@ -528,22 +557,22 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     if (nint > 0) {
       const SB_CONST int* rec = sched;
       qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
-      load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane);
-      load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane);
+      load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane, tl);
+      load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane, tl);
     }
     for (int i = 0; i < nint; ++i) {
       const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
       const Raw<T, AP> rl = ql, rr = qr, rd = qd;
-      const int tokp = aln[(long long)p * C];
+      const int tokp = tok_at(tl, aln, C, p, lane);
       bool fwd0 = false, fwd1 = false;
       if (i + 1 < nint) {
         const SB_CONST int* rec = sched + (i + 1) * 8;
         qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
         fwd0 = qp == l;
         fwd1 = qp == r;
-        if (!fwd0 && !fwd1) load_raw<T, AP>(qd, 0, i + 1, aln, C, Dreg, Dw, lane);
-        load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane);
-        load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane);
+        if (!fwd0 && !fwd1) load_raw<T, AP>(qd, 0, i + 1, aln, C, Dreg, Dw, lane, tl);
+        load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane, tl);
+        load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane, tl);
       }
       T x[AP];
       int ex;
@@ -895,8 +924,6 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
   }
 }
 
-// Deterministic reduction of per-tile gradient partials, one thread per
-// (block, branch) / (block, rate parameter).
 // Sum of the per-tile gradient partials.  A workgroup takes 16 nodes (or the
 // rate parameters) of one block; 16 threads per node add fixed tile strides
 // and a fixed tree adds their sums, so the result does not depend on timing.
@@ -987,22 +1014,23 @@ static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr,
   return k;
 }
 
-// Resident workgroups per CU for a block of `threads` threads (0 on error).
+// Resident workgroups per CU for a block of `threads` threads with `lds`
+// bytes of token stage (0 on error).
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        bool rows, int threads) {
+                                        bool rows, int threads, int lds) {
   void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
   if (!k) return 0;
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, size_t(lds)) != hipSuccess) return 0;
   return n;
 }
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            bool rows, dim3 grid, dim3 block, const SbLaunch* L,
-                                            hipStream_t s) {
+                                            bool rows, dim3 grid, dim3 block, int lds,
+                                            const SbLaunch* L, hipStream_t s) {
   void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
   if (!k) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k, grid, block, 0, s, L);
+  hipLaunchKernelGGL(k, grid, block, size_t(lds), s, L);
   return hipGetLastError();
 }
 

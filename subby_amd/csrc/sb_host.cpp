@@ -23,8 +23,8 @@
 #include "sb_plan.h"
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            bool rows, dim3 grid, dim3 block, const SbLaunch* L,
-                                            hipStream_t s);
+                                            bool rows, dim3 grid, dim3 block, int lds,
+                                            const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_gather_cols(const void* src, void* dst, int esz, int rows, int C,
                                             const int* const* cols, hipStream_t s);
 extern "C" hipError_t sb_launch_scatter_cols(const void* src, void* dst, int esz, int rows, int C,
@@ -39,7 +39,7 @@ extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, in
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81, int out);
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81, int out);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        bool rows, int threads);
+                                        bool rows, int threads, int lds);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad, int out);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
@@ -103,6 +103,12 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// A launch that may read a model's records: submission `seq` of the launcher
+// with serial number `serial`.
+struct Fence {
+  uint64_t serial, seq;
+};
+
 struct Model {
   bool alive = false;
   int A = 0, AP = 0;
@@ -111,6 +117,18 @@ struct Model {
   std::vector<double> mu, V, pi;   // host copies (plain path)
   double* d64 = nullptr;  // FP64 master
   void* rec[2] = {nullptr, nullptr};  // float / double records
+  char* base = nullptr;   // arena record (FP64 master + float + double records)
+  size_t bytes = 0;
+  // the last submission of every launcher that referenced the model
+  mutable std::vector<Fence> uses;
+};
+
+// Arena record of a dropped model; reused by a later sb_model_put of the same
+// size once every launch in `uses` has finished.
+struct Retired {
+  char* base;
+  size_t bytes;
+  std::vector<Fence> uses;
 };
 
 }  // namespace
@@ -133,6 +151,10 @@ struct sb_context {
   std::vector<void*> arena_chunks;
   char* arena_cur = nullptr;
   size_t arena_off = 0, arena_cap = 0;
+  size_t arena_reserved = 0;        // bytes handed out from chunks (live + retired records)
+  std::vector<Retired> retired;     // dropped records waiting for their launches to finish
+  std::vector<sb_batch*> live;      // launchers of this context (fences name them by serial)
+  uint64_t next_serial = 0;
   char* pend[2] = {nullptr, nullptr};
   size_t pend_cap[2] = {0, 0};
   size_t pend_len = 0;
@@ -150,16 +172,22 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows;
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds;
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds;
   }
 };
 
 struct sb_batch {
   sb_context* ctx = nullptr;
   hipStream_t stream = nullptr;
+  // submissions: seq counts them; ring_ev[k] was recorded after submission
+  // ring_seq[k] (the newest submissions, one event each)
+  static const int kRing = 8;
+  uint64_t serial = 0, seq = 0;
+  hipEvent_t ring_ev[kRing] = {};
+  uint64_t ring_seq[kRing] = {};
   SbLaunch* header = nullptr;       // start of the plan buffer (fixed until it grows)
   DevBuf plan, bt, scratch, part, stage;
   // calls with several models: gathered alignment / mask, staged outputs,
@@ -176,6 +204,7 @@ struct sb_batch {
   bool staged_pending = false;
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
+  bool stage_tokens = true;         // small-alphabet walk reads tokens from LDS
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
@@ -269,8 +298,49 @@ Model* get_model(sb_context* ctx, int id) {
   return &ctx->models[id];
 }
 
-// Reserve `bytes` (multiple of 256) in the model arena.
+// True when submission f.seq of launcher f.serial has finished (or the
+// launcher is gone: batch_free synchronises its stream first).  A launcher's
+// submissions finish in order, so the event of any later one also answers.
+bool fence_done(sb_context* ctx, const Fence& f) {
+  const sb_batch* b = nullptr;
+  for (const sb_batch* x : ctx->live)
+    if (x->serial == f.serial) b = x;
+  if (!b) return true;
+  int best = -1;
+  for (int k = 0; k < sb_batch::kRing; ++k)
+    if (b->ring_ev[k] && b->ring_seq[k] >= f.seq && (best < 0 || b->ring_seq[k] < b->ring_seq[best])) best = k;
+  if (best < 0) return false;   // no event recorded for it yet
+  const hipError_t e = hipEventQuery(b->ring_ev[best]);
+  if (e == hipSuccess) return true;
+  if (e != hipErrorNotReady) (void)hipGetLastError();
+  return false;
+}
+
+// Note that submission `seq` of launcher b may read model m.
+void note_use(const Model* m, const sb_batch* b, uint64_t seq) {
+  for (Fence& f : m->uses)
+    if (f.serial == b->serial) {
+      f.seq = seq;
+      return;
+    }
+  m->uses.push_back({b->serial, seq});
+}
+
+// Reserve `bytes` (multiple of 256) in the model arena: a finished retired
+// record of the same size if there is one, else fresh chunk space.
 int arena_alloc(sb_context* ctx, size_t bytes, char** out) {
+  for (size_t i = 0, checked = 0; i < ctx->retired.size() && checked < 8; ++i) {
+    Retired& r = ctx->retired[i];
+    if (r.bytes != bytes) continue;
+    ++checked;
+    bool done = true;
+    for (const Fence& f : r.uses) done = done && fence_done(ctx, f);
+    if (!done) continue;
+    *out = r.base;
+    ctx->retired.erase(ctx->retired.begin() + long(i));
+    return SB_OK;
+  }
+  ctx->arena_reserved += bytes;
   if (!ctx->arena_cur || ctx->arena_off + bytes > ctx->arena_cap) {
     const size_t chunk = std::max<size_t>(64u << 20, bytes);
     void* p = nullptr;
@@ -385,7 +455,7 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
                            b->header, s));
   else
     SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0, grid,
-                                 block, b->header, s));
+                                 block, key.lds, b->header, s));
   if (b->timing) {
     SB_HIP(hipEventRecord(ev.second, s));
     b->tev.push_back(ev);
@@ -603,12 +673,21 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   // persistent grid: resident waves per CU from the kernel's occupancy unless
   // the launcher was given an explicit value; never more waves than tiles
   const int wpg = (large || post) ? 1 : b->waves_per_group;
+  // small-alphabet fused kernel: each wave stages its tile's tokens (R_max
+  // rows, rounded up to 16, of 64 bytes) in LDS when a workgroup's stage fits
+  // 64 KiB
+  int tok_rows = 0;
+  if (!large && !post && b->stage_tokens) {
+    const int rows = (R_max + 15) / 16 * 16;
+    if (size_t(rows) * SB_WAVE * size_t(wpg) <= (64u << 10)) tok_rows = rows;
+  }
+  const int lds_tok = tok_rows * SB_WAVE * wpg;
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
     const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
                     : f81w  ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
-                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg);
+                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg, lds_tok);
     wpc = occ > 0 ? occ * wpg : 8;
   }
   int nwaves = ctx->num_cus * wpc;
@@ -751,6 +830,16 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   L->grad_stride = grad_stride;
   L->AP = AP;
   L->tile_cols = tile_cols;
+  L->tok_rows = tok_rows;
+
+  // every model this launch reads is fenced by its submission number
+  const uint64_t seq = b->seq + 1;
+  for (int i = 0; i < nb; ++i) {
+    note_use(mods[i], b, seq);
+    if (in[i].rows)
+      for (int n = 0; n < in[i].blk.R; ++n) note_use(in[i].rows[n], b, seq);
+    if (in[i].root_model) note_use(in[i].root_model, b, seq);
+  }
 
   hipStream_t s = b->stream;
   rc = flush_models(ctx, s);
@@ -761,7 +850,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
                lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0};
+               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -776,6 +865,13 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     if (rc) return rc;
   }
   SB_HIP(hipEventRecord(b->done, s));
+  {
+    const int k = int(seq % sb_batch::kRing);
+    if (!b->ring_ev[k]) SB_HIP(hipEventCreateWithFlags(&b->ring_ev[k], hipEventDisableTiming));
+    SB_HIP(hipEventRecord(b->ring_ev[k], s));
+    b->ring_seq[k] = seq;
+    b->seq = seq;
+  }
   return SB_OK;
 }
 
@@ -790,6 +886,8 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
   }
   SB_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
   SB_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  b->serial = ++ctx->next_serial;
+  ctx->live.push_back(b);
   *out = b;
   return SB_OK;
 }
@@ -797,6 +895,10 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
 void batch_free(sb_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
+  auto& live = b->ctx->live;
+  live.erase(std::remove(live.begin(), live.end(), b), live.end());
+  for (hipEvent_t e : b->ring_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
   for (auto* v : {&b->tev, &b->tfree})
     for (auto& e : *v) {
@@ -1248,6 +1350,8 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.d64 = reinterpret_cast<double*>(base);
   m.rec[0] = base + b64;
   m.rec[1] = base + b64 + bf;
+  m.base = base;
+  m.bytes = b64 + bf + bd;
   int id;
   if (!ctx->free_ids.empty()) {
     id = ctx->free_ids.back();
@@ -1366,6 +1470,8 @@ int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues, const 
   m.d64 = reinterpret_cast<double*>(base);
   m.rec[0] = base + b64;
   m.rec[1] = base + b64 + bf;
+  m.base = base;
+  m.bytes = b64 + bf + bd;
   int id;
   if (!ctx->free_ids.empty()) {
     id = ctx->free_ids.back();
@@ -1384,11 +1490,27 @@ int sb_model_drop(sb_context* ctx, int model_id) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   Model* m = get_model(ctx, model_id);
   if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
-  // arena space is reclaimed with the context (records may still be in use
-  // by queued launches)
+  // the record is reused by a later sb_model_put once every launch that may
+  // read it has finished
+  if (m->base) ctx->retired.push_back({m->base, m->bytes, std::move(m->uses)});
+  m->uses.clear();
+  m->base = nullptr;
   m->d64 = nullptr;
+  m->rec[0] = m->rec[1] = nullptr;
   m->alive = false;
   ctx->free_ids.push_back(model_id);
+  return SB_OK;
+}
+
+int sb_arena_stats(sb_context* ctx, long long* reserved_bytes, long long* live_models,
+                   int* retired_records) {
+  if (!ctx) return fail(SB_ERR_ARG, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  long long n = 0;
+  for (const Model& m : ctx->models) n += m.alive ? 1 : 0;
+  if (reserved_bytes) *reserved_bytes = (long long)ctx->arena_reserved;
+  if (live_models) *live_models = n;
+  if (retired_records) *retired_records = int(ctx->retired.size());
   return SB_OK;
 }
 
@@ -1578,6 +1700,29 @@ int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu) {
     b->waves_per_cu = waves_per_cu;
   }
   return SB_OK;
+}
+
+int sb_batch_set_tune(sb_batch* b, int knob, int value) {
+  if (!b) return fail(SB_ERR_ARG, "null launcher");
+  switch (knob) {
+    case SB_TUNE_WAVES_PER_GROUP: return sb_batch_set_tiling(b, value, -1);
+    case SB_TUNE_WAVES_PER_CU: return sb_batch_set_tiling(b, 0, value);
+    case SB_TUNE_STAGE_TOKENS:
+      if (value != 0 && value != 1) return fail(SB_ERR_ARG, "stage_tokens must be 0 or 1");
+      b->stage_tokens = value != 0;
+      return SB_OK;
+    default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
+  }
+}
+
+int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
+  if (!b || !value) return fail(SB_ERR_ARG, "null argument");
+  switch (knob) {
+    case SB_TUNE_WAVES_PER_GROUP: *value = b->waves_per_group; return SB_OK;
+    case SB_TUNE_WAVES_PER_CU: *value = b->waves_per_cu; return SB_OK;
+    case SB_TUNE_STAGE_TOKENS: *value = b->stage_tokens ? 1 : 0; return SB_OK;
+    default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
+  }
 }
 
 int sb_batch_submit(sb_batch* b, int n_blocks, const sb_block* blocks, int dtype, int use_graph) {

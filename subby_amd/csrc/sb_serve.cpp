@@ -25,10 +25,12 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <list>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/subby_amd.h"
@@ -117,8 +119,37 @@ class Pool {
   bool quit_ = false;
 };
 
+// A resident copy of a model that requests gave inline, keyed by content:
+// requests with the same eigensystem share it, and it is dropped (its arena
+// record then reused once the launches that read it have finished) when more
+// than kInlineModels are resident and no queued launch uses it.
+struct InlineModel {
+  int id = -1;
+  int refs = 0;                 // admitted requests not yet completed
+  uint64_t hash = 0;
+  std::vector<double> key;      // A, eigenvalues, eigenvectors, pi
+};
+const size_t kInlineModels = 4096;
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+void inline_key(const sb_request& r, std::vector<double>* key, uint64_t* hash) {
+  const size_t A = size_t(r.A);
+  key->resize(1 + 2 * A + A * A);
+  (*key)[0] = double(A);
+  std::memcpy(key->data() + 1, r.eigenvalues, A * 8);
+  std::memcpy(key->data() + 1 + A, r.eigenvectors, A * A * 8);
+  std::memcpy(key->data() + 1 + A + A * A, r.pi, A * 8);
+  *hash = fnv1a(1469598103934665603ull, key->data(), key->size() * 8);
+}
+
 struct Item {
   sb_request r;
+  InlineModel* im = nullptr;        // inline model this request holds a reference to
   int A = 0;
   int kind = 0;                     // 2 x padded alphabet + irreversible: one kind per launch
   size_t in_bytes = 0, out_bytes = 0;
@@ -164,6 +195,10 @@ struct sb_server {
   std::condition_variable idle_cv;  // drain: everything completed
   std::deque<Item> queue;
   std::vector<std::pair<long long, double>> done;
+  // inline models, least recently released first (guarded by mmu)
+  std::mutex mmu;
+  std::list<InlineModel> inline_models;
+  std::unordered_multimap<uint64_t, std::list<InlineModel>::iterator> inline_index;
   size_t in_flight = 0;
   long long pending = 0;            // submitted, not completed
   int err = 0;
@@ -208,8 +243,25 @@ int grow(sb_server* s, char** h, char** d, size_t* cap, size_t want) {
   return 0;
 }
 
+// Give back the inline-model references of a lane's finished requests.
+void release_models(sb_server* s, Lane& L) {
+  std::lock_guard<std::mutex> g(s->mmu);
+  for (Item& it : L.items) {
+    InlineModel* im = it.im;
+    it.im = nullptr;
+    if (!im || --im->refs > 0) continue;
+    // most recently released last: eviction takes from the front
+    for (auto p = s->inline_index.equal_range(im->hash); p.first != p.second; ++p.first)
+      if (&*p.first->second == im) {
+        s->inline_models.splice(s->inline_models.end(), s->inline_models, p.first->second);
+        break;
+      }
+  }
+}
+
 // Mark a lane's requests finished (after an error the outputs are undefined).
 void finish(sb_server* s, Lane& L) {
+  release_models(s, L);
   const double now = seconds(s->t0, Clock::now());
   std::lock_guard<std::mutex> g(s->mu);
   for (Item& it : L.items) {
@@ -222,6 +274,96 @@ void finish(sb_server* s, Lane& L) {
   L.bytes = 0;
   L.busy = false;
   s->idle_cv.notify_all();
+}
+
+// Give every inline-model request of the lane a resident model id, holding
+// a reference until the request completes; idle inline models beyond
+// kInlineModels are evicted first.
+int resolve_models(sb_server* s, Lane& L) {
+  const int n = int(L.items.size());
+  std::vector<std::vector<double>> keys(n);
+  std::vector<uint64_t> hashes(n, 0);
+  std::vector<int> miss, follow(n, -1);   // follow[k]: earlier miss of this batch with k's content
+  L.pool->run(n, [&](int k) {
+    const Item& it = L.items[k];
+    if (it.r.model_id < 0) inline_key(it.r, &keys[k], &hashes[k]);
+  });
+  {
+    std::lock_guard<std::mutex> g(s->mmu);
+    std::unordered_multimap<uint64_t, int> fresh;
+    for (int k = 0; k < n; ++k) {
+      Item& it = L.items[k];
+      if (it.r.model_id >= 0) continue;
+      bool found = false;
+      for (auto p = s->inline_index.equal_range(hashes[k]); p.first != p.second && !found; ++p.first) {
+        InlineModel& im = *p.first->second;
+        if (im.key == keys[k]) {
+          ++im.refs;
+          it.im = &im;
+          it.r.model_id = im.id;
+          found = true;
+        }
+      }
+      for (auto p = fresh.equal_range(hashes[k]); p.first != p.second && !found; ++p.first)
+        if (keys[p.first->second] == keys[k]) {
+          follow[k] = p.first->second;
+          found = true;
+        }
+      if (!found) {
+        fresh.emplace(hashes[k], k);
+        miss.push_back(k);
+      }
+    }
+    // make room: evict idle inline models, least recently released first, so
+    // that the new ones can take over their arena records
+    for (auto q = s->inline_models.begin();
+         q != s->inline_models.end() && s->inline_models.size() + miss.size() > kInlineModels;) {
+      if (q->refs > 0) {
+        ++q;
+        continue;
+      }
+      (void)sb_model_drop(s->ctx, q->id);
+      for (auto p = s->inline_index.equal_range(q->hash); p.first != p.second; ++p.first)
+        if (p.first->second == q) {
+          s->inline_index.erase(p.first);
+          break;
+        }
+      q = s->inline_models.erase(q);
+    }
+  }
+  std::atomic<int> mrc{0};
+  L.pool->run(int(miss.size()), [&](int q) {
+    Item& it = L.items[miss[q]];
+    int id = -1;
+    const int e = sb_model_put(s->ctx, it.A, it.r.eigenvalues, it.r.eigenvectors, it.r.pi, &id);
+    if (e) {
+      set_error(s, e, std::string("request model: ") + sb_last_error());
+      mrc.store(e);
+    }
+    it.r.model_id = id;
+  });
+  std::lock_guard<std::mutex> g(s->mmu);
+  for (int k : miss) {
+    Item& it = L.items[k];
+    if (it.r.model_id < 0) continue;
+    InlineModel im;
+    im.id = it.r.model_id;
+    im.refs = 1;
+    im.hash = hashes[k];
+    im.key.swap(keys[k]);
+    s->inline_models.push_back(std::move(im));
+    auto pos = std::prev(s->inline_models.end());
+    s->inline_index.emplace(pos->hash, pos);
+    it.im = &*pos;
+  }
+  for (int k = 0; k < n; ++k)
+    if (follow[k] >= 0) {
+      Item& it = L.items[k];
+      it.im = L.items[follow[k]].im;
+      it.r.model_id = L.items[follow[k]].r.model_id;
+      if (it.im) ++it.im->refs;
+    }
+  return mrc.load();
 }
 
 void launch(sb_server* s, Lane& L, std::vector<Item>& items, size_t bytes) {
@@ -238,24 +380,11 @@ void launch(sb_server* s, Lane& L, std::vector<Item>& items, size_t bytes) {
   }
   int rc = grow(s, &L.h_in, &L.d_in, &L.in_cap, in_total);
   if (!rc) rc = grow(s, &L.h_out, &L.d_out, &L.out_cap, out_total);
-  // models given inline go to the resident arena now (their host math runs
-  // on the pool; the context lock covers only the arena and the model table)
+  // models given inline: a resident copy with the same content is shared;
+  // the others go to the resident arena now (their host math runs on the
+  // pool; the context lock covers only the arena and the model table)
   std::vector<sb_block> blocks(n);
-  if (!rc) {
-    std::atomic<int> mrc{0};
-    L.pool->run(n, [&](int k) {
-      Item& it = L.items[k];
-      if (it.r.model_id >= 0) return;
-      int id = -1;
-      const int e = sb_model_put(s->ctx, it.A, it.r.eigenvalues, it.r.eigenvectors, it.r.pi, &id);
-      if (e) {
-        set_error(s, e, std::string("request model: ") + sb_last_error());
-        mrc.store(e);
-      }
-      it.r.model_id = id;
-    });
-    rc = mrc.load();
-  }
+  if (!rc) rc = resolve_models(s, L);
   if (!rc) {
     L.pool->run(n, [&](int k) {
       const Item& it = L.items[k];
@@ -495,6 +624,7 @@ int sb_server_destroy(sb_server* s) {
     if (L.ev) (void)hipEventDestroy(L.ev);
     if (L.stream) (void)hipStreamDestroy(L.stream);
   }
+  for (const InlineModel& im : s->inline_models) (void)sb_model_drop(s->ctx, im.id);
   const int err = s->err;
   std::string msg = s->errmsg;
   delete s;

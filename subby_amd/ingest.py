@@ -6,7 +6,8 @@ Mirrors the data path of the reference's batch driver
 (subby/formats.py:63-225).  The driver there buckets blocks by pruned-tree
 size, pads columns to 128 and runs jax.vmap(Counts) per bucket; here every
 block, whatever its tree, goes into one persistent launch (no bucketing, no
-padding), and features are formed on the GPU before one download per launch.
+padding), one download per launch brings the counts back, and the 20 features
+of each block are formed from them on the host.
 """
 from __future__ import annotations
 
@@ -291,7 +292,7 @@ def precompute_maf(maf_path, output_dir, model, guide_tree: dict, exclude_specie
     feature download, and writing, with the block, column and column·branch
     totals."""
     from .api import default_engine
-    eng = engine or default_engine()
+    eng = (engine or default_engine()).new_call()
     if len(np.asarray(model.pi)) != 4:
         raise ValueError('the MAF driver reads DNA blocks: the model needs A = 4')
     exclude = set(exclude_species or ())

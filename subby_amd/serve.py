@@ -120,6 +120,7 @@ class Server:
             lane.host_out = torch.empty(n, dtype=torch.uint8, pin_memory=True)
 
     def _launch(self, lane, reqs):
+        self.eng.new_call()
         sizes = [self._bytes(r) for r in reqs]
         nin = sum(s[0] for s in sizes)
         nout = sum(s[1] for s in sizes)
@@ -288,6 +289,7 @@ class NativeServer:
     def prepare(self, requests):
         """Host-side request records and output arrays (not timed)."""
         n = len(requests)
+        self.eng.new_call()
         arr = (_lib.SbRequest * n)()
         keep, outs = [], []
         for k, r in enumerate(requests):

@@ -119,6 +119,38 @@ def test_gamma_mixture_and_stacked_models(sb, eng):
         assert torch.equal(bc[k], sb.BranchCounts(aln, t, sb.scale_model(m, rates[k]), precision='fp64', engine=eng))
 
 
+def test_stacked_models_per_model_masks(sb, eng):
+    """Stacked models with an explicit (*H, R, C) mask: model h uses slice h
+    (subby/jax/__init__.py:355-356); an (R, C) mask is shared; wrong shapes raise
+    on the host before anything is launched."""
+    say('stacked models, per-model explicit masks')
+    rng, m, t, aln = _hky_case(sb, 72, leaves=20, C=500)
+    R, C = aln.shape
+    rates = np.array([0.5, 1.0, 2.0])
+    stack = sb.scale_model(m, rates)
+    oms = [O.scale_model(om(m), r) for r in rates]
+    masks = rng.random((3, R, C)) < np.array([0.3, 0.6, 0.9])[:, None, None]
+    for prec, tol in (('fp64', 1e-12), ('fp32', 1e-4)):
+        cnt = sb.Counts(aln, t, stack, branch_mask=masks, precision=prec, engine=eng)
+        bc = sb.BranchCounts(aln, t, stack, branch_mask=torch.as_tensor(masks), precision=prec, engine=eng)
+        shared = sb.Counts(aln, t, stack, branch_mask=masks[1], precision=prec, engine=eng)
+        for h in range(3):
+            ref = O.Counts(aln, otree(t), oms[h], branch_mask=masks[h])
+            assert err_counts(cnt[h], ref) <= tol, (prec, h)
+            ref_bc = O.BranchCounts(aln, otree(t), oms[h], branch_mask=masks[h])
+            assert err_bc(bc[h], ref_bc) <= tol, (prec, h)
+            ref = O.Counts(aln, otree(t), oms[h], branch_mask=masks[1])
+            assert err_counts(shared[h], ref) <= tol, (prec, h)
+    for bad in (masks[:2], masks[:, :-1], masks[0, :, :-1], masks[None]):
+        with pytest.raises(ValueError, match='branch_mask'):
+            sb.Counts(aln, t, stack, branch_mask=bad, engine=eng)
+    from subby_amd.batch import Block
+    with pytest.raises(ValueError, match='branch_mask'):
+        eng.launcher().submit([Block(aln, t, m, branch_mask=masks[0, :-1])])
+    with pytest.raises(ValueError, match='branch_mask'):
+        eng.launcher().submit([Block(aln, t, m, branch_mask='bogus')])
+
+
 def test_model_list(sb, eng):
     """Per-column models from three distinct models (different pi): counts per
     column under its own model; loglike / root with the first column's root

@@ -44,7 +44,7 @@ def test_served_equals_alone(eng):
     import subby_amd as sb
     from subby_amd.serve import Server, make_stream
     say('serve: 400 requests, Poisson arrivals')
-    reqs = make_stream(400, qps=4000.0, seed=11, leaves=(4, 60), mean_cols=300)
+    reqs = make_stream(400, qps=4000.0, seed=11, leaves=(4, 100), mean_cols=300)
     srv = Server(eng, hbm_budget=1 << 30, lanes=3)
     results, stats = srv.run(reqs)
     say('   stats', stats)
@@ -70,7 +70,7 @@ def test_native_served_equals_alone(eng):
     import subby_amd as sb
     from subby_amd.serve import NativeServer, make_stream
     say('native serve: 400 requests, Poisson arrivals')
-    reqs = make_stream(400, qps=4000.0, seed=13, leaves=(4, 60), mean_cols=300)
+    reqs = make_stream(400, qps=4000.0, seed=13, leaves=(4, 100), mean_cols=300)
     srv = NativeServer(eng, hbm_budget=1 << 30, lanes=3)
     try:
         results, stats = srv.run(reqs)
@@ -184,3 +184,68 @@ def test_native_served_mixed_alphabets(eng):
     say('   stats', stats)
     assert stats['requests'] == 90
     _check_alone_any(sb, eng, reqs, results)
+
+
+def test_native_arena_flat_with_distinct_models(eng):
+    """A stream whose every request carries its own inline model: the server keeps at most
+    4096 of them resident and reuses the arena records of evicted ones, so a second stream
+    of distinct models does not grow the model arena; equal models share one record."""
+    import subby_amd as sb
+    from subby_amd.serve import NativeServer, make_stream
+    say('native serve: arena over distinct inline models')
+    srv = NativeServer(eng, hbm_budget=1 << 30, lanes=3, max_blocks=256)
+    try:
+        _, n0, _ = eng.arena_stats()         # models of the engine itself (other calls)
+        reqs = make_stream(6000, qps=1e9, seed=31, leaves=(4, 12), mean_cols=40, pool=40)
+        res1, st1 = srv.run(reqs, realtime=False)
+        r1, n1, q1 = eng.arena_stats()
+        reqs2 = make_stream(6000, qps=1e9, seed=32, leaves=(4, 12), mean_cols=40, pool=40)
+        res2, st2 = srv.run(reqs2, realtime=False)
+        r2, n2, q2 = eng.arena_stats()
+        say(f'   arena after 6000: {r1} B, {n1} live, {q1} retired; after 12000: {r2} B, {n2} live, {q2} retired')
+        rec = 1536   # one A = 4 model record (FP64 master + float + double records)
+        assert n2 - n0 <= 4096 + 3 * 256
+        assert r2 - r1 <= 3 * 256 * rec
+        # equal models share one record
+        same = make_stream(300, qps=1e9, seed=33, leaves=(4, 12), mean_cols=40, pool=10)
+        for r in same:
+            r.model = same[0].model
+        res3, _ = srv.run(same, realtime=False)
+        r3, n3, _ = eng.arena_stats()
+        assert r3 <= r2 + rec and n3 <= n2 + 1
+    finally:
+        srv.close()
+    _check_alone(sb, eng, reqs[:100], res1[:100], oracle_every=50)
+    _check_alone(sb, eng, reqs2[-100:], res2[-100:], oracle_every=50)
+    _check_alone(sb, eng, same[:20], res3[:20], oracle_every=10)
+
+
+def test_engine_model_cache_bounded():
+    """An Engine keeps at most max_models eigensystems; evicted records are reused, and a
+    call that needs more distinct models than that (a model list) still runs."""
+    import subby_amd as sb
+    say('engine model cache bound')
+    e = sb.Engine(0, max_models=32)
+    try:
+        rng = np.random.default_rng(41)
+        t = sb.synthetic.random_tree(10, rng, 0.1)
+        aln = sb.synthetic.simulate(t, sb.hky85_diag(2.0, np.full(4, 0.25)), 64, rng)
+        models = [sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0))) for _ in range(300)]
+        for k, m in enumerate(models):
+            ll = sb.LogLike(aln, t, m, engine=e)
+            if k % 97 == 0:
+                assert err_ll_np(ll, O.LogLike(aln, O.Tree(*t), O.DiagModel(*m))) <= 1e-5
+        r, n, q = e.arena_stats()
+        say(f'   arena {r} B, {n} live, {q} retired')
+        assert n <= 32 and r <= 64 * 1536
+        lst = models[:64]
+        ll = sb.LogLike(aln, t, lst, engine=e)       # 64 per-column models > max_models
+        ref = O.LogLike_columns(aln, O.Tree(*t), [O.DiagModel(*m) for m in lst])
+        assert err_ll_np(ll, ref) <= 1e-5
+    finally:
+        e.close()
+
+
+def err_ll_np(got, ref):
+    got = got.detach().cpu().numpy()
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)))
