@@ -109,6 +109,27 @@ int sb_counts(sb_context* ctx, int model_id, const int8_t* d_alignment,
               int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
               void* d_counts, void* d_root, double* d_loglike, void* stream);
 
+/* sb_counts without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments, results
+ * and errors as sb_counts (bitwise the same results), but the call's plan,
+ * tile counters, branch records and per-wave scratch live in one static
+ * device array of 256 MiB per GPU (reserved when the library loads on that
+ * GPU, held until the process ends), and the call makes no hipMalloc,
+ * hipFree or other allocating call.  Limits: launch plan <= 4 MiB (about
+ * 500,000 column tiles), branch records <= 12 MiB - 64 KiB (R up to ~17,000
+ * nodes at A <= 4 in FP32), at least one wave's scratch slot in 240 MiB;
+ * beyond them it fails with SB_ERR_ARG and a message.  Fewer waves run when
+ * the slots of a full grid do not fit (slower, same results); general models
+ * above 8 states run the large-alphabet VALU kernel rather than the two-pass
+ * path.  Every call on a GPU (from any context) shares that GPU's array and
+ * leaves its contents behind: calls that overlap in time on one GPU race, so
+ * the caller keeps them in one stream or otherwise ordered. */
+int sb_counts_noalloc(sb_context* ctx, int model_id, const int8_t* d_alignment,
+                      const int32_t* parent_index, const double* distances, int R, int C,
+                      int dtype, int f81_fast, int mask_mode, const uint8_t* d_mask,
+                      void* d_counts, void* d_root, double* d_loglike, void* stream);
+
 /* Gradient of sum_c g_c logL_c (subby/jax/vjp.py:62-133, method 'eigen').
  * d_g: [C] f64 cotangent, or NULL for all ones.
  * d_dist_grad: [R] f64 output, entry 0 (root) is 0.

@@ -24,7 +24,7 @@ EXPORTS = [
     'sb_server_create', 'sb_server_destroy', 'sb_server_submit', 'sb_server_poll',
     'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
     'sb_counts_multi', 'sb_branch_counts_multi', 'sb_inside_outside_multi', 'sb_mixture_posterior',
-    'sb_arena_stats', 'sb_batch_set_tune', 'sb_batch_get_tune',
+    'sb_arena_stats', 'sb_batch_set_tune', 'sb_batch_get_tune', 'sb_counts_noalloc',
 ]
 
 
@@ -102,6 +102,7 @@ def load(path=LIB_PATH):
     lib.sb_loglike.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp]
     lib.sb_root_prob.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, vp]
     lib.sb_counts.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]
+    lib.sb_counts_noalloc.argtypes = [vp, i, vp, vp, vp, i, i, i, i, i, vp, vp, vp, vp, vp]
     lib.sb_loglike_grad.argtypes = [vp, i, vp, vp, vp, i, i, i, vp, vp, i, vp, vp, vp, vp]
     lib.sb_batch_create.argtypes = [vp, vp, P(vp)]
     lib.sb_batch_destroy.argtypes = [vp]

@@ -59,6 +59,7 @@ static const size_t kGen2StageBudget = size_t(8) << 30;
 extern "C" hipError_t sb_launch_branch_params(int dtype, bool irr, int nwg, const SbLaunch* L,
                                               hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
+extern "C" int sb_internal_noalloc_scratch(int device, void** ptr);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
 struct SbPlainBuffers {
   void* p = nullptr;
@@ -165,6 +166,10 @@ struct sb_context {
   hipEvent_t model_ev = nullptr;
   bool model_ev_valid = false;
   sb_batch* internal = nullptr;
+  // launcher of sb_counts_noalloc: every device buffer lies in the static
+  // no-allocation scratch array (null if its set-up failed: noalloc_err says why)
+  sb_batch* noalloc = nullptr;
+  std::string noalloc_err;
   SbPlainBuffers plain;
   DevBuf plain_bt;
   std::vector<float> plain_bt_host;
@@ -205,6 +210,8 @@ struct sb_batch {
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
   bool stage_tokens = true;         // small-alphabet walk reads tokens from LDS
+  // fixed buffers (the no-allocation launcher): never grown, never freed here
+  bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
@@ -218,6 +225,9 @@ namespace {
 // Grow a device buffer; waits for the launcher's stream before freeing the old one.
 int ensure(sb_batch* b, DevBuf& buf, size_t bytes) {
   if (bytes <= buf.cap) return SB_OK;
+  if (b->fixed)
+    return fail(SB_ERR_ARG, "the call needs %zu bytes of a buffer the no-allocation scratch holds %zu of "
+                "(scratch %llu MiB per GPU)", bytes, buf.cap, (unsigned long long)(SB_NOALLOC_BYTES >> 20));
   SB_HIP(hipStreamSynchronize(b->stream));
   size_t want = std::max(bytes, buf.cap * 3 / 2);
   if (buf.p) SB_HIP(hipFree(buf.p));
@@ -239,6 +249,9 @@ int ensure_host(sb_batch* b, size_t bytes) {
     b->staged_pending = false;
   }
   if (bytes <= b->host_cap) return SB_OK;
+  if (b->fixed)
+    return fail(SB_ERR_ARG, "the call's launch plan needs %zu bytes; the no-allocation plan area holds %zu",
+                bytes, b->host_cap);
   size_t want = std::max(bytes, b->host_cap * 3 / 2);
   if (b->host) SB_HIP(hipHostFree(b->host));
   b->host = nullptr;
@@ -628,7 +641,9 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   bool f81w = large && !lbc && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  const bool gen2 = large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad;
+  // (not on the no-allocation launcher: its stage area would not fit)
+  const bool gen2 = !b->fixed && large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 &&
+                    !grad;
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
@@ -709,6 +724,15 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   else if (large) wave_stride = sb_large_wave_stride(dtype, AP, nint_max, f81, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0);
   else wave_stride = (long long)nint_max * (2 * AP * esz * SB_WAVE + 2 * 4 * SB_WAVE);
   wave_stride = (long long)align_up(size_t(std::max<long long>(wave_stride, 256)), 256);
+  if (b->fixed && !gen2) {
+    // no-allocation launcher: as many resident waves as its scratch slots hold
+    const int g = f81w ? sb_f81w_waves_per_group() : wpg;
+    const long long fit = (long long)(b->scratch.cap / size_t(wave_stride)) / g * g;
+    if (fit < g)
+      return fail(SB_ERR_ARG, "one wave's scratch slot (%lld bytes for %d internal nodes) exceeds the "
+                  "no-allocation scratch area (%zu bytes)", wave_stride, nint_max, b->scratch.cap);
+    if (nwaves > fit) nwaves = int(fit);
+  }
   if (!gen2) {
     rc = ensure(b, b->scratch, size_t(wave_stride) * nwaves);
     if (rc) return rc;
@@ -867,7 +891,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   SB_HIP(hipEventRecord(b->done, s));
   {
     const int k = int(seq % sb_batch::kRing);
-    if (!b->ring_ev[k]) SB_HIP(hipEventCreateWithFlags(&b->ring_ev[k], hipEventDisableTiming));
     SB_HIP(hipEventRecord(b->ring_ev[k], s));
     b->ring_seq[k] = seq;
     b->seq = seq;
@@ -886,6 +909,7 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
   }
   SB_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
   SB_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  for (hipEvent_t& e : b->ring_ev) SB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   b->serial = ++ctx->next_serial;
   ctx->live.push_back(b);
   *out = b;
@@ -905,15 +929,58 @@ void batch_free(sb_batch* b) {
       (void)hipEventDestroy(e.first);
       (void)hipEventDestroy(e.second);
     }
-  for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask, &b->xout,
-                    &b->xcols})
-    if (d->p) (void)hipFree(d->p);
+  if (!b->fixed) {
+    for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask, &b->xout,
+                      &b->xcols})
+      if (d->p) (void)hipFree(d->p);
+    if (b->counter) (void)hipFree(b->counter);
+  }
   if (b->xev) (void)hipEventDestroy(b->xev);
-  if (b->counter) (void)hipFree(b->counter);
   if (b->host) (void)hipHostFree(b->host);
   if (b->staged) (void)hipEventDestroy(b->staged);
   if (b->done) (void)hipEventDestroy(b->done);
   delete b;
+}
+
+// The no-allocation launcher (set-up, from sb_create): its plan, tile
+// counters, branch records and wave scratch slots are fixed parts of this
+// GPU's static scratch array (whose device address the scratch source file
+// looks up with hipGetSymbolAddress), and its pinned plan staging is
+// allocated here, so a submission on it allocates nothing.
+int noalloc_create(sb_context* ctx) {
+  void* base = nullptr;
+  int rc = sb_internal_noalloc_scratch(ctx->device, &base);
+  if (rc) return rc;
+  sb_batch* b = new sb_batch();
+  b->ctx = ctx;
+  b->fixed = true;
+  char* p = static_cast<char*>(base);
+  b->plan.p = p + SB_NA_PLAN_OFF;
+  b->plan.cap = SB_NA_PLAN_BYTES;
+  b->header = static_cast<SbLaunch*>(b->plan.p);
+  b->counter = reinterpret_cast<int*>(p + SB_NA_COUNTER_OFF);
+  b->bt.p = p + SB_NA_BT_OFF;
+  b->bt.cap = SB_NA_BT_BYTES;
+  b->scratch.p = p + SB_NA_WAVES_OFF;
+  b->scratch.cap = SB_NA_WAVES_BYTES;
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->host), SB_NA_PLAN_BYTES, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    delete b;
+    return fail(SB_ERR_OOM, "pinned plan staging of the no-allocation launcher failed");
+  }
+  b->host_cap = SB_NA_PLAN_BYTES;
+  hipError_t e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  for (hipEvent_t& ev : b->ring_ev)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  b->serial = ++ctx->next_serial;
+  ctx->live.push_back(b);
+  if (e != hipSuccess) {
+    batch_free(b);
+    return fail(SB_ERR_HIP, "no-allocation launcher events: %s", hipGetErrorString(e));
+  }
+  ctx->noalloc = b;
+  return SB_OK;
 }
 
 // Single-block calls go through the context's own launcher; if the caller
@@ -1218,6 +1285,12 @@ int sb_create(sb_context** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cus = prop.multiProcessorCount;
+  // model-upload events up front, so no call creates them later
+  for (int k = 0; k < 2; ++k) SB_HIP(hipEventCreateWithFlags(&ctx->pend_ev[k], hipEventDisableTiming));
+  SB_HIP(hipEventCreateWithFlags(&ctx->model_ev, hipEventDisableTiming));
+  // the no-allocation entry point's launcher; if it cannot be set up, only
+  // that entry point fails (with this message)
+  if (noalloc_create(ctx) != SB_OK) ctx->noalloc_err = g_err;
   *out = ctx;
   return SB_OK;
 }
@@ -1226,6 +1299,7 @@ int sb_destroy(sb_context* ctx) {
   if (!ctx) return SB_OK;
   (void)hipSetDevice(ctx->device);
   batch_free(ctx->internal);
+  batch_free(ctx->noalloc);
   if (ctx->plain.p) (void)hipFree(ctx->plain.p);
   if (ctx->plain_bt.p) (void)hipFree(ctx->plain_bt.p);
   (void)hipDeviceSynchronize();
@@ -1547,6 +1621,32 @@ int sb_counts(sb_context* ctx, int model_id, const int8_t* d_alignment, const in
   bi.blk.d_root = d_root;
   bi.blk.d_loglike = d_loglike;
   return single(ctx, bi, dtype, stream);
+}
+
+int sb_counts_noalloc(sb_context* ctx, int model_id, const int8_t* d_alignment, const int32_t* parent_index,
+                      const double* distances, int R, int C, int dtype, int f81_fast, int mask_mode,
+                      const uint8_t* d_mask, void* d_counts, void* d_root, double* d_loglike, void* stream) {
+  if (!ctx) return fail(SB_ERR_ARG, "null context");
+  if (!d_counts && C > 0) return fail(SB_ERR_ARG, "null counts output");
+  BlockIn bi = make_block(model_id, d_alignment, parent_index, distances, R, C);
+  bi.flags = SB_F_COUNTS | (d_root ? SB_F_ROOT : 0) | (d_loglike ? SB_F_LL : 0);
+  bi.blk.f81_fast = f81_fast ? 1 : 0;
+  bi.blk.mask_mode = mask_mode;
+  bi.blk.d_mask = d_mask;
+  bi.blk.d_counts = d_counts;
+  bi.blk.d_root = d_root;
+  bi.blk.d_loglike = d_loglike;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  sb_batch* b = ctx->noalloc;
+  if (!b) return fail(SB_ERR_UNSUPPORTED, "no-allocation scratch unavailable: %s", ctx->noalloc_err.c_str());
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (b->stream != s) {
+    SB_HIP(hipStreamWaitEvent(s, b->done, 0));
+    b->stream = s;
+  }
+  std::vector<BlockIn> v(1, bi);
+  return submit(b, v, dtype, 0);
 }
 
 int sb_inside_outside(sb_context* ctx, int model_id, const int8_t* d_alignment,

@@ -152,6 +152,19 @@ struct SbBlockDesc {
   int A, tile_base, ntiles, n_rate;
 };
 
+// Device memory of the no-allocation Counts entry point: one static array of
+// SB_NOALLOC_BYTES per GPU, split into the launch plan (header first), the
+// tile counters, the branch records and the per-wave scratch slots.
+#define SB_NOALLOC_BYTES (256ull << 20)
+#define SB_NOALLOC_MAX_DEVICES 64
+#define SB_NA_PLAN_OFF 0ull
+#define SB_NA_PLAN_BYTES (4ull << 20)
+#define SB_NA_COUNTER_OFF (4ull << 20)
+#define SB_NA_BT_OFF ((4ull << 20) + (64ull << 10))
+#define SB_NA_BT_BYTES ((12ull << 20) - (64ull << 10))
+#define SB_NA_WAVES_OFF (16ull << 20)
+#define SB_NA_WAVES_BYTES (SB_NOALLOC_BYTES - SB_NA_WAVES_OFF)
+
 struct SbBranchJob {       // one per (block, node): M(t), J(t) ... in FP64
   const double* model64;
   double t;
