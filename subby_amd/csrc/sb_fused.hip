@@ -65,78 +65,84 @@ struct Tile {
   static __device__ inline long long vec_bytes(int nint) {
     return (long long)nint * NCH * SB_WAVE * 16;
   }
+  // (the slot's base is wave-uniform: a scalar base plus a 32-bit lane offset)
   static __device__ inline void store_vec(SB_GLOBAL char* region, int slot, int lane, const T* v) {
-    SB_GLOBAL CT* p = (SB_GLOBAL CT*)(region);
+    SB_GLOBAL CT* p = (SB_GLOBAL CT*)(region + (long long)slot * (NCH * SB_WAVE * 16));
 #pragma unroll
     for (int q = 0; q < NCH; ++q) {
       CT c;
       if constexpr (EPC == 4) { c.x = v[q * 4]; c.y = v[q * 4 + 1]; c.z = v[q * 4 + 2]; c.w = v[q * 4 + 3]; }
       else { c.x = v[q * 2]; c.y = v[q * 2 + 1]; }
-      p[((long long)slot * NCH + q) * SB_WAVE + lane] = c;
+      p[q * SB_WAVE + lane] = c;
     }
   }
   static __device__ inline void load_vec(const SB_GLOBAL char* region, int slot, int lane, T* v) {
-    const SB_GLOBAL CT* p = (const SB_GLOBAL CT*)(region);
+    const SB_GLOBAL CT* p = (const SB_GLOBAL CT*)(region + (long long)slot * (NCH * SB_WAVE * 16));
 #pragma unroll
     for (int q = 0; q < NCH; ++q) {
-      CT c = p[((long long)slot * NCH + q) * SB_WAVE + lane];
+      CT c = p[q * SB_WAVE + lane];
       if constexpr (EPC == 4) { v[q * 4] = c.x; v[q * 4 + 1] = c.y; v[q * 4 + 2] = c.z; v[q * 4 + 3] = c.w; }
       else { v[q * 2] = c.x; v[q * 2 + 1] = c.y; }
     }
   }
 };
 
-// Likelihood vector of a token: one-hot for 0..A-1, ones on real states otherwise.
+// Likelihood vectors of tokens by byte masks: word w of a token's mask holds
+// one byte per state 4w..4w+3 (1 if the state is possible), so each element
+// is one unsigned-byte-to-float conversion.  Observed tokens 0..A-1 give one
+// byte; every other token gives the real states (padded states stay zero).
 template <typename T, int AP>
-__device__ inline void token_vec(int tok, int A, const SB_CONST T* ones, T* u) {
-  const bool obs = (tok >= 0) && (tok < A);
+struct TokDec {
+  static constexpr int NW = (AP + 3) / 4;
+  unsigned unobs[NW];
+  int A;
+  __device__ inline void init(int A_) {
+    A = A_;
 #pragma unroll
-  for (int a = 0; a < AP; ++a) u[a] = obs ? (a == tok ? T(1) : T(0)) : ones[a];
-}
-
-// Branch-free variant with the real-state mask already in registers.
-template <typename T, int AP>
-__device__ inline void token_vec_r(int tok, int A, const T* ones, T* u) {
-  const T fill = (tok >= 0 && tok < A) ? T(0) : T(1);
+    for (int w = 0; w < NW; ++w) {
+      unsigned m = 0;
 #pragma unroll
-  for (int a = 0; a < AP; ++a) u[a] = (a == tok) ? T(1) : fill * ones[a];
-}
+      for (int b = 0; b < 4; ++b) m |= (4 * w + b < A_ ? 1u : 0u) << (8 * b);
+      unobs[w] = m;
+    }
+  }
+  __device__ inline void decode(int tok, T* u) const {
+    const bool obs = unsigned(tok) < unsigned(A);
+    const unsigned one = 1u << (8 * (tok & 3));
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      // (with one word, an observed token is always in it)
+      const unsigned m = obs ? ((NW == 1 || (tok >> 2) == w) ? one : 0u) : unobs[w];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) u[4 * w + b] = T(float((m >> (8 * b)) & 0xffu));
+    }
+  }
+};
 
 // Token of node n for this lane's column: from the wave's LDS stage when the
-// launch stages tokens (tl != null, uniform), else from the alignment.
-__device__ inline int tok_at(const int8_t* tl, const SB_GLOBAL int8_t* aln, long long C, int n, int lane) {
-  return tl ? int(tl[n * SB_WAVE + lane]) : int(aln[(long long)n * C]);
+// launch stages tokens (`staged`, wave-uniform), else from the alignment.
+struct TokSrc {
+  const int8_t* tl;                 // this lane's column in the wave's LDS stage (row stride 64)
+  const SB_GLOBAL int8_t* aln;      // this lane's column of the alignment (row stride C)
+  long long C;
+  bool staged;
+};
+__device__ inline int tok_at(const TokSrc& ts, int n) {
+  if (ts.staged) return int(ts.tl[n * SB_WAVE]);
+  return int(ts.aln[(long long)n * ts.C]);
 }
 
 // Copy the tile's tokens (R rows x 64 columns, one byte each) into the wave's
-// LDS stage: 32 rows of loads in flight, then their LDS writes.
+// LDS stage, 32 rows of loads in flight at a time, then their LDS writes.  The
+// stage holds a multiple of 32 rows >= R, so the last batch reads row R - 1
+// again for the rows past R instead of branching.
 __device__ inline void stage_tokens(int8_t* tl, const SB_GLOBAL int8_t* aln, long long C, int R, int lane) {
   for (int r0 = 0; r0 < R; r0 += 32) {
     int v[32];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) v[k] = (r0 + k < R) ? int(aln[(long long)(r0 + k) * C]) : 0;
+    for (int k = 0; k < 32; ++k) v[k] = int(aln[(long long)min(r0 + k, R - 1) * C]);
 #pragma unroll
-    for (int k = 0; k < 32; ++k)
-      if (r0 + k < R) tl[(r0 + k) * SB_WAVE + lane] = int8_t(v[k]);
-  }
-}
-
-// Child data as loaded (decoded when used): internal -> vector + word, leaf -> token in w.
-template <typename T, int AP>
-struct Raw {
-  T u[AP];
-  int w;
-};
-
-template <typename T, int AP>
-__device__ inline void load_raw(Raw<T, AP>& r, int n, int slot, const SB_GLOBAL int8_t* aln, long long C,
-                                const SB_GLOBAL char* reg, const SB_GLOBAL int* words, int lane,
-                                const int8_t* tl) {
-  if (slot < 0) {
-    r.w = tok_at(tl, aln, C, n, lane);
-  } else {
-    Tile<T, AP>::load_vec(reg, slot, lane, r.u);
-    r.w = words[slot * SB_WAVE + lane];
+    for (int k = 0; k < 32; ++k) tl[(r0 + k) * SB_WAVE + lane] = int8_t(v[k]);
   }
 }
 
@@ -192,43 +198,62 @@ struct Child {
   bool leaf;
 };
 
-template <typename T, int AP>
-__device__ inline void load_child(Child<T, AP>& ch, int n, int slot, const SB_GLOBAL int8_t* aln,
-                                  long long C, int A, const SB_CONST T* ones,
-                                  const SB_GLOBAL char* Ureg, const SB_GLOBAL int* Uw, int lane) {
-  if (slot < 0) {
-    const int tok = aln[(long long)n * C];
-    token_vec<T, AP>(tok, A, ones, ch.u);
-    ch.e = 0;
-    ch.has = (tok >= 0) && (tok <= A);
-    ch.branch = false;
-    ch.leaf = true;
-  } else {
-    Tile<T, AP>::load_vec(Ureg, slot, lane, ch.u);
-    const int w = Uw[slot * SB_WAVE + lane];
-    ch.e = w >> 2;
-    ch.has = (w & 1) != 0;
-    ch.branch = (w & 2) != 0;
-    ch.leaf = false;
-  }
+// One tree record as the walk reads it (scalar data).
+struct InRec {
+  int p, l, r, sl, sr;
+};
+__device__ inline InRec in_rec(const SB_CONST int* sched, int i) {
+  const SB_CONST int* q = sched + i * 8;
+  return InRec{q[0], q[1], q[2], q[4], q[5]};
 }
 
+// What the walk reads of a record's children and parent, fetched a record
+// ahead: both children's staged vectors and words and the tokens of both
+// children and the parent.  A fetch whose value will not be used (a leaf, or
+// a child forwarded in registers: slot == fwd_slot) reads slot 0 instead,
+// which stays in cache, so the instruction stream has no branch and the
+// wasted reads cost no memory round trip.
 template <typename T, int AP>
-__device__ inline void decode_child(const Raw<T, AP>& r, int slot, int A, const T* ones, Child<T, AP>& ch) {
-  if (slot < 0) {
-    token_vec_r<T, AP>(r.w, A, ones, ch.u);
-    ch.e = 0;
-    ch.has = (r.w >= 0) && (r.w <= A);
-    ch.branch = false;
-    ch.leaf = true;
-  } else {
+struct Fetch {
+  T ul[AP], ur[AP];
+  int wl, wr;
+  int tok;   // tokens: left child (bits 0-7), right child (8-15), parent (16-23)
+  __device__ inline int tl() const { return int(int8_t(tok)); }
+  __device__ inline int tr() const { return int(int8_t(tok >> 8)); }
+  __device__ inline int tp() const { return int(int8_t(tok >> 16)); }
+};
+
+template <typename T, int AP>
+__device__ inline void fetch_vec(T* v, int& w, int slot, const SB_GLOBAL char* reg, const SB_GLOBAL int* words,
+                                 int lane) {
+  Tile<T, AP>::load_vec(reg, slot, lane, v);
+  w = words[slot * SB_WAVE + lane];
+}
+
+template <typename T, int AP, bool LEFT = true>
+__device__ inline void fetch_children(Fetch<T, AP>& f, const InRec& q, int fwd_slot, const TokSrc& ts,
+                                      const SB_GLOBAL char* Ureg, const SB_GLOBAL int* Uw, int lane) {
+  if constexpr (LEFT) fetch_vec<T, AP>(f.ul, f.wl, (q.sl < 0 || q.sl == fwd_slot) ? 0 : q.sl, Ureg, Uw, lane);
+  fetch_vec<T, AP>(f.ur, f.wr, (q.sr < 0 || q.sr == fwd_slot) ? 0 : q.sr, Ureg, Uw, lane);
+  f.tok = (tok_at(ts, q.l) & 0xff) | ((tok_at(ts, q.r) & 0xff) << 8) | ((tok_at(ts, q.p) & 0xff) << 16);
+}
+
+// A child from fetched data, by selects (slot < 0: leaf from its token; fwd:
+// the forwarded vector fu / word fw; otherwise the staged vector and word).
+template <typename T, int AP>
+__device__ inline void pick_child(int slot, bool fwd, int tok, const T* su, int sw, const T* fu, int fw,
+                                  const TokDec<T, AP>& td, Child<T, AP>& ch) {
+  const bool leaf = slot < 0;
+  const int A = td.A;
+  T tv[AP];
+  td.decode(tok, tv);
 #pragma unroll
-    for (int a = 0; a < AP; ++a) ch.u[a] = r.u[a];
-    ch.e = r.w >> 2;
-    ch.has = (r.w & 1) != 0;
-    ch.branch = (r.w & 2) != 0;
-    ch.leaf = false;
-  }
+  for (int a = 0; a < AP; ++a) ch.u[a] = leaf ? tv[a] : (fwd ? fu[a] : su[a]);
+  const int w = fwd ? fw : sw;
+  ch.e = leaf ? 0 : (w >> 2);
+  ch.has = leaf ? ((tok >= 0) && (tok <= A)) : ((w & 1) != 0);
+  ch.branch = !leaf && ((w & 2) != 0);
+  ch.leaf = leaf;
 }
 
 template <typename T, int AP, bool F81, bool IRR>
@@ -404,8 +429,17 @@ __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
   if (threadIdx.x < SB_NCOUNTER && blockIdx.x == 0) L->counter[threadIdx.x] = 0;
 }
 
+// Register budget: the FP32 four-state reversible variants with one model
+// keep four waves per SIMD (at most 128 registers), so a full grid of tiles
+// is resident at once.
+template <typename T, int AP, bool IRR, bool ROWS>
+struct SmallWavesPerEU {
+  static constexpr int v = (sizeof(T) == 4 && AP == 4 && !IRR && !ROWS) ? 4 : 1;
+};
+
 template <typename T, int AP, bool F81, bool GRAD, bool IRR, bool ROWS>
-__global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SmallWavesPerEU<T, AP, IRR, ROWS>::v)))
+sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
   typedef Tile<T, AP> TL;
   const SbLaunch& args = *Lp;
   const int lane = threadIdx.x & (SB_WAVE - 1);
@@ -437,58 +471,60 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     const SB_CONST T* bt = cst<T>(B->bt);
     const SB_CONST T* md = cst<T>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
-    T ones[AP], pi[AP];
+    T pi[AP];
 #pragma unroll
-    for (int a = 0; a < AP; ++a) {
-      ones[a] = md[sb_mr_ones_k(AP, IRR) + a];
-      pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
-    }
+    for (int a = 0; a < AP; ++a) pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
     const int BTS = sb_bt_stride_k(AP, IRR);
     // the tile's tokens go to LDS once, so no token read of the walk waits on HBM
-    int8_t* tl = nullptr;
-    if (args.tok_rows > 0 && nint > 0) {
-      tl = sb_tok_lds + (threadIdx.x / SB_WAVE) * args.tok_rows * SB_WAVE;
-      stage_tokens(tl, aln, C, B->R, lane);
-    }
+    TokSrc ts;
+    ts.staged = args.tok_rows > 0 && nint > 0;
+    ts.aln = aln;
+    ts.C = C;
+    int8_t* const stage = sb_tok_lds + __builtin_amdgcn_readfirstlane(threadIdx.x / SB_WAVE) * args.tok_rows * SB_WAVE;
+    ts.tl = stage + lane;
+    if (ts.staged) stage_tokens(stage, aln, C, B->R, lane);
 
     const bool outside = (flags & (SB_F_COUNTS | SB_F_GRAD)) != 0;
 
     // ---------------- inside pass ----------------
-    // Internal nodes in postorder.  The next node's children are fetched one
-    // iteration ahead; a child that is the node being computed now (always the
-    // case for an internal left child) is forwarded from registers.
+    // Internal nodes in postorder (records walked backwards), software-
+    // pipelined: while record i is computed, everything record i - 1 reads
+    // (its right child's staged U and word, both children's tokens and the
+    // parent's token) is already in flight.  The fetches are unconditional (a leaf's slot is
+    // clamped to 0, a forwarded child's slot is read but not used) and the
+    // choice between token, staged and forwarded data is a select, so the
+    // wave waits only on fetches issued a whole record earlier.  A child whose
+    // slot is i + 1 (an internal left child, or an internal right child next
+    // to a leaf) is the record computed just before: forwarded in registers.
     T u0[AP];
     int e0 = 0;
     bool root_branch = false;
-    if (nint == 0) token_vec_r<T, AP>(aln[0], A, ones, u0);  // single-node tree
-    Raw<T, AP> nl, nr;
-    int np_ = 0, nl_ = 0, nr_ = 0, nsl = -1, nsr = -1;
+    TokDec<T, AP> td;
+    td.init(A);
+    if (nint == 0) td.decode(aln[0], u0);  // single-node tree
+    // (a tree of one node has no records: nothing is fetched)
+    InRec q{0, 0, 0, -1, -1};
+    Fetch<T, AP> cur;
     if (nint > 0) {
-      const SB_CONST int* rec = sched + (nint - 1) * 8;
-      np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
-      load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane, tl);
-      load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane, tl);
+      q = in_rec(sched, nint - 1);
+      fetch_children<T, AP, false>(cur, q, nint, ts, Ureg, Uw, lane);
     }
+    T fu[AP];                      // U and word of the record computed last
+    int fw = 0;
+#pragma unroll
+    for (int a = 0; a < AP; ++a) fu[a] = T(0);
     for (int i = nint - 1; i >= 0; --i) {
-      const int p = np_, l = nl_, r = nr_, sl = nsl, sr = nsr;
-      const Raw<T, AP> rl = nl, rr = nr;
-      const int tokp = tok_at(tl, aln, C, p, lane);
-      bool fwd_l = false, fwd_r = false;
-      if (i > 0) {
-        const SB_CONST int* rec = sched + (i - 1) * 8;
-        np_ = rec[0]; nl_ = rec[1]; nr_ = rec[2]; nsl = rec[4]; nsr = rec[5];
-        fwd_l = nsl == i;
-        fwd_r = nsr == i;
-        if (!fwd_l) load_raw<T, AP>(nl, nl_, nsl, aln, C, Ureg, Uw, lane, tl);
-        if (!fwd_r) load_raw<T, AP>(nr, nr_, nsr, aln, C, Ureg, Uw, lane, tl);
-      }
+      const InRec qn = in_rec(sched, i > 0 ? i - 1 : 0);
+      Fetch<T, AP> nxt;
+      fetch_children<T, AP, false>(nxt, qn, i, ts, Ureg, Uw, lane);
       Child<T, AP> cl, cr;
-      decode_child<T, AP>(rl, sl, A, ones, cl);
-      decode_child<T, AP>(rr, sr, A, ones, cr);
+      // (an internal left child is the next node in preorder: slot i + 1, forwarded)
+      pick_child<T, AP>(q.sl, true, cur.tl(), fu, fw, fu, fw, td, cl);
+      pick_child<T, AP>(q.sr, q.sr == i + 1, cur.tr(), cur.ur, cur.wr, fu, fw, td, cr);
       T wl[AP], wr[AP], up[AP];
-      matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), cl.u, wl);
-      matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), cr.u, wr);
-      token_vec_r<T, AP>(tokp, A, ones, up);
+      matvec<T, AP>(bt + (long long)q.l * BTS + sb_bt_M(AP), cl.u, wl);
+      matvec<T, AP>(bt + (long long)q.r * BTS + sb_bt_M(AP), cr.u, wr);
+      td.decode(cur.tp(), up);
 #pragma unroll
       for (int a = 0; a < AP; ++a) up[a] = up[a] * wl[a] * wr[a];
       const int e = rescale<T, AP>(up);
@@ -499,19 +535,9 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
       if (i > 0) {
         // the outside pass reads every internal U; without it, a U forwarded
         // to the next record is never read back
-        if (outside || !(fwd_l || fwd_r)) {
+        if (outside || !(qn.sl == i || qn.sr == i)) {
           TL::store_vec(Ureg, i, lane, up);
           Uw[i * SB_WAVE + lane] = word;
-        }
-        if (fwd_l) {
-#pragma unroll
-          for (int a = 0; a < AP; ++a) nl.u[a] = up[a];
-          nl.w = word;
-        }
-        if (fwd_r) {
-#pragma unroll
-          for (int a = 0; a < AP; ++a) nr.u[a] = up[a];
-          nr.w = word;
         }
       } else {
 #pragma unroll
@@ -519,6 +545,11 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         e0 = ep;
         root_branch = br;
       }
+#pragma unroll
+      for (int a = 0; a < AP; ++a) fu[a] = up[a];
+      fw = word;
+      q = qn;
+      cur = nxt;
     }
     // log-likelihood and root posterior (pruning.py:84-87; __init__.py:642-650)
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
@@ -549,35 +580,50 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
     Acc<T, AP, F81, IRR && !ROWS> acc;   // IRR + ROWS: natural-basis real counts
     acc.zero();
 
-    // Internal nodes in preorder.  The next node's children (read-only U
-    // region) and its own outside vector are fetched one iteration ahead; an
-    // outside vector computed in this iteration is forwarded from registers.
-    Raw<T, AP> ql, qr, qd;
-    int qp = 0, qls = 0, qrs = 0, qsl = -1, qsr = -1;
+    // Internal nodes in preorder, pipelined as the inside pass: record i + 1's
+    // own staged outside vector (slot i + 1), its children's staged U and words
+    // and its tokens are fetched while record i is computed.  An outside vector
+    // computed for a child of record i that is record i + 1's node is
+    // forwarded in registers and never stored (an internal left child always,
+    // a right child when the left one is a leaf).
+    InRec oq{0, 0, 0, -1, -1};
+    Fetch<T, AP> ocur;
+    T cd[AP];                      // staged outside vector of the record's node and its word
+    int cdw = 0;
     if (nint > 0) {
-      const SB_CONST int* rec = sched;
-      qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
-      load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane, tl);
-      load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane, tl);
+      oq = in_rec(sched, 0);
+      fetch_children<T, AP>(ocur, oq, -2, ts, Ureg, Uw, lane);
+      fetch_vec<T, AP>(cd, cdw, 0, Dreg, Dw, lane);
     }
+    T fd[AP];                      // the outside vector (and word) forwarded to this record
+    int fdw = 0;
+    bool pf = false;               // this record's node is a child of the last record
+#pragma unroll
+    for (int a = 0; a < AP; ++a) fd[a] = T(0);
     for (int i = 0; i < nint; ++i) {
-      const int p = qp, l = qls, r = qrs, sl = qsl, sr = qsr;
-      const Raw<T, AP> rl = ql, rr = qr, rd = qd;
-      const int tokp = tok_at(tl, aln, C, p, lane);
-      bool fwd0 = false, fwd1 = false;
-      if (i + 1 < nint) {
-        const SB_CONST int* rec = sched + (i + 1) * 8;
-        qp = rec[0]; qls = rec[1]; qrs = rec[2]; qsl = rec[4]; qsr = rec[5];
-        fwd0 = qp == l;
-        fwd1 = qp == r;
-        if (!fwd0 && !fwd1) load_raw<T, AP>(qd, 0, i + 1, aln, C, Dreg, Dw, lane, tl);
-        load_raw<T, AP>(ql, qls, qsl, aln, C, Ureg, Uw, lane, tl);
-        load_raw<T, AP>(qr, qrs, qsr, aln, C, Ureg, Uw, lane, tl);
+      const int inext = i + 1 < nint ? i + 1 : i;
+      const InRec oqn = in_rec(sched, inext);
+      Fetch<T, AP> onxt;
+      T nd[AP];
+      int ndw;
+      const bool last = i + 1 >= nint;
+      const bool fwd0 = !last && oqn.p == oq.l;
+      const bool fwd1 = !last && oqn.p == oq.r;
+      fetch_children<T, AP>(onxt, oqn, -2, ts, Ureg, Uw, lane);
+      fetch_vec<T, AP>(nd, ndw, (fwd0 || fwd1) ? 0 : inext, Dreg, Dw, lane);
+      const int p = oq.p, l = oq.l, r = oq.r, sl = oq.sl, sr = oq.sr;
+      // the node's own outside vector: staged, or forwarded by its parent's record
+      T rd[AP];
+      int rdw;
+      {
+#pragma unroll
+        for (int a = 0; a < AP; ++a) rd[a] = pf ? fd[a] : cd[a];
+        rdw = pf ? fdw : cdw;
       }
       T x[AP];
       int ex;
       bool stp;
-      token_vec_r<T, AP>(tokp, A, ones, x);
+      td.decode(ocur.tp(), x);
       if (i == 0) {
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= pi[a];
@@ -585,15 +631,15 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         stp = root_branch;
       } else {
         T xm[AP];
-        ex = rd.w >> 1;
-        stp = (rd.w & 1) != 0;
-        vecmat<T, AP>(rd.u, bt + (long long)p * BTS + sb_bt_M(AP), xm);
+        ex = rdw >> 1;
+        stp = (rdw & 1) != 0;
+        vecmat<T, AP>(rd, bt + (long long)p * BTS + sb_bt_M(AP), xm);
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= xm[a];
       }
       Child<T, AP> ch[2];
-      decode_child<T, AP>(rl, sl, A, ones, ch[0]);
-      decode_child<T, AP>(rr, sr, A, ones, ch[1]);
+      pick_child<T, AP>(sl, false, ocur.tl(), ocur.ul, ocur.wl, ocur.ul, 0, td, ch[0]);
+      pick_child<T, AP>(sr, false, ocur.tr(), ocur.ur, ocur.wr, ocur.ur, 0, td, ch[1]);
       T w[2][AP];
       matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
       matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
@@ -611,15 +657,15 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
         // an internal child's outside vector is read once, by its own record:
         // when that record is the next one it is forwarded in registers and
         // never stored
-        const bool fw = (s == 0) ? fwd0 : fwd1;
-        if (sn >= 0 && !fw) {
+        const bool fwc = (s == 0) ? fwd0 : fwd1;
+        if (sn >= 0 && !fwc) {
           TL::store_vec(Dreg, sn, lane, d);
           Dw[sn * SB_WAVE + lane] = dword;
         }
-        if (fw) {
+        if (fwc) {   // uniform
 #pragma unroll
-          for (int a = 0; a < AP; ++a) qd.u[a] = d[a];
-          qd.w = dword;
+          for (int a = 0; a < AP; ++a) fd[a] = d[a];
+          fdw = dword;
         }
         bool active;
         if (GRAD || mask_mode == SB_MASK_NONE) active = true;
@@ -824,6 +870,12 @@ __global__ void __launch_bounds__(256) sb_fused_small_kernel(const SbLaunch* __r
           }
         }
       }
+      pf = fwd0 || fwd1;
+      oq = oqn;
+      ocur = onxt;
+#pragma unroll
+      for (int a = 0; a < AP; ++a) cd[a] = nd[a];
+      cdw = ndw;
     }
 
     // ---------------- epilogue: natural-basis counts ----------------

@@ -689,11 +689,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   // the launcher was given an explicit value; never more waves than tiles
   const int wpg = (large || post) ? 1 : b->waves_per_group;
   // small-alphabet fused kernel: each wave stages its tile's tokens (R_max
-  // rows, rounded up to 16, of 64 bytes) in LDS when a workgroup's stage fits
-  // 64 KiB
+  // rows, rounded up to 32 as the kernel copies 32 rows at a time, of 64
+  // bytes) in LDS when a workgroup's stage fits 64 KiB
   int tok_rows = 0;
   if (!large && !post && b->stage_tokens) {
-    const int rows = (R_max + 15) / 16 * 16;
+    const int rows = (R_max + 31) / 32 * 32;
     if (size_t(rows) * SB_WAVE * size_t(wpg) <= (64u << 10)) tok_rows = rows;
   }
   const int lds_tok = tok_rows * SB_WAVE * wpg;

@@ -187,8 +187,8 @@ struct SbLaunch {
   long long stage_stride;  // bytes of stage per tile (two-pass general path)
   int nblocks, ntiles, njobs, nint_max;
   int grad_stride, AP, tile_cols, rmax;   // rmax: largest tree (nodes) in the launch
-  // small-alphabet kernels: rows of a wave's LDS token stage (>= rmax; the
-  // tile's R x 64 tokens are copied there at tile start), 0 = read tokens
-  // from the alignment in HBM
+  // small-alphabet kernels: rows of a wave's LDS token stage (a multiple of
+  // 32, >= rmax; the tile's R x 64 tokens are copied there at tile start),
+  // 0 = read tokens from the alignment in HBM
   int tok_rows, pad_;
 };
