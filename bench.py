@@ -528,7 +528,7 @@ def other_workload(args, log):
                             "kernel": "sb_large_kernel<float, 20, false, false, LOUT_POST>"},
                "branch_counts": {"stream_ms_per_call": bc_ms, "column*branches/s": units / (bc_ms * 1e-3),
                                  "output_GB_per_s": R * A * A * C * 4 / (bc_ms * 1e-3) / 1e9,
-                                 "kernel": "sb_large_kernel<float, 20, false, false, LOUT_BRANCH>"},
+                                 "kernel": "sb_gwalk_kernel<32> + sb_gbranch_kernel<32>"},
                "counts": {"stream_ms_per_call": counts_ms, "column*branches/s": units / (counts_ms * 1e-3)},
                "parity": {"node_posterior_abs": err_np, "branch_posterior_abs": err_bp,
                           "branch_counts_per_branch_normwise": err_bc, "columns_checked": len(cols)}}

@@ -321,7 +321,7 @@ def rate_matrix_symmetric(model):
 def back_transform(Ceig, model):
     """counts_ij = VCV_ij (i==j: dwell) or S_ij VCV_ij (oracle.py:1087-1137)."""
     V = model.eigenvectors
-    VCV = np.einsum('ik,klc,jl->ijc', V, Ceig, V)
+    VCV = np.einsum('ik,klc,jl->ijc', V, Ceig, V, optimize=True)
     S = rate_matrix_symmetric(model)
     A = len(model.pi)
     eye = np.eye(A, dtype=bool)[..., None]
@@ -459,8 +459,8 @@ def rate_matrix_irrev(model):
 def _vcv_irrev(Ceig, model, per_branch):
     V, Vi = model.eigenvectors, model.eigenvectors_inv
     if model.exact:     # VCV_ij = sum_kl Vinv_ki C_kl V_jl
-        return np.einsum('ki,...klc,jl->...ijc', Vi, Ceig, V)
-    return np.einsum('ik,...klc,lj->...ijc', V, Ceig, Vi)   # V C V^{-1}
+        return np.einsum('ki,...klc,jl->...ijc', Vi, Ceig, V, optimize=True)
+    return np.einsum('ik,...klc,lj->...ijc', V, Ceig, Vi, optimize=True)   # V C V^{-1}
 
 
 def back_transform_irrev(Ceig, model):
@@ -575,7 +575,7 @@ def accumulate_C_per_branch(D_tilde, U_tilde, J, logNormU, logNormD, logLike, br
 def back_transform_per_branch(Cn, model):
     """back_transform keeping the branch axis (eigensub.py:206-231)."""
     V = model.eigenvectors
-    VCV = np.einsum('ik,nklc,jl->nijc', V, Cn, V)
+    VCV = np.einsum('ik,nklc,jl->nijc', V, Cn, V, optimize=True)
     S = rate_matrix_symmetric(model)
     A = len(model.pi)
     eye = np.eye(A, dtype=bool)[None, ..., None]

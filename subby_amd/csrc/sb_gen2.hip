@@ -17,9 +17,11 @@
 // the operands of accumulate_C (eigensub.py:87-135), and writes logLike and
 // the root posterior.
 //
-// Pass 2 (sb_gacc_kernel): lane = eigen-state k, two columns per wave, eight
-// waves share J_n(t) staged in LDS:  C[k][q] = sum_n Dt_n[k] J_n[k][q] Ut_n[q],
-// then counts = (V C V^T) o S with the diagonal unscaled (eigensub.py:138-172).
+// Pass 2 (sb_gacc_kernel): two columns per wave, eight waves share J_n(t)
+// staged in LDS:  C[k][q] = sum_n Dt_n[k] J_n[k][q] Ut_n[q], the accumulator
+// held in the matrix cores' output layout; then counts = (V C V^T) o S with
+// the diagonal unscaled (eigensub.py:138-172) as two GEMMs on the matrix
+// cores.  With branch counts, pass 2 is sb_gbranch_kernel instead.
 //
 // Vector layout in pass 1: lane l holds column l % 16 of the wave and, in
 // register v, state 16 (v / 4) + 4 (l / 16) + v % 4 -- the output layout of
@@ -381,7 +383,7 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
       if (s < A) glb<float>(B->root)[(long long)s * C + cc] = spi[s] * u0[v] * inv_s0;
     }
   }
-  if (!(flags & SB_F_COUNTS)) return;
+  if (!(flags & (SB_F_COUNTS | SB_F_BCOUNTS))) return;
 
   // ---------------- outside pass + Dt (preorder) ----------------
   {
@@ -460,20 +462,32 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
 }
 
 // ------------------------------------------------------------------ pass 2
+// C[k][q] = sum_n Dt_n[k] J_n[k][q] Ut_n[q] for 2 columns per wave, 8 waves per
+// workgroup sharing J_n staged in LDS (double-buffered, prefetched a branch
+// ahead in registers).  The accumulator is held in the output layout of
+// v_mfma_f32_16x16x4f32: lane (c = l % 16, h = l / 16) holds C[16 mt + 4 h + r][16 nq + c]
+// for every tile (mt, nq) and r < 4, so J arrives in 16-byte LDS reads
+// (J[q][k..k+3], J symmetric) and the accumulator is already the B operand of
+// the back-transform's first GEMM.  Back-transform (eigensub.py:138-172) per
+// column on the matrix cores: T = V C, one transpose of T through LDS, then
+// (V C V^T)^T = V T^T; counts = (V C V^T) o S with the diagonal unscaled.
 template <int AW>
 __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   constexpr int TC = kTC2;
-  constexpr int JPT = AW * AW / (64 * kWPG2);   // J elements staged per thread
-  // LDS: J double buffer, reused after the branch loop as one AW x AW
-  // transpose buffer per wave; V^T for the back-transform; U~ broadcast rows
-  constexpr int kAcc = kWPG2 * AW * AW;
-  __shared__ __attribute__((aligned(16))) float sBig[kAcc];
-  __shared__ __attribute__((aligned(16))) float sV[AW * AW];
-  __shared__ __attribute__((aligned(16))) float sb[kWPG2][TC][AW];
+  constexpr int MT = AW / 16;
+  constexpr int NA = MT * MT * 4;                 // accumulator registers per column
+  constexpr int PS = AW + 4;
+  constexpr int NTH = 64 * kWPG2;
+  constexpr int JPT = (AW * AW / 4 + NTH - 1) / NTH;   // float4 of J staged per thread
+  // LDS: J double buffer during the branch loop, then one T buffer per wave; V
+  constexpr int kBig = kWPG2 * AW * PS > 2 * AW * PS ? kWPG2 * AW * PS : 2 * AW * PS;
+  __shared__ __attribute__((aligned(16))) float sBig[kBig];
+  __shared__ __attribute__((aligned(16))) float sV[AW * PS];   // V[i][k]
   float* sJ0 = sBig;
-  float* sJ1 = sBig + AW * AW;
+  float* sJ1 = sBig + AW * PS;
   const SbLaunch& L = *Lp;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int jc = lane & 15, h = lane >> 4;
   const int tl = blockIdx.x / kQ2, qq = blockIdx.x % kQ2;
   const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * (tile0 + tl);
   const int bidx = tbp[0], tidx = tbp[1];
@@ -490,72 +504,54 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
   const SB_GLOBAL float* DT = UT + 2 * vplane;
   const SB_GLOBAL float* bt = glb<float>(B->bt);
   const SB_GLOBAL float* md = glb<float>(B->model);
-  const SB_CONST float* mdc = cst<float>(B->model);
   const int BTS = sb_bt_stride(AP);
-  const int k = lane;
-  const bool on = k < AW;
 
-  // V^T: sV[kk * AW + i] = V[i][kk]
-  for (int idx = tid; idx < AW * AW; idx += 64 * kWPG2) {
-    const int i = idx / AW, kk = idx % AW;
-    sV[kk * AW + i] = (i < AP && kk < AP) ? md[sb_mr_V(AP) + i * AP + kk] : 0.f;
-  }
-  // J_n staged as sJ[q * AW + kk] = J_n[q][kk] (J is symmetric), pads zero
-  // thread tid holds elements 4 (tid + 512 m) + z (z < 4, m < JPT / 4) when
-  // JPT is a multiple of 4, else elements tid + 512 m (m < JPT)
-  float jr[JPT];
+  stage<AW, false>(sV, md + sb_mr_V(AP), AP, tid, NTH);
+  // J_n (AP x AP) as AW x AW with row stride PS, zero pads: thread tid holds
+  // float4 e = tid + NTH m of the padded matrix
+  f32x4 jr[JPT];
   auto load_J = [&](int n) {
     const SB_GLOBAL float* J = bt + (long long)n * BTS + sb_bt_J(AP);
-    if constexpr (JPT % 4 == 0) {
 #pragma unroll
-      for (int m = 0; m < JPT / 4; ++m) {
-        const int idx = 4 * (tid + m * 64 * kWPG2);
-        const int q = idx / AW, kk = idx % AW;
-        if (AP == AW) {
-          const f32x4 v = *(const SB_GLOBAL f32x4*)(J + q * AW + kk);
+    for (int m = 0; m < JPT; ++m) {
+      const int e = 4 * (tid + m * NTH);
+      const int q = e / AW, kk = e % AW;
+      if (e >= AW * AW) continue;
+      if (AP == AW) {
+        jr[m] = *(const SB_GLOBAL f32x4*)(J + q * AW + kk);
+      } else {
 #pragma unroll
-          for (int z = 0; z < 4; ++z) jr[4 * m + z] = v[z];
-        } else {
-#pragma unroll
-          for (int z = 0; z < 4; ++z) jr[4 * m + z] = (q < AP && kk + z < AP) ? J[q * AP + kk + z] : 0.f;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int m = 0; m < JPT; ++m) {
-        const int idx = tid + m * 64 * kWPG2;
-        const int q = idx / AW, kk = idx % AW;
-        jr[m] = (q < AP && kk < AP) ? J[q * AP + kk] : 0.f;
+        for (int z = 0; z < 4; ++z) jr[m][z] = (q < AP && kk + z < AP) ? J[q * AP + kk + z] : 0.f;
       }
     }
   };
   auto put_J = [&](float* dst) {
-    if constexpr (JPT % 4 == 0) {
 #pragma unroll
-      for (int m = 0; m < JPT / 4; ++m)
-        *(f32x4*)(dst + 4 * (tid + m * 64 * kWPG2)) =
-            f32x4{jr[4 * m], jr[4 * m + 1], jr[4 * m + 2], jr[4 * m + 3]};
-    } else {
-#pragma unroll
-      for (int m = 0; m < JPT; ++m) dst[tid + m * 64 * kWPG2] = jr[m];
+    for (int m = 0; m < JPT; ++m) {
+      const int e = 4 * (tid + m * NTH);
+      if (e < AW * AW) *(f32x4*)(dst + (e / AW) * PS + e % AW) = jr[m];
     }
   };
-
-  // Dt[k] and Ut[k] of branch n, fetched one branch ahead
-  float dkn[TC], ukn[TC];
+  // this lane's factors of branch n, fetched one branch ahead:
+  // Dt_n[16 mt + 4 h + r] (float4 per mt) and Ut_n[16 nq + c]
+  f32x4 dkn[TC][MT];
+  float ukn[TC][MT];
   auto load_vecs = [&](int n) {
 #pragma unroll
     for (int c = 0; c < TC; ++c) {
-      const long long base = ((long long)n * kTW + wcol0 + c) * AW + (on ? k : 0);
-      dkn[c] = DT[base];
-      ukn[c] = UT[base];
+      const long long base = ((long long)n * kTW + wcol0 + c) * AW;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        dkn[c][mt] = *(const SB_GLOBAL f32x4*)(DT + base + 16 * mt + 4 * h);
+        ukn[c][mt] = UT[base + 16 * mt + jc];
+      }
     }
   };
-  float acc[TC][AW];
+  float acc[TC][NA];
 #pragma unroll
   for (int c = 0; c < TC; ++c)
 #pragma unroll
-    for (int q = 0; q < AW; ++q) acc[c][q] = 0.f;
+    for (int x = 0; x < NA; ++x) acc[c][x] = 0.f;
   if (R > 1) {
     load_J(1);
     put_J(sJ1);
@@ -564,72 +560,208 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
   }
   for (int n = 1; n < R; ++n) {
     lds_barrier();                       // J_n is in buffer n & 1; buffer (n+1) & 1 is free
-    float dk[TC];
+    f32x4 dk[TC][MT];
+    float uk[TC][MT];
 #pragma unroll
-    for (int c = 0; c < TC; ++c) {
-      dk[c] = on ? dkn[c] : 0.f;
-      if (on) sb[wv][c][k] = ukn[c];
-    }
+    for (int c = 0; c < TC; ++c)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        dk[c][mt] = dkn[c][mt];
+        uk[c][mt] = ukn[c][mt];
+      }
     if (n + 1 < R) {
       put_J((n + 1) & 1 ? sJ1 : sJ0);
       if (n + 2 < R) load_J(n + 2);
       load_vecs(n + 1);
     }
     const float* J = (n & 1) ? sJ1 : sJ0;
-    wave_lds_sync();
 #pragma unroll
-    for (int q = 0; q < AW; q += 4) {
-      float jq[4];
+    for (int nq = 0; nq < MT; ++nq)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) jq[m] = J[(q + m) * AW + (on ? k : 0)];
+      for (int mt = 0; mt < MT; ++mt) {
+        // J[16 nq + c][16 mt + 4 h + r] = J_n[k][q] (J symmetric)
+        const f32x4 jv = *(const f32x4*)(J + (16 * nq + jc) * PS + 16 * mt + 4 * h);
 #pragma unroll
-      for (int c = 0; c < TC; ++c) {
-        const f32x4 u4 = *(const f32x4*)&sb[wv][c][q];
+        for (int c = 0; c < TC; ++c)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[c][q + m] = fmaf(dk[c] * jq[m], u4[m], acc[c][q + m]);
+          for (int r = 0; r < 4; ++r) {
+            float& a = acc[c][(mt * MT + nq) * 4 + r];
+            a = fmaf(dk[c][mt][r] * jv[r], uk[c][nq], a);
+          }
       }
-    }
-    wave_lds_sync();
   }
 
-  // ---------------- back-transform: counts = (V C V^T) o S ----------------
+  // ---------------- back-transform on the matrix cores ----------------
   lds_barrier();                         // every wave is done with the J buffers
-  float* accb = sBig + wv * AW * AW;     // this wave's transpose buffer
+  float* Tb = sBig + wv * AW * PS;       // this wave's transpose buffer
   SB_GLOBAL float* out = glb<float>(B->counts);
   const SB_GLOBAL float* S = md + sb_mr_S(AP);
-  const SB_CONST float* Vc = mdc + sb_mr_V(AP);
-  const int i = lane;
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
-    if (on) {
+    // T = V C: tile (mi, nq), lane (c', h) holds T[16 mi + 4 h + r][16 nq + c']
 #pragma unroll
-      for (int q = 0; q < AW; q += 4)
-        *(f32x4*)&accb[k * AW + q] = f32x4{acc[c][q], acc[c][q + 1], acc[c][q + 2], acc[c][q + 3]};
-    }
-    wave_lds_sync();
-    // tmp[q] = sum_kk V[i][kk] C[kk][q]
-    float tmp[AW];
+    for (int mi = 0; mi < MT; ++mi) {
+      f32x4 a[MT];
 #pragma unroll
-    for (int q = 0; q < AW; ++q) tmp[q] = 0.f;
-    for (int kk = 0; kk < AP; ++kk) {
-      const float vik = sV[kk * AW + (on ? i : 0)];
+      for (int t2 = 0; t2 < MT; ++t2) a[t2] = *(const f32x4*)(sV + (16 * mi + jc) * PS + 16 * t2 + 4 * h);
 #pragma unroll
-      for (int q = 0; q < AW; q += 4) {
-        const f32x4 a4 = *(const f32x4*)&accb[kk * AW + q];
+      for (int nq = 0; nq < MT; ++nq) {
+        f32x4 t = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int m = 0; m < 4; ++m) tmp[q + m] = fmaf(vik, a4[m], tmp[q + m]);
+        for (int t2 = 0; t2 < MT; ++t2)
+#pragma unroll
+          for (int r2 = 0; r2 < 4; ++r2)
+            t = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t2][r2], acc[c][(t2 * MT + nq) * 4 + r2], t, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Tb[(16 * mi + 4 * h + r) * PS + 16 * nq + jc] = t[r];
       }
     }
     wave_lds_sync();
+    // (V C V^T)^T = V T^T: tile (mj, ni), lane (c', h) holds VCV[16 ni + c'][16 mj + 4 h + r]
     const long long col = (long long)tidx * kTW + wcol0 + c;
-    if (col < C && i < A) {
-      for (int jj = 0; jj < A; ++jj) {
-        float sacc = 0.f;
-        // tmp[q] = 0 for q >= A (pads of U~ are zero), so reading past row jj is harmless
 #pragma unroll
-        for (int q = 0; q < AW; ++q) sacc = fmaf(tmp[q], Vc[jj * AP + q], sacc);
-        const float v = jj == i ? sacc : S[jj * AP + i] * sacc;   // S is symmetric
-        out[((long long)i * A + jj) * C + col] = v;
+    for (int mj = 0; mj < MT; ++mj) {
+      if (16 * mj >= A) break;
+      f32x4 a[MT];
+#pragma unroll
+      for (int t2 = 0; t2 < MT; ++t2) a[t2] = *(const f32x4*)(sV + (16 * mj + jc) * PS + 16 * t2 + 4 * h);
+#pragma unroll
+      for (int ni = 0; ni < MT; ++ni) {
+        if (16 * ni >= A) break;
+        f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t2 = 0; t2 < MT; ++t2) {
+          const f32x4 bv = *(const f32x4*)(Tb + (16 * ni + jc) * PS + 16 * t2 + 4 * h);
+#pragma unroll
+          for (int r2 = 0; r2 < 4; ++r2) o = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t2][r2], bv[r2], o, 0, 0, 0);
+        }
+        const int i = 16 * ni + jc;
+        if (col < C && i < A) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int j = 16 * mj + 4 * h + r;
+            if (j < A) out[((long long)i * A + j) * C + col] = i == j ? o[r] : S[i * AP + j] * o[r];
+          }
+        }
+      }
+    }
+    wave_lds_sync();                     // Tb is rewritten by the next column
+  }
+}
+
+// ------------------------------------------------------------------ pass 2, per-branch counts
+// BranchCounts (subby/jax/__init__.py:450-540): for branch n >= 1 and column c,
+//     counts_n,c = (V diag(Dt_c) J_n diag(Ut_c) V^T) o S     (diagonal unscaled)
+// with Ut_c = Vs^T U_n, Dt_c = Vi^T D_n * scale * mask from pass 1.  Over a
+// wave's 16 columns both products are GEMMs with a shared left operand:
+//     W[k][(j,c)]   = sum_q J_n[k][q] (Ut_c[q] V[j][q])
+//     out[i][(j,c)] = sum_k V[i][k] (Dt_c[k] W[k][(j,c)])
+// on v_mfma_f32_16x16x4f32, N index (j, c) with c fastest: one N tile is one j
+// for the wave's 16 columns, so every (i, j) row of the output is a 64-byte
+// run of 16 consecutive columns.  K steps use the permuted index of pass 1
+// (lane l, step (t2, r2): k = 16 t2 + 4 (l / 16) + r2), under which the output
+// layout of the first GEMM (lane l: rows 16 mt + 4 (l / 16) + r) is already the
+// B operand of the second, and Ut / Dt in pass 1's register layout are the
+// factors each lane needs.  One workgroup of 4 waves per (tile, branch);
+// branch 0 writes zeros.
+template <int AW>
+__global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+  constexpr int NV = AW / 4;
+  constexpr int MT = AW / 16;
+  constexpr int PS = AW + 4;
+  constexpr int NTH = 64 * kWPG;
+  __shared__ __attribute__((aligned(16))) float sJ[AW * PS];   // J_n[k][q]
+  __shared__ __attribute__((aligned(16))) float sV[AW * PS];   // V[i][k]
+  __shared__ __attribute__((aligned(16))) float sS[AW * PS];   // S[j][i]
+  const SbLaunch& L = *Lp;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int jc = lane & 15, h = lane >> 4;
+  const int RM = L.rmax;
+  const int tl = blockIdx.x / RM, n = blockIdx.x % RM;
+  const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * (tile0 + tl);
+  const int bidx = tbp[0], tidx = tbp[1];
+  const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(L.blocks) + bidx;
+  if (!(B->flags & SB_F_BCOUNTS) || !B->bcounts || n >= B->R) return;
+  const long long C = B->C;
+  const int A = B->A, AP = L.AP;
+  const int wcol = wv * kWC + jc;
+  const long long col = (long long)tidx * kTW + wcol;
+  const bool valid = col < C;
+  SB_GLOBAL float* out = glb<float>(B->bcounts) + (long long)n * A * A * C;
+  if (n == 0) {   // the root has no branch: zeros
+    if (valid)
+      for (int e = h; e < A * A; e += 4) out[(long long)e * C + col] = 0.f;
+    return;
+  }
+  const SB_GLOBAL float* md = glb<float>(B->model);
+  const SB_GLOBAL float* bt = glb<float>(B->bt);
+  stage<AW, false>(sJ, bt + (long long)n * sb_bt_stride(AP) + sb_bt_J(AP), AP, tid, NTH);
+  stage<AW, false>(sV, md + sb_mr_V(AP), AP, tid, NTH);
+  stage<AW, false>(sS, md + sb_mr_S(AP), AP, tid, NTH);   // S symmetric: S[j][i] = S[i][j]
+  // this lane's factors: Ut_c and Dt_c at states 16 (v / 4) + 4 h + v % 4
+  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
+  const long long vplane = (long long)RM * kTW * AW;
+  const SB_GLOBAL float* UT = (const SB_GLOBAL float*)st + vplane;
+  const SB_GLOBAL float* DT = UT + 2 * vplane;
+  float ut[NV], dt[NV];
+  {
+    const SB_GLOBAL float* pu = UT + ((long long)n * kTW + wcol) * AW + 4 * h;
+    const SB_GLOBAL float* pd = DT + ((long long)n * kTW + wcol) * AW + 4 * h;
+#pragma unroll
+    for (int t = 0; t < NV / 4; ++t) {
+      const f32x4 x = *(const SB_GLOBAL f32x4*)(pu + 16 * t);
+      const f32x4 y = *(const SB_GLOBAL f32x4*)(pd + 16 * t);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        ut[4 * t + r] = x[r];
+        dt[4 * t + r] = y[r];
+      }
+    }
+  }
+  lds_barrier();
+  for (int j = 0; j < A; ++j) {
+    // B operand of the first GEMM: Ut_c[k] V[j][k] at this lane's K indices
+    float b1[NV];
+#pragma unroll
+    for (int t2 = 0; t2 < MT; ++t2) {
+      const f32x4 vj = *(const f32x4*)(sV + j * PS + 16 * t2 + 4 * h);
+#pragma unroll
+      for (int r2 = 0; r2 < 4; ++r2) b1[4 * t2 + r2] = ut[4 * t2 + r2] * vj[r2];
+    }
+    // W = J_n (Ut o V[j]) for the wave's 16 columns, rows scaled by Dt_c
+    float w[NV];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t2 = 0; t2 < MT; ++t2) {
+        const f32x4 a = *(const f32x4*)(sJ + (16 * mt + jc) * PS + 16 * t2 + 4 * h);
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r2], b1[4 * t2 + r2], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) w[4 * mt + r] = acc[r] * dt[4 * mt + r];
+    }
+    // out = V W; lane l holds rows i = 16 mi + 4 h + r of column c = l % 16
+#pragma unroll
+    for (int mi = 0; mi < MT; ++mi) {
+      if (16 * mi >= A) break;
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t2 = 0; t2 < MT; ++t2) {
+        const f32x4 a = *(const f32x4*)(sV + (16 * mi + jc) * PS + 16 * t2 + 4 * h);
+#pragma unroll
+        for (int r2 = 0; r2 < 4; ++r2)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r2], w[4 * t2 + r2], acc, 0, 0, 0);
+      }
+      const f32x4 sj = *(const f32x4*)(sS + j * PS + 16 * mi + 4 * h);
+      if (valid) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * mi + 4 * h + r;
+          if (i < A) out[((long long)i * A + j) * C + col] = i == j ? acc[r] : sj[r] * acc[r];
+        }
       }
     }
   }
@@ -639,6 +771,8 @@ template __global__ void sb_gwalk_kernel<32>(const SbLaunch*, int);
 template __global__ void sb_gwalk_kernel<64>(const SbLaunch*, int);
 template __global__ void sb_gacc_kernel<32>(const SbLaunch*, int);
 template __global__ void sb_gacc_kernel<64>(const SbLaunch*, int);
+template __global__ void sb_gbranch_kernel<32>(const SbLaunch*, int);
+template __global__ void sb_gbranch_kernel<64>(const SbLaunch*, int);
 
 extern "C" int sb_gen2_tile_cols() { return kTW; }
 
@@ -648,14 +782,23 @@ extern "C" long long sb_gen2_tile_bytes(int AP, int rmax) {
   return (b + 255) / 256 * 256;
 }
 
-extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaunch* L, hipStream_t s) {
-  if (AP > 64 || ntiles <= 0) return hipErrorInvalidValue;
+// Both passes of one round of tiles; branch != 0: pass 2 writes per-branch
+// counts (rmax workgroups per tile) instead of summed counts.
+extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, int branch, const SbLaunch* L,
+                                     hipStream_t s) {
+  if (AP > 64 || ntiles <= 0 || rmax <= 0) return hipErrorInvalidValue;
   if (AP <= 32) {
     hipLaunchKernelGGL(sb_gwalk_kernel<32>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0);
-    hipLaunchKernelGGL(sb_gacc_kernel<32>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
+    if (branch)
+      hipLaunchKernelGGL(sb_gbranch_kernel<32>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
+    else
+      hipLaunchKernelGGL(sb_gacc_kernel<32>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
   } else {
     hipLaunchKernelGGL(sb_gwalk_kernel<64>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0);
-    hipLaunchKernelGGL(sb_gacc_kernel<64>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
+    if (branch)
+      hipLaunchKernelGGL(sb_gbranch_kernel<64>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
+    else
+      hipLaunchKernelGGL(sb_gacc_kernel<64>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
   }
   return hipGetLastError();
 }

@@ -51,7 +51,8 @@ extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, bool irr, int 
                                      const SbLaunch* L, hipStream_t s);
 extern "C" int sb_gen2_tile_cols();
 extern "C" long long sb_gen2_tile_bytes(int AP, int rmax);
-extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, int branch, const SbLaunch* L,
+                                     hipStream_t s);
 
 // Stage memory of the two-pass general path: tiles are processed in rounds
 // that fit this many bytes.
@@ -213,6 +214,7 @@ struct sb_batch {
   // fixed buffers (the no-allocation launcher): never grown, never freed here
   bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
+  int g2_rmax = 1;                  // and the largest tree of the launch
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
   bool timing = false;
@@ -457,9 +459,10 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
   }
   if (key.large == 4) {
     SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, key.irr != 0, nwaves, b->header, s));
-  } else if (key.large == 3) {
+  } else if (key.large == 3 || key.large == 8) {
     for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
-      SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->header, s));
+      SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->g2_rmax,
+                            key.large == 8 ? 1 : 0, b->header, s));
   } else if (key.large == 2)
     SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
   else if (key.large)   // 1 counts, 5 posteriors, 6 per-branch counts, 7 irreversible counts
@@ -641,9 +644,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   bool f81w = large && !lbc && f81 && dtype == SB_F32 && !grad;
   for (int i = 0; i < nb && f81w; ++i) f81w = mods[i]->f81s;
   // general models in FP32 (Counts, log-likelihood, root): two-pass matrix-core path
-  // (not on the no-allocation launcher: its stage area would not fit)
-  const bool gen2 = !b->fixed && large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 &&
-                    !grad;
+  // (not on the no-allocation launcher: its stage area would not fit); with
+  // branch counts (one model per block) its second pass writes them per branch
+  const bool gen2b = !b->fixed && large && lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad;
+  const bool gen2 = gen2b ||
+                    (!b->fixed && large && !lpost && !lbc && !irr && !any_rows && !f81 && dtype == SB_F32 && !grad);
   const int tile_cols = gen2    ? sb_gen2_tile_cols()
                         : f81w  ? sb_f81w_tile_cols()
                         : large ? sb_large_tile_cols(dtype, AP, f81, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
@@ -745,6 +750,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     const size_t rounds = (nt + cap - 1) / cap;
     b->g2_ntiles = int(ntiles);
     b->g2_round = int((nt + rounds - 1) / rounds);   // equal rounds
+    b->g2_rmax = R_max;
     rc = ensure(b, b->stage, size_t(b->g2_round) * size_t(g2_tile_bytes));
     if (rc) return rc;
   }
@@ -873,7 +879,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   b->staged_pending = true;
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
-               lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
+               gen2b ? 8 : lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
                nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);

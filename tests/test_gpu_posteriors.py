@@ -113,10 +113,10 @@ def test_branch_counts_sum_to_golden_counts(sb, eng, case):
 
 @pytest.mark.parametrize('A,C', [(12, 70), (20, 5), (20, 131), (64, 40)])
 def test_branch_counts_large_alphabets(sb, eng, A, C):
-    """Per-branch counts above A = 8 (one branch-selecting block per branch on the
-    large-alphabet kernels): general model FP64 / FP32, all mask modes, the F81
-    closed form, and sum over branches = Counts (the reference's
-    test_large_alphabet, tests/test_api.py:1406-1419, at A = 20)."""
+    """Per-branch counts above A = 8 (FP32 general models: the two-pass matrix-core
+    path; FP64 and the F81 closed form: the large-alphabet kernel's per-branch
+    counts variant): all mask modes, and sum over branches = Counts (the
+    reference's test_large_alphabet, tests/test_api.py:1406-1419, at A = 20)."""
     say('branch counts, A =', A, 'C =', C)
     t, m, aln, pi = _block(sb, A, C, seed=80 + A + C, leaves=7)
     ot, om = otree(t), omodel(m)
@@ -139,6 +139,27 @@ def test_branch_counts_large_alphabets(sb, eng, A, C):
         e = per_branch_err(got, ref)
         say('   ', precision, 'f81', f'{e:.2e}')
         assert e <= tol
+
+
+@pytest.mark.parametrize('A,C', [(20, 300), (64, 70)])
+def test_branch_counts_on_199_node_tree(sb, eng, A, C):
+    """BranchCounts of a general model on the large-alphabet configurations' 100-leaf
+    (R = 199) tree, FP32 (two-pass matrix-core path: per-branch GEMMs over 16-column
+    groups) against the oracle, masks none and auto, ragged last tile."""
+    say('branch counts on the 199-node tree, A =', A, 'C =', C)
+    rng = np.random.default_rng(900 + A)
+    pi = rng.dirichlet(np.full(A, 2.0))
+    m = sb.diagonalize_rate_matrix(sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi), pi)
+    t = sb.synthetic.random_tree(100, rng, 0.1)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+    ot, om = otree(t), omodel(m)
+    for mask in (None, 'auto'):
+        ref = O.BranchCounts(aln, ot, om, branch_mask=mask)
+        got = sb.BranchCounts(aln, t, m, branch_mask=mask, precision='fp32', engine=eng)
+        e = per_branch_err(got, ref)
+        say(f'    mask {mask}: {e:.2e}')
+        assert float(got[0].abs().max()) == 0.0
+        assert e <= 1e-4
 
 
 @pytest.mark.parametrize('A,C', [(12, 70), (20, 131), (33, 45), (64, 40)])
