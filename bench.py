@@ -170,6 +170,12 @@ def other_workload(args, log):
         aln = torch.from_numpy(aln_np).to(dev)
         L = Launcher(eng)
         blk = Block(aln, t, m, f81_fast=f81)
+        tuned = None
+        if not args.no_tune:
+            from subby_amd.autotune import tune
+            tuned = tune(L, [blk], reps=2)
+            log(f"bench {w}: autotuned {tuned['family']} {tuned['best']} ({tuned['ms']:.3f} ms; "
+                f"slowest candidate {tuned['worst_ms']:.3f} ms)")
         prep = L.prepare([blk])
         L.launch(prep, use_graph=False)
         torch.cuda.synchronize()
@@ -209,6 +215,9 @@ def other_workload(args, log):
                "ms_per_step": wall * 1e3, "higher_is_better": True, "dtype": "f32",
                "config": {"workload": w, "A": A, "R": R, "C": C}, "roofline": roof,
                "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": len(cols)}}
+        if tuned is not None:
+            out["launch_shape"] = {"family": tuned['family'], "best": tuned['best'], "best_ms": tuned['ms'],
+                                   "slowest_ms": tuned['worst_ms'], "autotune_table": tuned['table']}
     elif w == 'irrev':
         # irreversible A=4 model (complex eigensystem) on a block of the headline's shape
         rng = np.random.default_rng(6000)
@@ -670,10 +679,11 @@ def main():
     blk = Block(aln, t, m)
     from subby_amd.autotune import tune
     if args.no_tune:
-        tuned = {'best': (4, 0), 'ms': None, 'table': []}
+        tuned = {'best': {}, 'ms': None, 'worst_ms': None, 'table': [], 'family': 'small'}
     else:
         tuned = tune(L, [blk])
-        log(f"bench: autotuned launch shape {tuned['best']} ({tuned['ms']:.3f} ms)")
+        log(f"bench: autotuned launch shape {tuned['best']} ({tuned['ms']:.3f} ms; "
+            f"slowest candidate {tuned['worst_ms']:.3f} ms)")
     prep = L.prepare([blk])
     torch.cuda.synchronize()
 
@@ -724,8 +734,8 @@ def main():
                      "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
         "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
-        "launch_shape": {"waves_per_group": tuned['best'][0], "waves_per_cu": tuned['best'][1],
-                         "autotune_table": tuned['table']},
+        "launch_shape": {"family": tuned['family'], "best": tuned['best'], "best_ms": tuned['ms'],
+                         "slowest_ms": tuned['worst_ms'], "autotune_table": tuned['table']},
     }
 
     if not args.no_plain:

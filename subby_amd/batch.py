@@ -57,6 +57,15 @@ class Launcher:
     def set_tiling(self, waves_per_group=0, waves_per_cu=-1):
         _lib.check(self.eng.lib.sb_batch_set_tiling(self.h, waves_per_group, waves_per_cu))
 
+    def set_tune(self, knob, value):
+        """Set a launch-shape knob (SB_TUNE_*); results do not depend on it."""
+        _lib.check(self.eng.lib.sb_batch_set_tune(self.h, int(knob), int(value)))
+
+    def get_tune(self, knob):
+        v = ctypes.c_int()
+        _lib.check(self.eng.lib.sb_batch_get_tune(self.h, int(knob), ctypes.byref(v)))
+        return v.value
+
     def close(self):
         if self.h:
             self.eng.lib.sb_batch_destroy(self.h)

@@ -210,7 +210,8 @@ struct sb_batch {
   bool staged_pending = false;
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
-  bool stage_tokens = true;         // small-alphabet walk reads tokens from LDS
+  bool stage_tokens = false;        // small-alphabet walk copies tile tokens to LDS first
+  int gen2_round = 0;               // two-pass path: tiles per round (0: as the stage budget allows)
   // fixed buffers (the no-allocation launcher): never grown, never freed here
   bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
@@ -745,7 +746,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   long long g2_tile_bytes = 0;
   if (gen2) {
     g2_tile_bytes = sb_gen2_tile_bytes(AP, R_max);
-    const size_t cap = std::max<size_t>(1, kGen2StageBudget / size_t(g2_tile_bytes));
+    size_t cap = std::max<size_t>(1, kGen2StageBudget / size_t(g2_tile_bytes));
+    if (b->gen2_round > 0) cap = std::min(cap, size_t(b->gen2_round));   // tuned round size
     const size_t nt = std::max<size_t>(ntiles, 1);
     const size_t rounds = (nt + cap - 1) / cap;
     b->g2_ntiles = int(ntiles);
@@ -1817,6 +1819,10 @@ int sb_batch_set_tune(sb_batch* b, int knob, int value) {
       if (value != 0 && value != 1) return fail(SB_ERR_ARG, "stage_tokens must be 0 or 1");
       b->stage_tokens = value != 0;
       return SB_OK;
+    case SB_TUNE_GEN2_ROUND:
+      if (value < 0) return fail(SB_ERR_ARG, "gen2 round must be >= 0 tiles");
+      b->gen2_round = value;
+      return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
 }
@@ -1827,6 +1833,7 @@ int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
     case SB_TUNE_WAVES_PER_GROUP: *value = b->waves_per_group; return SB_OK;
     case SB_TUNE_WAVES_PER_CU: *value = b->waves_per_cu; return SB_OK;
     case SB_TUNE_STAGE_TOKENS: *value = b->stage_tokens ? 1 : 0; return SB_OK;
+    case SB_TUNE_GEN2_ROUND: *value = b->gen2_round; return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
 }

@@ -1,0 +1,65 @@
+"""Launch-shape and tile-shape knobs: every candidate of the autotuner gives
+bitwise the same results (only the time may differ), and tune() times the whole
+search space of the batch's kernel family and applies the fastest."""
+import numpy as np
+import pytest
+import torch
+
+from tests.test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def eng():
+    import subby_amd
+    return subby_amd.Engine(0)
+
+
+def _blocks(sb, A, C, seed, leaves):
+    rng = np.random.default_rng(seed)
+    pi = rng.dirichlet(np.full(A, 3.0))
+    if A == 4:
+        m = sb.hky85_diag(2.0, pi)
+    else:
+        m = sb.diagonalize_rate_matrix(sb.reversible_rate_matrix(rng.lognormal(0, 1, (A, A)), pi), pi)
+    t = sb.synthetic.random_tree(leaves, rng, 0.1)
+    aln = sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+    return t, m, aln
+
+
+@pytest.mark.parametrize('A,C,leaves', [(4, 3000, 64), (20, 700, 30)])
+def test_every_candidate_gives_the_same_bits(eng, A, C, leaves):
+    import subby_amd as sb
+    from subby_amd.autotune import apply, candidates, family, tune
+    from subby_amd.batch import Block, Launcher
+    say('tuning knobs, A =', A)
+    t, m, aln = _blocks(sb, A, C, 300 + A, leaves)
+    aln_d = torch.from_numpy(aln).to(eng.device)
+    L = Launcher(eng)
+    fam = family([Block(aln_d, t, m)])
+    assert fam == ('small' if A == 4 else 'gen2')
+    cands = candidates(fam)
+    if fam == 'small':
+        cands = cands[::5]
+    cands = cands + [dict(gen2_round=1)] if fam == 'gen2' else cands
+    ref = None
+    for cand in cands:
+        apply(L, cand)
+        blk = Block(aln_d, t, m)
+        L.submit([blk])
+        L.sync()
+        got = (blk.counts.clone(), blk.loglike.clone(), blk.root.clone())
+        if ref is None:
+            ref = got
+        else:
+            for a, b in zip(ref, got):
+                assert torch.equal(a, b), cand
+    res = tune(L, [Block(aln_d, t, m)], reps=1)
+    assert res['family'] == fam and len(res['table']) == len(candidates(fam))
+    assert res['ms'] == min(r['ms'] for r in res['table'])
+    say('   best', res['best'], f"{res['ms']:.3f} ms, slowest {res['worst_ms']:.3f} ms")
+    blk = Block(aln_d, t, m)
+    L.submit([blk])
+    L.sync()
+    assert torch.equal(blk.counts, ref[0])
