@@ -101,6 +101,14 @@ def _cpu_chunk(bounds):
     return c1 - c0
 
 
+def _host_cores(max_workers=16):
+    return max(1, min(len(os.sched_getaffinity(0)), max_workers))
+
+
+def _round(d, nd=4):
+    return {k: (round(v, nd) if isinstance(v, float) else v) for k, v in d.items()}
+
+
 def cpu_baseline(t, m, aln, reps=5, max_workers=16):
     """The FP64 oracle (subby.jax data flow: 128-column inside chunks, einsum
     contractions, Counts with mask auto) over the whole block on every core of
@@ -626,8 +634,11 @@ def main():
     ap.add_argument('--stream-blocks', type=int, default=300)
     ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
     ap.add_argument('--serve-requests', type=int, default=20000)
-    ap.add_argument('--serve-pool', type=int, default=500,
-                    help='distinct simulated (tree, alignment) pairs the request stream draws from')
+    ap.add_argument('--serve-pool', type=int, default=0,
+                    help='if set, the request stream draws (tree, alignment) pairs from this many '
+                         'simulated ones (default: every request distinct)')
+    ap.add_argument('--serve-plain-sample', type=int, default=1000,
+                    help='served blocks timed through the plain path')
     ap.add_argument('--workload', default='hky85',
                     choices=['hky85', 'f81-a64', 'general-a64', 'grad-hky85', 'branch-hky85', 'post-a20',
                              'irrev', 'mixture', 'rows', 'maf'],
@@ -672,6 +683,13 @@ def main():
         log('bench: CPU baseline (FP64 oracle, whole block, all cores)')
         cpu = cpu_baseline(t, m, aln_np)
         log(f"bench: CPU baseline {cpu['value']:.3e} col*br/s on {cpu['cores']} cores")
+    serve_reqs = None
+    if args.serve_requests > 0:   # simulated by forked workers, before the GPU is touched
+        from subby_amd.serve import make_stream as make_requests
+        nw = 1 if torch.cuda.is_initialized() else _host_cores()
+        log(f'bench: simulating {args.serve_requests} serving requests ({nw} workers)')
+        serve_reqs = make_requests(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool or None,
+                                   workers=nw)
     eng = sb.Engine(0)
     dev = eng.device
     aln = torch.from_numpy(aln_np).to(dev)
@@ -780,12 +798,13 @@ def main():
             "unit": "column*branches/s"}
 
     if args.serve_requests > 0:
-        from subby_amd.serve import NativeServer, Server, make_stream as make_requests
-        log(f'bench: serving {args.serve_requests} requests (continuous batching)')
-        reqs = make_requests(args.serve_requests, qps=1e9, seed=99, pool=args.serve_pool)
+        from subby_amd.serve import NativeServer, Server
+        reqs = serve_reqs
         units = sum(int(r.alignment.shape[1]) * (len(r.tree.parentIndex) - 1) for r in reqs)
-        # GPU-only reference: the same blocks as one persistent batch, inputs resident
+        ncols = sum(int(r.alignment.shape[1]) for r in reqs)
         from subby_amd.batch import Block
+        # GPU-only reference: the same blocks as one persistent batch, inputs resident
+        log('bench: serving stream as one resident batch')
         blocks = [Block(torch.from_numpy(r.alignment).to(dev), r.tree, r.model) for r in reqs]
         Lg = Launcher(eng)
         prep = Lg.prepare(blocks)
@@ -796,34 +815,77 @@ def main():
             Lg.launch(prep, use_graph=False)
         torch.cuda.synchronize()
         gpu_only = units / ((time.perf_counter() - s_) / 3)
-        del blocks, prep, Lg
-        nsrv = NativeServer(eng, hbm_budget=16 << 30, lanes=3)
-        nsrv.run(reqs[:500], realtime=False)                    # warm-up (graphs, buffers)
-        _, sat = nsrv.run(reqs, realtime=False)
+        del prep, Lg
+        # plain path (one block per call, per-level launches) on the same blocks:
+        # a sample of them, inputs resident, outputs left on the device
+        nplain = min(len(blocks), args.serve_plain_sample)
+        log(f'bench: plain path on {nplain} of the served blocks')
+        for b_ in blocks[:20]:
+            counts_plain(b_.alignment, b_.tree, b_.model, engine=eng)
+        torch.cuda.synchronize()
+        s_ = time.perf_counter()
+        for b_ in blocks[:nplain]:
+            counts_plain(b_.alignment, b_.tree, b_.model, engine=eng)
+        torch.cuda.synchronize()
+        pt = time.perf_counter() - s_
+        plain_units = sum(int(b_.alignment.shape[1]) * (len(b_.tree.parentIndex) - 1) for b_ in blocks[:nplain])
+        plain = plain_units / pt
+        del blocks
+        eng.trim()              # the resident batch resolved every request's model
+        torch.cuda.empty_cache()
+
+        log(f'bench: serving {len(reqs)} distinct requests (native continuous batching)')
+        nsrv = NativeServer(eng, hbm_budget=16 << 30, lanes=4, max_cols=1 << 17)
+        nsrv.run(reqs[:2000], realtime=False)                   # warm-up (graphs, buffers)
+        arena = [eng.arena_stats()]
+        _, sat = nsrv.run(reqs, realtime=False, timed=True)
+        arena.append(eng.arena_stats())
         rate = sat['requests'] / sat['makespan_s']
-        qps = 0.5 * rate
-        arr = np.cumsum(np.random.default_rng(5).exponential(1.0 / qps, size=len(reqs)))
-        for r, a in zip(reqs, arr):
-            r.arrival = float(a)
-        _, lat = nsrv.run(reqs, realtime=True)
+        log(f"bench: saturated {rate:.0f} requests/s, {sat['throughput']:.3e} col*br/s")
+        sweep = []
+        for k, frac in enumerate((0.25, 0.5, 0.75, 0.9)):
+            qps = frac * rate
+            arr = np.cumsum(np.random.default_rng(5 + k).exponential(1.0 / qps, size=len(reqs)))
+            for r, a_ in zip(reqs, arr):
+                r.arrival = float(a_)
+            _, st = nsrv.run(reqs, realtime=True, timed=True)
+            arena.append(eng.arena_stats())
+            bd = st['breakdown']
+            sweep.append({"load": frac, "qps": qps, "achieved_qps": st['requests'] / st['makespan_s'],
+                          "throughput": st['throughput'], "p50_ms": st['p50_ms'], "p99_ms": st['p99_ms'],
+                          "mean_ms": st['mean_ms'], "breakdown_p45_p55": _round(bd['p45_p55']),
+                          "breakdown_slowest_1pct": _round(bd['slowest_1pct'])})
+            log(f"bench: load {frac:.2f} ({qps:.0f} req/s): p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
         nsrv.close()
         psrv = Server(eng, hbm_budget=16 << 30, lanes=3)       # the Python loop, for comparison
+        npy = min(len(reqs), 5000)
         psrv.run(reqs[:200], realtime=False)
-        _, psat = psrv.run(reqs, realtime=False)
+        _, psat = psrv.run(reqs[:npy], realtime=False)
+        del psrv
         out["served"] = {
-            "workload": "stream of HKY85 blocks: random trees with 4-100 leaves, block length LogNormal "
-                        "(mean 500, sd 1.0) clipped to [20, 10000], kappa~U(1,5), pi~Dir(5,5,5,5) per request; "
-                        f"{args.serve_pool} simulated (tree, alignment) pairs drawn with replacement; "
+            "workload": "stream of distinct HKY85 blocks: random trees with 4-100 leaves, block length LogNormal "
+                        "(mean 500, sd 1.0) clipped to [20, 10000], 3% gaps and 1% ambiguous tokens, "
+                        "kappa~U(1,5), pi~Dir(5,5,5,5) per request, each request simulated from its own seed; "
                         "FP32 Counts+logLike+root, alignments from host memory, results copied back to host",
-            "server": "native loop (sb_server_*), 3 lanes, arrivals replayed from native code",
-            "requests": len(reqs),
+            "server": "native loop (sb_server_*), 4 lanes, at most 131072 columns per launch, "
+                      "arrivals replayed from native code",
+            "requests": len(reqs), "distinct_blocks": len(reqs) if not args.serve_pool else args.serve_pool,
+            "columns": ncols,
             "saturated": {"throughput": sat['throughput'], "requests_per_s": rate,
-                          "p50_ms": sat['p50_ms'], "p99_ms": sat['p99_ms']},
-            "poisson": {"qps": qps, "throughput": lat['throughput'], "p50_ms": lat['p50_ms'],
-                        "p99_ms": lat['p99_ms']},
-            "python_loop_saturated": {"throughput": psat['throughput'],
-                                      "requests_per_s": psat['requests'] / psat['makespan_s']},
+                          "p50_ms": sat['p50_ms'], "p99_ms": sat['p99_ms'],
+                          "breakdown_all": _round(sat['breakdown']['all'])},
+            "sweep": sweep,
+            "arena": {"note": "model arena after the warm-up and after each of the 5 runs above "
+                              "(saturated, then the sweep); inline models are capped at 4096 per server",
+                      "reserved_bytes": [a_[0] for a_ in arena], "live_models": [a_[1] for a_ in arena]},
+            "plain": {"throughput": plain, "blocks": nplain,
+                      "note": "sb_counts_plain per block on the first blocks of the same stream, "
+                              "alignments resident in HBM, outputs left in HBM"},
+            "served_vs_plain": sat['throughput'] / plain,
             "gpu_only_batch": gpu_only,
+            "gpu_only_batch_vs_plain": gpu_only / plain,
+            "python_loop_saturated": {"throughput": psat['throughput'], "requests": npy,
+                                      "requests_per_s": psat['requests'] / psat['makespan_s']},
             "unit": "column*branches/s"}
 
     if cpu is not None:

@@ -317,6 +317,27 @@ int sb_server_drain(sb_server* s);     /* wait until every submitted request com
  * their index.  (No reference counterpart: measurement of the serving loop.) */
 int sb_server_replay(sb_server* s, int n, sb_request* requests, const double* arrival_s,
                      double* done_s);
+/* Where a request's latency went.  Times are seconds after the replay call,
+ * like done_s; the *_ms fields are the request's batch's GPU spans (HIP
+ * events on the lane's stream) and host planning time. */
+typedef struct sb_request_timing {
+  double admitted_s;     /* a lane took it off the queue */
+  double enqueued_s;     /* its batch's upload, launch and download were enqueued */
+  double gpu_done_s;     /* the lane saw its batch's download complete */
+  double done_s;         /* its results were copied into the caller's arrays */
+  double upload_ms;      /* batch alignments host -> device */
+  double kernel_ms;      /* end of upload -> end of the batch's kernels (plan copy included) */
+  double download_ms;    /* batch outputs device -> host */
+  double plan_ms;        /* host time in the batch submission (planning, model table, launch) */
+  double models_ms;      /* host time resolving the batch's inline models (sharing, evictions, arena puts) */
+  long long batch_columns;
+  int batch_requests;
+  int lane;
+} sb_request_timing;
+/* sb_server_replay with the per-request breakdown above written to
+ * timing[i].  (No reference counterpart: measurement of the serving loop.) */
+int sb_server_replay_timed(sb_server* s, int n, sb_request* requests, const double* arrival_s,
+                           sb_request_timing* timing);
 
 /* Plain path: the same statistics through one launch per tree level per pass
  * with inside/outside partials materialised in HBM (the data flow of the

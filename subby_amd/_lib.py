@@ -6,6 +6,8 @@ to a CPU path: a missing library or a missing GPU raises.
 import ctypes
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libsubby_amd.so')
 
@@ -25,6 +27,7 @@ EXPORTS = [
     'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
     'sb_counts_multi', 'sb_branch_counts_multi', 'sb_inside_outside_multi', 'sb_mixture_posterior',
     'sb_arena_stats', 'sb_batch_set_tune', 'sb_batch_get_tune', 'sb_counts_noalloc',
+    'sb_server_replay_timed',
 ]
 
 
@@ -67,6 +70,26 @@ class SbRequest(ctypes.Structure):
         ('loglike', ctypes.c_void_p),
         ('tag', ctypes.c_longlong),
     ]
+
+
+class SbRequestTiming(ctypes.Structure):
+    """sb_request_timing of include/subby_amd.h."""
+    _fields_ = [(f, ctypes.c_double) for f in (
+        'admitted_s', 'enqueued_s', 'gpu_done_s', 'done_s', 'upload_ms', 'kernel_ms', 'download_ms',
+        'plan_ms', 'models_ms')] + [
+        ('batch_columns', ctypes.c_longlong),
+        ('batch_requests', ctypes.c_int),
+        ('lane', ctypes.c_int),
+    ]
+
+
+TIMING_DTYPE = np.dtype({
+    'names': ['admitted_s', 'enqueued_s', 'gpu_done_s', 'done_s', 'upload_ms', 'kernel_ms',
+              'download_ms', 'plan_ms', 'models_ms', 'batch_columns', 'batch_requests', 'lane'],
+    'formats': ['<f8'] * 9 + ['<i8', '<i4', '<i4'],
+    'offsets': [0, 8, 16, 24, 32, 40, 48, 56, 64, 72, 80, 84],
+    'itemsize': 88})
+assert TIMING_DTYPE.itemsize == ctypes.sizeof(SbRequestTiming)
 
 
 class SubbyError(RuntimeError):
@@ -126,6 +149,7 @@ def load(path=LIB_PATH):
     lib.sb_server_poll.argtypes = [vp, i, vp, vp, P(i)]
     lib.sb_server_drain.argtypes = [vp]
     lib.sb_server_replay.argtypes = [vp, i, vp, vp, vp]
+    lib.sb_server_replay_timed.argtypes = [vp, i, vp, vp, vp]
     for name in EXPORTS:
         if name not in ('sb_last_error',):
             getattr(lib, name).restype = i

@@ -120,10 +120,17 @@ class Engine:
                 self._models[key] = [mid, self._gen]
         return mid
 
-    def _evict(self):
+    def trim(self):
+        """Start a call and drop cached models beyond ``max_models`` (least
+        recently used first), e.g. after one call resolved many models."""
+        self.new_call()
+        with self._lock:
+            self._evict(0)
+
+    def _evict(self, slack: int = 1):
         if self.max_models is None:
             return
-        while len(self._models) >= self.max_models:
+        while len(self._models) > self.max_models - slack:
             key, (mid, gen) = next(iter(self._models.items()))
             if gen >= self._gen:        # everything left was asked for by the current call
                 return

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, 'csrc')
 OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libsubby_amd.so')
-SOURCES = ['sb_host.cpp', 'sb_serve.cpp', 'sb_fused.hip', 'sb_large.hip', 'sb_f81w.hip', 'sb_gen2.hip', 'sb_post.hip', 'sb_plain.hip', 'sb_multi.hip', 'sb_noalloc.hip']
+SOURCES = ['sb_host.cpp', 'sb_serve.cpp', 'sb_fused.hip', 'sb_large.hip', 'sb_f81w.hip', 'sb_gen2.hip', 'sb_post.hip', 'sb_plain.hip', 'sb_multi.hip', 'sb_noalloc.hip',
+           'sb_stage.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = [

@@ -14,9 +14,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/subby_amd.h"
@@ -60,6 +62,8 @@ static const size_t kGen2StageBudget = size_t(8) << 30;
 extern "C" hipError_t sb_launch_branch_params(int dtype, bool irr, int nwg, const SbLaunch* L,
                                               hipStream_t s);
 extern "C" hipError_t sb_launch_reset(const SbLaunch* L, hipStream_t s);
+extern "C" hipError_t sb_launch_scatter_runs(const char* stage, const SbScatterRun* runs, int nruns,
+                                             hipStream_t s);
 extern "C" int sb_internal_noalloc_scratch(int device, void** ptr);
 extern "C" hipError_t sb_launch_grad_reduce(int nwg, const SbLaunch* L, hipStream_t s);
 struct SbPlainBuffers {
@@ -147,22 +151,28 @@ struct sb_context {
   std::vector<int> free_ids;        // ids of dropped models, reused by sb_model_put
   std::mutex mu;
   // Model arena: records of every model live in large device chunks that never
-  // move.  sb_model_put stages the records in pinned memory; the next
-  // submission on any stream flushes them (merged copies) before its kernels,
-  // and every submission waits for the last flush.
+  // move.  sb_model_put stages the records in pinned memory (a rotation of
+  // kPend buffers); the next submission on any stream flushes them before its
+  // kernels (a few merged copies, or one copy plus a scatter kernel), and
+  // every submission waits for the last flush.
   std::vector<void*> arena_chunks;
   char* arena_cur = nullptr;
   size_t arena_off = 0, arena_cap = 0;
   size_t arena_reserved = 0;        // bytes handed out from chunks (live + retired records)
-  std::vector<Retired> retired;     // dropped records waiting for their launches to finish
+  // dropped records by size, oldest first, waiting for their launches to finish
+  std::unordered_map<size_t, std::list<Retired>> retired;
+  size_t n_retired = 0;
   std::vector<sb_batch*> live;      // launchers of this context (fences name them by serial)
   uint64_t next_serial = 0;
-  char* pend[2] = {nullptr, nullptr};
-  size_t pend_cap[2] = {0, 0};
+  static constexpr int kPend = 4;    // staging buffers in rotation
+  char* pend[kPend] = {};
+  size_t pend_cap[kPend] = {};
   size_t pend_len = 0;
   int pend_k = 0;
-  bool pend_busy[2] = {false, false};
-  hipEvent_t pend_ev[2] = {nullptr, nullptr};
+  bool pend_busy[kPend] = {};
+  hipEvent_t pend_ev[kPend] = {};
+  char* dstage[kPend] = {};          // device copies of pend[] for the scattered upload
+  size_t dstage_cap[kPend] = {};
   std::vector<ModelRun> runs;
   hipEvent_t model_ev = nullptr;
   bool model_ev_valid = false;
@@ -345,21 +355,25 @@ void note_use(const Model* m, const sb_batch* b, uint64_t seq) {
 // Reserve `bytes` (multiple of 256) in the model arena: a finished retired
 // record of the same size if there is one, else fresh chunk space.
 int arena_alloc(sb_context* ctx, size_t bytes, char** out) {
-  for (size_t i = 0, checked = 0; i < ctx->retired.size() && checked < 8; ++i) {
-    Retired& r = ctx->retired[i];
-    if (r.bytes != bytes) continue;
-    ++checked;
-    bool done = true;
-    for (const Fence& f : r.uses) done = done && fence_done(ctx, f);
-    if (!done) continue;
-    *out = r.base;
-    ctx->retired.erase(ctx->retired.begin() + long(i));
-    return SB_OK;
+  auto bucket = ctx->retired.find(bytes);
+  if (bucket != ctx->retired.end()) {
+    std::list<Retired>& l = bucket->second;
+    int checked = 0;
+    for (auto it = l.begin(); it != l.end() && checked < 8; ++it, ++checked) {
+      bool done = true;
+      for (const Fence& f : it->uses) done = done && fence_done(ctx, f);
+      if (!done) continue;
+      *out = it->base;
+      l.erase(it);
+      --ctx->n_retired;
+      return SB_OK;
+    }
   }
   ctx->arena_reserved += bytes;
   if (!ctx->arena_cur || ctx->arena_off + bytes > ctx->arena_cap) {
     const size_t chunk = std::max<size_t>(64u << 20, bytes);
     void* p = nullptr;
+    SB_HIP(hipSetDevice(ctx->device));
     if (hipMalloc(&p, chunk) != hipSuccess) {
       (void)hipGetLastError();
       return fail(SB_ERR_OOM, "model arena allocation of %zu bytes failed", chunk);
@@ -374,6 +388,26 @@ int arena_alloc(sb_context* ctx, size_t bytes, char** out) {
   return SB_OK;
 }
 
+// Make pinned staging buffer k hold `need` bytes, keeping its first `keep`
+// (buffer k is not in use by the GPU here).
+int grow_pend(sb_context* ctx, int k, size_t need, size_t keep) {
+  if (need <= ctx->pend_cap[k]) return SB_OK;
+  size_t want = std::max<size_t>({need, ctx->pend_cap[k] * 2, size_t(1) << 20});
+  char* nb = nullptr;
+  SB_HIP(hipSetDevice(ctx->device));
+  if (hipHostMalloc(reinterpret_cast<void**>(&nb), want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SB_ERR_OOM, "pinned model staging allocation failed");
+  }
+  if (ctx->pend[k]) {
+    std::memcpy(nb, ctx->pend[k], keep);
+    SB_HIP(hipHostFree(ctx->pend[k]));
+  }
+  ctx->pend[k] = nb;
+  ctx->pend_cap[k] = want;
+  return SB_OK;
+}
+
 // Append bytes destined for device address dst to the pending model staging.
 int stage_model_bytes(sb_context* ctx, char* dst, const void* src, size_t len) {
   int k = ctx->pend_k;
@@ -381,20 +415,8 @@ int stage_model_bytes(sb_context* ctx, char* dst, const void* src, size_t len) {
     SB_HIP(hipEventSynchronize(ctx->pend_ev[k]));
     ctx->pend_busy[k] = false;
   }
-  if (ctx->pend_len + len > ctx->pend_cap[k]) {
-    size_t want = std::max<size_t>({ctx->pend_len + len, ctx->pend_cap[k] * 2, size_t(1) << 20});
-    char* nb = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void**>(&nb), want, hipHostMallocDefault) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(SB_ERR_OOM, "pinned model staging allocation failed");
-    }
-    if (ctx->pend[k]) {
-      std::memcpy(nb, ctx->pend[k], ctx->pend_len);
-      SB_HIP(hipHostFree(ctx->pend[k]));
-    }
-    ctx->pend[k] = nb;
-    ctx->pend_cap[k] = want;
-  }
+  const int rc = grow_pend(ctx, k, ctx->pend_len + len, ctx->pend_len);
+  if (rc) return rc;
   std::memcpy(ctx->pend[k] + ctx->pend_len, src, len);
   if (!ctx->runs.empty() && ctx->runs.back().dst + ctx->runs.back().len == dst &&
       ctx->runs.back().src + ctx->runs.back().len == ctx->pend_len)
@@ -406,12 +428,49 @@ int stage_model_bytes(sb_context* ctx, char* dst, const void* src, size_t len) {
 }
 
 // Enqueue pending model uploads on stream s and make s wait for all uploads.
-int flush_models(sb_context* ctx, hipStream_t s) {
+// A few runs are copied directly; more (new models scattered over reused
+// arena records) go up as one copy of the staging buffer plus its run table,
+// and one kernel moves each run into place.  With `may_alloc` false (the
+// no-allocation launcher) only direct copies are used.
+int flush_models(sb_context* ctx, hipStream_t s, bool may_alloc = true) {
   if (ctx->model_ev_valid) SB_HIP(hipStreamWaitEvent(s, ctx->model_ev, 0));
   if (!ctx->runs.empty()) {
     const int k = ctx->pend_k;
-    for (const ModelRun& r : ctx->runs)
-      SB_HIP(hipMemcpyAsync(r.dst, ctx->pend[k] + r.src, r.len, hipMemcpyHostToDevice, s));
+    const size_t nruns = ctx->runs.size();
+    const size_t toff = align_up(ctx->pend_len, 256);
+    const size_t total = toff + nruns * sizeof(SbScatterRun);
+    bool scatter = may_alloc && nruns > 8;
+    if (scatter && grow_pend(ctx, k, total, ctx->pend_len)) {
+      (void)sb_last_error();
+      scatter = false;
+    }
+    if (scatter && total > ctx->dstage_cap[k]) {
+      // buffer k's last scatter has finished: stage_model_bytes waited for pend_ev[k]
+      if (ctx->dstage[k]) SB_HIP(hipFree(ctx->dstage[k]));
+      ctx->dstage[k] = nullptr;
+      ctx->dstage_cap[k] = 0;
+      const size_t want = std::max<size_t>(total * 2, size_t(1) << 20);
+      if (hipMalloc(reinterpret_cast<void**>(&ctx->dstage[k]), want) == hipSuccess) {
+        ctx->dstage_cap[k] = want;
+      } else {
+        (void)hipGetLastError();
+        ctx->dstage[k] = nullptr;
+        scatter = false;
+      }
+    }
+    if (scatter) {
+      SbScatterRun* t = reinterpret_cast<SbScatterRun*>(ctx->pend[k] + toff);
+      for (size_t i = 0; i < nruns; ++i) {
+        const ModelRun& r = ctx->runs[i];
+        t[i] = {reinterpret_cast<unsigned long long>(r.dst), r.src, r.len};
+      }
+      SB_HIP(hipMemcpyAsync(ctx->dstage[k], ctx->pend[k], total, hipMemcpyHostToDevice, s));
+      SB_HIP(sb_launch_scatter_runs(ctx->dstage[k], reinterpret_cast<const SbScatterRun*>(ctx->dstage[k] + toff),
+                                    int(nruns), s));
+    } else {
+      for (const ModelRun& r : ctx->runs)
+        SB_HIP(hipMemcpyAsync(r.dst, ctx->pend[k] + r.src, r.len, hipMemcpyHostToDevice, s));
+    }
     if (!ctx->pend_ev[k]) SB_HIP(hipEventCreateWithFlags(&ctx->pend_ev[k], hipEventDisableTiming));
     if (!ctx->model_ev) SB_HIP(hipEventCreateWithFlags(&ctx->model_ev, hipEventDisableTiming));
     SB_HIP(hipEventRecord(ctx->pend_ev[k], s));
@@ -420,7 +479,7 @@ int flush_models(sb_context* ctx, hipStream_t s) {
     ctx->pend_busy[k] = true;
     ctx->runs.clear();
     ctx->pend_len = 0;
-    ctx->pend_k = k ^ 1;
+    ctx->pend_k = (k + 1) % sb_context::kPend;
   }
   return SB_OK;
 }
@@ -874,7 +933,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
 
   hipStream_t s = b->stream;
-  rc = flush_models(ctx, s);
+  rc = flush_models(ctx, s, !b->fixed);
   if (rc) return rc;
   SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
   SB_HIP(hipEventRecord(b->staged, s));
@@ -1257,73 +1316,17 @@ int multi(sb_context* ctx, int n_cols, int n_rows, const int* ids, const int8_t*
   return SB_OK;
 }
 
-}  // namespace
+// A reversible model's device record and table entry, built on the host
+// (validation and host math: no context state, so callers may run it on
+// several threads); commit_model then places it under the context lock.
+struct PreparedModel {
+  Model m;
+  std::vector<char> blob;
+};
 
-extern "C" {
-
-const char* sb_last_error(void) { return g_err.c_str(); }
-
-// Internal hooks for the serving loop (sb_serve.cpp): set the calling
-// thread's error message; the device a context was created on.
-int sb_internal_error(int code, const char* msg) {
-  g_err = msg;
-  return code;
-}
-int sb_internal_device(sb_context* ctx) { return ctx ? ctx->device : -1; }
-// Padded alphabet of an alphabet size (-1 outside 2..64): requests of one
-// padded alphabet (and reversibility) share a launch.
-int sb_internal_pad_alphabet(int A) { return A < 2 ? -1 : pad_alphabet(A); }
-// 1 for an irreversible model, 0 for a reversible one, -1 for an unknown id.
-int sb_internal_model_irr(sb_context* ctx, int model_id) {
-  if (!ctx) return -1;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  const Model* m = get_model(ctx, model_id);
-  return m ? (m->irr ? 1 : 0) : -1;
-}
-int sb_version(void) { return 1; }
-
-int sb_create(sb_context** out, int device) {
-  if (!out) return fail(SB_ERR_ARG, "null output pointer");
-  int n = 0;
-  SB_HIP(hipGetDeviceCount(&n));
-  if (device < 0 || device >= n) return fail(SB_ERR_ARG, "device %d not present (%d devices)", device, n);
-  SB_HIP(hipSetDevice(device));
-  sb_context* ctx = new sb_context();
-  ctx->device = device;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-    ctx->num_cus = prop.multiProcessorCount;
-  // model-upload events up front, so no call creates them later
-  for (int k = 0; k < 2; ++k) SB_HIP(hipEventCreateWithFlags(&ctx->pend_ev[k], hipEventDisableTiming));
-  SB_HIP(hipEventCreateWithFlags(&ctx->model_ev, hipEventDisableTiming));
-  // the no-allocation entry point's launcher; if it cannot be set up, only
-  // that entry point fails (with this message)
-  if (noalloc_create(ctx) != SB_OK) ctx->noalloc_err = g_err;
-  *out = ctx;
-  return SB_OK;
-}
-
-int sb_destroy(sb_context* ctx) {
-  if (!ctx) return SB_OK;
-  (void)hipSetDevice(ctx->device);
-  batch_free(ctx->internal);
-  batch_free(ctx->noalloc);
-  if (ctx->plain.p) (void)hipFree(ctx->plain.p);
-  if (ctx->plain_bt.p) (void)hipFree(ctx->plain_bt.p);
-  (void)hipDeviceSynchronize();
-  for (void* c : ctx->arena_chunks) (void)hipFree(c);
-  for (int k = 0; k < 2; ++k) {
-    if (ctx->pend[k]) (void)hipHostFree(ctx->pend[k]);
-    if (ctx->pend_ev[k]) (void)hipEventDestroy(ctx->pend_ev[k]);
-  }
-  if (ctx->model_ev) (void)hipEventDestroy(ctx->model_ev);
-  delete ctx;
-  return SB_OK;
-}
-
-int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double* eigenvectors,
-                 const double* pi, int* model_id) {
-  if (!ctx || !eigenvalues || !eigenvectors || !pi || !model_id) return fail(SB_ERR_ARG, "null argument");
+int prepare_model(int A, const double* eigenvalues, const double* eigenvectors, const double* pi,
+                  PreparedModel* out) {
+  if (!eigenvalues || !eigenvectors || !pi) return fail(SB_ERR_ARG, "null argument");
   const int AP = pad_alphabet(A);
   if (A < 2 || AP < 0) return fail(SB_ERR_ALPHABET, "alphabet size %d outside 2..64", A);
   double psum = 0.0;
@@ -1407,7 +1410,7 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   }
   std::vector<float> recf(rec.begin(), rec.end());
 
-  Model m;
+  Model& m = out->m;
   m.alive = true;
   m.A = A;
   m.AP = AP;
@@ -1417,34 +1420,173 @@ int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double
   m.pi.assign(pi, pi + A);
   const size_t b64 = align_up(m64.size() * 8, 256), bf = align_up(recf.size() * 4, 256),
                bd = align_up(rec.size() * 8, 256);
-  std::vector<char> blob(b64 + bf + bd, 0);
+  std::vector<char>& blob = out->blob;
+  blob.assign(b64 + bf + bd, 0);
   std::memcpy(blob.data(), m64.data(), m64.size() * 8);
   std::memcpy(blob.data() + b64, recf.data(), recf.size() * 4);
   std::memcpy(blob.data() + b64 + bf, rec.data(), rec.size() * 8);
 
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  SB_HIP(hipSetDevice(ctx->device));
+  return SB_OK;
+}
+
+// Place a prepared model: arena record, staged upload, table entry (the
+// caller holds ctx->mu).
+int commit_model(sb_context* ctx, PreparedModel& pm, int* model_id) {
+  Model& m = pm.m;
   char* base = nullptr;
-  int rc = arena_alloc(ctx, b64 + bf + bd, &base);
+  int rc = arena_alloc(ctx, pm.blob.size(), &base);
   if (rc) return rc;
-  rc = stage_model_bytes(ctx, base, blob.data(), blob.size());
+  rc = stage_model_bytes(ctx, base, pm.blob.data(), pm.blob.size());
   if (rc) return rc;
+  const size_t b64 = align_up(sb_m64_size(m.AP) * 8, 256), bf = align_up(sb_mr_size(m.AP) * 4, 256);
   m.d64 = reinterpret_cast<double*>(base);
   m.rec[0] = base + b64;
   m.rec[1] = base + b64 + bf;
   m.base = base;
-  m.bytes = b64 + bf + bd;
+  m.bytes = pm.blob.size();
   int id;
   if (!ctx->free_ids.empty()) {
     id = ctx->free_ids.back();
     ctx->free_ids.pop_back();
-    ctx->models[id] = m;
+    ctx->models[id] = std::move(m);
   } else {
     id = int(ctx->models.size());
-    ctx->models.push_back(m);
+    ctx->models.push_back(std::move(m));
   }
   *model_id = id;
   return SB_OK;
+}
+
+int drop_model(sb_context* ctx, int model_id) {
+  Model* m = get_model(ctx, model_id);
+  if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
+  // the record is reused by a later model put once every launch that may
+  // read it has finished
+  if (m->base) {
+    ctx->retired[m->bytes].push_back({m->base, m->bytes, std::move(m->uses)});
+    ++ctx->n_retired;
+  }
+  m->uses.clear();
+  m->base = nullptr;
+  m->d64 = nullptr;
+  m->rec[0] = m->rec[1] = nullptr;
+  m->alive = false;
+  ctx->free_ids.push_back(model_id);
+  return SB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sb_last_error(void) { return g_err.c_str(); }
+
+// Internal hooks for the serving loop (sb_serve.cpp): set the calling
+// thread's error message; the device a context was created on.
+int sb_internal_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+int sb_internal_device(sb_context* ctx) { return ctx ? ctx->device : -1; }
+// Padded alphabet of an alphabet size (-1 outside 2..64): requests of one
+// padded alphabet (and reversibility) share a launch.
+int sb_internal_pad_alphabet(int A) { return A < 2 ? -1 : pad_alphabet(A); }
+// 1 for an irreversible model, 0 for a reversible one, -1 for an unknown id.
+int sb_internal_model_irr(sb_context* ctx, int model_id) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const Model* m = get_model(ctx, model_id);
+  return m ? (m->irr ? 1 : 0) : -1;
+}
+int sb_version(void) { return 1; }
+
+int sb_create(sb_context** out, int device) {
+  if (!out) return fail(SB_ERR_ARG, "null output pointer");
+  int n = 0;
+  SB_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(SB_ERR_ARG, "device %d not present (%d devices)", device, n);
+  SB_HIP(hipSetDevice(device));
+  sb_context* ctx = new sb_context();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  // model-upload events up front, so no call creates them later
+  for (int k = 0; k < sb_context::kPend; ++k)
+    SB_HIP(hipEventCreateWithFlags(&ctx->pend_ev[k], hipEventDisableTiming));
+  SB_HIP(hipEventCreateWithFlags(&ctx->model_ev, hipEventDisableTiming));
+  // the no-allocation entry point's launcher; if it cannot be set up, only
+  // that entry point fails (with this message)
+  if (noalloc_create(ctx) != SB_OK) ctx->noalloc_err = g_err;
+  *out = ctx;
+  return SB_OK;
+}
+
+int sb_destroy(sb_context* ctx) {
+  if (!ctx) return SB_OK;
+  (void)hipSetDevice(ctx->device);
+  batch_free(ctx->internal);
+  batch_free(ctx->noalloc);
+  if (ctx->plain.p) (void)hipFree(ctx->plain.p);
+  if (ctx->plain_bt.p) (void)hipFree(ctx->plain_bt.p);
+  (void)hipDeviceSynchronize();
+  for (void* c : ctx->arena_chunks) (void)hipFree(c);
+  for (int k = 0; k < sb_context::kPend; ++k) {
+    if (ctx->pend[k]) (void)hipHostFree(ctx->pend[k]);
+    if (ctx->pend_ev[k]) (void)hipEventDestroy(ctx->pend_ev[k]);
+    if (ctx->dstage[k]) (void)hipFree(ctx->dstage[k]);
+  }
+  if (ctx->model_ev) (void)hipEventDestroy(ctx->model_ev);
+  delete ctx;
+  return SB_OK;
+}
+
+int sb_model_put(sb_context* ctx, int A, const double* eigenvalues, const double* eigenvectors,
+                 const double* pi, int* model_id) {
+  if (!ctx || !model_id) return fail(SB_ERR_ARG, "null argument");
+  PreparedModel pm;
+  int rc = prepare_model(A, eigenvalues, eigenvectors, pi, &pm);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  return commit_model(ctx, pm, model_id);
+}
+
+// Batched puts and drops for the serving loop (one context lock per batch):
+// sb_internal_model_prepare builds a model off the lock (any thread) and
+// returns it as an opaque handle; sb_internal_model_commit places n of them
+// (consuming the handles, also on error); sb_internal_model_drop_many drops n ids.
+void* sb_internal_model_prepare(int A, const double* eigenvalues, const double* eigenvectors,
+                                const double* pi, int* rc) {
+  PreparedModel* pm = new PreparedModel();
+  *rc = prepare_model(A, eigenvalues, eigenvectors, pi, pm);
+  if (*rc) {
+    delete pm;
+    return nullptr;
+  }
+  return pm;
+}
+
+int sb_internal_model_commit(sb_context* ctx, int n, void* const* handles, int* ids) {
+  int rc = SB_OK;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (int i = 0; i < n; ++i) {
+      ids[i] = -1;
+      if (!rc && handles[i]) rc = commit_model(ctx, *static_cast<PreparedModel*>(handles[i]), &ids[i]);
+    }
+  }
+  for (int i = 0; i < n; ++i) delete static_cast<PreparedModel*>(handles[i]);
+  return rc;
+}
+
+int sb_internal_model_drop_many(sb_context* ctx, int n, const int* ids) {
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  int rc = SB_OK;
+  for (int i = 0; i < n; ++i) {
+    const int e = drop_model(ctx, ids[i]);
+    if (e && !rc) rc = e;
+  }
+  return rc;
 }
 
 int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues, const double* eigenvectors,
@@ -1570,18 +1712,7 @@ int sb_model_put_irrev(sb_context* ctx, int A, const double* eigenvalues, const 
 int sb_model_drop(sb_context* ctx, int model_id) {
   if (!ctx) return fail(SB_ERR_ARG, "null context");
   std::lock_guard<std::mutex> lock(ctx->mu);
-  Model* m = get_model(ctx, model_id);
-  if (!m) return fail(SB_ERR_MODEL, "unknown model id %d", model_id);
-  // the record is reused by a later sb_model_put once every launch that may
-  // read it has finished
-  if (m->base) ctx->retired.push_back({m->base, m->bytes, std::move(m->uses)});
-  m->uses.clear();
-  m->base = nullptr;
-  m->d64 = nullptr;
-  m->rec[0] = m->rec[1] = nullptr;
-  m->alive = false;
-  ctx->free_ids.push_back(model_id);
-  return SB_OK;
+  return drop_model(ctx, model_id);
 }
 
 int sb_arena_stats(sb_context* ctx, long long* reserved_bytes, long long* live_models,
@@ -1592,7 +1723,7 @@ int sb_arena_stats(sb_context* ctx, long long* reserved_bytes, long long* live_m
   for (const Model& m : ctx->models) n += m.alive ? 1 : 0;
   if (reserved_bytes) *reserved_bytes = (long long)ctx->arena_reserved;
   if (live_models) *live_models = n;
-  if (retired_records) *retired_records = int(ctx->retired.size());
+  if (retired_records) *retired_records = int(ctx->n_retired);
   return SB_OK;
 }
 

@@ -152,6 +152,12 @@ struct SbBlockDesc {
   int A, tile_base, ntiles, n_rate;
 };
 
+// One run of a model-record upload: `len` bytes from offset `src` of the
+// device staging copy to device address `dst`.
+struct SbScatterRun {
+  unsigned long long dst, src, len;
+};
+
 // Device memory of the no-allocation Counts entry point: one static array of
 // SB_NOALLOC_BYTES per GPU, split into the launch plan (header first), the
 // tile counters, the branch records and the per-wave scratch slots.

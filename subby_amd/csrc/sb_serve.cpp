@@ -39,6 +39,10 @@ extern "C" int sb_internal_error(int code, const char* msg);
 extern "C" int sb_internal_device(sb_context* ctx);
 extern "C" int sb_internal_model_irr(sb_context* ctx, int model_id);
 extern "C" int sb_internal_pad_alphabet(int A);
+extern "C" void* sb_internal_model_prepare(int A, const double* eigenvalues, const double* eigenvectors,
+                                           const double* pi, int* rc);
+extern "C" int sb_internal_model_commit(sb_context* ctx, int n, void* const* handles, int* ids);
+extern "C" int sb_internal_model_drop_many(sb_context* ctx, int n, const int* ids);
 
 namespace {
 
@@ -155,6 +159,7 @@ struct Item {
   size_t in_bytes = 0, out_bytes = 0;
   size_t in_off = 0, out_off = 0;   // within the lane buffers
   double* done_slot = nullptr;      // replay: completion time (seconds since server start)
+  sb_request_timing* tslot = nullptr;  // timed replay: where the latency went
 };
 
 struct Lane {
@@ -163,6 +168,11 @@ struct Lane {
   std::unique_ptr<Pool> pool;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
+  hipEvent_t tev[4] = {};           // timed batches: before upload, after upload, kernels, download
+  bool timed = false;               // this batch records tev
+  double t_admit = 0, t_enq = 0, t_gpu = 0, plan_ms = 0, models_ms = 0;   // seconds since server start; ms
+  long long cols = 0;
+  int index = 0;
   char* h_in = nullptr;
   char* d_in = nullptr;
   char* h_out = nullptr;
@@ -263,9 +273,30 @@ void release_models(sb_server* s, Lane& L) {
 void finish(sb_server* s, Lane& L) {
   release_models(s, L);
   const double now = seconds(s->t0, Clock::now());
+  float ms[3] = {0.f, 0.f, 0.f};
+  if (L.timed)
+    for (int k = 0; k < 3; ++k)
+      if (hipEventElapsedTime(&ms[k], L.tev[k], L.tev[k + 1]) != hipSuccess) {
+        (void)hipGetLastError();
+        ms[k] = -1.f;
+      }
   std::lock_guard<std::mutex> g(s->mu);
   for (Item& it : L.items) {
     if (it.done_slot) *it.done_slot = now;
+    if (sb_request_timing* t = it.tslot) {
+      t->admitted_s = L.t_admit;
+      t->enqueued_s = L.t_enq;
+      t->gpu_done_s = L.t_gpu;
+      t->done_s = now;
+      t->upload_ms = ms[0];
+      t->kernel_ms = ms[1];
+      t->download_ms = ms[2];
+      t->plan_ms = L.plan_ms;
+      t->models_ms = L.models_ms;
+      t->batch_columns = L.cols;
+      t->batch_requests = int(L.items.size());
+      t->lane = L.index;
+    }
     s->done.emplace_back(it.r.tag, now);
   }
   s->in_flight -= L.bytes;
@@ -284,6 +315,7 @@ int resolve_models(sb_server* s, Lane& L) {
   std::vector<std::vector<double>> keys(n);
   std::vector<uint64_t> hashes(n, 0);
   std::vector<int> miss, follow(n, -1);   // follow[k]: earlier miss of this batch with k's content
+  std::vector<int> evicted;
   L.pool->run(n, [&](int k) {
     const Item& it = L.items[k];
     if (it.r.model_id < 0) inline_key(it.r, &keys[k], &hashes[k]);
@@ -322,7 +354,7 @@ int resolve_models(sb_server* s, Lane& L) {
         ++q;
         continue;
       }
-      (void)sb_model_drop(s->ctx, q->id);
+      evicted.push_back(q->id);
       for (auto p = s->inline_index.equal_range(q->hash); p.first != p.second; ++p.first)
         if (p.first->second == q) {
           s->inline_index.erase(p.first);
@@ -331,17 +363,29 @@ int resolve_models(sb_server* s, Lane& L) {
       q = s->inline_models.erase(q);
     }
   }
+  // evicted records first, so that the new models can reuse them; host math
+  // of the new models on the pool, then one context lock for all of them
+  if (!evicted.empty()) (void)sb_internal_model_drop_many(s->ctx, int(evicted.size()), evicted.data());
   std::atomic<int> mrc{0};
+  std::vector<void*> handles(miss.size(), nullptr);
+  std::vector<int> ids(miss.size(), -1);
   L.pool->run(int(miss.size()), [&](int q) {
-    Item& it = L.items[miss[q]];
-    int id = -1;
-    const int e = sb_model_put(s->ctx, it.A, it.r.eigenvalues, it.r.eigenvectors, it.r.pi, &id);
+    const Item& it = L.items[miss[q]];
+    int e = 0;
+    handles[q] = sb_internal_model_prepare(it.A, it.r.eigenvalues, it.r.eigenvectors, it.r.pi, &e);
     if (e) {
       set_error(s, e, std::string("request model: ") + sb_last_error());
       mrc.store(e);
     }
-    it.r.model_id = id;
   });
+  if (!miss.empty()) {
+    const int e = sb_internal_model_commit(s->ctx, int(miss.size()), handles.data(), ids.data());
+    if (e) {
+      set_error(s, e, std::string("request model: ") + sb_last_error());
+      mrc.store(e);
+    }
+  }
+  for (size_t q = 0; q < miss.size(); ++q) L.items[miss[q]].r.model_id = ids[q];
   std::lock_guard<std::mutex> g(s->mmu);
   for (int k : miss) {
     Item& it = L.items[k];
@@ -371,8 +415,13 @@ void launch(sb_server* s, Lane& L, std::vector<Item>& items, size_t bytes) {
   L.bytes = bytes;
   L.busy = true;
   const int n = int(L.items.size());
+  L.timed = n > 0 && L.items[0].tslot != nullptr;
+  L.t_admit = seconds(s->t0, Clock::now());
+  L.t_enq = L.t_gpu = L.plan_ms = L.models_ms = 0;
+  L.cols = 0;
   size_t in_total = 0, out_total = 0;
   for (Item& it : L.items) {
+    L.cols += it.r.C;
     it.in_off = in_total;
     in_total += it.in_bytes;
     it.out_off = out_total;
@@ -384,14 +433,21 @@ void launch(sb_server* s, Lane& L, std::vector<Item>& items, size_t bytes) {
   // the others go to the resident arena now (their host math runs on the
   // pool; the context lock covers only the arena and the model table)
   std::vector<sb_block> blocks(n);
-  if (!rc) rc = resolve_models(s, L);
+  if (!rc) {
+    const Clock::time_point m0 = Clock::now();
+    rc = resolve_models(s, L);
+    L.models_ms = 1e3 * seconds(m0, Clock::now());
+  }
   if (!rc) {
     L.pool->run(n, [&](int k) {
       const Item& it = L.items[k];
       std::memcpy(L.h_in + it.in_off, it.r.alignment, size_t(it.r.R) * size_t(it.r.C));
     });
-    rc = hip_check(s, hipMemcpyAsync(L.d_in, L.h_in, in_total, hipMemcpyHostToDevice, L.stream),
-                   "upload");
+    if (L.timed) rc = hip_check(s, hipEventRecord(L.tev[0], L.stream), "timing event");
+    if (!rc)
+      rc = hip_check(s, hipMemcpyAsync(L.d_in, L.h_in, in_total, hipMemcpyHostToDevice, L.stream),
+                     "upload");
+    if (!rc && L.timed) rc = hip_check(s, hipEventRecord(L.tev[1], L.stream), "timing event");
   }
   if (!rc) {
     for (int k = 0; k < n; ++k) {
@@ -413,16 +469,22 @@ void launch(sb_server* s, Lane& L, std::vector<Item>& items, size_t bytes) {
       b.d_loglike = reinterpret_cast<double*>(base + ol);
       b.d_root = base + orr;
     }
+    const Clock::time_point p0 = Clock::now();
     const int src = sb_batch_submit(L.batch, n, blocks.data(), SB_F32, 1);
+    L.plan_ms = 1e3 * seconds(p0, Clock::now());
     if (src) {
       set_error(s, src, std::string("batch: ") + sb_last_error());
       rc = src;
     }
   }
+  if (!rc && L.timed) rc = hip_check(s, hipEventRecord(L.tev[2], L.stream), "timing event");
   if (!rc)
     rc = hip_check(s, hipMemcpyAsync(L.h_out, L.d_out, out_total, hipMemcpyDeviceToHost, L.stream),
                    "download");
+  if (!rc && L.timed) rc = hip_check(s, hipEventRecord(L.tev[3], L.stream), "timing event");
   if (!rc) rc = hip_check(s, hipEventRecord(L.ev, L.stream), "event");
+  L.t_enq = seconds(s->t0, Clock::now());
+  if (rc) L.timed = false;
   if (rc) {
     (void)hipStreamSynchronize(L.stream);
     finish(s, L);
@@ -490,6 +552,7 @@ void run_lane(sb_server* s, Lane& L) {
     for (;;) {
       const hipError_t q = hipEventQuery(L.ev);
       if (q == hipSuccess) {
+        L.t_gpu = seconds(s->t0, Clock::now());
         complete(s, L);
         break;
       }
@@ -528,7 +591,8 @@ int validate(const sb_request& r, int* A) {
   return SB_OK;
 }
 
-int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
+int push(sb_server* s, int n, const sb_request* reqs, double* done_slots,
+         sb_request_timing* tslots) {
   std::vector<Item> items(n);
   for (int i = 0; i < n; ++i) {
     Item& it = items[i];
@@ -549,6 +613,7 @@ int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
     size_t oc, ol, orr;
     it.out_bytes = out_layout(it, &oc, &ol, &orr);
     it.done_slot = done_slots ? done_slots + i : nullptr;
+    it.tslot = tslots ? tslots + i : nullptr;
   }
   {
     std::lock_guard<std::mutex> g(s->mu);
@@ -558,6 +623,49 @@ int push(sb_server* s, int n, const sb_request* reqs, double* done_slots) {
   }
   s->cv.notify_all();
   return SB_OK;
+}
+
+// Submit request i at arrival_s[i] seconds after the call, wait for all;
+// completion times (and, with `timing`, the breakdown) relative to the call.
+int replay(sb_server* s, int n, sb_request* requests, const double* arrival_s, double* done_s,
+           sb_request_timing* timing) {
+  for (int i = 0; i < n; ++i) requests[i].tag = i;
+  if (timing) std::memset(timing, 0, sizeof(sb_request_timing) * size_t(n));
+  const Clock::time_point start = Clock::now();
+  const double base = seconds(s->t0, start);
+  int i = 0;
+  while (i < n) {
+    const double now = seconds(start, Clock::now());
+    if (arrival_s[i] > now) {
+      const double wait = arrival_s[i] - now;
+      if (wait > 3e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 2e-4));
+      else std::this_thread::yield();
+      continue;
+    }
+    int j = i;
+    while (j < n && arrival_s[j] <= now) ++j;   // everything that has arrived
+    const int rc = push(s, j - i, requests + i, done_s ? done_s + i : nullptr,
+                        timing ? timing + i : nullptr);
+    if (rc) return rc;
+    i = j;
+  }
+  const int rc = sb_server_drain(s);
+  for (int k = 0; k < n; ++k) {
+    if (done_s) done_s[k] -= base;
+    if (timing) {
+      sb_request_timing& t = timing[k];
+      t.admitted_s -= base;
+      t.enqueued_s -= base;
+      t.gpu_done_s -= base;
+      t.done_s -= base;
+    }
+  }
+  {
+    // replay completions are reported through done_s / timing, not poll
+    std::lock_guard<std::mutex> g(s->mu);
+    s->done.clear();
+  }
+  return rc;
 }
 
 }  // namespace
@@ -581,8 +689,11 @@ int sb_server_create(sb_context* ctx, int lanes, long long hbm_budget, int max_b
   }
   s->lanes.resize(lanes);
   for (Lane& L : s->lanes) {
-    if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) != hipSuccess) {
+    L.index = int(&L - s->lanes.data());
+    bool ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) == hipSuccess;
+    for (hipEvent_t& e : L.tev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    if (!ok) {
       sb_server_destroy(s);
       return sb_internal_error(SB_ERR_HIP, "sb_server_create: stream/event creation failed");
     }
@@ -622,6 +733,8 @@ int sb_server_destroy(sb_server* s) {
     if (L.d_in) (void)hipFree(L.d_in);
     if (L.d_out) (void)hipFree(L.d_out);
     if (L.ev) (void)hipEventDestroy(L.ev);
+    for (hipEvent_t e : L.tev)
+      if (e) (void)hipEventDestroy(e);
     if (L.stream) (void)hipStreamDestroy(L.stream);
   }
   for (const InlineModel& im : s->inline_models) (void)sb_model_drop(s->ctx, im.id);
@@ -633,7 +746,7 @@ int sb_server_destroy(sb_server* s) {
 
 int sb_server_submit(sb_server* s, int n, const sb_request* requests) {
   if (!s || n < 0 || (n > 0 && !requests)) return sb_internal_error(SB_ERR_ARG, "sb_server_submit: bad argument");
-  return push(s, n, requests, nullptr);
+  return push(s, n, requests, nullptr, nullptr);
 }
 
 int sb_server_poll(sb_server* s, int max, long long* tags, double* done_s, int* n_done) {
@@ -663,32 +776,14 @@ int sb_server_replay(sb_server* s, int n, sb_request* requests, const double* ar
                      double* done_s) {
   if (!s || n < 0 || (n > 0 && (!requests || !arrival_s || !done_s)))
     return sb_internal_error(SB_ERR_ARG, "sb_server_replay: bad argument");
-  for (int i = 0; i < n; ++i) requests[i].tag = i;
-  const Clock::time_point start = Clock::now();
-  const double base = seconds(s->t0, start);
-  int i = 0;
-  while (i < n) {
-    const double now = seconds(start, Clock::now());
-    if (arrival_s[i] > now) {
-      const double wait = arrival_s[i] - now;
-      if (wait > 3e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 2e-4));
-      else std::this_thread::yield();
-      continue;
-    }
-    int j = i;
-    while (j < n && arrival_s[j] <= now) ++j;   // everything that has arrived
-    const int rc = push(s, j - i, requests + i, done_s + i);
-    if (rc) return rc;
-    i = j;
-  }
-  const int rc = sb_server_drain(s);
-  for (int k = 0; k < n; ++k) done_s[k] -= base;
-  {
-    // replay completions are reported through done_s, not poll
-    std::lock_guard<std::mutex> g(s->mu);
-    s->done.clear();
-  }
-  return rc;
+  return replay(s, n, requests, arrival_s, done_s, nullptr);
+}
+
+int sb_server_replay_timed(sb_server* s, int n, sb_request* requests, const double* arrival_s,
+                           sb_request_timing* timing) {
+  if (!s || n < 0 || (n > 0 && (!requests || !arrival_s || !timing)))
+    return sb_internal_error(SB_ERR_ARG, "sb_server_replay_timed: bad argument");
+  return replay(s, n, requests, arrival_s, nullptr, timing);
 }
 
 }  // extern "C"

@@ -266,8 +266,8 @@ class NativeServer:
     inline (uploaded into the resident arena when admitted).  FP32 outputs.
     """
 
-    def __init__(self, engine: Engine | None = None, hbm_budget: int = 8 << 30, lanes: int = 3,
-                 max_blocks: int = 2048, max_cols: int = 1 << 19):
+    def __init__(self, engine: Engine | None = None, hbm_budget: int = 8 << 30, lanes: int = 4,
+                 max_blocks: int = 2048, max_cols: int = 1 << 17):
         self.eng = engine or default_engine()
         self.lib = self.eng.lib
         h = ctypes.c_void_p()
@@ -324,14 +324,23 @@ class NativeServer:
             outs.append((cnt, ll, root))
         return arr, keep, outs
 
-    def run(self, requests, realtime: bool = True):
-        """Serve ``requests`` (sorted by arrival).  Returns (results, stats) like Server.run."""
+    def run(self, requests, realtime: bool = True, timed: bool = False):
+        """Serve ``requests`` (sorted by arrival).  Returns (results, stats) like
+        Server.run; with ``timed`` the stats also hold ``breakdown`` (where the
+        latency went, see `breakdown`) and ``timing`` (the per-request records)."""
         n = len(requests)
         arr, keep, outs = self.prepare(requests)
         arrival = np.array([r.arrival if realtime else 0.0 for r in requests], dtype=np.float64)
-        done = np.zeros(n, dtype=np.float64)
-        _lib.check(self.lib.sb_server_replay(self.h, n, ctypes.cast(arr, ctypes.c_void_p),
-                                             arrival.ctypes.data, done.ctypes.data))
+        timing = None
+        if timed:
+            timing = np.zeros(n, dtype=_lib.TIMING_DTYPE)
+            _lib.check(self.lib.sb_server_replay_timed(self.h, n, ctypes.cast(arr, ctypes.c_void_p),
+                                                       arrival.ctypes.data, timing.ctypes.data))
+            done = timing['done_s'].copy()
+        else:
+            done = np.zeros(n, dtype=np.float64)
+            _lib.check(self.lib.sb_server_replay(self.h, n, ctypes.cast(arr, ctypes.c_void_p),
+                                                 arrival.ctypes.data, done.ctypes.data))
         del keep
         results = [Result(r.rid, cnt, ll, root, float(arrival[k]), float(done[k]))
                    for k, (r, (cnt, ll, root)) in enumerate(zip(requests, outs))]
@@ -342,8 +351,48 @@ class NativeServer:
                  'throughput': units / makespan if makespan > 0 else 0.0,
                  'p50_ms': float(np.percentile(lat, 50) * 1e3), 'p99_ms': float(np.percentile(lat, 99) * 1e3),
                  'mean_ms': float(lat.mean() * 1e3)}
+        if timed and n:
+            stats['breakdown'] = breakdown(timing, arrival)
+            stats['timing'] = timing
         results.sort(key=lambda r: r.rid)
         return results, stats
+
+
+def breakdown(timing, arrival):
+    """Split each request's latency (arrival -> results in the caller's arrays)
+    into consecutive host-clock spans:
+
+      queue      arrival -> a lane admits it (waiting for a free lane / budget)
+      models     resolving the batch's inline models (sharing by content,
+                 evictions, puts into the resident arena)
+      pack       admission -> its batch is enqueued, less ``models`` and
+                 ``plan`` (alignment copies into pinned staging, upload)
+      plan       host time of the batch submission (tree planning, launch)
+      gpu        enqueued -> the lane sees the batch's download complete
+      unpack     results copied from pinned staging into the caller's arrays
+
+    plus the batch's GPU spans from HIP events (upload, kernel, download) and its
+    size.  Returns the mean of every field over all requests, over the median
+    band (45th-55th percentile of latency) and over the slowest 1%."""
+    t = timing
+    lat = (t['done_s'] - arrival) * 1e3
+    f = {
+        'latency_ms': lat,
+        'queue_ms': (t['admitted_s'] - arrival) * 1e3,
+        'models_ms': t['models_ms'],
+        'pack_ms': (t['enqueued_s'] - t['admitted_s']) * 1e3 - t['plan_ms'] - t['models_ms'],
+        'plan_ms': t['plan_ms'],
+        'gpu_ms': (t['gpu_done_s'] - t['enqueued_s']) * 1e3,
+        'unpack_ms': (t['done_s'] - t['gpu_done_s']) * 1e3,
+        'gpu_upload_ms': t['upload_ms'],
+        'gpu_kernel_ms': t['kernel_ms'],
+        'gpu_download_ms': t['download_ms'],
+        'batch_requests': t['batch_requests'].astype(np.float64),
+        'batch_columns': t['batch_columns'].astype(np.float64),
+    }
+    lo, mid, hi = np.percentile(lat, [45, 55, 99])
+    bands = {'all': np.ones(len(lat), bool), 'p45_p55': (lat >= lo) & (lat <= mid), 'slowest_1pct': lat >= hi}
+    return {b: {k: float(v[m].mean()) if m.any() else None for k, v in f.items()} for b, m in bands.items()}
 
 
 _PADDED = (4, 8, 16, 20, 32, 64)
@@ -357,36 +406,57 @@ def _kind(r):
     return next(p for p in _PADDED if A <= p), isinstance(m, IrrevDiagModel)
 
 
+def _simulate(args):
+    """Requests ``lo .. hi-1`` of a stream (each from its own seeded generator)."""
+    seed, lo, hi, leaves, mu, sd, clip, shapes = args
+    from . import synthetic
+    from .models import hky85_diag
+    out = []
+    for i in range(lo, hi):
+        rng = np.random.default_rng([seed, i])
+        m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+        if shapes is not None:
+            out.append((int(rng.integers(shapes)), m))
+            continue
+        nl = int(rng.integers(leaves[0], leaves[1] + 1))
+        C = int(np.clip(round(rng.lognormal(mu, sd)), clip[0], clip[1]))
+        tr = synthetic.random_tree(nl, rng, 0.1)
+        aln = synthetic.simulate(tr, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
+        out.append((tr, m, aln))
+    return out
+
+
 def make_stream(n, qps, seed=4, leaves=(4, 100), mean_cols=500.0, sd=1.0, clip=(20, 10000),
-                pool=None):
+                pool=None, workers=None):
     """Synthetic request stream: Poisson arrivals at ``qps``; each request has a
     random tree (leaves uniform in ``leaves``), LogNormal block length with
     arithmetic mean ``mean_cols`` (log-sd ``sd``) clipped to ``clip``, and its own
-    HKY85 model with kappa ~ U(1,5), pi ~ Dirichlet(5,5,5,5).  With ``pool`` set,
-    only that many (tree, alignment) pairs are simulated and requests draw from
-    them (each request still gets its own model), to keep generation fast."""
-    from . import synthetic
-    from .models import hky85_diag
-    rng = np.random.default_rng(seed)
+    HKY85 model with kappa ~ U(1,5), pi ~ Dirichlet(5,5,5,5).  Request i is
+    simulated from its own generator (seeded by (seed, i)), so every request is
+    distinct and the stream does not depend on ``workers``: with ``workers`` > 1
+    the requests are simulated by that many forked processes (call it before
+    anything initialises the GPU).  With ``pool`` set, only that many (tree,
+    alignment) pairs are simulated and requests draw from them (each request
+    still gets its own model), to keep generation fast in tests."""
     mu = np.log(mean_cols) - 0.5 * sd * sd
-
-    def one():
-        nl = int(rng.integers(leaves[0], leaves[1] + 1))
-        C = int(np.clip(round(rng.lognormal(mu, sd)), clip[0], clip[1]))
-        m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
-        tr = synthetic.random_tree(nl, rng, 0.1)
-        aln = synthetic.simulate(tr, m, C, rng, gap_frac=0.03, ambig_frac=0.01)
-        return tr, m, aln
-
-    shapes = [one() for _ in range(pool)] if pool else None
-    t = 0.0
+    shapes = None
+    if pool:
+        shapes = [(tr, aln) for tr, _, aln in _simulate((seed + 1, 0, pool, leaves, mu, sd, clip, None))]
+    args = (seed, leaves, mu, sd, clip, len(shapes) if shapes else None)
+    if workers and workers > 1 and n > workers:
+        import multiprocessing as mp
+        step = -(-n // (4 * workers))
+        parts = [(args[0], lo, min(n, lo + step)) + args[1:] for lo in range(0, n, step)]
+        with mp.get_context('fork').Pool(workers) as p:
+            sims = [x for part in p.map(_simulate, parts) for x in part]
+    else:
+        sims = _simulate((args[0], 0, n) + args[1:])
+    arrivals = np.cumsum(np.random.default_rng([seed, n]).exponential(1.0 / qps, size=n))
     out = []
-    for i in range(n):
-        t += rng.exponential(1.0 / qps)
-        if shapes:
-            tr, _, aln = shapes[int(rng.integers(len(shapes)))]
-            m = hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0)))
+    for i, x in enumerate(sims):
+        if shapes is not None:
+            (tr, aln), m = shapes[x[0]], x[1]
         else:
-            tr, m, aln = one()
-        out.append(Request(aln, tr, m, arrival=t, rid=i))
+            tr, m, aln = x
+        out.append(Request(aln, tr, m, arrival=float(arrivals[i]), rid=i))
     return out

@@ -562,3 +562,46 @@ def test_gradient_config_fd(sb, eng):
     richk = (4 * fd_k(h / 2, ok) - fd_k(h, ok)) / 3
     assert abs(float(sr64[0]) - richk) <= 1e-4 * abs(richk)
     assert abs(float(sr32[0]) - richk) <= 1e-4 * abs(richk)
+
+
+def test_models_uploaded_into_reused_records(sb):
+    """Many new models whose arena records are scattered over reused ones go up
+    in one staged copy plus a scatter; every model must arrive intact (results
+    bitwise those of a fresh engine, a sample against the oracle)."""
+    from subby_amd.batch import Launcher, Block
+    say('model upload into scattered reused records')
+    rng = np.random.default_rng(21)
+    t = sb.synthetic.random_tree(20, rng, 0.1)
+    base = sb.hky85_diag(2.0, np.full(4, 0.25))
+    aln = sb.synthetic.simulate(t, base, 300, rng, gap_frac=0.03)
+
+    def models(n):
+        return [sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0))) for _ in range(n)]
+
+    def run(e, ms):
+        bl = [Block(aln, t, m) for m in ms]
+        L = Launcher(e)
+        L.submit(bl)
+        L.sync()
+        return bl
+
+    eng = sb.Engine(0, max_models=24)
+    first = models(24)
+    run(eng, first)
+    run(eng, first[1::2])            # the odd ones become the recently used
+    fresh = models(12)               # evicts the even ones: 12 scattered records
+    got = run(eng, fresh)
+    r0, n0, q0 = eng.arena_stats()
+    ref_eng = sb.Engine(0, max_models=None)
+    ref = run(ref_eng, fresh)
+    for a, b in zip(got, ref):
+        assert torch.equal(a.counts, b.counts) and torch.equal(a.loglike, b.loglike)
+        assert torch.equal(a.root, b.root)
+    for b in got[:3]:
+        rc, rl, _ = O.counts_loglike_root(b.alignment, otree(b.tree), omodel(b.model))
+        assert err_counts(b.counts, rc) <= 1e-4 and err_ll(b.loglike, rl) <= 1e-5
+    # the new models took over the dropped records: the arena did not grow
+    run(eng, models(12))
+    assert eng.arena_stats()[0] == r0
+    ref_eng.close()
+    eng.close()

@@ -82,6 +82,45 @@ def test_native_served_equals_alone(eng):
     _check_alone(sb, eng, reqs, results)
 
 
+def test_native_timed_replay(eng):
+    """sb_server_replay_timed: same results as the untimed replay (bitwise), and
+    a consistent per-request breakdown (ordered host times, GPU spans, batch
+    sizes) whose spans add up to the latency."""
+    from subby_amd.serve import NativeServer, make_stream
+    say('native serve: timed replay, 300 requests')
+    reqs = make_stream(300, qps=3000.0, seed=15, leaves=(4, 100), mean_cols=300)
+    srv = NativeServer(eng, hbm_budget=1 << 30, lanes=3)
+    try:
+        plain, _ = srv.run(reqs)
+        timed, stats = srv.run(reqs, timed=True)
+    finally:
+        srv.close()
+    for a, b in zip(plain, timed):
+        assert np.array_equal(a.counts, b.counts) and np.array_equal(a.loglike, b.loglike)
+        assert np.array_equal(a.root, b.root)
+    t = stats['timing']
+    arrival = np.array([r.arrival for r in reqs])
+    eps = 1e-6
+    assert np.all(t['admitted_s'] >= arrival - eps)
+    assert np.all(t['enqueued_s'] >= t['admitted_s'] - eps)
+    assert np.all(t['gpu_done_s'] >= t['enqueued_s'] - eps)
+    assert np.all(t['done_s'] >= t['gpu_done_s'] - eps)
+    for k in ('upload_ms', 'kernel_ms', 'download_ms', 'plan_ms', 'models_ms'):
+        assert np.all(t[k] >= 0) and np.all(np.isfinite(t[k])), k
+    assert np.all(t['kernel_ms'] > 0)
+    assert np.all((t['lane'] >= 0) & (t['lane'] < 3))
+    assert np.all(t['batch_requests'] >= 1)
+    # requests of one batch share its record; the batches' sizes add up
+    key = np.stack([t['lane'], np.round(t['enqueued_s'] * 1e7).astype(np.int64)], 1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    assert np.array_equal(cnt[inv.ravel()], t['batch_requests'])
+    bd = stats['breakdown']['all']
+    spans = bd['queue_ms'] + bd['models_ms'] + bd['pack_ms'] + bd['plan_ms'] + bd['gpu_ms'] + bd['unpack_ms']
+    assert abs(spans - bd['latency_ms']) <= 1e-6 * max(1.0, bd['latency_ms'])
+    assert abs(bd['latency_ms'] - stats['mean_ms']) <= 1e-6 * max(1.0, stats['mean_ms'])
+    say('   breakdown', stats['breakdown'])
+
+
 def test_native_tight_budget_burst_and_poll(eng):
     """Burst under a budget that admits few requests at a time; then the
     submit / drain / poll interface with caller tags."""
