@@ -251,15 +251,12 @@ int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
  * budget only, scripts/batch_eigensub_maf.py:296-304):
  *   SB_TUNE_WAVES_PER_GROUP  1, 2 or 4 (small-alphabet kernels)
  *   SB_TUNE_WAVES_PER_CU     0..32, 0 = from the kernel's occupancy
- *   SB_TUNE_STAGE_TOKENS     1 = the small-alphabet walk copies each tile's
- *                            tokens to LDS first, 0 = reads them from HBM as
- *                            it goes (default)
  *   SB_TUNE_GEN2_ROUND       two-pass path (general models above 8 states,
  *                            FP32): 64-column tiles staged per round, 0 = as
  *                            many as an 8 GiB stage holds (default); smaller
  *                            rounds keep the stage in cache
  * Results do not depend on any knob (bitwise). */
-enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_STAGE_TOKENS = 2, SB_TUNE_GEN2_ROUND = 3 };
+enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_GEN2_ROUND = 3 };
 int sb_batch_set_tune(sb_batch* b, int knob, int value);
 int sb_batch_get_tune(sb_batch* b, int knob, int* value);
 /* Block until the launcher's last submission has finished. */

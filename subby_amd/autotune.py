@@ -5,8 +5,7 @@ batch runs on (HIP events around the main kernels on the launcher's stream),
 keeps the fastest, and applies it to the launcher.  The search space per
 family:
 
-  small  (padded alphabet <= 8): LDS token stage on/off x waves per workgroup
-         x resident waves per CU;
+  small  (padded alphabet <= 8): waves per workgroup x resident waves per CU;
   gen2   (general models above 8 states, FP32 Counts / BranchCounts): 64-column
          tiles per round of the two-pass path (the stage footprint: small rounds
          keep it in cache, large ones expose more parallelism);
@@ -35,8 +34,8 @@ WAVES_PER_CU = (0, 8, 12, 16, 20, 24)          # 0 = from the kernel's occupancy
 GEN2_ROUNDS = (0, 4, 8, 16, 32, 64, 128)        # 0 = as many tiles as the stage budget holds
 LARGE_WAVES_PER_CU = (0, 1, 2, 4, 8)
 
-KNOBS = {'stage_tokens': _lib.SB_TUNE_STAGE_TOKENS, 'waves_per_group': _lib.SB_TUNE_WAVES_PER_GROUP,
-         'waves_per_cu': _lib.SB_TUNE_WAVES_PER_CU, 'gen2_round': _lib.SB_TUNE_GEN2_ROUND}
+KNOBS = {'waves_per_group': _lib.SB_TUNE_WAVES_PER_GROUP, 'waves_per_cu': _lib.SB_TUNE_WAVES_PER_CU,
+         'gen2_round': _lib.SB_TUNE_GEN2_ROUND}
 
 
 def family(blocks, precision='fp32'):
@@ -53,8 +52,8 @@ def family(blocks, precision='fp32'):
 
 def candidates(fam):
     if fam == 'small':
-        return [dict(stage_tokens=s, waves_per_group=g, waves_per_cu=c)
-                for s, g, c in itertools.product((1, 0), WAVES_PER_GROUP, WAVES_PER_CU)]
+        return [dict(waves_per_group=g, waves_per_cu=c)
+                for g, c in itertools.product(WAVES_PER_GROUP, WAVES_PER_CU)]
     if fam == 'gen2':
         return [dict(gen2_round=r) for r in GEN2_ROUNDS]
     return [dict(waves_per_cu=c) for c in LARGE_WAVES_PER_CU]

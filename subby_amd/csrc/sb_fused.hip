@@ -119,32 +119,16 @@ struct TokDec {
   }
 };
 
-// Token of node n for this lane's column: from the wave's LDS stage when the
-// launch stages tokens (`staged`, wave-uniform), else from the alignment.
+// Token of node n for this lane's column, as an unsigned byte: a gap token
+// -1 reads as 255, which every test below treats as a gap (not observed,
+// not ungapped).  Reading bytes unsigned leaves nothing to do between the
+// load and the use, so a token fetched a record ahead is not waited for at
+// the end of the record that fetched it.
 struct TokSrc {
-  const int8_t* tl;                 // this lane's column in the wave's LDS stage (row stride 64)
-  const SB_GLOBAL int8_t* aln;      // this lane's column of the alignment (row stride C)
+  const SB_GLOBAL uint8_t* aln;     // this lane's column of the alignment (row stride C)
   long long C;
-  bool staged;
 };
-__device__ inline int tok_at(const TokSrc& ts, int n) {
-  if (ts.staged) return int(ts.tl[n * SB_WAVE]);
-  return int(ts.aln[(long long)n * ts.C]);
-}
-
-// Copy the tile's tokens (R rows x 64 columns, one byte each) into the wave's
-// LDS stage, 32 rows of loads in flight at a time, then their LDS writes.  The
-// stage holds a multiple of 32 rows >= R, so the last batch reads row R - 1
-// again for the rows past R instead of branching.
-__device__ inline void stage_tokens(int8_t* tl, const SB_GLOBAL int8_t* aln, long long C, int R, int lane) {
-  for (int r0 = 0; r0 < R; r0 += 32) {
-    int v[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) v[k] = int(aln[(long long)min(r0 + k, R - 1) * C]);
-#pragma unroll
-    for (int k = 0; k < 32; ++k) tl[(r0 + k) * SB_WAVE + lane] = int8_t(v[k]);
-  }
-}
+__device__ inline int tok_at(const TokSrc& ts, int n) { return int(ts.aln[(long long)n * ts.C]); }
 
 // w = M u  (w_a = sum_j M[a][j] u_j)
 template <typename T, int AP>
@@ -217,10 +201,12 @@ template <typename T, int AP>
 struct Fetch {
   T ul[AP], ur[AP];
   int wl, wr;
-  int tok;   // tokens: left child (bits 0-7), right child (8-15), parent (16-23)
-  __device__ inline int tl() const { return int(int8_t(tok)); }
-  __device__ inline int tr() const { return int(int8_t(tok >> 8)); }
-  __device__ inline int tp() const { return int(int8_t(tok >> 16)); }
+  // tokens of the left child, right child and parent, each in its own
+  // register: combining them at fetch time would wait for the loads there
+  int tkl, tkr, tkp;
+  __device__ inline int tl() const { return tkl; }
+  __device__ inline int tr() const { return tkr; }
+  __device__ inline int tp() const { return tkp; }
 };
 
 template <typename T, int AP>
@@ -235,7 +221,9 @@ __device__ inline void fetch_children(Fetch<T, AP>& f, const InRec& q, int fwd_s
                                       const SB_GLOBAL char* Ureg, const SB_GLOBAL int* Uw, int lane) {
   if constexpr (LEFT) fetch_vec<T, AP>(f.ul, f.wl, (q.sl < 0 || q.sl == fwd_slot) ? 0 : q.sl, Ureg, Uw, lane);
   fetch_vec<T, AP>(f.ur, f.wr, (q.sr < 0 || q.sr == fwd_slot) ? 0 : q.sr, Ureg, Uw, lane);
-  f.tok = (tok_at(ts, q.l) & 0xff) | ((tok_at(ts, q.r) & 0xff) << 8) | ((tok_at(ts, q.p) & 0xff) << 16);
+  f.tkl = tok_at(ts, q.l);
+  f.tkr = tok_at(ts, q.r);
+  f.tkp = tok_at(ts, q.p);
 }
 
 // A child from fetched data, by selects (slot < 0: leaf from its token; fwd:
@@ -251,7 +239,7 @@ __device__ inline void pick_child(int slot, bool fwd, int tok, const T* su, int 
   for (int a = 0; a < AP; ++a) ch.u[a] = leaf ? tv[a] : (fwd ? fu[a] : su[a]);
   const int w = fwd ? fw : sw;
   ch.e = leaf ? 0 : (w >> 2);
-  ch.has = leaf ? ((tok >= 0) && (tok <= A)) : ((w & 1) != 0);
+  ch.has = leaf ? (unsigned(tok) <= unsigned(A)) : ((w & 1) != 0);
   ch.branch = !leaf && ((w & 2) != 0);
   ch.leaf = leaf;
 }
@@ -422,9 +410,6 @@ __global__ void __launch_bounds__(256) sb_branch_params_irrev_kernel(const SbLau
   }
 }
 
-// LDS token stage of the small-alphabet walk: tok_rows x 64 bytes per wave.
-extern __shared__ int8_t sb_tok_lds[];
-
 __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
   if (threadIdx.x < SB_NCOUNTER && blockIdx.x == 0) L->counter[threadIdx.x] = 0;
 }
@@ -475,14 +460,9 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
 #pragma unroll
     for (int a = 0; a < AP; ++a) pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
     const int BTS = sb_bt_stride_k(AP, IRR);
-    // the tile's tokens go to LDS once, so no token read of the walk waits on HBM
     TokSrc ts;
-    ts.staged = args.tok_rows > 0 && nint > 0;
-    ts.aln = aln;
+    ts.aln = (const SB_GLOBAL uint8_t*)aln;
     ts.C = C;
-    int8_t* const stage = sb_tok_lds + __builtin_amdgcn_readfirstlane(threadIdx.x / SB_WAVE) * args.tok_rows * SB_WAVE;
-    ts.tl = stage + lane;
-    if (ts.staged) stage_tokens(stage, aln, C, B->R, lane);
 
     const bool outside = (flags & (SB_F_COUNTS | SB_F_GRAD)) != 0;
 
@@ -501,7 +481,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     bool root_branch = false;
     TokDec<T, AP> td;
     td.init(A);
-    if (nint == 0) td.decode(aln[0], u0);  // single-node tree
+    if (nint == 0) td.decode(tok_at(ts, 0), u0);  // single-node tree
     // (a tree of one node has no records: nothing is fetched)
     InRec q{0, 0, 0, -1, -1};
     Fetch<T, AP> cur;
@@ -587,9 +567,9 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     // forwarded in registers and never stored (an internal left child always,
     // a right child when the left one is a leaf).
     InRec oq{0, 0, 0, -1, -1};
-    Fetch<T, AP> ocur;
-    T cd[AP];                      // staged outside vector of the record's node and its word
-    int cdw = 0;
+    Fetch<T, AP> ocur, onxt;
+    T cd[AP], nd[AP];              // staged outside vector of the record's node and its word
+    int cdw = 0, ndw = 0;
     if (nint > 0) {
       oq = in_rec(sched, 0);
       fetch_children<T, AP>(ocur, oq, -2, ts, Ureg, Uw, lane);
@@ -600,30 +580,31 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     bool pf = false;               // this record's node is a child of the last record
 #pragma unroll
     for (int a = 0; a < AP; ++a) fd[a] = T(0);
-    for (int i = 0; i < nint; ++i) {
+    // The loop is unrolled by two over ping-pong buffers (ocur / onxt, cd / nd): the record
+    // computed reads one buffer while the next record's fetches land in the
+    // other, so no loaded value is copied (a copy would wait for the load).
+    auto ostep = [&](int i, const Fetch<T, AP>& cur, const T* cdv, int cdwv, Fetch<T, AP>& nxt, T* ndv,
+                     int& ndwv) __attribute__((always_inline)) {
       const int inext = i + 1 < nint ? i + 1 : i;
       const InRec oqn = in_rec(sched, inext);
-      Fetch<T, AP> onxt;
-      T nd[AP];
-      int ndw;
       const bool last = i + 1 >= nint;
       const bool fwd0 = !last && oqn.p == oq.l;
       const bool fwd1 = !last && oqn.p == oq.r;
-      fetch_children<T, AP>(onxt, oqn, -2, ts, Ureg, Uw, lane);
-      fetch_vec<T, AP>(nd, ndw, (fwd0 || fwd1) ? 0 : inext, Dreg, Dw, lane);
+      fetch_children<T, AP>(nxt, oqn, -2, ts, Ureg, Uw, lane);
+      fetch_vec<T, AP>(ndv, ndwv, (fwd0 || fwd1) ? 0 : inext, Dreg, Dw, lane);
       const int p = oq.p, l = oq.l, r = oq.r, sl = oq.sl, sr = oq.sr;
       // the node's own outside vector: staged, or forwarded by its parent's record
       T rd[AP];
       int rdw;
       {
 #pragma unroll
-        for (int a = 0; a < AP; ++a) rd[a] = pf ? fd[a] : cd[a];
-        rdw = pf ? fdw : cdw;
+        for (int a = 0; a < AP; ++a) rd[a] = pf ? fd[a] : cdv[a];
+        rdw = pf ? fdw : cdwv;
       }
       T x[AP];
       int ex;
       bool stp;
-      td.decode(ocur.tp(), x);
+      td.decode(cur.tp(), x);
       if (i == 0) {
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= pi[a];
@@ -638,8 +619,8 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         for (int a = 0; a < AP; ++a) x[a] *= xm[a];
       }
       Child<T, AP> ch[2];
-      pick_child<T, AP>(sl, false, ocur.tl(), ocur.ul, ocur.wl, ocur.ul, 0, td, ch[0]);
-      pick_child<T, AP>(sr, false, ocur.tr(), ocur.ur, ocur.wr, ocur.ur, 0, td, ch[1]);
+      pick_child<T, AP>(sl, false, cur.tl(), cur.ul, cur.wl, cur.ul, 0, td, ch[0]);
+      pick_child<T, AP>(sr, false, cur.tr(), cur.ur, cur.wr, cur.ur, 0, td, ch[1]);
       T w[2][AP];
       matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
       matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
@@ -872,10 +853,11 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       }
       pf = fwd0 || fwd1;
       oq = oqn;
-      ocur = onxt;
-#pragma unroll
-      for (int a = 0; a < AP; ++a) cd[a] = nd[a];
-      cdw = ndw;
+    };
+    for (int i = 0; i < nint; i += 2) {
+      ostep(i, ocur, cd, cdw, onxt, nd, ndw);
+      if (i + 1 >= nint) break;
+      ostep(i + 1, onxt, nd, ndw, ocur, cd, cdw);
     }
 
     // ---------------- epilogue: natural-basis counts ----------------

@@ -220,7 +220,6 @@ struct sb_batch {
   bool staged_pending = false;
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
-  bool stage_tokens = false;        // small-alphabet walk copies tile tokens to LDS first
   int gen2_round = 0;               // two-pass path: tiles per round (0: as the stage budget allows)
   // fixed buffers (the no-allocation launcher): never grown, never freed here
   bool fixed = false;
@@ -753,15 +752,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   // persistent grid: resident waves per CU from the kernel's occupancy unless
   // the launcher was given an explicit value; never more waves than tiles
   const int wpg = (large || post) ? 1 : b->waves_per_group;
-  // small-alphabet fused kernel: each wave stages its tile's tokens (R_max
-  // rows, rounded up to 32 as the kernel copies 32 rows at a time, of 64
-  // bytes) in LDS when a workgroup's stage fits 64 KiB
-  int tok_rows = 0;
-  if (!large && !post && b->stage_tokens) {
-    const int rows = (R_max + 31) / 32 * 32;
-    if (size_t(rows) * SB_WAVE * size_t(wpg) <= (64u << 10)) tok_rows = rows;
-  }
-  const int lds_tok = tok_rows * SB_WAVE * wpg;
+  const int lds_tok = 0;   // (no dynamic LDS in the small-alphabet kernels)
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
     const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
@@ -921,7 +912,6 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   L->grad_stride = grad_stride;
   L->AP = AP;
   L->tile_cols = tile_cols;
-  L->tok_rows = tok_rows;
 
   // every model this launch reads is fenced by its submission number
   const uint64_t seq = b->seq + 1;
@@ -1946,10 +1936,6 @@ int sb_batch_set_tune(sb_batch* b, int knob, int value) {
   switch (knob) {
     case SB_TUNE_WAVES_PER_GROUP: return sb_batch_set_tiling(b, value, -1);
     case SB_TUNE_WAVES_PER_CU: return sb_batch_set_tiling(b, 0, value);
-    case SB_TUNE_STAGE_TOKENS:
-      if (value != 0 && value != 1) return fail(SB_ERR_ARG, "stage_tokens must be 0 or 1");
-      b->stage_tokens = value != 0;
-      return SB_OK;
     case SB_TUNE_GEN2_ROUND:
       if (value < 0) return fail(SB_ERR_ARG, "gen2 round must be >= 0 tiles");
       b->gen2_round = value;
@@ -1963,7 +1949,6 @@ int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
   switch (knob) {
     case SB_TUNE_WAVES_PER_GROUP: *value = b->waves_per_group; return SB_OK;
     case SB_TUNE_WAVES_PER_CU: *value = b->waves_per_cu; return SB_OK;
-    case SB_TUNE_STAGE_TOKENS: *value = b->stage_tokens ? 1 : 0; return SB_OK;
     case SB_TUNE_GEN2_ROUND: *value = b->gen2_round; return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }

@@ -196,5 +196,4 @@ struct SbLaunch {
   // small-alphabet kernels: rows of a wave's LDS token stage (a multiple of
   // 32, >= rmax; the tile's R x 64 tokens are copied there at tile start),
   // 0 = read tokens from the alignment in HBM
-  int tok_rows, pad_;
 };
