@@ -42,9 +42,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWC = 16;            // columns per wave, pass 1
 constexpr int kWPG = 4;            // waves per workgroup, pass 1
 constexpr int kTW = kWC * kWPG;    // columns per tile
-constexpr int kTC2 = 2;            // columns per wave, pass 2
-constexpr int kWPG2 = 8;           // waves per workgroup, pass 2
-constexpr int kQ2 = kTW / (kTC2 * kWPG2);   // pass-2 workgroups per tile
+constexpr int kWPG2 = 4;           // waves per workgroup, pass 2
+// columns per wave, pass 2 (two waves per SIMD: at AW = 64 two columns' 128
+// accumulators are half of the registers a wave can address)
+template <int AW> struct Gacc { static constexpr int TC = AW == 64 ? 2 : 4; };
 
 template <typename T>
 __device__ inline const SB_CONST T* cst(const void* p) { return (const SB_CONST T*)(p); }
@@ -462,33 +463,50 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
 }
 
 // ------------------------------------------------------------------ pass 2
-// C[k][q] = sum_n Dt_n[k] J_n[k][q] Ut_n[q] for 2 columns per wave, 8 waves per
-// workgroup sharing J_n staged in LDS (double-buffered, prefetched a branch
-// ahead in registers).  The accumulator is held in the output layout of
-// v_mfma_f32_16x16x4f32: lane (c = l % 16, h = l / 16) holds C[16 mt + 4 h + r][16 nq + c]
-// for every tile (mt, nq) and r < 4, so J arrives in 16-byte LDS reads
-// (J[q][k..k+3], J symmetric) and the accumulator is already the B operand of
-// the back-transform's first GEMM.  Back-transform (eigensub.py:138-172) per
-// column on the matrix cores: T = V C, one transpose of T through LDS, then
-// (V C V^T)^T = V T^T; counts = (V C V^T) o S with the diagonal unscaled.
-template <int AW>
-__global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
-  constexpr int TC = kTC2;
+// C[k][q] = sum_n Dt_n[k] J_n[k][q] Ut_n[q] for TC = 2 columns per wave and
+// WPG waves per workgroup.  The accumulator is held in the output layout of
+// v_mfma_f32_16x16x4f32: lane (c = l % 16, h = l / 16) holds
+// C[16 mt + 4 h + r][16 nq + c] for every tile (mt, nq) and r < 4.
+//
+// Operands arrive by direct-to-LDS loads (global_load_lds_dwordx4), one branch
+// ahead into double buffers, so no register holds a prefetched operand and the
+// registers left over keep LDS reads in flight:
+//   * J_n (shared by the workgroup): AW x AW, 16-byte chunks stored densely
+//     with chunk j of row r at r * CPR + (j ^ (r % CPR)), CPR = AW / 4 chunks
+//     per row; a lane (c, h) reads J[16 nq + c][16 mt + 4 h .. + 3] (J is
+//     symmetric: that is J_n[k][q] for its four k), and the sixteen lanes of a
+//     quarter hit sixteen different chunk columns, i.e. distinct banks.  Pad
+//     rows and columns (AP < AW) are never loaded and stay zero.
+//   * Dt_n and Ut_n of the wave's two columns (per wave): 4 AW floats.
+//
+// Back-transform (eigensub.py:138-172) on the matrix cores without any
+// transpose: as an A operand, a value in the output layout is the transpose of
+// what it holds as a B operand.  So with the accumulator as A (it reads as
+// C^T) and V^T from LDS as B, D1 = C^T V^T = (V C)^T; D1 as A reads as V C,
+// and D2 = (V C) V^T = V C V^T; counts = D2 o S with the diagonal unscaled.
+template <int AW, int WPG, int TC>
+__global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(2)))
+sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   constexpr int MT = AW / 16;
   constexpr int NA = MT * MT * 4;                 // accumulator registers per column
   constexpr int PS = AW + 4;
-  constexpr int NTH = 64 * kWPG2;
-  constexpr int JPT = (AW * AW / 4 + NTH - 1) / NTH;   // float4 of J staged per thread
-  // LDS: J double buffer during the branch loop, then one T buffer per wave; V
-  constexpr int kBig = kWPG2 * AW * PS > 2 * AW * PS ? kWPG2 * AW * PS : 2 * AW * PS;
-  __shared__ __attribute__((aligned(16))) float sBig[kBig];
+  constexpr int NTH = 64 * WPG;
+  constexpr int CPR = AW / 4;                     // 16-byte chunks per row of J
+  constexpr int JCH = AW * AW / 4;                // chunks of J
+  constexpr int VF = 2 * TC * AW;                 // floats of a wave's Dt / Ut block
+  constexpr int Q2 = kTW / (TC * WPG);            // workgroups per tile
+  // (separate objects per buffer, selected statically, so the compiler can
+  // tell the buffer being loaded from the one being read)
+  __shared__ __attribute__((aligned(16))) float sJ0[AW * AW];
+  __shared__ __attribute__((aligned(16))) float sJ1[AW * AW];
+  __shared__ __attribute__((aligned(16))) float sW0[WPG][VF];
+  __shared__ __attribute__((aligned(16))) float sW1[WPG][VF];
   __shared__ __attribute__((aligned(16))) float sV[AW * PS];   // V[i][k]
-  float* sJ0 = sBig;
-  float* sJ1 = sBig + AW * PS;
   const SbLaunch& L = *Lp;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int jc = lane & 15, h = lane >> 4;
-  const int tl = blockIdx.x / kQ2, qq = blockIdx.x % kQ2;
+  const int tl = blockIdx.x / Q2, qq = blockIdx.x % Q2;
   const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * (tile0 + tl);
   const int bidx = tbp[0], tidx = tbp[1];
   const SB_CONST SbBlockDesc* B = cst<SbBlockDesc>(L.blocks) + bidx;
@@ -497,8 +515,8 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
   const long long C = B->C;
   const int A = B->A, R = B->R;
   const int AP = L.AP, RM = L.rmax;
-  const int wcol0 = qq * (TC * kWPG2) + wv * TC;
-  SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
+  const int wcol0 = qq * (TC * WPG) + wv * TC;
+  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
   const SB_GLOBAL float* UT = (const SB_GLOBAL float*)st + vplane;
   const SB_GLOBAL float* DT = UT + 2 * vplane;
@@ -507,103 +525,104 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
   const int BTS = sb_bt_stride(AP);
 
   stage<AW, false>(sV, md + sb_mr_V(AP), AP, tid, NTH);
-  // J_n (AP x AP) as AW x AW with row stride PS, zero pads: thread tid holds
-  // float4 e = tid + NTH m of the padded matrix
-  f32x4 jr[JPT];
-  auto load_J = [&](int n) {
+
+  // this lane's J chunks: position p = (wv + WPG m) * 64 + lane -> row r, chunk j.
+  // Every lane loads (no branches in the loop): a pad chunk (AP < AW) reads
+  // J_n[0][0..3] instead, finite values that only ever multiply the pad
+  // states of Dt and Ut, which are exactly zero.
+  constexpr int JPW = JCH / NTH;                 // load instructions per wave
+  static_assert(JCH % NTH == 0 && (JPW > 0), "J chunks must split evenly over the workgroup");
+  int joff[JPW];                                 // float offset of the chunk in J_n
+#pragma unroll
+  for (int m = 0; m < JPW; ++m) {
+    const int p = (wv + WPG * m) * 64 + lane;
+    const int r = p / CPR, j = (p % CPR) ^ (r % CPR);
+    joff[m] = (r < AP && 4 * j < AP) ? r * AP + 4 * j : 0;
+  }
+  // the wave's Dt / Ut block: chunks p = 64 m + lane, Dt first
+  constexpr int VPW = VF / 4 / 64;               // load instructions per wave
+  static_assert(VF % 256 == 0, "the Dt / Ut block must be whole wave loads");
+  auto issue = [&](int n, float* sJb, float (*sWb)[VF]) __attribute__((always_inline)) {
     const SB_GLOBAL float* J = bt + (long long)n * BTS + sb_bt_J(AP);
 #pragma unroll
-    for (int m = 0; m < JPT; ++m) {
-      const int e = 4 * (tid + m * NTH);
-      const int q = e / AW, kk = e % AW;
-      if (e >= AW * AW) continue;
-      if (AP == AW) {
-        jr[m] = *(const SB_GLOBAL f32x4*)(J + q * AW + kk);
-      } else {
+    for (int m = 0; m < JPW; ++m)
+      __builtin_amdgcn_global_load_lds((const SB_GLOBAL void*)(J + joff[m]),
+                                       (__attribute__((address_space(3))) void*)(sJb + 4 * 64 * (wv + WPG * m)),
+                                       16, 0, 0);
+    const long long base = ((long long)n * kTW + wcol0) * AW;
 #pragma unroll
-        for (int z = 0; z < 4; ++z) jr[m][z] = (q < AP && kk + z < AP) ? J[q * AP + kk + z] : 0.f;
-      }
+    for (int m = 0; m < VPW; ++m) {
+      const int p = 64 * m + lane;
+      __builtin_amdgcn_global_load_lds(
+          (const SB_GLOBAL void*)(p < VF / 8 ? DT + base + 4 * p : UT + base + 4 * (p - VF / 8)),
+          (__attribute__((address_space(3))) void*)(&sWb[wv][0] + 4 * 64 * m), 16, 0, 0);
     }
   };
-  auto put_J = [&](float* dst) {
-#pragma unroll
-    for (int m = 0; m < JPT; ++m) {
-      const int e = 4 * (tid + m * NTH);
-      if (e < AW * AW) *(f32x4*)(dst + (e / AW) * PS + e % AW) = jr[m];
-    }
-  };
-  // this lane's factors of branch n, fetched one branch ahead:
-  // Dt_n[16 mt + 4 h + r] (float4 per mt) and Ut_n[16 nq + c]
-  f32x4 dkn[TC][MT];
-  float ukn[TC][MT];
-  auto load_vecs = [&](int n) {
-#pragma unroll
-    for (int c = 0; c < TC; ++c) {
-      const long long base = ((long long)n * kTW + wcol0 + c) * AW;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        dkn[c][mt] = *(const SB_GLOBAL f32x4*)(DT + base + 16 * mt + 4 * h);
-        ukn[c][mt] = UT[base + 16 * mt + jc];
-      }
-    }
-  };
+
   float acc[TC][NA];
 #pragma unroll
   for (int c = 0; c < TC; ++c)
 #pragma unroll
     for (int x = 0; x < NA; ++x) acc[c][x] = 0.f;
-  if (R > 1) {
-    load_J(1);
-    put_J(sJ1);
-    if (R > 2) load_J(2);
-    load_vecs(1);
-  }
-  for (int n = 1; n < R; ++n) {
-    lds_barrier();                       // J_n is in buffer n & 1; buffer (n+1) & 1 is free
-    f32x4 dk[TC][MT];
+
+  auto step = [&](int n, const float* J, const float (*Wb)[VF], float* sJn, float (*sWn)[VF])
+      __attribute__((always_inline)) {
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): this wave's loads of branch n have landed
+    lds_barrier();                       // ... and every wave's; the other buffers are free
+    issue(n + 1 < R ? n + 1 : n, sJn, sWn);   // (the last step reloads its own branch)
+    const float* W = &Wb[wv][0];
     float uk[TC][MT];
 #pragma unroll
     for (int c = 0; c < TC; ++c)
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        dk[c][mt] = dkn[c][mt];
-        uk[c][mt] = ukn[c][mt];
+      for (int nq = 0; nq < MT; ++nq) uk[c][nq] = W[TC * AW + c * AW + 16 * nq + jc];
+    // one K block mt of the state k at a time: its J rows and Dt chunks are
+    // read, then used; a scheduling barrier between blocks keeps the
+    // compiler from hoisting every read (and spilling accumulators): the
+    // other wave on the SIMD covers the LDS latency
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      f32x4 jv[MT], dk[TC];
+#pragma unroll
+      for (int c = 0; c < TC; ++c) dk[c] = *(const f32x4*)(W + c * AW + 16 * mt + 4 * h);
+#pragma unroll
+      for (int nq = 0; nq < MT; ++nq) {
+        const int r = 16 * nq + jc;
+        jv[nq] = *(const f32x4*)(J + 4 * (r * CPR + ((4 * mt + h) ^ (r % CPR))));
       }
-    if (n + 1 < R) {
-      put_J((n + 1) & 1 ? sJ1 : sJ0);
-      if (n + 2 < R) load_J(n + 2);
-      load_vecs(n + 1);
-    }
-    const float* J = (n & 1) ? sJ1 : sJ0;
 #pragma unroll
-    for (int nq = 0; nq < MT; ++nq)
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        // J[16 nq + c][16 mt + 4 h + r] = J_n[k][q] (J symmetric)
-        const f32x4 jv = *(const f32x4*)(J + (16 * nq + jc) * PS + 16 * mt + 4 * h);
+      for (int nq = 0; nq < MT; ++nq)
 #pragma unroll
         for (int c = 0; c < TC; ++c)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float& a = acc[c][(mt * MT + nq) * 4 + r];
-            a = fmaf(dk[c][mt][r] * jv[r], uk[c][nq], a);
+          for (int x = 0; x < 4; ++x) {
+            float& a = acc[c][(mt * MT + nq) * 4 + x];
+            a = fmaf(dk[c][x] * jv[nq][x], uk[c][nq], a);
           }
-      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  if (R > 1) issue(1, sJ1, sW1);
+  int n = 1;
+  for (; n + 1 < R; n += 2) {
+    step(n, sJ1, sW1, sJ0, sW0);
+    step(n + 1, sJ0, sW0, sJ1, sW1);
   }
+  if (n < R) step(n, sJ1, sW1, sJ0, sW0);
+  __builtin_amdgcn_s_waitcnt(0x0f70);    // no load into LDS outlives the loop
 
   // ---------------- back-transform on the matrix cores ----------------
-  lds_barrier();                         // every wave is done with the J buffers
-  float* Tb = sBig + wv * AW * PS;       // this wave's transpose buffer
   SB_GLOBAL float* out = glb<float>(B->counts);
   const SB_GLOBAL float* S = md + sb_mr_S(AP);
 #pragma unroll
   for (int c = 0; c < TC; ++c) {
-    // T = V C: tile (mi, nq), lane (c', h) holds T[16 mi + 4 h + r][16 nq + c']
+    // D1 = C^T V^T: tile (nq, ni), lane (jc, h) holds D1[16 nq + 4 h + r][16 ni + jc]
+    f32x4 d1[MT][MT];
 #pragma unroll
-    for (int mi = 0; mi < MT; ++mi) {
-      f32x4 a[MT];
+    for (int ni = 0; ni < MT; ++ni) {
+      f32x4 b[MT];
 #pragma unroll
-      for (int t2 = 0; t2 < MT; ++t2) a[t2] = *(const f32x4*)(sV + (16 * mi + jc) * PS + 16 * t2 + 4 * h);
+      for (int t2 = 0; t2 < MT; ++t2) b[t2] = *(const f32x4*)(sV + (16 * ni + jc) * PS + 16 * t2 + 4 * h);
 #pragma unroll
       for (int nq = 0; nq < MT; ++nq) {
         f32x4 t = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -611,41 +630,36 @@ __global__ void __launch_bounds__(512) sb_gacc_kernel(const SbLaunch* __restrict
         for (int t2 = 0; t2 < MT; ++t2)
 #pragma unroll
           for (int r2 = 0; r2 < 4; ++r2)
-            t = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t2][r2], acc[c][(t2 * MT + nq) * 4 + r2], t, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Tb[(16 * mi + 4 * h + r) * PS + 16 * nq + jc] = t[r];
+            t = __builtin_amdgcn_mfma_f32_16x16x4f32(acc[c][(t2 * MT + nq) * 4 + r2], b[t2][r2], t, 0, 0, 0);
+        d1[nq][ni] = t;
       }
     }
-    wave_lds_sync();
-    // (V C V^T)^T = V T^T: tile (mj, ni), lane (c', h) holds VCV[16 ni + c'][16 mj + 4 h + r]
+    // D2 = (V C) V^T: tile (mi, nj), lane (jc, h) holds VCV[16 mi + 4 h + r][16 nj + jc]
     const long long col = (long long)tidx * kTW + wcol0 + c;
 #pragma unroll
-    for (int mj = 0; mj < MT; ++mj) {
-      if (16 * mj >= A) break;
-      f32x4 a[MT];
+    for (int nj = 0; nj < MT; ++nj) {
+      if (16 * nj >= A) break;
+      f32x4 b[MT];
 #pragma unroll
-      for (int t2 = 0; t2 < MT; ++t2) a[t2] = *(const f32x4*)(sV + (16 * mj + jc) * PS + 16 * t2 + 4 * h);
+      for (int t2 = 0; t2 < MT; ++t2) b[t2] = *(const f32x4*)(sV + (16 * nj + jc) * PS + 16 * t2 + 4 * h);
+      const int j = 16 * nj + jc;
 #pragma unroll
-      for (int ni = 0; ni < MT; ++ni) {
-        if (16 * ni >= A) break;
+      for (int mi = 0; mi < MT; ++mi) {
+        if (16 * mi >= A) break;
         f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t2 = 0; t2 < MT; ++t2) {
-          const f32x4 bv = *(const f32x4*)(Tb + (16 * ni + jc) * PS + 16 * t2 + 4 * h);
+        for (int t2 = 0; t2 < MT; ++t2)
 #pragma unroll
-          for (int r2 = 0; r2 < 4; ++r2) o = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t2][r2], bv[r2], o, 0, 0, 0);
-        }
-        const int i = 16 * ni + jc;
-        if (col < C && i < A) {
+          for (int r2 = 0; r2 < 4; ++r2) o = __builtin_amdgcn_mfma_f32_16x16x4f32(d1[t2][mi][r2], b[t2][r2], o, 0, 0, 0);
+        if (col < C && j < A) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int j = 16 * mj + 4 * h + r;
-            if (j < A) out[((long long)i * A + j) * C + col] = i == j ? o[r] : S[i * AP + j] * o[r];
+            const int i = 16 * mi + 4 * h + r;
+            if (i < A) out[((long long)i * A + j) * C + col] = i == j ? o[r] : S[i * AP + j] * o[r];
           }
         }
       }
     }
-    wave_lds_sync();                     // Tb is rewritten by the next column
   }
 }
 
@@ -769,8 +783,8 @@ __global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restr
 
 template __global__ void sb_gwalk_kernel<32>(const SbLaunch*, int);
 template __global__ void sb_gwalk_kernel<64>(const SbLaunch*, int);
-template __global__ void sb_gacc_kernel<32>(const SbLaunch*, int);
-template __global__ void sb_gacc_kernel<64>(const SbLaunch*, int);
+template __global__ void sb_gacc_kernel<32, kWPG2, Gacc<32>::TC>(const SbLaunch*, int);
+template __global__ void sb_gacc_kernel<64, kWPG2, Gacc<64>::TC>(const SbLaunch*, int);
 template __global__ void sb_gbranch_kernel<32>(const SbLaunch*, int);
 template __global__ void sb_gbranch_kernel<64>(const SbLaunch*, int);
 
@@ -792,13 +806,15 @@ extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, in
     if (branch)
       hipLaunchKernelGGL(sb_gbranch_kernel<32>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
     else
-      hipLaunchKernelGGL(sb_gacc_kernel<32>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
+      hipLaunchKernelGGL((sb_gacc_kernel<32, kWPG2, Gacc<32>::TC>), dim3(ntiles * (kTW / (Gacc<32>::TC * kWPG2))),
+                         dim3(64 * kWPG2), 0, s, L, tile0);
   } else {
     hipLaunchKernelGGL(sb_gwalk_kernel<64>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0);
     if (branch)
       hipLaunchKernelGGL(sb_gbranch_kernel<64>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
     else
-      hipLaunchKernelGGL(sb_gacc_kernel<64>, dim3(ntiles * kQ2), dim3(64 * kWPG2), 0, s, L, tile0);
+      hipLaunchKernelGGL((sb_gacc_kernel<64, kWPG2, Gacc<64>::TC>), dim3(ntiles * (kTW / (Gacc<64>::TC * kWPG2))),
+                         dim3(64 * kWPG2), 0, s, L, tile0);
   }
   return hipGetLastError();
 }
