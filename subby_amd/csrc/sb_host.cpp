@@ -297,12 +297,26 @@ int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
     else if (second[p] < 0) second[p] = n;
     else return fail(SB_ERR_TREE, "Node %d has more than 2 children; tree must be binary", p);
   }
-  std::vector<int> slot(R, -1);
-  int nint = 0;
-  for (int n = 0; n < R; ++n) {
+  for (int n = 0; n < R; ++n)
     if (first[n] >= 0 && second[n] < 0)
       return fail(SB_ERR_TREE, "Node %d has 1 child; expected 0 or 2", n);
-    if (first[n] >= 0) slot[n] = nint++;
+  // Records (internal nodes) in depth-first preorder, first child's subtree
+  // first: the walks rely on an internal first child's record following its
+  // parent's.  parentIndex only guarantees parents before children
+  // (docs/architecture.md:39-48), and the reference's own balanced trees
+  // (parentIndex[i] = (i - 1) // 2, benchmarks/run_benchmarks.py:119-128)
+  // are breadth-first; node indices, alignment rows and outputs keep the
+  // caller's numbering, only the walk order changes.
+  std::vector<int> slot(R, -1), stack;
+  int nint = 0;
+  stack.push_back(0);
+  while (!stack.empty()) {
+    const int n = stack.back();
+    stack.pop_back();
+    if (first[n] < 0) continue;
+    slot[n] = nint++;
+    stack.push_back(second[n]);
+    stack.push_back(first[n]);
   }
   tp.nint = nint;
   tp.recs.assign(size_t(nint) * 8, 0);
