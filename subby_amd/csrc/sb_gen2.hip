@@ -128,6 +128,53 @@ __device__ inline void gemm(const float* P, const float* u, float* w, int lane) 
     for (int r = 0; r < 4; ++r) w[4 * t + r] = acc[t][r];
 }
 
+// Dense swizzled LDS layout of an AW x AW matrix: the 16-byte chunk j of row r
+// sits at chunk r * (AW / 4) + (j ^ (r % (AW / 4))).  A wave's A-operand read
+// (rows 16 t + i, chunk 4 t2 + h) puts its sixteen lanes of a quarter on
+// sixteen different chunk columns, i.e. distinct banks.
+template <int AW>
+__device__ inline int swz(int r, int c) {
+  constexpr int CPR = AW / 4;
+  return r * AW + 4 * (((c >> 2) ^ r) & (CPR - 1)) + (c & 3);
+}
+
+// Stage an AP x AP row-major matrix into LDS as AW x AW in the swizzled
+// layout with zero pads, transposed when TR.
+template <int AW, bool TR>
+__device__ inline void stage_sw(float* dst, const SB_GLOBAL float* src, int AP, int tid, int nthr) {
+#pragma unroll 4
+  for (int idx = tid; idx < AW * AW; idx += nthr) {
+    const int r = idx / AW, c = idx % AW;
+    const float v = (r < AP && c < AP) ? src[r * AP + c] : 0.f;
+    dst[TR ? swz<AW>(c, r) : swz<AW>(r, c)] = v;
+  }
+}
+
+// w = P u as gemm() below, P in the swizzled layout.
+template <int AW>
+__device__ inline void gemm_sw(const float* P, const float* u, float* w, int lane) {
+  constexpr int NT = AW / 16;
+  const int i = lane & 15, h = lane >> 4;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t2 = 0; t2 < NT; ++t2) {
+    f32x4 a[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = *(const f32x4*)(P + swz<AW>(16 * t + i, 16 * t2 + 4 * h));
+#pragma unroll
+    for (int r2 = 0; r2 < 4; ++r2)
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][r2], u[4 * t2 + r2], acc[t], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[4 * t + r] = acc[t][r];
+}
+
 struct Kid {
   int n, tok, e;
   bool leaf, has, branch;
@@ -143,21 +190,35 @@ struct Kid {
 //   DD[R][64][AW]  D_n of internal nodes
 //   DT[R][64][AW]  Dt_n = Vi^T D_n * scale * mask of every branch
 //   UW[R][64], DW[R][64]  ints: U word {exp << 2 | both | any}, D word {exp << 1 | steiner}
-// Rows of leaves that are never written may still be loaded one record ahead
-// (unconditional loads); those values are discarded.
+//
+// Records are in depth-first preorder (the planner's order), so the first
+// internal child of a record's node is the node of the record computed just
+// before it in the inside pass (postorder, records walked backwards) and just
+// after it in the outside pass: that vector is forwarded in registers.  Per
+// record, the matrices arrive by direct-to-LDS loads one record ahead into
+// double buffers (A: M_l or M_p^T, B: M_r), and the staged vectors it reads
+// (the other internal child's U, both children's W, the node's D) by register
+// loads one record ahead, so a record waits only on loads issued a whole record
+// earlier.  One LDS matrix holds Vs^T during the inside pass and Vi^T during
+// the outside pass: at AW = 64 the workgroup uses 80 KiB, two per CU.
+// Rows that are loaded but not needed (a leaf's, a forwarded child's) are
+// discarded by selects.
 template <int AW>
-__global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AW == 64 ? 1 : 2)))
+sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   constexpr int NV = AW / 4;
-  constexpr int PS = AW + 4;
+  constexpr int CPR = AW / 4;
   constexpr int NTH = 64 * kWPG;
-  constexpr int MPT = AW * AW / 4 / NTH;   // float4 of a staged matrix per thread
-  __shared__ __attribute__((aligned(16))) float sVs[AW * PS];   // Vs^T
-  __shared__ __attribute__((aligned(16))) float sVi[AW * PS];   // Vi^T
-  __shared__ __attribute__((aligned(16))) float sP[2][AW * PS];
-  __shared__ float sv1[AW];                                     // Vs^T (all ones on real states)
-  __shared__ float spi[AW];
+  constexpr int MPW = AW * AW / 4 / NTH;   // direct-to-LDS loads per thread per matrix
+  static_assert(AW * AW / 4 % NTH == 0, "matrix chunks must split evenly over the workgroup");
+  __shared__ __attribute__((aligned(16))) float sV[AW * AW];     // Vs^T, then Vi^T
+  __shared__ __attribute__((aligned(16))) float sA0[AW * AW];
+  __shared__ __attribute__((aligned(16))) float sA1[AW * AW];
+  __shared__ __attribute__((aligned(16))) float sB0[AW * AW];
+  __shared__ __attribute__((aligned(16))) float sB1[AW * AW];
   const SbLaunch& L = *Lp;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int jc = lane & 15, h = lane >> 4;
   const int tl = blockIdx.x;
   const SB_CONST int* tbp = cst<int>(L.tiles) + 2 * (tile0 + tl);
@@ -180,55 +241,49 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
   SB_GLOBAL float* DT = DD + vplane;
   SB_GLOBAL int* UW = (SB_GLOBAL int*)(DT + vplane);
   SB_GLOBAL int* DWd = UW + (long long)RM * kTW;
-  const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln);
+  // per-lane data is addressed as a uniform base plus a 32-bit lane offset;
+  // tokens and mask bytes are read unsigned (a gap token -1 reads as 255,
+  // which every test below treats as a gap), so a prefetched byte needs no
+  // conversion that would wait for its load
+  const SB_GLOBAL uint8_t* aln = glb<uint8_t>(B->aln) + (long long)tidx * kTW;
+  const unsigned lcol = unsigned(cc - (long long)tidx * kTW);     // this lane's column in the tile
   const SB_GLOBAL float* bt = glb<float>(B->bt);
   const SB_GLOBAL float* md = glb<float>(B->model);
   const SB_CONST int* sched = cst<int>(B->sched);
   const int BTS = sb_bt_stride(AP);
   const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
   // rows of the explicit mask (the alignment stands in when there is none; never used then)
-  const SB_GLOBAL int8_t* mrow = B->mask ? glb<int8_t>(B->mask) : aln;
+  const SB_GLOBAL uint8_t* mrow = (B->mask ? glb<uint8_t>(B->mask) : glb<uint8_t>(B->aln)) + (long long)tidx * kTW;
 
   auto state = [&](int v) { return 16 * (v >> 2) + 4 * h + (v & 3); };
   auto real = [&](int v) { return state(v) < A ? 1.f : 0.f; };
-  stage<AW, true>(sVs, md + sb_mr_Vs(AP), AP, tid, NTH);
-  stage<AW, true>(sVi, md + sb_mr_Vi(AP), AP, tid, NTH);
-  if (tid < AW) spi[tid] = tid < A ? md[sb_mr_pi(AP) + tid] : 0.f;
-  lds_barrier();
-  if (tid < AW) {
-    float s = 0.f;
-    for (int a = 0; a < A; ++a) s += sVs[tid * PS + a];
-    sv1[tid] = s;
+
+  // this thread's chunks of a staged matrix (position p -> row r, chunk j);
+  // a pad chunk (AP < AW) reads the matrix's first chunk: finite values that
+  // only ever meet pad states, which are exactly zero in every vector or are
+  // zeroed by the observation vector before use
+  // (byte offsets, unsigned: a uniform base plus a 32-bit lane offset)
+  unsigned moff[MPW];
+#pragma unroll
+  for (int m = 0; m < MPW; ++m) {
+    const int p = (wv + kWPG * m) * 64 + lane;
+    const int r = p / CPR, j = (p % CPR) ^ (r % CPR);
+    moff[m] = (r < AP && 4 * j < AP) ? unsigned(4 * (r * AP + 4 * j)) : 0u;
   }
-  // (sv1 is read after the next barrier)
+  auto dma = [&](float* dst, int n, int off) __attribute__((always_inline)) {
+    const SB_GLOBAL char* src = (const SB_GLOBAL char*)(bt + (long long)n * BTS + off);
+#pragma unroll
+    for (int m = 0; m < MPW; ++m)
+      __builtin_amdgcn_global_load_lds((const SB_GLOBAL void*)(src + moff[m]),
+                                       (__attribute__((address_space(3))) void*)(dst + 4 * 64 * (wv + kWPG * m)),
+                                       16, 0, 0);
+  };
 
-  // matrices are fetched into registers one record ahead and written to LDS
-  struct Mat {
-    f32x4 v[MPT];
+  const unsigned lvec = unsigned((wcol * AW + 4 * h) * 4);   // lane byte offset in a node's row of a vector plane
+  auto vptr = [&](SB_GLOBAL float* base, int n) {
+    return (SB_GLOBAL float*)((SB_GLOBAL char*)(base + (long long)n * kTW * AW) + lvec);
   };
-  auto mload = [&](Mat& m, const SB_GLOBAL float* src) {
-#pragma unroll
-    for (int q = 0; q < MPT; ++q) {
-      const int e = 4 * (tid + q * NTH);
-      const int r = e / AW, c = e % AW;
-      if (AP == AW) {
-        m.v[q] = *(const SB_GLOBAL f32x4*)(src + r * AW + c);
-      } else {
-#pragma unroll
-        for (int z = 0; z < 4; ++z) m.v[q][z] = (r < AP && c + z < AP) ? src[r * AP + c + z] : 0.f;
-      }
-    }
-  };
-  auto mstore = [&](float* dst, const Mat& m) {
-#pragma unroll
-    for (int q = 0; q < MPT; ++q) {
-      const int e = 4 * (tid + q * NTH);
-      *(f32x4*)(dst + (e / AW) * PS + e % AW) = m.v[q];
-    }
-  };
-  auto mat = [&](int n, int off) { return bt + (long long)n * BTS + off; };
-
-  auto vptr = [&](SB_GLOBAL float* base, int n) { return base + ((long long)n * kTW + wcol) * AW + 4 * h; };
+  auto word_at = [&](SB_GLOBAL int* base, int n) -> SB_GLOBAL int& { return (base + (long long)n * kTW)[wcol]; };
   auto load_vec = [&](SB_GLOBAL float* base, int n, float* u) {
     const SB_GLOBAL float* p = vptr(base, n);
 #pragma unroll
@@ -244,9 +299,9 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
     for (int t = 0; t < NV / 4; ++t)
       *(SB_GLOBAL f32x4*)(p + 16 * t) = f32x4{u[4 * t], u[4 * t + 1], u[4 * t + 2], u[4 * t + 3]};
   };
-  auto token = [&](int n) -> int { return aln[(long long)n * C + cc]; };
+  auto token = [&](int n) -> uint8_t { return (aln + (long long)n * C)[lcol]; };
   auto token_vec = [&](int tok, float* u) {
-    const bool obs = tok >= 0 && tok < A;
+    const bool obs = tok < A;
 #pragma unroll
     for (int v = 0; v < NV; ++v) u[v] = obs ? (state(v) == tok ? 1.f : 0.f) : real(v);
   };
@@ -260,11 +315,11 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
     return x;
   };
 
-  // One tree record's indices, tokens and words, fetched one record ahead
-  // with unconditional loads.
+  // One tree record's indices, tokens and words (scalar record, per-lane loads)
   struct Rec {
     int p, l, r, sl, sr;
-    int tp, tl, tr, wl, wr, mkl, mkr;
+    uint8_t tp, tl, tr, mkl, mkr;     // (bytes as loaded: widened where used)
+    int wl, wr;
   };
   auto fetch = [&](Rec& x, int k, bool outside) {
     const SB_CONST int* rec = sched + k * 8;
@@ -272,12 +327,10 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
     x.tp = token(x.p);
     x.tl = token(x.l);
     x.tr = token(x.r);
-    x.wl = UW[(long long)x.l * kTW + wcol];
-    x.wr = UW[(long long)x.r * kTW + wcol];
-    if (outside) {
-      x.mkl = mrow[(long long)x.l * C + cc];
-      x.mkr = mrow[(long long)x.r * C + cc];
-    }
+    x.wr = word_at(UW, x.r);
+    x.wl = outside ? word_at(UW, x.l) : 0;
+    x.mkl = outside ? (mrow + (long long)x.l * C)[lcol] : uint8_t(0);
+    x.mkr = outside ? (mrow + (long long)x.r * C)[lcol] : uint8_t(0);
   };
   auto kid = [&](int n, int slot, int tok, int w) {
     Kid k;
@@ -285,58 +338,78 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
     k.leaf = slot < 0;
     k.tok = tok;
     k.e = k.leaf ? 0 : w >> 2;
-    k.has = k.leaf ? (tok >= 0 && tok <= A) : (w & 1) != 0;
+    k.has = k.leaf ? tok <= A : (w & 1) != 0;
     k.branch = !k.leaf && (w & 2) != 0;
     return k;
   };
 
-  // ---------------- inside pass (postorder) ----------------
+  stage_sw<AW, true>(sV, md + sb_mr_Vs(AP), AP, tid, NTH);
+  lds_barrier();
+  // Vs^T u for an unobserved leaf: the sums of the rows of Vs^T over real states
+  float sv1[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    float s = 0.f;
+    for (int a = 0; a < A; ++a) s += sV[swz<AW>(state(v), a)];
+    sv1[v] = s;
+  }
+
+  // ---------------- inside pass (postorder: records walked backwards) ----------------
   float u0[NV];
   int e0 = 0;
   bool rbr = false;
   if (nint == 0) token_vec(token(0), u0);
   if (nint > 0) {
-    Rec cur, nxt;
-    Mat ml, mr;
-    fetch(cur, nint - 1, false);
-    mload(ml, mat(cur.l, sb_bt_M(AP)));
-    mload(mr, mat(cur.r, sb_bt_M(AP)));
-    for (int i = nint - 1; i >= 0; --i) {
-      float wl[NV], wr[NV];
-      load_vec(UU, cur.l, wl);       // U of internal children (leaf rows: discarded)
-      load_vec(UU, cur.r, wr);
-      lds_barrier();                 // previous readers of sP are done
-      mstore(sP[0], ml);
-      mstore(sP[1], mr);
-      const int kn = i > 0 ? i - 1 : 0;   // next record (clamped)
-      fetch(nxt, kn, false);
-      {
-        const SB_CONST int* rn = sched + kn * 8;
-        mload(ml, mat(rn[1], sb_bt_M(AP)));
-        mload(mr, mat(rn[2], sb_bt_M(AP)));
+    // ping-pong records: record i reads one while record i - 1's token and word
+    // loads land in the other, so those are never copied (a copy of a loaded
+    // value waits for its load); the prefetched vector is copied at the end of
+    // the record, after its compute
+    Rec ra, rb;
+    float ur[NV], nur[NV];           // the right child's staged U (both children internal)
+    float fu[NV];                    // U of the record computed last (its node is the first internal child)
+    int fw = 0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fu[v] = 0.f;
+    fetch(ra, nint - 1, false);
+    if (ra.sl >= 0 && ra.sr >= 0) load_vec(UU, ra.r, ur);
+    dma((nint - 1) & 1 ? sA1 : sA0, ra.l, sb_bt_M(AP));
+    dma((nint - 1) & 1 ? sB1 : sB0, ra.r, sb_bt_M(AP));
+    auto istep = [&](int i, const Rec& cur, Rec& nxt) __attribute__((always_inline)) {
+      __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): this wave's loads for record i have landed
+      lds_barrier();                        // ... and every wave's; the other buffers are free
+      if (i > 0) {
+        fetch(nxt, i - 1, false);
+        if (nxt.sl >= 0 && nxt.sr >= 0) load_vec(UU, nxt.r, nur);   // (uniform)
+        dma((i - 1) & 1 ? sA1 : sA0, nxt.l, sb_bt_M(AP));
+        dma((i - 1) & 1 ? sB1 : sB0, nxt.r, sb_bt_M(AP));
       }
-      lds_barrier();
-      const Kid kl = kid(cur.l, cur.sl, cur.tl, cur.wl), kr = kid(cur.r, cur.sr, cur.tr, cur.wr);
-      // W = M u for both children; leaves: a column of M (or ones), and their Ut
+      const float* MA = i & 1 ? sA1 : sA0;
+      const float* MB = i & 1 ? sB1 : sB0;
+      // first internal child: the record computed last; the right child, when the
+      // left one is internal too, from the stage area
+      const bool lint = cur.sl >= 0;
+      const Kid kl = kid(cur.l, cur.sl, cur.tl, fw);
+      const Kid kr = kid(cur.r, cur.sr, cur.tr, lint ? cur.wr : fw);
+      float wl[NV], wr[NV];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const Kid& k = s == 0 ? kl : kr;
         float* w = s == 0 ? wl : wr;
-        const float* P = sP[s];
+        const float* P = s == 0 ? MA : MB;
         if (k.leaf) {
-          const bool obs = k.tok >= 0 && k.tok < A;
+          const bool obs = k.tok < A;
           float t[NV];
 #pragma unroll
           for (int v = 0; v < NV; ++v) {
-            w[v] = obs ? P[state(v) * PS + k.tok] : real(v);
-            t[v] = obs ? sVs[state(v) * PS + k.tok] : sv1[state(v)];
+            w[v] = obs ? P[swz<AW>(state(v), k.tok)] : real(v);
+            t[v] = obs ? sV[swz<AW>(state(v), k.tok)] : sv1[v];
           }
           store_vec(UT, k.n, t);
         } else {
           float u[NV];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) u[v] = w[v];
-          gemm<AW>(P, u, w, lane);
+          for (int v = 0; v < NV; ++v) u[v] = (s == 1 && lint) ? ur[v] : fu[v];
+          gemm_sw<AW>(P, u, w, lane);
         }
         store_vec(UU, k.n, w);       // W_n replaces U_n
       }
@@ -346,15 +419,12 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
       for (int v = 0; v < NV; ++v) up[v] *= wl[v] * wr[v];
       const int e = rescale(up);
       const int ep = kl.e + kr.e + e;
+      const int word = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
       if (i > 0) {
-        const int word = (ep << 2) | ((kl.has && kr.has) ? 2 : 0) | ((kl.has || kr.has) ? 1 : 0);
         store_vec(UU, cur.p, up);
-        if (h == 0) UW[(long long)cur.p * kTW + wcol] = word;
-        // the next record's words were fetched before this one was written
-        if (nxt.l == cur.p) nxt.wl = word;
-        if (nxt.r == cur.p) nxt.wr = word;
+        if (h == 0) word_at(UW, cur.p) = word;
         float t[NV];
-        gemm<AW>(sVs, up, t, lane);
+        gemm_sw<AW>(sV, up, t, lane);
         store_vec(UT, cur.p, t);
       } else {
 #pragma unroll
@@ -362,11 +432,22 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
         e0 = ep;
         rbr = kl.has && kr.has;
       }
-      cur = nxt;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        fu[v] = up[v];
+        ur[v] = nur[v];
+      }
+      fw = word;
+    };
+    for (int i = nint - 1; i >= 0; i -= 2) {
+      istep(i, ra, rb);
+      if (i == 0) break;
+      istep(i - 1, rb, ra);
     }
   }
   // log-likelihood (FP64 pi . u), root posterior
   const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi(AP);
+  const SB_GLOBAL float* pif = md + sb_mr_pi(AP);
   double s0 = 0.0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -381,46 +462,66 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int s = state(v);
-      if (s < A) glb<float>(B->root)[(long long)s * C + cc] = spi[s] * u0[v] * inv_s0;
+      if (s < A) glb<float>(B->root)[(long long)s * C + cc] = pif[s] * u0[v] * inv_s0;
     }
   }
   if (!(flags & (SB_F_COUNTS | SB_F_BCOUNTS))) return;
 
   // ---------------- outside pass + Dt (preorder) ----------------
+  __builtin_amdgcn_s_waitcnt(0x0f70);
+  lds_barrier();                       // every wave is done with Vs^T
+  stage_sw<AW, true>(sV, md + sb_mr_Vi(AP), AP, tid, NTH);
   {
-    Rec cur, nxt;
-    Mat mt;              // M_p^T of the next record
-    fetch(cur, 0, true);
-    if (nint > 1) {
-      const SB_CONST int* rn = sched + 8;
-      mload(mt, mat(rn[0], sb_bt_Mt(AP)));
+    // ping-pong records as in the inside pass; the prefetched vectors (both
+    // children's W, the node's D: loaded, or forwarded from the parent's record)
+    // are copied at the end of the record
+    Rec ra, rb;
+    float wl[NV], wr[NV], dp[NV], nwl[NV], nwr[NV], ndp[NV];
+    int dw = 0, ndw = 0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      dp[v] = 0.f;
+      ndp[v] = 0.f;
     }
-    for (int i = 0; i < nint; ++i) {
-      float x[NV], wl[NV], wr[NV];
-      load_vec(UU, cur.l, wl);
-      load_vec(UU, cur.r, wr);
+    fetch(ra, 0, true);
+    load_vec(UU, ra.l, wl);
+    load_vec(UU, ra.r, wr);
+    auto ostep = [&](int i, const Rec& cur, Rec& nxt) __attribute__((always_inline)) {
+      __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): this wave's loads for record i have landed
+      lds_barrier();                        // ... and every wave's (and Vi^T is staged)
+      // the next record's node is a child of this one (forwarded) or was reached earlier
+      const SB_CONST int* rn = sched + (i + 1 < nint ? i + 1 : i) * 8;
+      const int np = rn[0];
+      const bool nfwd = np == cur.l || np == cur.r;
+      if (i + 1 < nint) {
+        fetch(nxt, i + 1, true);
+        load_vec(UU, nxt.l, nwl);
+        load_vec(UU, nxt.r, nwr);
+        if (!nfwd) {                   // (uniform)
+          load_vec(DD, np, ndp);
+          ndw = word_at(DWd, np);
+        }
+        dma((i + 1) & 1 ? sA1 : sA0, np, sb_bt_Mt(AP));
+      }
+      float x[NV];
       int ex;
       bool stp;
       if (i == 0) {
         token_vec(cur.tp, x);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) x[v] *= spi[state(v)];
+        for (int v = 0; v < NV; ++v) x[v] *= state(v) < A ? pif[state(v)] : 0.f;
         ex = 0;
         stp = rbr;
       } else {
-        // x = obs(p) * M_p^T d_p, with M_p^T staged at the end of the previous record
-        float dp[NV], o[NV];
-        load_vec(DD, cur.p, dp);
-        const int dw = DWd[(long long)cur.p * kTW + wcol];
+        // x = obs(p) * M_p^T d_p
+        float o[NV];
         ex = dw >> 1;
         stp = (dw & 1) != 0;
-        gemm<AW>(sP[0], dp, x, lane);
+        gemm_sw<AW>(i & 1 ? sA1 : sA0, dp, x, lane);
         token_vec(cur.tp, o);
 #pragma unroll
         for (int v = 0; v < NV; ++v) x[v] *= o[v];
       }
-      const int kn = i + 1 < nint ? i + 1 : i;   // next record (clamped)
-      fetch(nxt, kn, true);
       const Kid kl = kid(cur.l, cur.sl, cur.tl, cur.wl), kr = kid(cur.r, cur.sr, cur.tr, cur.wr);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
@@ -433,9 +534,16 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
         const int e = rescale(d);
         const int ed = ks.e + ex + e;
         const bool stn = (kn_.leaf ? kn_.has : kn_.branch) || (stp && kn_.has);
+        const int dword = (ed << 1) | (stn ? 1 : 0);
         if (sn >= 0) {
-          store_vec(DD, kn_.n, d);
-          if (h == 0) DWd[(long long)kn_.n * kTW + wcol] = (ed << 1) | (stn ? 1 : 0);
+          if (i + 1 < nint && np == kn_.n) {   // the next record's node: forwarded
+#pragma unroll
+            for (int v = 0; v < NV; ++v) ndp[v] = d[v];
+            ndw = dword;
+          } else {
+            store_vec(DD, kn_.n, d);
+            if (h == 0) word_at(DWd, kn_.n) = dword;
+          }
         }
         bool act;
         if (mask_mode == SB_MASK_NONE) act = true;
@@ -443,21 +551,23 @@ __global__ void __launch_bounds__(256, 2) sb_gwalk_kernel(const SbLaunch* __rest
         else act = (s == 0 ? cur.mkl : cur.mkr) != 0;
         const float sc = act ? __builtin_amdgcn_ldexpf(inv_s0, ed + kn_.e - e0) : 0.f;
         float t[NV];
-        gemm<AW>(sVi, d, t, lane);
+        gemm_sw<AW>(sV, d, t, lane);
 #pragma unroll
         for (int v = 0; v < NV; ++v) t[v] *= sc;
         store_vec(DT, kn_.n, t);
       }
-      if (i + 1 < nint) {
-        lds_barrier();               // every wave is done with M_p^T of this record
-        mstore(sP[0], mt);
-        if (i + 2 < nint) {
-          const SB_CONST int* rn = sched + (i + 2) * 8;
-          mload(mt, mat(rn[0], sb_bt_Mt(AP)));
-        }
-        lds_barrier();
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        wl[v] = nwl[v];
+        wr[v] = nwr[v];
+        dp[v] = ndp[v];
       }
-      cur = nxt;
+      dw = ndw;
+    };
+    for (int i = 0; i < nint; i += 2) {
+      ostep(i, ra, rb);
+      if (i + 1 >= nint) break;
+      ostep(i + 1, rb, ra);
     }
   }
 }
@@ -532,18 +642,18 @@ sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   // states of Dt and Ut, which are exactly zero.
   constexpr int JPW = JCH / NTH;                 // load instructions per wave
   static_assert(JCH % NTH == 0 && (JPW > 0), "J chunks must split evenly over the workgroup");
-  int joff[JPW];                                 // float offset of the chunk in J_n
+  unsigned joff[JPW];                            // byte offset of the chunk in J_n
 #pragma unroll
   for (int m = 0; m < JPW; ++m) {
     const int p = (wv + WPG * m) * 64 + lane;
     const int r = p / CPR, j = (p % CPR) ^ (r % CPR);
-    joff[m] = (r < AP && 4 * j < AP) ? r * AP + 4 * j : 0;
+    joff[m] = (r < AP && 4 * j < AP) ? unsigned(4 * (r * AP + 4 * j)) : 0u;
   }
   // the wave's Dt / Ut block: chunks p = 64 m + lane, Dt first
   constexpr int VPW = VF / 4 / 64;               // load instructions per wave
   static_assert(VF % 256 == 0, "the Dt / Ut block must be whole wave loads");
   auto issue = [&](int n, float* sJb, float (*sWb)[VF]) __attribute__((always_inline)) {
-    const SB_GLOBAL float* J = bt + (long long)n * BTS + sb_bt_J(AP);
+    const SB_GLOBAL char* J = (const SB_GLOBAL char*)(bt + (long long)n * BTS + sb_bt_J(AP));
 #pragma unroll
     for (int m = 0; m < JPW; ++m)
       __builtin_amdgcn_global_load_lds((const SB_GLOBAL void*)(J + joff[m]),
