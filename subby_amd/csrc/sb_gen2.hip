@@ -150,24 +150,31 @@ __device__ inline void stage_sw(float* dst, const SB_GLOBAL float* src, int AP, 
   }
 }
 
-// w = P u as gemm() below, P in the swizzled layout.
+// w = P u as gemm() below, P in the swizzled layout.  The A operands are read
+// one K block ahead into two register sets, and scheduling barriers keep the
+// compiler from hoisting more.
 template <int AW>
 __device__ inline void gemm_sw(const float* P, const float* u, float* w, int lane) {
   constexpr int NT = AW / 16;
   const int i = lane & 15, h = lane >> 4;
-  f32x4 acc[NT];
+  f32x4 acc[NT], a[2][NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int t2 = 0; t2 < NT; ++t2) {
-    f32x4 a[NT];
+  for (int t = 0; t < NT; ++t) a[0][t] = *(const f32x4*)(P + swz<AW>(16 * t + i, 4 * h));
 #pragma unroll
-    for (int t = 0; t < NT; ++t) a[t] = *(const f32x4*)(P + swz<AW>(16 * t + i, 16 * t2 + 4 * h));
+  for (int t2 = 0; t2 < NT; ++t2) {
+    if (t2 + 1 < NT) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        a[(t2 + 1) & 1][t] = *(const f32x4*)(P + swz<AW>(16 * t + i, 16 * (t2 + 1) + 4 * h));
+    }
 #pragma unroll
     for (int r2 = 0; r2 < 4; ++r2)
 #pragma unroll
       for (int t = 0; t < NT; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][r2], u[4 * t2 + r2], acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t2 & 1][t][r2], u[4 * t2 + r2], acc[t], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t)
@@ -205,7 +212,7 @@ struct Kid {
 // discarded by selects.
 template <int AW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AW == 64 ? 1 : 2)))
-sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0, int slot0) {
   constexpr int NV = AW / 4;
   constexpr int CPR = AW / 4;
   constexpr int NTH = 64 * kWPG;
@@ -233,7 +240,7 @@ sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   const long long col = (long long)tidx * kTW + wcol;
   const bool valid = col < C;
   const long long cc = valid ? col : C - 1;
-  SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
+  SB_GLOBAL char* st = glb<char>(L.stage) + (long long)(slot0 + tl) * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
   SB_GLOBAL float* UU = (SB_GLOBAL float*)st;
   SB_GLOBAL float* UT = UU + vplane;
@@ -596,7 +603,7 @@ sb_gwalk_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
 // and D2 = (V C) V^T = V C V^T; counts = D2 o S with the diagonal unscaled.
 template <int AW, int WPG, int TC>
 __global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(2)))
-sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0, int slot0) {
   constexpr int MT = AW / 16;
   constexpr int NA = MT * MT * 4;                 // accumulator registers per column
   constexpr int PS = AW + 4;
@@ -626,7 +633,7 @@ sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
   const int A = B->A, R = B->R;
   const int AP = L.AP, RM = L.rmax;
   const int wcol0 = qq * (TC * WPG) + wv * TC;
-  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
+  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)(slot0 + tl) * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
   const SB_GLOBAL float* UT = (const SB_GLOBAL float*)st + vplane;
   const SB_GLOBAL float* DT = UT + 2 * vplane;
@@ -789,7 +796,7 @@ sb_gacc_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
 // factors each lane needs.  One workgroup of 4 waves per (tile, branch);
 // branch 0 writes zeros.
 template <int AW>
-__global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restrict__ Lp, int tile0) {
+__global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restrict__ Lp, int tile0, int slot0) {
   constexpr int NV = AW / 4;
   constexpr int MT = AW / 16;
   constexpr int PS = AW + 4;
@@ -823,7 +830,7 @@ __global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restr
   stage<AW, false>(sV, md + sb_mr_V(AP), AP, tid, NTH);
   stage<AW, false>(sS, md + sb_mr_S(AP), AP, tid, NTH);   // S symmetric: S[j][i] = S[i][j]
   // this lane's factors: Ut_c and Dt_c at states 16 (v / 4) + 4 h + v % 4
-  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)tl * L.stage_stride;
+  const SB_GLOBAL char* st = glb<char>(L.stage) + (long long)(slot0 + tl) * L.stage_stride;
   const long long vplane = (long long)RM * kTW * AW;
   const SB_GLOBAL float* UT = (const SB_GLOBAL float*)st + vplane;
   const SB_GLOBAL float* DT = UT + 2 * vplane;
@@ -891,12 +898,12 @@ __global__ void __launch_bounds__(256) sb_gbranch_kernel(const SbLaunch* __restr
   }
 }
 
-template __global__ void sb_gwalk_kernel<32>(const SbLaunch*, int);
-template __global__ void sb_gwalk_kernel<64>(const SbLaunch*, int);
-template __global__ void sb_gacc_kernel<32, kWPG2, Gacc<32>::TC>(const SbLaunch*, int);
-template __global__ void sb_gacc_kernel<64, kWPG2, Gacc<64>::TC>(const SbLaunch*, int);
-template __global__ void sb_gbranch_kernel<32>(const SbLaunch*, int);
-template __global__ void sb_gbranch_kernel<64>(const SbLaunch*, int);
+template __global__ void sb_gwalk_kernel<32>(const SbLaunch*, int, int);
+template __global__ void sb_gwalk_kernel<64>(const SbLaunch*, int, int);
+template __global__ void sb_gacc_kernel<32, kWPG2, Gacc<32>::TC>(const SbLaunch*, int, int);
+template __global__ void sb_gacc_kernel<64, kWPG2, Gacc<64>::TC>(const SbLaunch*, int, int);
+template __global__ void sb_gbranch_kernel<32>(const SbLaunch*, int, int);
+template __global__ void sb_gbranch_kernel<64>(const SbLaunch*, int, int);
 
 extern "C" int sb_gen2_tile_cols() { return kTW; }
 
@@ -906,25 +913,39 @@ extern "C" long long sb_gen2_tile_bytes(int AP, int rmax) {
   return (b + 255) / 256 * 256;
 }
 
-// Both passes of one round of tiles; branch != 0: pass 2 writes per-branch
-// counts (rmax workgroups per tile) instead of summed counts.
-extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, int branch, const SbLaunch* L,
-                                     hipStream_t s) {
-  if (AP > 64 || ntiles <= 0 || rmax <= 0) return hipErrorInvalidValue;
+// One pass of tiles tile0 .. tile0 + ntiles - 1 of the launch, whose stage
+// slots are slot0 .. slot0 + ntiles - 1: pass 1 (the walk) or pass 2 (branch
+// != 0: per-branch counts, rmax workgroups per tile; else summed counts).
+extern "C" hipError_t sb_launch_gen2_pass(int AP, int tile0, int slot0, int ntiles, int rmax, int branch, int pass,
+                                          const SbLaunch* L, hipStream_t s) {
+  if (AP > 64 || ntiles <= 0 || rmax <= 0 || (pass != 1 && pass != 2)) return hipErrorInvalidValue;
   if (AP <= 32) {
-    hipLaunchKernelGGL(sb_gwalk_kernel<32>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0);
-    if (branch)
-      hipLaunchKernelGGL(sb_gbranch_kernel<32>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
+    if (pass == 1)
+      hipLaunchKernelGGL(sb_gwalk_kernel<32>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0, slot0);
+    else if (branch)
+      hipLaunchKernelGGL(sb_gbranch_kernel<32>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0, slot0);
     else
       hipLaunchKernelGGL((sb_gacc_kernel<32, kWPG2, Gacc<32>::TC>), dim3(ntiles * (kTW / (Gacc<32>::TC * kWPG2))),
-                         dim3(64 * kWPG2), 0, s, L, tile0);
+                         dim3(64 * kWPG2), 0, s, L, tile0, slot0);
   } else {
-    hipLaunchKernelGGL(sb_gwalk_kernel<64>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0);
-    if (branch)
-      hipLaunchKernelGGL(sb_gbranch_kernel<64>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0);
+    if (pass == 1)
+      hipLaunchKernelGGL(sb_gwalk_kernel<64>, dim3(ntiles), dim3(64 * kWPG), 0, s, L, tile0, slot0);
+    else if (branch)
+      hipLaunchKernelGGL(sb_gbranch_kernel<64>, dim3(ntiles * rmax), dim3(64 * kWPG), 0, s, L, tile0, slot0);
     else
       hipLaunchKernelGGL((sb_gacc_kernel<64, kWPG2, Gacc<64>::TC>), dim3(ntiles * (kTW / (Gacc<64>::TC * kWPG2))),
-                         dim3(64 * kWPG2), 0, s, L, tile0);
+                         dim3(64 * kWPG2), 0, s, L, tile0, slot0);
   }
   return hipGetLastError();
+}
+
+// Pass-1 workgroups one CU holds at once (its register and LDS budget).
+extern "C" int sb_gen2_pass1_per_cu(int AP) { return AP <= 32 ? 2 : 1; }
+
+// Both passes of one round of tiles (stage slots from 0).
+extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, int branch, const SbLaunch* L,
+                                     hipStream_t s) {
+  hipError_t e = sb_launch_gen2_pass(AP, tile0, 0, ntiles, rmax, branch, 1, L, s);
+  if (e == hipSuccess) e = sb_launch_gen2_pass(AP, tile0, 0, ntiles, rmax, branch, 2, L, s);
+  return e;
 }

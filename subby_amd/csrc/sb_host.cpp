@@ -53,6 +53,9 @@ extern "C" hipError_t sb_launch_post(int dtype, int AP, bool f81, bool irr, int 
                                      const SbLaunch* L, hipStream_t s);
 extern "C" int sb_gen2_tile_cols();
 extern "C" long long sb_gen2_tile_bytes(int AP, int rmax);
+extern "C" hipError_t sb_launch_gen2_pass(int AP, int tile0, int slot0, int ntiles, int rmax, int branch, int pass,
+                                          const SbLaunch* L, hipStream_t s);
+extern "C" int sb_gen2_pass1_per_cu(int AP);
 extern "C" hipError_t sb_launch_gen2(int AP, int tile0, int ntiles, int rmax, int branch, const SbLaunch* L,
                                      hipStream_t s);
 
@@ -225,6 +228,10 @@ struct sb_batch {
   bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
   int g2_rmax = 1;                  // and the largest tree of the launch
+  // second stream of the two-pass path: pass 2 of a round's first tiles beside
+  // the tail of its pass 1 (created on first use)
+  hipStream_t aux = nullptr;
+  hipEvent_t g2_ev[2] = {};
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
   // kernel timing (eager submissions only)
   bool timing = false;
@@ -533,9 +540,31 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
   if (key.large == 4) {
     SB_HIP(sb_launch_post(key.dtype, key.AP, key.f81, key.irr != 0, nwaves, b->header, s));
   } else if (key.large == 3 || key.large == 8) {
-    for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round)
-      SB_HIP(sb_launch_gen2(key.AP, t0, std::min(b->g2_round, b->g2_ntiles - t0), b->g2_rmax,
-                            key.large == 8 ? 1 : 0, b->header, s));
+    const int branch = key.large == 8 ? 1 : 0, rmax = b->g2_rmax;
+    const int wave1 = b->ctx->num_cus * sb_gen2_pass1_per_cu(key.AP);   // pass-1 workgroups held at once
+    for (int t0 = 0; t0 < b->g2_ntiles; t0 += b->g2_round) {
+      const int n = std::min(b->g2_round, b->g2_ntiles - t0);
+      const int k = n / wave1 * wave1;   // tiles of whole pass-1 waves
+      if (b->fixed || k == 0 || k == n) {
+        SB_HIP(sb_launch_gen2(key.AP, t0, n, rmax, branch, b->header, s));
+        continue;
+      }
+      // The last n - k pass-1 workgroups leave most CUs idle: pass 2 of the
+      // first k tiles runs beside them on the second stream.  The round ends
+      // when both streams are done (the next round reuses the stage slots).
+      if (!b->aux) {
+        SB_HIP(hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking));
+        for (hipEvent_t& e : b->g2_ev) SB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      }
+      SB_HIP(sb_launch_gen2_pass(key.AP, t0, 0, k, rmax, branch, 1, b->header, s));
+      SB_HIP(hipEventRecord(b->g2_ev[0], s));
+      SB_HIP(sb_launch_gen2_pass(key.AP, t0 + k, k, n - k, rmax, branch, 1, b->header, s));
+      SB_HIP(hipStreamWaitEvent(b->aux, b->g2_ev[0], 0));
+      SB_HIP(sb_launch_gen2_pass(key.AP, t0, 0, k, rmax, branch, 2, b->header, b->aux));
+      SB_HIP(hipEventRecord(b->g2_ev[1], b->aux));
+      SB_HIP(sb_launch_gen2_pass(key.AP, t0 + k, k, n - k, rmax, branch, 2, b->header, s));
+      SB_HIP(hipStreamWaitEvent(s, b->g2_ev[1], 0));
+    }
   } else if (key.large == 2)
     SB_HIP(sb_launch_f81w(key.AP, nwaves, b->header, s));
   else if (key.large)   // 1 counts, 5 posteriors, 6 per-branch counts, 7 irreversible counts
@@ -994,6 +1023,9 @@ void batch_free(sb_batch* b) {
   live.erase(std::remove(live.begin(), live.end(), b), live.end());
   for (hipEvent_t e : b->ring_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->g2_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->aux) (void)hipStreamDestroy(b->aux);
   for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
   for (auto* v : {&b->tev, &b->tfree})
     for (auto& e : *v) {

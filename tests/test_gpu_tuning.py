@@ -63,3 +63,39 @@ def test_every_candidate_gives_the_same_bits(eng, A, C, leaves):
     L.submit([blk])
     L.sync()
     assert torch.equal(blk.counts, ref[0])
+
+
+@pytest.mark.parametrize('A,C', [(64, 20000), (20, 36000)])
+def test_two_pass_tail_overlap(eng, A, C):
+    """More pass-1 tiles than the CUs hold at once (A = 64: 313 tiles of 64 columns on 256
+    CUs; A = 20: 563 tiles on 512 slots): the last pass-1 workgroups run beside pass 2 of
+    the first tiles on a second stream.  The results equal, bitwise, the same call in
+    rounds too small to split (gen2_round below one CU-wave of tiles), and the oracle on
+    columns taken from the first tiles and from the tail."""
+    import subby_amd as sb
+    from oracle import subby_oracle as O
+    from subby_amd.autotune import apply
+    from subby_amd.batch import Block, Launcher
+    say('two-pass tail overlap, A =', A, 'C =', C)
+    t, m, aln = _blocks(sb, A, C, 910 + A, 7)
+    aln_d = torch.from_numpy(aln).to(eng.device)
+    L = Launcher(eng)
+    out = []
+    for rnd in (0, 100):
+        apply(L, dict(gen2_round=rnd))
+        blk = Block(aln_d, t, m)
+        L.submit([blk])
+        L.sync()
+        out.append((blk.counts.clone(), blk.loglike.clone(), blk.root.clone()))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+    cols = np.r_[0:40, 16380:16420, C - 60:C]
+    ot = O.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent))
+    om = O.DiagModel(*(np.asarray(x, dtype=np.float64) for x in m))
+    rc, rl, _ = O.counts_loglike_root(aln[:, cols], ot, om)
+    got = out[0][0][:, :, torch.from_numpy(cols).to(eng.device)].double().cpu().numpy()
+    num = np.abs(got - rc).max(axis=(0, 1))
+    den = np.abs(rc).max(axis=(0, 1))
+    assert np.max(num / den) <= 1e-4
+    ll = out[0][1].cpu().numpy()[cols]
+    assert np.max(np.abs(ll - rl) / np.maximum(np.abs(rl), 1)) <= 1e-5
