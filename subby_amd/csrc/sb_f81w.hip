@@ -14,11 +14,14 @@
 // matrix cores (v_mfma_f32_32x32x2f32) with the AW x AW accumulator of each
 // column in 16 * (AW/32)^2 accumulation registers.
 //
-// Layout: one wave owns a tile of 4 columns.  Lane l holds, for column l/16,
-// states (l%16) + 16 j, j < AW/16, of every state vector, so per-column sums and
-// maxima are four DPP steps inside a 16-lane row and every lane of the row
-// ends with the identical total.  Scaled child vectors go through LDS into the
-// MFMA operand layout (lane = state within a 32-state half x branch of the pair).
+// Layout: one wave owns a tile of 2 columns.  Lane l holds, for column l/32,
+// states (l%32) + 32 j, j < AW/32, of every state vector, so per-column sums and
+// maxima are four DPP steps inside a 16-lane row plus one row-pair swap, and
+// every lane of the column ends with the identical total.  Two columns'
+// accumulators fill half of the accumulation registers, so two waves share a
+// SIMD and one's walk latency hides behind the other's work.  Scaled child
+// vectors go through LDS into the MFMA operand layout (lane = state within a
+// 32-state half x branch of the pair).
 //
 // Rescaling, Steiner flags and scales are those of the other kernels: exact
 // power-of-two rescaling with integer exponents, scale = ldexp(1/s0, eD+eU-e0).
@@ -36,8 +39,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kTC = 4;   // columns per wave
-constexpr int kWG = 4;   // waves per workgroup: a group tile is kTC * kWG = 16 consecutive columns
+constexpr int kLPC = 32;             // lanes per column
+constexpr int kTC = SB_WAVE / kLPC;  // columns per wave
+constexpr int kWG = 8;               // waves per workgroup: a group tile is kTC * kWG = 16 consecutive columns
 constexpr int kXS = 20;  // row stride (floats) of the count exchange buffer: 16 columns + pad
 
 template <typename T>
@@ -95,6 +99,39 @@ __device__ inline float row_max(float v) {
   v = fmaxf(v, dppf<0x140>(v));
   return v;
 }
+// Over the 32 lanes of a column (two rows): the row result, then the pair of
+// rows traded by v_permlane16_swap; both rows see (row 0, row 1) in the same
+// order, so every lane ends with the same bits.
+__device__ inline float col_sum(float v) {
+  v = row_sum(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(a[0]) + __uint_as_float(a[1]);
+}
+__device__ inline double col_sum(double v) {
+  v = row_sum(v);
+  const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+__device__ inline float col_max(float v) {
+  v = row_max(v);
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+}
+
+// A lane offset the compiler cannot hoist out of the walk: a loop-invariant
+// per-lane 64-bit address (base + offset) would cost a 64-bit add per access,
+// while a uniform base plus this 32-bit offset is the load's own addressing.
+__device__ inline unsigned opq(unsigned x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+// the element at a uniform base plus a byte offset
+template <typename T, typename P>
+__device__ inline T& at(P* base, unsigned off) {
+  return *(T*)((SB_GLOBAL char*)base + off);
+}
 
 template <int NS>
 struct Child {
@@ -106,15 +143,16 @@ struct Child {
 
 }  // namespace
 
-// A workgroup of four waves takes a group tile of 16 consecutive columns of
-// one block; wave w owns columns 4w .. 4w + 3 and walks the tree on its own
-// (wave-scope LDS exchanges only).  At the end the four waves trade counts
+// A workgroup of eight waves takes a group tile of 16 consecutive columns of
+// one block; wave w owns columns 2w, 2w + 1 and walks the tree on its own
+// (wave-scope LDS exchanges only).  At the end the eight waves trade counts
 // through LDS so that each count plane gets 64-byte stores of 16 columns.
 // Scratch per wave: U and D vectors of internal nodes ([slot][NS][64] floats)
-// followed by per-column words ([slot][4] int pairs: {word, pi.u or sum d}).
+// followed by per-column words ([slot][kTC] int pairs: {word, pi.u or sum d}).
 template <int AW>
-__global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
-  constexpr int NS = AW / 16;   // states per lane
+__global__ void __launch_bounds__(64 * kWG) __attribute__((amdgpu_waves_per_eu(2)))
+sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
+  constexpr int NS = AW / kLPC;   // states per lane
   constexpr int NT = AW / 32;   // MFMA tiles per dimension
   typedef Child<NS> C_;
   const SbLaunch& L = *Lp;
@@ -123,10 +161,10 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
   __shared__ float piT[AW + 1];
   __shared__ int gtile;
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // (wave-uniform)
   auto& opnd = opnd_all[wv];
-  const int cl = lane >> 4;      // column of this lane
-  const int rl = lane & 15;
+  const int cl = lane / kLPC;     // column of this lane
+  const int rl = lane % kLPC;
   const int AP = L.AP;
   const int NI = L.nint_max;
   const long long vbytes = (long long)NI * NS * SB_WAVE * 4;
@@ -156,6 +194,13 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     const bool valid = mycol < C;
     const long long cc = valid ? mycol : C - 1;
     const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln);
+    // per-lane data addressed as a uniform base plus an unsigned 32-bit lane offset
+    // (the group tile's first column is < C, so the clamped column cc of a lane
+    // past the block's end is still at or after it: lcl is in 0 .. 15)
+    const SB_GLOBAL int8_t* alnt = aln + gcol0;         // the group tile's first column
+    const unsigned lcl = unsigned(cc - gcol0);          // this lane's column in the tile (clamped)
+    const unsigned ul = unsigned(4 * lane);        // byte offsets
+    const unsigned ucl = unsigned(8 * cl);
     const SB_CONST float* bt = cst<float>(B->bt);
     const SB_CONST float* mdc = cst<float>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
@@ -163,7 +208,7 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     float pis[NS], ones[NS];
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const int s = rl + 16 * j;
+      const int s = rl + kLPC * j;
       pis[j] = s < A ? mdc[sb_mr_pi(AP) + s] : 0.f;
       ones[j] = s < A ? 1.f : 0.f;
     }
@@ -173,11 +218,11 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     }
     lds_barrier();
 
-    auto token = [&](int n) -> int { return aln[(long long)n * C + cc]; };
+    auto token = [&](int n) -> int { return (alnt + (long long)n * C)[opq(lcl)]; };
     auto emul = [&](int n) -> float { return bt[(long long)n * BTS + sb_bt_E(AP) + kstar]; };
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
     // rows of the explicit mask (the alignment stands in when there is none; never used then)
-    const SB_GLOBAL uint8_t* mrow = B->mask ? glb<uint8_t>(B->mask) : (const SB_GLOBAL uint8_t*)aln;
+    const SB_GLOBAL uint8_t* mrow = (B->mask ? glb<uint8_t>(B->mask) : (const SB_GLOBAL uint8_t*)aln) + gcol0;
 
     // Everything one tree record needs is fetched one record ahead (its node
     // indices two ahead) with unconditional loads: slots of leaf children are
@@ -206,11 +251,11 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
       const long long a = ix.sl < 0 ? 0 : ix.sl, b = ix.sr < 0 ? 0 : ix.sr;
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
-        x.ul[j] = Ureg[(a * NS + j) * SB_WAVE + lane];
-        x.ur[j] = Ureg[(b * NS + j) * SB_WAVE + lane];
+        x.ul[j] = at<SB_GLOBAL float>(Ureg + (a * NS + j) * SB_WAVE, opq(ul));
+        x.ur[j] = at<SB_GLOBAL float>(Ureg + (b * NS + j) * SB_WAVE, opq(ul));
       }
-      x.wl = Uw[a * kTC + cl];
-      x.wr = Uw[b * kTC + cl];
+      x.wl = at<SB_GLOBAL i32x2>(Uw + a * kTC, opq(ucl));
+      x.wr = at<SB_GLOBAL i32x2>(Uw + b * kTC, opq(ucl));
       x.el = emul(ix.l);
       x.er = emul(ix.r);
     };
@@ -218,7 +263,7 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     auto token_vec = [&](int tok, float* u) -> float {
       const bool obs = tok >= 0 && tok < A;
 #pragma unroll
-      for (int j = 0; j < NS; ++j) u[j] = obs ? ((rl + 16 * j) == tok ? 1.f : 0.f) : ones[j];
+      for (int j = 0; j < NS; ++j) u[j] = obs ? ((rl + kLPC * j) == tok ? 1.f : 0.f) : ones[j];
       return piT[obs ? tok : AW];
     };
     // child of a record: leaf (token), internal computed by the previous
@@ -253,7 +298,7 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
       float m = v[0];
 #pragma unroll
       for (int j = 1; j < NS; ++j) m = fmaxf(m, v[j]);
-      const int x = __builtin_amdgcn_frexp_expf(row_max(m));
+      const int x = __builtin_amdgcn_frexp_expf(col_max(m));
 #pragma unroll
       for (int j = 0; j < NS; ++j) v[j] = __builtin_amdgcn_ldexpf(v[j], -x);
       return x;
@@ -262,7 +307,7 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
       float s = 0.f;
 #pragma unroll
       for (int j = 0; j < NS; ++j) s = fmaf(pis[j], v[j], s);
-      return row_sum(s);
+      return col_sum(s);
     };
 
     // ---------------- inside pass (postorder) ----------------
@@ -293,14 +338,19 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
         (void)token_vec(cur.tp, o);
 #pragma unroll
         for (int j = 0; j < NS; ++j) up[j] = o[j] * wl[j] * up[j];
-        const int e = rescale(up);
+        // scaled by the power of two that puts pi . u in [0.5, 1): the sum is
+        // needed anyway, and an exact power-of-two scale changes no rounding
+        const float pu0 = dot_pi(up);
+        const int e = __builtin_amdgcn_frexp_expf(pu0);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) up[j] = __builtin_amdgcn_ldexpf(up[j], -e);
+        const float pu = __builtin_amdgcn_ldexpf(pu0, -e);
         const int ep = fl.e + fr.e + e;
         const int word = (ep << 2) | ((fl.has && fr.has) ? 2 : 0) | ((fl.has || fr.has) ? 1 : 0);
-        const float pu = dot_pi(up);
         // (the root's own slot 0 only ever serves as the clamped dummy)
 #pragma unroll
-        for (int j = 0; j < NS; ++j) Ureg[((long long)i * NS + j) * SB_WAVE + lane] = up[j];
-        Uw[i * kTC + cl] = i32x2{word, __float_as_int(pu)};   // same value from the whole row
+        for (int j = 0; j < NS; ++j) at<SB_GLOBAL float>(Ureg + ((long long)i * NS + j) * SB_WAVE, opq(ul)) = up[j];
+        at<SB_GLOBAL i32x2>(Uw + i * kTC, opq(ucl)) = i32x2{word, __float_as_int(pu)};   // same value from the whole column
 #pragma unroll
         for (int j = 0; j < NS; ++j) pv[j] = up[j];
         pw = word;
@@ -324,17 +374,17 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     double s0 = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const int s = rl + 16 * j;
+      const int s = rl + kLPC * j;
       if (s < A) s0 = fma(pi64[s], double(u0[j]), s0);
     }
-    s0 = row_sum(s0);
+    s0 = col_sum(s0);
     const float inv_s0 = float(1.0 / s0);
     if (valid && rl == 0 && (flags & SB_F_LL) && B->loglike)
       glb<double>(B->loglike)[cc] = double(e0) * 0.69314718055994530942 + log(s0);
     if (valid && (flags & SB_F_ROOT) && B->root) {
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
-        const int s = rl + 16 * j;
+        const int s = rl + kLPC * j;
         if (s < A) glb<float>(B->root)[(long long)s * C + cc] = pis[j] * u0[j] * inv_s0;
       }
     }
@@ -362,8 +412,8 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     };
     auto fetch_out = [&](Out& y, const Idx& ix, int k) {
 #pragma unroll
-      for (int j = 0; j < NS; ++j) y.dp[j] = Dreg[((long long)k * NS + j) * SB_WAVE + lane];
-      y.wp = Dw[k * kTC + cl];
+      for (int j = 0; j < NS; ++j) y.dp[j] = at<SB_GLOBAL float>(Dreg + ((long long)k * NS + j) * SB_WAVE, opq(ul));
+      y.wp = at<SB_GLOBAL i32x2>(Dw + k * kTC, opq(ucl));
       y.ep = emul(ix.p);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
@@ -372,7 +422,7 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
         y.al[s] = cf[0];
         y.be[s] = cf[1];
         y.ga[s] = cf[2];
-        y.mk[s] = mrow[(long long)n * C + cc];
+        y.mk[s] = (mrow + (long long)n * C)[opq(lcl)];
       }
     };
     if (nint > 0) {
@@ -436,19 +486,24 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
           apply_M(s == 0 ? cur.er : cur.el, fs, d);
 #pragma unroll
           for (int j = 0; j < NS; ++j) d[j] *= x[j];
-          const int e = rescale(d);
+          // scaled by the power of two that puts sum(d) in [0.5, 1) (the sum
+          // is needed anyway; an exact power-of-two scale changes no rounding)
           float sd = 0.f;
 #pragma unroll
           for (int j = 0; j < NS; ++j) sd += d[j];
-          sd = row_sum(sd);
+          sd = col_sum(sd);
+          const int e = __builtin_amdgcn_frexp_expf(sd);
+#pragma unroll
+          for (int j = 0; j < NS; ++j) d[j] = __builtin_amdgcn_ldexpf(d[j], -e);
+          sd = __builtin_amdgcn_ldexpf(sd, -e);
           const int ed = fs.e + ex + e;
           const bool st = (fn.leaf ? fn.has : fn.branch) || (stp && fn.has);
           const int dword = (ed << 1) | (st ? 1 : 0);
           // leaf children write the never-read slot 0 (the root has no D)
           const long long ds_slot = sn < 0 ? 0 : sn;
 #pragma unroll
-          for (int j = 0; j < NS; ++j) Dreg[(ds_slot * NS + j) * SB_WAVE + lane] = d[j];
-          Dw[ds_slot * kTC + cl] = i32x2{dword, __float_as_int(sd)};
+          for (int j = 0; j < NS; ++j) at<SB_GLOBAL float>(Dreg + (ds_slot * NS + j) * SB_WAVE, opq(ul)) = d[j];
+          at<SB_GLOBAL i32x2>(Dw + ds_slot * kTC, opq(ucl)) = i32x2{dword, __float_as_int(sd)};
 #pragma unroll
           for (int j = 0; j < NS; ++j) pd[s][j] = d[j];
           pdw[s] = dword;
@@ -466,8 +521,8 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
             const float ds = d[j] * sc;
             t2a[j] = fmaf(be * ds, fn.pu, t2a[j]);
             t2b[j] = fmaf(be * dsum, fn.u[j], t2b[j]);
-            opnd[0][s][cl][rl + 16 * j] = al * ds;
-            opnd[1][s][cl][rl + 16 * j] = fn.u[j];
+            opnd[0][s][cl][rl + kLPC * j] = al * ds;
+            opnd[1][s][cl][rl + kLPC * j] = fn.u[j];
           }
           t3 = fmaf(oc.ga[s] * dsum, fn.pu, t3);
         }
@@ -504,10 +559,10 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     // [t2a pi] [1 (t2b + t3)]^T.
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      opnd[0][0][cl][rl + 16 * j] = t2a[j];
-      opnd[0][1][cl][rl + 16 * j] = pis[j];
-      opnd[1][0][cl][rl + 16 * j] = 1.f;
-      opnd[1][1][cl][rl + 16 * j] = t2b[j] + t3;
+      opnd[0][0][cl][rl + kLPC * j] = t2a[j];
+      opnd[0][1][cl][rl + kLPC * j] = pis[j];
+      opnd[1][0][cl][rl + kLPC * j] = 1.f;
+      opnd[1][1][cl][rl + kLPC * j] = t2b[j] + t3;
     }
     wave_sync();
     const int kb = lane >> 5, ii = lane & 31;
@@ -528,8 +583,8 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
     // counts_ii = I_ii, counts_ij = mu pi_j I_ij; accumulator element q of tile
     // (a, b) in this lane is I[32a + 8(q/4) + 4(lane/32) + q%4][32b + lane%32].
     // Chunks of 256 planes (tile (a, b), q / 4 fixed) go through LDS: wave w
-    // writes its 4 columns of each plane, then the workgroup stores 16 columns
-    // per plane.
+    // writes its kTC columns of each plane, then the workgroup stores 16
+    // columns per plane.
     const float fmu = mdc[sb_mr_misc(AP)];
     SB_GLOBAL float* out = glb<float>(B->counts);
     const bool full = (C % 4 == 0) && (gcol0 + kTC * kWG <= C);
@@ -548,12 +603,12 @@ __global__ void __launch_bounds__(64 * kWG) sb_f81w_kernel(const SbLaunch* __res
               const int q = 4 * qg + r;
               const int i = 32 * a + 8 * qg + 4 * kb + r;
               const float f = i == j ? 1.f : fj;
-              *(f32x4*)&xb[(r * 64 + lane) * kXS + kTC * wv] =
-                  f32x4{f * acc[0][a][b][q], f * acc[1][a][b][q], f * acc[2][a][b][q], f * acc[3][a][b][q]};
+#pragma unroll
+              for (int c = 0; c < kTC; ++c) xb[(r * 64 + lane) * kXS + kTC * wv + c] = f * acc[c][a][b][q];
             }
             lds_barrier();
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
+            for (int m = 0; m < 256 * 4 / (64 * kWG); ++m) {
               const int e = threadIdx.x + 64 * kWG * m;   // float4 index: plane e / 4, part e % 4
               const int pl = e >> 2, part = e & 3;
               const int r = pl >> 6, Lp_ = pl & 63;
@@ -593,7 +648,7 @@ extern "C" int sb_f81w_waves_per_group() { return kWG; }
 
 extern "C" long long sb_f81w_wave_stride(int AP, int nint_max) {
   const int AW = AP <= 32 ? 32 : 64;
-  const long long NS = AW / 16;
+  const long long NS = AW / kLPC;
   return (long long)nint_max * (2 * NS * SB_WAVE * 4 + 2 * kTC * 8);
 }
 

@@ -66,21 +66,23 @@ struct Tile {
     return (long long)nint * NCH * SB_WAVE * 16;
   }
   // (the slot's base is wave-uniform: a scalar base plus a 32-bit lane offset)
+  // (a uniform slot address plus an unsigned 32-bit lane offset: no 64-bit
+  // address arithmetic per lane)
   static __device__ inline void store_vec(SB_GLOBAL char* region, int slot, int lane, const T* v) {
-    SB_GLOBAL CT* p = (SB_GLOBAL CT*)(region + (long long)slot * (NCH * SB_WAVE * 16));
+    SB_GLOBAL char* p = region + (long long)slot * (NCH * SB_WAVE * 16);
 #pragma unroll
     for (int q = 0; q < NCH; ++q) {
       CT c;
       if constexpr (EPC == 4) { c.x = v[q * 4]; c.y = v[q * 4 + 1]; c.z = v[q * 4 + 2]; c.w = v[q * 4 + 3]; }
       else { c.x = v[q * 2]; c.y = v[q * 2 + 1]; }
-      p[q * SB_WAVE + lane] = c;
+      *(SB_GLOBAL CT*)(p + unsigned((q * SB_WAVE + lane) * 16)) = c;
     }
   }
   static __device__ inline void load_vec(const SB_GLOBAL char* region, int slot, int lane, T* v) {
-    const SB_GLOBAL CT* p = (const SB_GLOBAL CT*)(region + (long long)slot * (NCH * SB_WAVE * 16));
+    const SB_GLOBAL char* p = region + (long long)slot * (NCH * SB_WAVE * 16);
 #pragma unroll
     for (int q = 0; q < NCH; ++q) {
-      CT c = p[q * SB_WAVE + lane];
+      CT c = *(const SB_GLOBAL CT*)(p + unsigned((q * SB_WAVE + lane) * 16));
       if constexpr (EPC == 4) { v[q * 4] = c.x; v[q * 4 + 1] = c.y; v[q * 4 + 2] = c.z; v[q * 4 + 3] = c.w; }
       else { v[q * 2] = c.x; v[q * 2 + 1] = c.y; }
     }
@@ -125,10 +127,11 @@ struct TokDec {
 // load and the use, so a token fetched a record ahead is not waited for at
 // the end of the record that fetched it.
 struct TokSrc {
-  const SB_GLOBAL uint8_t* aln;     // this lane's column of the alignment (row stride C)
+  const SB_GLOBAL uint8_t* aln;     // the tile's first column of the alignment (row stride C)
+  unsigned lc;                      // this lane's column in the tile (clamped to the block)
   long long C;
 };
-__device__ inline int tok_at(const TokSrc& ts, int n) { return int(ts.aln[(long long)n * ts.C]); }
+__device__ inline int tok_at(const TokSrc& ts, int n) { return int((ts.aln + (long long)n * ts.C)[ts.lc]); }
 
 // w = M u  (w_a = sum_j M[a][j] u_j)
 template <typename T, int AP>
@@ -213,7 +216,7 @@ template <typename T, int AP>
 __device__ inline void fetch_vec(T* v, int& w, int slot, const SB_GLOBAL char* reg, const SB_GLOBAL int* words,
                                  int lane) {
   Tile<T, AP>::load_vec(reg, slot, lane, v);
-  w = words[slot * SB_WAVE + lane];
+  w = (words + slot * SB_WAVE)[unsigned(lane)];
 }
 
 template <typename T, int AP, bool LEFT = true>
@@ -453,6 +456,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     const bool valid = c < C;
     const long long cc = valid ? c : C - 1;
     const SB_GLOBAL int8_t* aln = glb<int8_t>(B->aln) + cc;
+    const long long tcol0 = (long long)tb.y * SB_WAVE;   // the tile's first column
     const SB_CONST T* bt = cst<T>(B->bt);
     const SB_CONST T* md = cst<T>(B->model);
     const SB_CONST int* sched = cst<int>(B->sched);
@@ -461,7 +465,8 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     for (int a = 0; a < AP; ++a) pi[a] = md[sb_mr_pi_k(AP, IRR) + a];
     const int BTS = sb_bt_stride_k(AP, IRR);
     TokSrc ts;
-    ts.aln = (const SB_GLOBAL uint8_t*)aln;
+    ts.aln = glb<uint8_t>(B->aln) + tcol0;
+    ts.lc = unsigned(cc - tcol0);
     ts.C = C;
 
     const bool outside = (flags & (SB_F_COUNTS | SB_F_GRAD)) != 0;
@@ -517,7 +522,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         // to the next record is never read back
         if (outside || !(qn.sl == i || qn.sr == i)) {
           TL::store_vec(Ureg, i, lane, up);
-          Uw[i * SB_WAVE + lane] = word;
+          (Uw + i * SB_WAVE)[unsigned(lane)] = word;
         }
       } else {
 #pragma unroll
@@ -641,7 +646,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         const bool fwc = (s == 0) ? fwd0 : fwd1;
         if (sn >= 0 && !fwc) {
           TL::store_vec(Dreg, sn, lane, d);
-          Dw[sn * SB_WAVE + lane] = dword;
+          (Dw + sn * SB_WAVE)[unsigned(lane)] = dword;
         }
         if (fwc) {   // uniform
 #pragma unroll
