@@ -92,16 +92,11 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-__device__ inline float row_max(float v) {
-  v = fmaxf(v, dppf<0xb1>(v));
-  v = fmaxf(v, dppf<0x4e>(v));
-  v = fmaxf(v, dppf<0x141>(v));
-  v = fmaxf(v, dppf<0x140>(v));
-  return v;
-}
 // Over the 32 lanes of a column (two rows): the row result, then the pair of
 // rows traded by v_permlane16_swap; both rows see (row 0, row 1) in the same
-// order, so every lane ends with the same bits.
+// order, so every lane ends with the same bits.  (Vectors are rescaled by the
+// power of two of their sum, which the walk needs anyway, so no maximum is
+// reduced.)
 __device__ inline float col_sum(float v) {
   v = row_sum(v);
   const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
@@ -114,12 +109,6 @@ __device__ inline double col_sum(double v) {
   const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
   return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
 }
-__device__ inline float col_max(float v) {
-  v = row_max(v);
-  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
-  return fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
-}
-
 // A lane offset the compiler cannot hoist out of the walk: a loop-invariant
 // per-lane 64-bit address (base + offset) would cost a 64-bit add per access,
 // while a uniform base plus this 32-bit offset is the load's own addressing.
@@ -293,15 +282,6 @@ sb_f81w_kernel(const SbLaunch* __restrict__ Lp) {
       const float b = (1.f - e) * ch.pu;
 #pragma unroll
       for (int j = 0; j < NS; ++j) w[j] = ones[j] * fmaf(e, ch.u[j], b);
-    };
-    auto rescale = [&](float* v) -> int {
-      float m = v[0];
-#pragma unroll
-      for (int j = 1; j < NS; ++j) m = fmaxf(m, v[j]);
-      const int x = __builtin_amdgcn_frexp_expf(col_max(m));
-#pragma unroll
-      for (int j = 0; j < NS; ++j) v[j] = __builtin_amdgcn_ldexpf(v[j], -x);
-      return x;
     };
     auto dot_pi = [&](const float* v) -> float {
       float s = 0.f;

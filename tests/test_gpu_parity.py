@@ -605,3 +605,23 @@ def test_models_uploaded_into_reused_records(sb):
     assert eng.arena_stats()[0] == r0
     ref_eng.close()
     eng.close()
+
+
+@pytest.mark.parametrize('A', [20, 64])
+def test_f81_partial_tiles(sb, eng, A):
+    """Blocks narrower than one 16-column group tile of the F81 matrix-core kernel, and
+    ragged tails: waves whose columns all lie past the block's last column read the
+    clamped last column through offsets from the tile's first column (a regression test
+    for offsets taken from a wave's own first column, which went negative there)."""
+    say('F81 partial tiles, A =', A)
+    rng = np.random.default_rng(780 + A)
+    f = sb.f81_model(rng.dirichlet(np.full(A, 2.0)))
+    t = sb.synthetic.random_tree(9, rng, 0.1)
+    for C in (1, 2, 3, 6, 15, 17, 33):
+        aln = sb.synthetic.simulate(t, f, C, rng, gap_frac=0.1, ambig_frac=0.05)
+        for mask in ('auto', None):
+            check_all(sb, eng, aln, t, f, 'fp32', mask=mask, f81=True)
+        mk = rng.random(aln.shape) < 0.5
+        ref = O.Counts(aln, otree(t), omodel(f), f81_fast=True, branch_mask=mk)
+        got = sb.Counts(aln, t, f, f81_fast_flag=True, branch_mask=mk, precision='fp32', engine=eng)
+        assert err_counts(got, ref) <= 1e-4
