@@ -50,6 +50,14 @@ template <typename T> struct Chunk;
 template <> struct Chunk<float> { typedef f32x4 type; static constexpr int n = 4; };
 template <> struct Chunk<double> { typedef f64x2 type; static constexpr int n = 2; };
 
+// A lane offset the compiler cannot hoist out of the walk: a loop-invariant
+// per-lane 64-bit address (base + offset) would cost a 64-bit add per access,
+// while a uniform base plus this 32-bit offset is the load's own addressing.
+__device__ inline unsigned opq(unsigned x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 __device__ inline int frexp_exp(float x) { return __builtin_amdgcn_frexp_expf(x); }
 __device__ inline int frexp_exp(double x) { return __builtin_amdgcn_frexp_exp(x); }
 __device__ inline float ldexp_(float x, int e) { return __builtin_amdgcn_ldexpf(x, e); }
@@ -75,14 +83,14 @@ struct Tile {
       CT c;
       if constexpr (EPC == 4) { c.x = v[q * 4]; c.y = v[q * 4 + 1]; c.z = v[q * 4 + 2]; c.w = v[q * 4 + 3]; }
       else { c.x = v[q * 2]; c.y = v[q * 2 + 1]; }
-      *(SB_GLOBAL CT*)(p + unsigned((q * SB_WAVE + lane) * 16)) = c;
+      *(SB_GLOBAL CT*)(p + q * SB_WAVE * 16 + opq(unsigned(lane) * 16u)) = c;
     }
   }
   static __device__ inline void load_vec(const SB_GLOBAL char* region, int slot, int lane, T* v) {
     const SB_GLOBAL char* p = region + (long long)slot * (NCH * SB_WAVE * 16);
 #pragma unroll
     for (int q = 0; q < NCH; ++q) {
-      CT c = *(const SB_GLOBAL CT*)(p + unsigned((q * SB_WAVE + lane) * 16));
+      CT c = *(const SB_GLOBAL CT*)(p + q * SB_WAVE * 16 + opq(unsigned(lane) * 16u));
       if constexpr (EPC == 4) { v[q * 4] = c.x; v[q * 4 + 1] = c.y; v[q * 4 + 2] = c.z; v[q * 4 + 3] = c.w; }
       else { v[q * 2] = c.x; v[q * 2 + 1] = c.y; }
     }
@@ -131,7 +139,7 @@ struct TokSrc {
   unsigned lc;                      // this lane's column in the tile (clamped to the block)
   long long C;
 };
-__device__ inline int tok_at(const TokSrc& ts, int n) { return int((ts.aln + (long long)n * ts.C)[ts.lc]); }
+__device__ inline int tok_at(const TokSrc& ts, int n) { return int((ts.aln + (long long)n * ts.C)[opq(ts.lc)]); }
 
 // w = M u  (w_a = sum_j M[a][j] u_j)
 template <typename T, int AP>
@@ -216,7 +224,7 @@ template <typename T, int AP>
 __device__ inline void fetch_vec(T* v, int& w, int slot, const SB_GLOBAL char* reg, const SB_GLOBAL int* words,
                                  int lane) {
   Tile<T, AP>::load_vec(reg, slot, lane, v);
-  w = (words + slot * SB_WAVE)[unsigned(lane)];
+  w = *(const SB_GLOBAL int*)((const SB_GLOBAL char*)(words + slot * SB_WAVE) + opq(unsigned(lane) * 4u));
 }
 
 template <typename T, int AP, bool LEFT = true>
@@ -522,7 +530,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         // to the next record is never read back
         if (outside || !(qn.sl == i || qn.sr == i)) {
           TL::store_vec(Ureg, i, lane, up);
-          (Uw + i * SB_WAVE)[unsigned(lane)] = word;
+          *(SB_GLOBAL int*)((SB_GLOBAL char*)(Uw + i * SB_WAVE) + opq(unsigned(lane) * 4u)) = word;
         }
       } else {
 #pragma unroll
@@ -554,10 +562,17 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     if (!outside) continue;
 
     // ---------------- outside pass + accumulation ----------------
+    // Reversible models with one model per block carry each outside vector
+    // divided by pi (HAT): detailed balance, pi_i M_ia = pi_a M_ai, gives
+    // (d^T M)_a / pi_a = (M (d / pi))_a, so the parent's outside product is a
+    // matvec, the root's outside vector is its token alone, and the outside
+    // projection Vi^T d (Vi = V / sqrt(pi)) equals Vs^T (d / pi): one
+    // projection matrix instead of two.  The stored and forwarded outside
+    // vectors of the walk are these; nothing else reads them.
+    constexpr bool HAT = !F81 && !IRR && !ROWS;
     const int mask_mode = (flags >> SB_F_MASK_SHIFT) & 3;
     const SB_GLOBAL uint8_t* emask = B->mask ? glb<uint8_t>(B->mask) + cc : nullptr;
     const SB_CONST T* Vs = md + sb_mr_Vs(AP);
-    const SB_CONST T* Vi = md + sb_mr_Vi(AP);
     const double gc = GRAD ? ((B->g && valid) ? glb<double>(B->g)[cc] : (valid ? 1.0 : 0.0)) : 0.0;
     SB_GLOBAL double* gpart =
         GRAD ? glb<double>(args.grad_part) + (long long)(B->tile_base + tb.y) * args.grad_stride : nullptr;
@@ -611,15 +626,18 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       bool stp;
       td.decode(cur.tp(), x);
       if (i == 0) {
+        if constexpr (!HAT) {
 #pragma unroll
-        for (int a = 0; a < AP; ++a) x[a] *= pi[a];
+          for (int a = 0; a < AP; ++a) x[a] *= pi[a];
+        }
         ex = 0;
         stp = root_branch;
       } else {
         T xm[AP];
         ex = rdw >> 1;
         stp = (rdw & 1) != 0;
-        vecmat<T, AP>(rd, bt + (long long)p * BTS + sb_bt_M(AP), xm);
+        if constexpr (HAT) matvec<T, AP>(bt + (long long)p * BTS + sb_bt_M(AP), rd, xm);
+        else vecmat<T, AP>(rd, bt + (long long)p * BTS + sb_bt_M(AP), xm);
 #pragma unroll
         for (int a = 0; a < AP; ++a) x[a] *= xm[a];
       }
@@ -646,7 +664,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         const bool fwc = (s == 0) ? fwd0 : fwd1;
         if (sn >= 0 && !fwc) {
           TL::store_vec(Dreg, sn, lane, d);
-          (Dw + sn * SB_WAVE)[unsigned(lane)] = dword;
+          *(SB_GLOBAL int*)((SB_GLOBAL char*)(Dw + sn * SB_WAVE) + opq(unsigned(lane) * 4u)) = dword;
         }
         if (fwc) {   // uniform
 #pragma unroll
@@ -832,8 +850,9 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
             T su = T(0), sd = T(0);
 #pragma unroll
             for (int b = 0; b < AP; ++b) {
+              // (d here is D / pi, so the outside projection uses Vs too)
               su = fma(u[b], Vs[b * AP + k], su);
-              sd = fma(d[b], Vi[b * AP + k], sd);
+              sd = fma(d[b], Vs[b * AP + k], sd);
             }
             Ut[k] = su;
             Dt[k] = sd;
