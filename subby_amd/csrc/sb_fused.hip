@@ -153,6 +153,17 @@ __device__ inline void matvec(const SB_CONST T* M, const T* u, T* w) {
   }
 }
 
+template <typename T, int AP>
+__device__ inline void matvec(const T* M, const T* u, T* w) {
+#pragma unroll
+  for (int a = 0; a < AP; ++a) {
+    T s = M[a * AP] * u[0];
+#pragma unroll
+    for (int j = 1; j < AP; ++j) s = fma(M[a * AP + j], u[j], s);
+    w[a] = s;
+  }
+}
+
 // x = d^T M  (x_a = sum_i d_i M[i][a])
 template <typename T, int AP>
 __device__ inline void vecmat(const T* d, const SB_CONST T* M, T* x) {
@@ -237,22 +248,65 @@ __device__ inline void fetch_children(Fetch<T, AP>& f, const InRec& q, int fwd_s
   f.tkp = tok_at(ts, q.p);
 }
 
-// A child from fetched data, by selects (slot < 0: leaf from its token; fwd:
-// the forwarded vector fu / word fw; otherwise the staged vector and word).
-template <typename T, int AP>
-__device__ inline void pick_child(int slot, bool fwd, int tok, const T* su, int sw, const T* fu, int fw,
+// Where a record's child vector comes from.  The kind is uniform per record,
+// so the walk runs one specialised copy of the record per combination of
+// kinds and neither decodes a token nor selects a vector it does not use.
+template <int K>
+struct Kind {
+  static constexpr int v = K;
+};
+constexpr int kLeaf = 0;    // decoded from the leaf's token
+constexpr int kFwd = 1;     // forwarded in registers from the record computed just before (fu / fw)
+constexpr int kStaged = 2;  // staged in the wave's scratch (su / sw)
+constexpr int kAny = 3;     // chosen per record by selects (slot < 0: leaf; fwd: forwarded; else staged)
+
+template <int K, typename T, int AP>
+__device__ inline void take_child(int slot, bool fwd, int tok, const T* su, int sw, const T* fu, int fw,
                                   const TokDec<T, AP>& td, Child<T, AP>& ch) {
-  const bool leaf = slot < 0;
-  const int A = td.A;
-  T tv[AP];
-  td.decode(tok, tv);
+  if constexpr (K == kAny) {
+    const bool leaf = slot < 0;
+    const int A = td.A;
+    T tv[AP];
+    td.decode(tok, tv);
 #pragma unroll
-  for (int a = 0; a < AP; ++a) ch.u[a] = leaf ? tv[a] : (fwd ? fu[a] : su[a]);
-  const int w = fwd ? fw : sw;
-  ch.e = leaf ? 0 : (w >> 2);
-  ch.has = leaf ? (unsigned(tok) <= unsigned(A)) : ((w & 1) != 0);
-  ch.branch = !leaf && ((w & 2) != 0);
-  ch.leaf = leaf;
+    for (int a = 0; a < AP; ++a) ch.u[a] = leaf ? tv[a] : (fwd ? fu[a] : su[a]);
+    const int w = fwd ? fw : sw;
+    ch.e = leaf ? 0 : (w >> 2);
+    ch.has = leaf ? (unsigned(tok) <= unsigned(A)) : ((w & 1) != 0);
+    ch.branch = !leaf && ((w & 2) != 0);
+    ch.leaf = leaf;
+  } else if constexpr (K == kLeaf) {
+    td.decode(tok, ch.u);
+    ch.e = 0;
+    ch.has = unsigned(tok) <= unsigned(td.A);
+    ch.branch = false;
+    ch.leaf = true;
+  } else {
+    const T* v = K == kFwd ? fu : su;
+    const int w = K == kFwd ? fw : sw;
+#pragma unroll
+    for (int a = 0; a < AP; ++a) ch.u[a] = v[a];
+    ch.e = w >> 2;
+    ch.has = (w & 1) != 0;
+    ch.branch = (w & 2) != 0;
+    ch.leaf = false;
+  }
+}
+
+// v *= the likelihood vector of an internal node's token.  Internal nodes'
+// rows are usually all unobserved (token A, or a gap), whose vector is 1 on
+// every real state (padded states are zero in every vector anyway), so the
+// product is skipped unless a lane of the wave observes the node.  Token
+// vectors hold 0 or 1, so the product is exact and the result does not
+// depend on the skip.
+template <typename T, int AP>
+__device__ inline void times_token(int tok, const TokDec<T, AP>& td, T* v) {
+  if (__any(unsigned(tok) < unsigned(td.A))) {
+    T tv[AP];
+    td.decode(tok, tv);
+#pragma unroll
+    for (int a = 0; a < AP; ++a) v[a] = tv[a] * v[a];
+  }
 }
 
 template <typename T, int AP, bool F81, bool IRR>
@@ -484,11 +538,14 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     // pipelined: while record i is computed, everything record i - 1 reads
     // (its right child's staged U and word, both children's tokens and the
     // parent's token) is already in flight.  The fetches are unconditional (a leaf's slot is
-    // clamped to 0, a forwarded child's slot is read but not used) and the
-    // choice between token, staged and forwarded data is a select, so the
+    // clamped to 0, a forwarded child's slot is read but not used), so the
     // wave waits only on fetches issued a whole record earlier.  A child whose
     // slot is i + 1 (an internal left child, or an internal right child next
     // to a leaf) is the record computed just before: forwarded in registers.
+    // Where each child's vector comes from is uniform per record: the
+    // single-model reversible and F81 variants (SPEC) branch to a copy of the
+    // record for its kinds; the others choose by selects.
+    constexpr bool SPEC = !IRR && !ROWS;
     T u0[AP];
     int e0 = 0;
     bool root_branch = false;
@@ -497,29 +554,49 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     if (nint == 0) td.decode(tok_at(ts, 0), u0);  // single-node tree
     // (a tree of one node has no records: nothing is fetched)
     InRec q{0, 0, 0, -1, -1};
-    Fetch<T, AP> cur;
+    Fetch<T, AP> fa, fb;
     if (nint > 0) {
       q = in_rec(sched, nint - 1);
-      fetch_children<T, AP, false>(cur, q, nint, ts, Ureg, Uw, lane);
+      fetch_children<T, AP, false>(fa, q, nint, ts, Ureg, Uw, lane);
     }
     T fu[AP];                      // U and word of the record computed last
     int fw = 0;
 #pragma unroll
     for (int a = 0; a < AP; ++a) fu[a] = T(0);
-    for (int i = nint - 1; i >= 0; --i) {
+    // (an internal left child is the next node in preorder: slot i + 1, forwarded)
+    auto istep = [&](int i, const Fetch<T, AP>& cur, Fetch<T, AP>& nxt, auto LK, auto RK)
+                     __attribute__((always_inline)) {
+      // the record's two matrices and the next record are requested together,
+      // so one scalar round trip covers them
+      T ml[AP * AP], mr[AP * AP];
+      {
+        const SB_CONST T* Ml = bt + (long long)q.l * BTS + sb_bt_M(AP);
+        const SB_CONST T* Mr = bt + (long long)q.r * BTS + sb_bt_M(AP);
+#pragma unroll
+        for (int k = 0; k < AP * AP; ++k) {
+          ml[k] = Ml[k];
+          mr[k] = Mr[k];
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
       const InRec qn = in_rec(sched, i > 0 ? i - 1 : 0);
-      Fetch<T, AP> nxt;
+      __builtin_amdgcn_sched_barrier(0);
       fetch_children<T, AP, false>(nxt, qn, i, ts, Ureg, Uw, lane);
       Child<T, AP> cl, cr;
-      // (an internal left child is the next node in preorder: slot i + 1, forwarded)
-      pick_child<T, AP>(q.sl, true, cur.tl(), fu, fw, fu, fw, td, cl);
-      pick_child<T, AP>(q.sr, q.sr == i + 1, cur.tr(), cur.ur, cur.wr, fu, fw, td, cr);
+      take_child<decltype(LK)::v, T, AP>(q.sl, true, cur.tl(), fu, fw, fu, fw, td, cl);
+      take_child<decltype(RK)::v, T, AP>(q.sr, q.sr == i + 1, cur.tr(), cur.ur, cur.wr, fu, fw, td, cr);
       T wl[AP], wr[AP], up[AP];
-      matvec<T, AP>(bt + (long long)q.l * BTS + sb_bt_M(AP), cl.u, wl);
-      matvec<T, AP>(bt + (long long)q.r * BTS + sb_bt_M(AP), cr.u, wr);
-      td.decode(cur.tp(), up);
+      matvec<T, AP>(ml, cl.u, wl);
+      matvec<T, AP>(mr, cr.u, wr);
+      if constexpr (SPEC) {
 #pragma unroll
-      for (int a = 0; a < AP; ++a) up[a] = up[a] * wl[a] * wr[a];
+        for (int a = 0; a < AP; ++a) up[a] = wl[a] * wr[a];
+        times_token<T, AP>(cur.tp(), td, up);
+      } else {
+        td.decode(cur.tp(), up);
+#pragma unroll
+        for (int a = 0; a < AP; ++a) up[a] = up[a] * wl[a] * wr[a];
+      }
       const int e = rescale<T, AP>(up);
       const int ep = cl.e + cr.e + e;
       const bool has = cl.has || cr.has;
@@ -542,7 +619,28 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       for (int a = 0; a < AP; ++a) fu[a] = up[a];
       fw = word;
       q = qn;
-      cur = nxt;
+    };
+    auto irec = [&](int i, const Fetch<T, AP>& cur, Fetch<T, AP>& nxt) __attribute__((always_inline)) {
+      if constexpr (SPEC) {
+        const int rk = q.sr < 0 ? kLeaf : (q.sr == i + 1 ? kFwd : kStaged);
+        if (q.sl < 0) {
+          if (rk == kLeaf) istep(i, cur, nxt, Kind<kLeaf>{}, Kind<kLeaf>{});
+          else if (rk == kFwd) istep(i, cur, nxt, Kind<kLeaf>{}, Kind<kFwd>{});
+          else istep(i, cur, nxt, Kind<kLeaf>{}, Kind<kStaged>{});
+        } else {
+          if (rk == kLeaf) istep(i, cur, nxt, Kind<kFwd>{}, Kind<kLeaf>{});
+          else if (rk == kFwd) istep(i, cur, nxt, Kind<kFwd>{}, Kind<kFwd>{});
+          else istep(i, cur, nxt, Kind<kFwd>{}, Kind<kStaged>{});
+        }
+      } else {
+        istep(i, cur, nxt, Kind<kAny>{}, Kind<kAny>{});
+      }
+    };
+    // unrolled by two over ping-pong fetch buffers, so no loaded value is copied
+    for (int i = nint - 1; i >= 0; i -= 2) {
+      irec(i, fa, fb);
+      if (i == 0) break;
+      irec(i - 1, fb, fa);
     }
     // log-likelihood and root posterior (pruning.py:84-87; __init__.py:642-650)
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
@@ -605,8 +703,24 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     // other, so no loaded value is copied (a copy would wait for the load).
     auto ostep = [&](int i, const Fetch<T, AP>& cur, const T* cdv, int cdwv, Fetch<T, AP>& nxt, T* ndv,
                      int& ndwv) __attribute__((always_inline)) {
+      // HAT: the record's three transition matrices and the next record are
+      // requested together, so one scalar round trip covers them
+      T mp[HAT ? AP * AP : 1], mlr[HAT ? 2 : 1][HAT ? AP * AP : 1];
+      if constexpr (HAT) {
+        const SB_CONST T* Mp = bt + (long long)oq.p * BTS + sb_bt_M(AP);
+        const SB_CONST T* Ml = bt + (long long)oq.l * BTS + sb_bt_M(AP);
+        const SB_CONST T* Mr = bt + (long long)oq.r * BTS + sb_bt_M(AP);
+#pragma unroll
+        for (int k = 0; k < AP * AP; ++k) {
+          mp[k] = Mp[k];
+          mlr[0][k] = Ml[k];
+          mlr[1][k] = Mr[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
       const int inext = i + 1 < nint ? i + 1 : i;
       const InRec oqn = in_rec(sched, inext);
+      if constexpr (HAT) __builtin_amdgcn_sched_barrier(0);
       const bool last = i + 1 >= nint;
       const bool fwd0 = !last && oqn.p == oq.l;
       const bool fwd1 = !last && oqn.p == oq.r;
@@ -624,8 +738,8 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       T x[AP];
       int ex;
       bool stp;
-      td.decode(cur.tp(), x);
       if (i == 0) {
+        td.decode(cur.tp(), x);
         if constexpr (!HAT) {
 #pragma unroll
           for (int a = 0; a < AP; ++a) x[a] *= pi[a];
@@ -636,17 +750,29 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         T xm[AP];
         ex = rdw >> 1;
         stp = (rdw & 1) != 0;
-        if constexpr (HAT) matvec<T, AP>(bt + (long long)p * BTS + sb_bt_M(AP), rd, xm);
+        if constexpr (HAT) matvec<T, AP>(mp, rd, xm);
         else vecmat<T, AP>(rd, bt + (long long)p * BTS + sb_bt_M(AP), xm);
+        if constexpr (SPEC) {
 #pragma unroll
-        for (int a = 0; a < AP; ++a) x[a] *= xm[a];
+          for (int a = 0; a < AP; ++a) x[a] = xm[a];
+          times_token<T, AP>(cur.tp(), td, x);
+        } else {
+          td.decode(cur.tp(), x);
+#pragma unroll
+          for (int a = 0; a < AP; ++a) x[a] *= xm[a];
+        }
       }
       Child<T, AP> ch[2];
-      pick_child<T, AP>(sl, false, cur.tl(), cur.ul, cur.wl, cur.ul, 0, td, ch[0]);
-      pick_child<T, AP>(sr, false, cur.tr(), cur.ur, cur.wr, cur.ur, 0, td, ch[1]);
+      take_child<kAny, T, AP>(sl, false, cur.tl(), cur.ul, cur.wl, cur.ul, 0, td, ch[0]);
+      take_child<kAny, T, AP>(sr, false, cur.tr(), cur.ur, cur.wr, cur.ur, 0, td, ch[1]);
       T w[2][AP];
-      matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
-      matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
+      if constexpr (HAT) {
+        matvec<T, AP>(mlr[0], ch[0].u, w[0]);
+        matvec<T, AP>(mlr[1], ch[1].u, w[1]);
+      } else {
+        matvec<T, AP>(bt + (long long)l * BTS + sb_bt_M(AP), ch[0].u, w[0]);
+        matvec<T, AP>(bt + (long long)r * BTS + sb_bt_M(AP), ch[1].u, w[1]);
+      }
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const int n = s == 0 ? l : r;
@@ -662,7 +788,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         // when that record is the next one it is forwarded in registers and
         // never stored
         const bool fwc = (s == 0) ? fwd0 : fwd1;
-        if (sn >= 0 && !fwc) {
+        if (!ch[s].leaf && !fwc) {
           TL::store_vec(Dreg, sn, lane, d);
           *(SB_GLOBAL int*)((SB_GLOBAL char*)(Dw + sn * SB_WAVE) + opq(unsigned(lane) * 4u)) = dword;
         }
