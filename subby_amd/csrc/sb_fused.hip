@@ -188,9 +188,35 @@ __device__ inline int rescale(T* v) {
   return e;
 }
 
+template <int CTRL>
+__device__ inline double dppd(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+// Sum over the wave, the same bits in every lane, in registers only: a
+// butterfly inside each 16-lane row (quad swaps, half-row mirror, row mirror),
+// then the row pairs and the wave halves traded by v_permlane16_swap and
+// v_permlane32_swap, the two lanes of each trade adding the same two values
+// in the same order.  (A shuffle butterfly would be six dependent LDS round
+// trips per sum.)
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, SB_WAVE);
+  v += dppd<0xb1>(v);    // quad_perm [1,0,3,2]
+  v += dppd<0x4e>(v);    // quad_perm [2,3,0,1]
+  v += dppd<0x141>(v);   // row_half_mirror
+  v += dppd<0x140>(v);   // row_mirror
+  {
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    v = __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  }
+  {
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    v = __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+  }
   return v;
 }
 
@@ -259,6 +285,7 @@ constexpr int kLeaf = 0;    // decoded from the leaf's token
 constexpr int kFwd = 1;     // forwarded in registers from the record computed just before (fu / fw)
 constexpr int kStaged = 2;  // staged in the wave's scratch (su / sw)
 constexpr int kAny = 3;     // chosen per record by selects (slot < 0: leaf; fwd: forwarded; else staged)
+constexpr int kLeafOrStaged = 4;  // leaf or staged, by a uniform branch
 
 template <int K, typename T, int AP>
 __device__ inline void take_child(int slot, bool fwd, int tok, const T* su, int sw, const T* fu, int fw,
@@ -275,6 +302,20 @@ __device__ inline void take_child(int slot, bool fwd, int tok, const T* su, int 
     ch.has = leaf ? (unsigned(tok) <= unsigned(A)) : ((w & 1) != 0);
     ch.branch = !leaf && ((w & 2) != 0);
     ch.leaf = leaf;
+  } else if constexpr (K == kLeafOrStaged) {
+    if (slot < 0) {
+      td.decode(tok, ch.u);
+      ch.e = 0;
+      ch.has = unsigned(tok) <= unsigned(td.A);
+      ch.branch = false;
+    } else {
+#pragma unroll
+      for (int a = 0; a < AP; ++a) ch.u[a] = su[a];
+      ch.e = sw >> 2;
+      ch.has = (sw & 1) != 0;
+      ch.branch = (sw & 2) != 0;
+    }
+    ch.leaf = slot < 0;
   } else if constexpr (K == kLeaf) {
     td.decode(tok, ch.u);
     ch.e = 0;
@@ -763,8 +804,8 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         }
       }
       Child<T, AP> ch[2];
-      take_child<kAny, T, AP>(sl, false, cur.tl(), cur.ul, cur.wl, cur.ul, 0, td, ch[0]);
-      take_child<kAny, T, AP>(sr, false, cur.tr(), cur.ur, cur.wr, cur.ur, 0, td, ch[1]);
+      take_child<SPEC ? kLeafOrStaged : kAny, T, AP>(sl, false, cur.tl(), cur.ul, cur.wl, cur.ul, 0, td, ch[0]);
+      take_child<SPEC ? kLeafOrStaged : kAny, T, AP>(sr, false, cur.tr(), cur.ur, cur.wr, cur.ur, 0, td, ch[1]);
       T w[2][AP];
       if constexpr (HAT) {
         matvec<T, AP>(mlr[0], ch[0].u, w[0]);
