@@ -808,7 +808,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   {
     // small batches launch fewer waves (a power of two >= the tile count, so
     // few distinct grids and hence few captured graphs).  The small-alphabet
-    // kernel (one wave walks one tile) launches at most the waves per CU its
+    // kernels (one wave walks one tile) launch at most the waves per CU their
     // tiles need, rounded up: idle waves beside them only let the dispatcher
     // stack four walking waves on some SIMDs while others hold two (3,125
     // tiles of the 127-node workload: 0.275 ms at 13 waves per CU, 0.289 ms
@@ -816,7 +816,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     const long long wpt = f81w ? sb_f81w_waves_per_group() : 1;   // waves per tile
     long long want = 64;
     while (want < (long long)ntiles * wpt) want *= 2;
-    if (!large && !post)
+    if (!large)
       want = std::min(want, ((long long)ntiles + ctx->num_cus - 1) / ctx->num_cus * ctx->num_cus);
     if (want < nwaves) nwaves = int(want);
   }
