@@ -649,6 +649,9 @@ def main():
                          'posteriors, general A=20 R=199 C=2e4')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
+    ap.add_argument('--graph', action='store_true',
+                    help='time the headline steps as hipGraph replays (default: direct launches, which '
+                         'queue fewer packets between steps on ROCm)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build '
@@ -707,16 +710,26 @@ def main():
 
     log(f'bench: warmup {args.warmup}, timed {args.steps} steps')
     for _ in range(args.warmup):
-        L.launch(prep, use_graph=True)
+        L.launch(prep, use_graph=args.graph)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        L.launch(prep, use_graph=True)
+        L.launch(prep, use_graph=args.graph)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     units = C * (R - 1)
     value = units / (wall / args.steps)
+
+    # the other launch mode, for comparison (same prepared batch)
+    for _ in range(3):
+        L.launch(prep, use_graph=not args.graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        L.launch(prep, use_graph=not args.graph)
+    torch.cuda.synchronize()
+    ms_other = (time.perf_counter() - t0) / args.steps * 1e3
 
     # kernel duration with HIP events on the launcher's stream (eager launches)
     L.set_timing(True)
@@ -746,12 +759,13 @@ def main():
         "config": {"workload": "HKY85 kappa=2 pi~Dir(5,5,5,5); 64-leaf random tree (R=127), "
                                "t~Exp(0.05); C=200000; 3% gaps, 1% ambiguous; int8 alignment; "
                                "Counts+logLike+root, mask auto", "R": R, "C": C, "A": A,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "launch": "hipGraph replay" if args.graph else "direct"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
         "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
+        ("ms_per_step_graph" if not args.graph else "ms_per_step_direct"): ms_other,
         "launch_shape": {"family": tuned['family'], "best": tuned['best'], "best_ms": tuned['ms'],
                          "slowest_ms": tuned['worst_ms'], "autotune_table": tuned['table']},
     }

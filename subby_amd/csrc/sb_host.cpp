@@ -191,10 +191,11 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds;
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds, slot;   // slot: plan buffer read
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds &&
+           slot == o.slot;
   }
 };
 
@@ -207,8 +208,14 @@ struct sb_batch {
   uint64_t serial = 0, seq = 0;
   hipEvent_t ring_ev[kRing] = {};
   uint64_t ring_seq[kRing] = {};
-  SbLaunch* header = nullptr;       // start of the plan buffer (fixed until it grows)
-  DevBuf plan, bt, scratch, part, stage;
+  SbLaunch* header = nullptr;       // start of the plan buffer of the current submission
+  // Plans are double-buffered (growable launchers): submission k builds its
+  // plan in host[k % 2] and uploads it to plan[k % 2] on the copy stream while
+  // submission k - 1 still runs, so the launches follow each other without a
+  // copy between them; used[s] fences plan[s] until the launch reading it is
+  // done.  The no-allocation launcher keeps one slot and copies in order.
+  DevBuf plan[2];
+  DevBuf bt, scratch, part, stage;
   // calls with several models: gathered alignment / mask, staged outputs,
   // column maps (sb_counts_multi, sb_inside_outside_multi, sb_mixture_posterior)
   DevBuf xaln, xmask, xout, xcols;
@@ -216,11 +223,17 @@ struct sb_batch {
   hipEvent_t xev = nullptr;
   bool xpending = false;
   int* counter = nullptr;
-  char* host = nullptr;             // pinned staging: header + plan
-  size_t host_cap = 0;
-  hipEvent_t staged = nullptr;      // host staging may be rewritten after this
-  hipEvent_t done = nullptr;        // last submission finished
-  bool staged_pending = false;
+  char* host[2] = {};               // pinned staging: header + plan
+  size_t host_cap[2] = {};
+  hipEvent_t staged[2] = {};        // host[s] may be rewritten and plan[s] read after this
+  bool staged_pending[2] = {};
+  hipEvent_t used[2] = {};          // the last launch reading plan[s] has finished (a ring event)
+  bool used_pending[2] = {};
+  int pslot = 0;                    // plan slot of the next submission
+  hipStream_t cstream = nullptr;    // plan uploads (created on first use)
+  std::vector<char> uploaded[2];    // the bytes plan[s] holds (a repeated plan is not sent again)
+  hipEvent_t done = nullptr;        // last submission finished (its ring event, or done_own)
+  hipEvent_t done_own = nullptr;
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
   int gen2_round = 0;               // two-pass path: tiles per round (0: as the stage budget allows)
@@ -262,26 +275,26 @@ int ensure(sb_batch* b, DevBuf& buf, size_t bytes) {
   return SB_OK;
 }
 
-int ensure_host(sb_batch* b, size_t bytes) {
-  if (b->staged_pending) {
-    SB_HIP(hipEventSynchronize(b->staged));
-    b->staged_pending = false;
+int ensure_host(sb_batch* b, int sl, size_t bytes) {
+  if (b->staged_pending[sl]) {
+    SB_HIP(hipEventSynchronize(b->staged[sl]));
+    b->staged_pending[sl] = false;
   }
-  if (bytes <= b->host_cap) return SB_OK;
+  if (bytes <= b->host_cap[sl]) return SB_OK;
   if (b->fixed)
     return fail(SB_ERR_ARG, "the call's launch plan needs %zu bytes; the no-allocation plan area holds %zu",
-                bytes, b->host_cap);
-  size_t want = std::max(bytes, b->host_cap * 3 / 2);
-  if (b->host) SB_HIP(hipHostFree(b->host));
-  b->host = nullptr;
+                bytes, b->host_cap[sl]);
+  size_t want = std::max(bytes, b->host_cap[sl] * 3 / 2);
+  if (b->host[sl]) SB_HIP(hipHostFree(b->host[sl]));
+  b->host[sl] = nullptr;
   want = (want + 4095) & ~size_t(4095);
-  if (hipHostMalloc(reinterpret_cast<void**>(&b->host), want, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->host[sl]), want, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
-    b->host = nullptr;
-    b->host_cap = 0;
+    b->host[sl] = nullptr;
+    b->host_cap[sl] = 0;
     return fail(SB_ERR_OOM, "pinned allocation of %zu bytes failed", want);
   }
-  b->host_cap = want;
+  b->host_cap[sl] = want;
   return SB_OK;
 }
 
@@ -775,18 +788,20 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   off = align_up(off + sizeof(void*) * nrowm, 256);
   const size_t total = off;
 
-  int rc = ensure_host(b, total);
+  const int sl = b->fixed ? 0 : b->pslot;   // this submission's plan slot
+  int rc = ensure_host(b, sl, total);
   if (rc) return rc;
   {
-    void* before = b->plan.p;
-    rc = ensure(b, b->plan, total);
+    void* before = b->plan[sl].p;
+    rc = ensure(b, b->plan[sl], total);
     if (rc) return rc;
-    if (b->plan.p != before) {
+    if (b->plan[sl].p != before) {
       // captured graphs embed the header address: drop them
       for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
       b->graphs.clear();
-      b->header = static_cast<SbLaunch*>(b->plan.p);
+      b->uploaded[sl].clear();
     }
+    b->header = static_cast<SbLaunch*>(b->plan[sl].p);
   }
   const int bts = sb_bt_stride_k(AP, irr);
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
@@ -862,8 +877,8 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     if (rc) return rc;
   }
 
-  char* h = b->host;
-  char* dplan = static_cast<char*>(b->plan.p);
+  char* h = b->host[sl];
+  char* dplan = static_cast<char*>(b->plan[sl].p);
   auto dev = [&](size_t o) { return dplan + o; };
   SbBlockDesc* hb = reinterpret_cast<SbBlockDesc*>(h + o_blocks);
   int* hs = reinterpret_cast<int*>(h + o_sched);
@@ -975,13 +990,30 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   hipStream_t s = b->stream;
   rc = flush_models(ctx, s, !b->fixed);
   if (rc) return rc;
-  SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
-  SB_HIP(hipEventRecord(b->staged, s));
-  b->staged_pending = true;
+  if (b->fixed) {
+    SB_HIP(hipMemcpyAsync(b->plan[0].p, h, total, hipMemcpyHostToDevice, s));
+    SB_HIP(hipEventRecord(b->staged[0], s));
+    b->staged_pending[0] = true;
+  } else {
+    // the upload waits only for the launch that last read this slot (two
+    // submissions back), so it overlaps the previous launch
+    // (a plan equal to the one this slot holds, e.g. the same batch launched
+    // again, is not sent again: no copy and no wait on the copy engine)
+    std::vector<char>& up = b->uploaded[sl];
+    if (up.size() != total || std::memcmp(up.data(), h, total) != 0) {
+      if (!b->cstream) SB_HIP(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
+      if (b->used_pending[sl]) SB_HIP(hipStreamWaitEvent(b->cstream, b->used[sl], 0));
+      SB_HIP(hipMemcpyAsync(b->plan[sl].p, h, total, hipMemcpyHostToDevice, b->cstream));
+      SB_HIP(hipEventRecord(b->staged[sl], b->cstream));
+      b->staged_pending[sl] = true;
+      SB_HIP(hipStreamWaitEvent(s, b->staged[sl], 0));
+      up.assign(h, h + total);
+    }
+  }
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
                gen2b ? 8 : lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok};
+               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok, sl};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -995,12 +1027,20 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     rc = enqueue(b, key, grid, block, nwaves, nwg_params, nwg_reduce);
     if (rc) return rc;
   }
-  SB_HIP(hipEventRecord(b->done, s));
   {
+    // one event marks the submission: the ring's, which is also "last
+    // submission finished" and "this plan slot's last reader" (each queued
+    // event record costs the GPU time between launches)
     const int k = int(seq % sb_batch::kRing);
     SB_HIP(hipEventRecord(b->ring_ev[k], s));
     b->ring_seq[k] = seq;
     b->seq = seq;
+    b->done = b->ring_ev[k];
+    if (!b->fixed) {
+      b->used[sl] = b->ring_ev[k];
+      b->used_pending[sl] = true;
+      b->pslot = sl ^ 1;
+    }
   }
   return SB_OK;
 }
@@ -1014,8 +1054,9 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
     delete b;
     return fail(SB_ERR_OOM, "launcher allocation failed");
   }
-  SB_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
-  SB_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  for (hipEvent_t& e : b->staged) SB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  SB_HIP(hipEventCreateWithFlags(&b->done_own, hipEventDisableTiming));
+  b->done = b->done_own;
   for (hipEvent_t& e : b->ring_ev) SB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   b->serial = ++ctx->next_serial;
   ctx->live.push_back(b);
@@ -1026,6 +1067,7 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
 void batch_free(sb_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
+  if (b->cstream) (void)hipStreamSynchronize(b->cstream);
   auto& live = b->ctx->live;
   live.erase(std::remove(live.begin(), live.end(), b), live.end());
   for (hipEvent_t e : b->ring_ev)
@@ -1040,15 +1082,18 @@ void batch_free(sb_batch* b) {
       (void)hipEventDestroy(e.second);
     }
   if (!b->fixed) {
-    for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask, &b->xout,
-                      &b->xcols})
+    for (DevBuf* d : {&b->plan[0], &b->plan[1], &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask,
+                      &b->xout, &b->xcols})
       if (d->p) (void)hipFree(d->p);
     if (b->counter) (void)hipFree(b->counter);
   }
   if (b->xev) (void)hipEventDestroy(b->xev);
-  if (b->host) (void)hipHostFree(b->host);
-  if (b->staged) (void)hipEventDestroy(b->staged);
-  if (b->done) (void)hipEventDestroy(b->done);
+  for (int k = 0; k < 2; ++k) {
+    if (b->host[k]) (void)hipHostFree(b->host[k]);
+    if (b->staged[k]) (void)hipEventDestroy(b->staged[k]);
+  }
+  if (b->cstream) (void)hipStreamDestroy(b->cstream);
+  if (b->done_own) (void)hipEventDestroy(b->done_own);
   delete b;
 }
 
@@ -1065,22 +1110,23 @@ int noalloc_create(sb_context* ctx) {
   b->ctx = ctx;
   b->fixed = true;
   char* p = static_cast<char*>(base);
-  b->plan.p = p + SB_NA_PLAN_OFF;
-  b->plan.cap = SB_NA_PLAN_BYTES;
-  b->header = static_cast<SbLaunch*>(b->plan.p);
+  b->plan[0].p = p + SB_NA_PLAN_OFF;
+  b->plan[0].cap = SB_NA_PLAN_BYTES;
+  b->header = static_cast<SbLaunch*>(b->plan[0].p);
   b->counter = reinterpret_cast<int*>(p + SB_NA_COUNTER_OFF);
   b->bt.p = p + SB_NA_BT_OFF;
   b->bt.cap = SB_NA_BT_BYTES;
   b->scratch.p = p + SB_NA_WAVES_OFF;
   b->scratch.cap = SB_NA_WAVES_BYTES;
-  if (hipHostMalloc(reinterpret_cast<void**>(&b->host), SB_NA_PLAN_BYTES, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc(reinterpret_cast<void**>(&b->host[0]), SB_NA_PLAN_BYTES, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     delete b;
     return fail(SB_ERR_OOM, "pinned plan staging of the no-allocation launcher failed");
   }
-  b->host_cap = SB_NA_PLAN_BYTES;
-  hipError_t e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  b->host_cap[0] = SB_NA_PLAN_BYTES;
+  hipError_t e = hipEventCreateWithFlags(&b->staged[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done_own, hipEventDisableTiming);
+  b->done = b->done_own;
   for (hipEvent_t& ev : b->ring_ev)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
   b->serial = ++ctx->next_serial;
