@@ -191,11 +191,10 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds, slot;   // slot: plan buffer read
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds;
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds &&
-           slot == o.slot;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds;
   }
 };
 
@@ -208,14 +207,17 @@ struct sb_batch {
   uint64_t serial = 0, seq = 0;
   hipEvent_t ring_ev[kRing] = {};
   uint64_t ring_seq[kRing] = {};
-  SbLaunch* header = nullptr;       // start of the plan buffer of the current submission
-  // Plans are double-buffered (growable launchers): submission k builds its
-  // plan in host[k % 2] and uploads it to plan[k % 2] on the copy stream while
-  // submission k - 1 still runs, so the launches follow each other without a
-  // copy between them; used[s] fences plan[s] until the launch reading it is
-  // done.  The no-allocation launcher keeps one slot and copies in order.
-  DevBuf plan[2];
-  DevBuf bt, scratch, part, stage;
+  SbLaunch* header = nullptr;       // start of the plan buffer (fixed until it grows)
+  // The plan is staged in two pinned host buffers (growable launchers):
+  // submission k builds it in host[k % 2] while the copy of submission
+  // k - 1's plan may still be pending, and uploads it in order, unless the
+  // device plan already holds the same bytes (the same batch launched again;
+  // plans up to kPlanCache bytes are compared).  (An upload on a second
+  // stream, overlapping the previous launch, cost the serving lanes more than
+  // it saved: their extra streams share the process's four hardware queues.)
+  // The no-allocation launcher keeps one staging buffer.
+  static const size_t kPlanCache = 256 << 10;
+  DevBuf plan, bt, scratch, part, stage;
   // calls with several models: gathered alignment / mask, staged outputs,
   // column maps (sb_counts_multi, sb_inside_outside_multi, sb_mixture_posterior)
   DevBuf xaln, xmask, xout, xcols;
@@ -225,13 +227,10 @@ struct sb_batch {
   int* counter = nullptr;
   char* host[2] = {};               // pinned staging: header + plan
   size_t host_cap[2] = {};
-  hipEvent_t staged[2] = {};        // host[s] may be rewritten and plan[s] read after this
+  hipEvent_t staged[2] = {};        // host[s] may be rewritten after this
   bool staged_pending[2] = {};
-  hipEvent_t used[2] = {};          // the last launch reading plan[s] has finished (a ring event)
-  bool used_pending[2] = {};
-  int pslot = 0;                    // plan slot of the next submission
-  hipStream_t cstream = nullptr;    // plan uploads (created on first use)
-  std::vector<char> uploaded[2];    // the bytes plan[s] holds (a repeated plan is not sent again)
+  int pslot = 0;                    // staging buffer of the next submission
+  std::vector<char> uploaded;       // the bytes the device plan holds (small plans)
   hipEvent_t done = nullptr;        // last submission finished (its ring event, or done_own)
   hipEvent_t done_own = nullptr;
   int waves_per_group = 4;
@@ -788,20 +787,20 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   off = align_up(off + sizeof(void*) * nrowm, 256);
   const size_t total = off;
 
-  const int sl = b->fixed ? 0 : b->pslot;   // this submission's plan slot
+  const int sl = b->fixed ? 0 : b->pslot;   // this submission's staging buffer
   int rc = ensure_host(b, sl, total);
   if (rc) return rc;
   {
-    void* before = b->plan[sl].p;
-    rc = ensure(b, b->plan[sl], total);
+    void* before = b->plan.p;
+    rc = ensure(b, b->plan, total);
     if (rc) return rc;
-    if (b->plan[sl].p != before) {
+    if (b->plan.p != before) {
       // captured graphs embed the header address: drop them
       for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.second);
       b->graphs.clear();
-      b->uploaded[sl].clear();
+      b->uploaded.clear();
+      b->header = static_cast<SbLaunch*>(b->plan.p);
     }
-    b->header = static_cast<SbLaunch*>(b->plan[sl].p);
   }
   const int bts = sb_bt_stride_k(AP, irr);
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
@@ -878,7 +877,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
 
   char* h = b->host[sl];
-  char* dplan = static_cast<char*>(b->plan[sl].p);
+  char* dplan = static_cast<char*>(b->plan.p);
   auto dev = [&](size_t o) { return dplan + o; };
   SbBlockDesc* hb = reinterpret_cast<SbBlockDesc*>(h + o_blocks);
   int* hs = reinterpret_cast<int*>(h + o_sched);
@@ -990,30 +989,23 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   hipStream_t s = b->stream;
   rc = flush_models(ctx, s, !b->fixed);
   if (rc) return rc;
-  if (b->fixed) {
-    SB_HIP(hipMemcpyAsync(b->plan[0].p, h, total, hipMemcpyHostToDevice, s));
-    SB_HIP(hipEventRecord(b->staged[0], s));
-    b->staged_pending[0] = true;
-  } else {
-    // the upload waits only for the launch that last read this slot (two
-    // submissions back), so it overlaps the previous launch
-    // (a plan equal to the one this slot holds, e.g. the same batch launched
-    // again, is not sent again: no copy and no wait on the copy engine)
-    std::vector<char>& up = b->uploaded[sl];
-    if (up.size() != total || std::memcmp(up.data(), h, total) != 0) {
-      if (!b->cstream) SB_HIP(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
-      if (b->used_pending[sl]) SB_HIP(hipStreamWaitEvent(b->cstream, b->used[sl], 0));
-      SB_HIP(hipMemcpyAsync(b->plan[sl].p, h, total, hipMemcpyHostToDevice, b->cstream));
-      SB_HIP(hipEventRecord(b->staged[sl], b->cstream));
+  {
+    // (a small plan equal to the one the device holds, e.g. the same batch
+    // launched again, is not sent again: no copy between the launches)
+    std::vector<char>& up = b->uploaded;
+    const bool cached = !b->fixed && total <= sb_batch::kPlanCache;
+    if (!cached || up.size() != total || std::memcmp(up.data(), h, total) != 0) {
+      SB_HIP(hipMemcpyAsync(b->plan.p, h, total, hipMemcpyHostToDevice, s));
+      SB_HIP(hipEventRecord(b->staged[sl], s));
       b->staged_pending[sl] = true;
-      SB_HIP(hipStreamWaitEvent(s, b->staged[sl], 0));
-      up.assign(h, h + total);
+      if (cached) up.assign(h, h + total);
+      else up.clear();
     }
   }
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
                gen2b ? 8 : lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok, sl};
+               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -1029,18 +1021,14 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   }
   {
     // one event marks the submission: the ring's, which is also "last
-    // submission finished" and "this plan slot's last reader" (each queued
-    // event record costs the GPU time between launches)
+    // submission finished" (each queued event record costs the GPU time
+    // between launches)
     const int k = int(seq % sb_batch::kRing);
     SB_HIP(hipEventRecord(b->ring_ev[k], s));
     b->ring_seq[k] = seq;
     b->seq = seq;
     b->done = b->ring_ev[k];
-    if (!b->fixed) {
-      b->used[sl] = b->ring_ev[k];
-      b->used_pending[sl] = true;
-      b->pslot = sl ^ 1;
-    }
+    if (!b->fixed) b->pslot = sl ^ 1;
   }
   return SB_OK;
 }
@@ -1067,7 +1055,6 @@ int batch_create(sb_context* ctx, hipStream_t stream, sb_batch** out) {
 void batch_free(sb_batch* b) {
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
-  if (b->cstream) (void)hipStreamSynchronize(b->cstream);
   auto& live = b->ctx->live;
   live.erase(std::remove(live.begin(), live.end(), b), live.end());
   for (hipEvent_t e : b->ring_ev)
@@ -1082,8 +1069,8 @@ void batch_free(sb_batch* b) {
       (void)hipEventDestroy(e.second);
     }
   if (!b->fixed) {
-    for (DevBuf* d : {&b->plan[0], &b->plan[1], &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask,
-                      &b->xout, &b->xcols})
+    for (DevBuf* d : {&b->plan, &b->bt, &b->scratch, &b->part, &b->stage, &b->xaln, &b->xmask, &b->xout,
+                      &b->xcols})
       if (d->p) (void)hipFree(d->p);
     if (b->counter) (void)hipFree(b->counter);
   }
@@ -1092,7 +1079,6 @@ void batch_free(sb_batch* b) {
     if (b->host[k]) (void)hipHostFree(b->host[k]);
     if (b->staged[k]) (void)hipEventDestroy(b->staged[k]);
   }
-  if (b->cstream) (void)hipStreamDestroy(b->cstream);
   if (b->done_own) (void)hipEventDestroy(b->done_own);
   delete b;
 }
@@ -1110,9 +1096,9 @@ int noalloc_create(sb_context* ctx) {
   b->ctx = ctx;
   b->fixed = true;
   char* p = static_cast<char*>(base);
-  b->plan[0].p = p + SB_NA_PLAN_OFF;
-  b->plan[0].cap = SB_NA_PLAN_BYTES;
-  b->header = static_cast<SbLaunch*>(b->plan[0].p);
+  b->plan.p = p + SB_NA_PLAN_OFF;
+  b->plan.cap = SB_NA_PLAN_BYTES;
+  b->header = static_cast<SbLaunch*>(b->plan.p);
   b->counter = reinterpret_cast<int*>(p + SB_NA_COUNTER_OFF);
   b->bt.p = p + SB_NA_BT_OFF;
   b->bt.cap = SB_NA_BT_BYTES;
