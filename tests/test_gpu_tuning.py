@@ -99,3 +99,48 @@ def test_two_pass_tail_overlap(eng, A, C):
     assert np.max(num / den) <= 1e-4
     ll = out[0][1].cpu().numpy()[cols]
     assert np.max(np.abs(ll - rl) / np.maximum(np.abs(rl), 1)) <= 1e-5
+
+
+def test_repeated_and_changed_plans(eng):
+    """A launcher that skips re-sending a plan equal to the one the device holds:
+    the same prepared batch launched again (direct and replayed) gives the same
+    bits, and a batch that differs only in its branch lengths or its output
+    buffers (a different plan of the same size) gets its own results, against
+    the oracle; two launchers alternate on one stream."""
+    import subby_amd as sb
+    from oracle import subby_oracle as O
+    from subby_amd.batch import Block, Launcher
+    from tests.test_gpu_parity import err_counts, err_ll, omodel, otree
+    say('repeated and changed plans')
+    t, m, aln = _blocks(sb, 4, 2000, 911, 40)
+    aln_d = torch.from_numpy(aln).to(eng.device)
+    L = Launcher(eng)
+    blk = Block(aln_d, t, m)
+    prep = L.prepare([blk])
+    L.launch(prep, use_graph=False)
+    L.sync()
+    ref = (blk.counts.clone(), blk.loglike.clone())
+    for g in (False, True, False, True):
+        blk.counts.zero_()
+        blk.loglike.zero_()
+        torch.cuda.synchronize()
+        L.launch(prep, use_graph=g)
+        L.sync()
+        assert torch.equal(blk.counts, ref[0]) and torch.equal(blk.loglike, ref[1]), g
+    oc, ol, _ = O.counts_loglike_root(aln, otree(t), omodel(m))
+    assert err_counts(ref[0], oc) <= 1e-4 and err_ll(ref[1], ol) <= 1e-5
+    # same tree and sizes, other branch lengths: the plan differs only in its jobs
+    t2 = sb.Tree(np.asarray(t.parentIndex), np.asarray(t.distanceToParent) * 1.7)
+    blk2 = Block(aln_d, t2, m)
+    L2 = Launcher(eng)
+    for _ in range(2):
+        L.submit([blk2])
+        L2.submit([blk])
+        L.sync()
+        L2.sync()
+        oc2, ol2, _ = O.counts_loglike_root(aln, otree(t2), omodel(m))
+        assert err_counts(blk2.counts, oc2) <= 1e-4 and err_ll(blk2.loglike, ol2) <= 1e-5
+        assert torch.equal(blk.counts, ref[0]) and torch.equal(blk.loglike, ref[1])
+        L.launch(prep, use_graph=True)
+        L.sync()
+        assert torch.equal(blk.counts, ref[0])
