@@ -762,7 +762,8 @@ def main():
                    "parallelism": "single GPU", "launch": "hipGraph replay" if args.graph else "direct"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "kernel": "sb_fused_small_kernel<float,4,false,false>", "kernel_ms": kavg,
+                     "kernel": "sb_fused_small_kernel<float,4,false,false,false,false,true> (two waves per tile)",
+                     "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
         "parity": {"counts_normwise": err_c, "loglike_rel": err_l, "columns_checked": int(len(cols))},
         ("ms_per_step_graph" if not args.graph else "ms_per_step_direct"): ms_other,

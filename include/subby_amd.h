@@ -255,10 +255,25 @@ int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
  *                            FP32): 64-column tiles staged per round, 0 = as
  *                            many as an 8 GiB stage holds (default); smaller
  *                            rounds keep the stage in cache
- * Results do not depend on any knob (bitwise). */
-enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_GEN2_ROUND = 3 };
+ *   SB_TUNE_SPLIT            small-alphabet single-model kinds at four states:
+ *                            two waves walk each tile (a subtree each) -- 1
+ *                            wherever a tree names a subtree (default; two
+ *                            waves per workgroup, waves_per_group is then
+ *                            ignored), 0 never
+ *   SB_TUNE_LAST_SPLIT       read only: 1 when the launcher's last submission
+ *                            walked its tiles with two waves
+ * Results do not depend on any knob (bitwise), except that a split walk sums
+ * the subtree's branch terms apart: its Counts (and rate gradients) may differ
+ * from the one-wave walk's in the last bits; log-likelihoods, root
+ * posteriors and branch-length gradients do not. */
+enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_GEN2_ROUND = 3, SB_TUNE_SPLIT = 4,
+       SB_TUNE_LAST_SPLIT = 5 };
 int sb_batch_set_tune(sb_batch* b, int knob, int value);
 int sb_batch_get_tune(sb_batch* b, int knob, int* value);
+/* The same knobs on the launcher behind the single-call entry points
+ * (sb_loglike, sb_counts, sb_loglike_grad, ...), e.g. SB_TUNE_SPLIT for the
+ * latency of small calls.  No reference counterpart. */
+int sb_context_set_tune(sb_context* ctx, int knob, int value);
 /* Block until the launcher's last submission has finished. */
 int sb_batch_sync(sb_batch* b);
 /* Kernel timing: when enabled, eager (non-graph) submissions bracket the

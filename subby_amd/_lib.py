@@ -15,7 +15,8 @@ SB_OK = 0
 SB_F32, SB_F64 = 0, 1
 SB_MASK_NONE, SB_MASK_AUTO, SB_MASK_EXPLICIT = 0, 1, 2
 SB_WANT_LOGLIKE, SB_WANT_ROOT, SB_WANT_COUNTS = 1, 2, 4
-SB_TUNE_WAVES_PER_GROUP, SB_TUNE_WAVES_PER_CU, SB_TUNE_GEN2_ROUND = 0, 1, 3
+SB_TUNE_WAVES_PER_GROUP, SB_TUNE_WAVES_PER_CU, SB_TUNE_GEN2_ROUND, SB_TUNE_SPLIT = 0, 1, 3, 4
+SB_TUNE_LAST_SPLIT = 5
 
 # Every exported symbol of include/subby_amd.h.
 EXPORTS = [
@@ -27,7 +28,7 @@ EXPORTS = [
     'sb_server_drain', 'sb_server_replay', 'sb_inside_outside', 'sb_model_put_irrev',
     'sb_counts_multi', 'sb_branch_counts_multi', 'sb_inside_outside_multi', 'sb_mixture_posterior',
     'sb_arena_stats', 'sb_batch_set_tune', 'sb_batch_get_tune', 'sb_counts_noalloc',
-    'sb_server_replay_timed',
+    'sb_server_replay_timed', 'sb_context_set_tune',
 ]
 
 
@@ -133,6 +134,7 @@ def load(path=LIB_PATH):
     lib.sb_batch_set_tiling.argtypes = [vp, i, i]
     lib.sb_batch_set_tune.argtypes = [vp, i, i]
     lib.sb_batch_get_tune.argtypes = [vp, i, P(i)]
+    lib.sb_context_set_tune.argtypes = [vp, i, i]
     lib.sb_batch_sync.argtypes = [vp]
     lib.sb_batch_set_timing.argtypes = [vp, i]
     lib.sb_batch_kernel_ms.argtypes = [vp, P(d), P(i)]

@@ -366,6 +366,25 @@ struct Acc {
     }
     t3 = T(0);
   }
+  // The accumulators as one list of NV values (c, then t2a, t2b, t3 for F81),
+  // kept in LDS one value per lane: v[x * 64 + lane].
+  static constexpr int NV = NC + (F81 ? 2 * AP + 1 : 0);
+  __device__ inline T& at(int x) {
+    if constexpr (F81) {
+      if (x >= NC + 2 * AP) return t3;
+      if (x >= NC + AP) return t2b[x - NC - AP];
+      if (x >= NC) return t2a[x - NC];
+    }
+    return c[x];
+  }
+  __device__ inline void add_lds(const T* v, int lane) {
+#pragma unroll
+    for (int x = 0; x < NV; ++x) at(x) += v[x * SB_WAVE + lane];
+  }
+  __device__ inline void store_lds(T* v, int lane) {
+#pragma unroll
+    for (int x = 0; x < NV; ++x) v[x * SB_WAVE + lane] = at(x);
+  }
 };
 
 }  // namespace
@@ -523,19 +542,43 @@ __global__ void sb_reset_kernel(const SbLaunch* __restrict__ L) {
 // Register budget: the FP32 four-state reversible variants with one model
 // keep four waves per SIMD (at most 128 registers), so a full grid of tiles
 // is resident at once.
-template <typename T, int AP, bool IRR, bool ROWS>
+template <typename T, int AP, bool IRR, bool ROWS, bool SPLIT>
 struct SmallWavesPerEU {
-  static constexpr int v = (sizeof(T) == 4 && AP == 4 && !IRR && !ROWS) ? 4 : 1;
+  // (split walks fill the GPU with half as many tiles: two waves per SIMD hold them)
+  static constexpr int v = (sizeof(T) == 4 && AP == 4 && !IRR && !ROWS) ? (SPLIT ? 2 : 4) : 1;
 };
 
-template <typename T, int AP, bool F81, bool GRAD, bool IRR, bool ROWS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SmallWavesPerEU<T, AP, IRR, ROWS>::v)))
+// SPLIT: a workgroup of two waves walks each tile together.  The planner
+// names a subtree of internal records [s0, s1) (records in depth-first
+// preorder, so a subtree is contiguous) that the second wave walks while the
+// first walks the rest: inside pass, first wave [nint-1 .. s1], then after a
+// barrier [s0-1 .. 0]; second wave [s1-1 .. s0]; outside pass, first wave
+// [0 .. s0-1], then after a barrier [s1 .. nint-1]; second wave, after it,
+// [s0 .. s1-1].  Nothing is forwarded in registers across a range boundary
+// (a range's first record reads its children staged, its last record stores
+// what it computes), so the waves exchange only through the tile's scratch
+// slot (the first wave's) and, for the log-likelihood scale and the
+// accumulators, through LDS.  For launches with spare wave slots: it cuts the
+// latency of a tile's walk (s0 = s1 = 0: the first wave walks all).  The
+// subtree's branch terms are summed apart and added last, so counts may differ
+// from the one-wave walk's in the last bits.
+template <typename T, int AP, bool F81, bool GRAD, bool IRR, bool ROWS, bool SPLIT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SmallWavesPerEU<T, AP, IRR, ROWS, SPLIT>::v)))
 sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
   typedef Tile<T, AP> TL;
+  static_assert(!SPLIT || (!IRR && !ROWS), "split walks are built for the single-model kinds only");
   const SbLaunch& args = *Lp;
   const int lane = threadIdx.x & (SB_WAVE - 1);
   const int wpb = blockDim.x / SB_WAVE;
-  const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + threadIdx.x / SB_WAVE);
+  const int wig = __builtin_amdgcn_readfirstlane(threadIdx.x / SB_WAVE);   // wave in the workgroup
+  const int role = SPLIT ? wig : 0;            // 0: first wave, 1: second (split walks)
+  // (split walks share the first wave's scratch slot)
+  const int gw = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (SPLIT ? 0 : wig));
+  constexpr int NCC = AP * AP + (F81 ? 2 * AP + 1 : 0);   // accumulator values per lane
+  __shared__ T split_acc[SPLIT ? NCC * SB_WAVE : 1];
+  __shared__ T split_inv[SPLIT ? SB_WAVE : 1];
+  __shared__ int split_e0[SPLIT ? SB_WAVE : 1];
+  __shared__ int split_tile;
   SB_GLOBAL char* wscr = glb<char>(args.scratch) + (long long)gw * args.wave_stride;
   const int NI = args.nint_max;
   SB_GLOBAL char* Ureg = wscr;
@@ -545,8 +588,15 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
 
   for (;;) {
     int t = 0;
-    if (lane == 0) t = atomicAdd(args.counter, 1);
-    t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    if constexpr (SPLIT) {
+      // (every later LDS read of the previous tile came before this tile's barriers)
+      if (threadIdx.x == 0) split_tile = atomicAdd(args.counter, 1);
+      __syncthreads();
+      t = __builtin_amdgcn_readfirstlane(split_tile);
+    } else {
+      if (lane == 0) t = atomicAdd(args.counter, 1);
+      t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, SB_WAVE));
+    }
     if (t >= args.ntiles) break;
     const SB_CONST int* tbp = cst<int>(args.tiles) + 2 * t;
     const int2 tb = make_int2(tbp[0], tbp[1]);
@@ -589,19 +639,27 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     constexpr bool SPEC = !IRR && !ROWS;
     T u0[AP];
     int e0 = 0;
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int a = 0; a < AP; ++a) u0[a] = T(1);   // (the second wave never reaches the root)
+    }
     bool root_branch = false;
     TokDec<T, AP> td;
     td.init(A);
     if (nint == 0) td.decode(tok_at(ts, 0), u0);  // single-node tree
     // (a tree of one node has no records: nothing is fetched)
+    // split walks: the second wave's subtree of records [s0, s1) (from the
+    // root record; s0 = s1 = 0: none, the first wave walks the whole tree)
+    const int s0 = SPLIT && nint > 0 ? sched[6] : 0, s1 = SPLIT && nint > 0 ? sched[7] : 0;
     InRec q{0, 0, 0, -1, -1};
     Fetch<T, AP> fa, fb;
-    if (nint > 0) {
+    if (!SPLIT && nint > 0) {
       q = in_rec(sched, nint - 1);
       fetch_children<T, AP, false>(fa, q, nint, ts, Ureg, Uw, lane);
     }
     T fu[AP];                      // U and word of the record computed last
     int fw = 0;
+    int lo_in = 0;                 // the last record of the range walked
 #pragma unroll
     for (int a = 0; a < AP; ++a) fu[a] = T(0);
     // (an internal left child is the next node in preorder: slot i + 1, forwarded)
@@ -624,7 +682,7 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       __builtin_amdgcn_sched_barrier(0);
       fetch_children<T, AP, false>(nxt, qn, i, ts, Ureg, Uw, lane);
       Child<T, AP> cl, cr;
-      take_child<decltype(LK)::v, T, AP>(q.sl, true, cur.tl(), fu, fw, fu, fw, td, cl);
+      take_child<decltype(LK)::v, T, AP>(q.sl, true, cur.tl(), cur.ul, cur.wl, fu, fw, td, cl);
       take_child<decltype(RK)::v, T, AP>(q.sr, q.sr == i + 1, cur.tr(), cur.ur, cur.wr, fu, fw, td, cr);
       T wl[AP], wr[AP], up[AP];
       matvec<T, AP>(ml, cl.u, wl);
@@ -645,8 +703,9 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       const int word = (ep << 2) | (br ? 2 : 0) | (has ? 1 : 0);
       if (i > 0) {
         // the outside pass reads every internal U; without it, a U forwarded
-        // to the next record is never read back
-        if (outside || !(qn.sl == i || qn.sr == i)) {
+        // to the next record is never read back (a range's last record has no
+        // next record in this wave)
+        if (outside || i == lo_in || !(qn.sl == i || qn.sr == i)) {
           TL::store_vec(Ureg, i, lane, up);
           *(SB_GLOBAL int*)((SB_GLOBAL char*)(Uw + i * SB_WAVE) + opq(unsigned(lane) * 4u)) = word;
         }
@@ -678,10 +737,31 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       }
     };
     // unrolled by two over ping-pong fetch buffers, so no loaded value is copied
-    for (int i = nint - 1; i >= 0; i -= 2) {
-      irec(i, fa, fb);
-      if (i == 0) break;
-      irec(i - 1, fb, fa);
+    if constexpr (!SPLIT) {
+      for (int i = nint - 1; i >= 0; i -= 2) {
+        irec(i, fa, fb);
+        if (i == 0) break;
+        irec(i - 1, fb, fa);
+      }
+    } else {
+      // records hi down to lo; the first one takes both children from their
+      // tokens or staged (nothing computed in this range before it)
+      auto walk_in = [&](int hi, int lo) __attribute__((always_inline)) {
+        if (hi < lo) return;
+        lo_in = lo;
+        q = in_rec(sched, hi);
+        fetch_children<T, AP, true>(fa, q, -3, ts, Ureg, Uw, lane);
+        istep(hi, fa, fb, Kind<kLeafOrStaged>{}, Kind<kLeafOrStaged>{});
+        for (int i = hi - 1; i >= lo; i -= 2) {
+          irec(i, fb, fa);
+          if (i == lo) break;
+          irec(i - 1, fa, fb);
+        }
+      };
+      if (role == 0) walk_in(nint - 1, s1);
+      else walk_in(s1 - 1, s0);
+      __syncthreads();               // the second wave's inside vectors are stored
+      if (role == 0) walk_in(s0 - 1, 0);
     }
     // log-likelihood and root posterior (pruning.py:84-87; __init__.py:642-650)
     const SB_CONST double* pi64 = cst<double>(B->model64) + sb_m64_pi_k(AP, IRR);
@@ -690,15 +770,22 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     for (int a = 0; a < AP; ++a)
       if (a < A) s0d = fma(pi64[a], double(u0[a]), s0d);
     const double ll = double(e0) * 0.69314718055994530942 + log(s0d);
-    const T inv_s0 = T(1.0 / s0d);
-    if (valid && (flags & SB_F_LL) && B->loglike) glb<double>(B->loglike)[c] = ll;
-    if (valid && (flags & SB_F_ROOT) && B->root) {
+    T inv_s0 = T(1.0 / s0d);
+    if (valid && role == 0 && (flags & SB_F_LL) && B->loglike) glb<double>(B->loglike)[c] = ll;
+    if (valid && role == 0 && (flags & SB_F_ROOT) && B->root) {
       SB_GLOBAL T* root = glb<T>(B->root);
 #pragma unroll
       for (int a = 0; a < AP; ++a)
         if (a < A) root[(long long)a * C + c] = pi[a] * u0[a] * inv_s0;
     }
     if (!outside) continue;
+    if constexpr (SPLIT) {
+      // the second wave scales its branches by the root's likelihood too
+      if (role == 0) {
+        split_e0[lane] = e0;
+        split_inv[lane] = inv_s0;
+      }
+    }
 
     // ---------------- outside pass + accumulation ----------------
     // Reversible models with one model per block carry each outside vector
@@ -729,7 +816,8 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
     Fetch<T, AP> ocur, onxt;
     T cd[AP], nd[AP];              // staged outside vector of the record's node and its word
     int cdw = 0, ndw = 0;
-    if (nint > 0) {
+    int hi_out = nint - 1;         // the last record of the range walked
+    if (!SPLIT && nint > 0) {
       oq = in_rec(sched, 0);
       fetch_children<T, AP>(ocur, oq, -2, ts, Ureg, Uw, lane);
       fetch_vec<T, AP>(cd, cdw, 0, Dreg, Dw, lane);
@@ -759,10 +847,10 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      const int inext = i + 1 < nint ? i + 1 : i;
+      const int inext = i + 1 <= hi_out ? i + 1 : i;
       const InRec oqn = in_rec(sched, inext);
       if constexpr (HAT) __builtin_amdgcn_sched_barrier(0);
-      const bool last = i + 1 >= nint;
+      const bool last = i >= hi_out;   // (no next record in this wave: store what is computed)
       const bool fwd0 = !last && oqn.p == oq.l;
       const bool fwd1 = !last && oqn.p == oq.r;
       fetch_children<T, AP>(nxt, oqn, -2, ts, Ureg, Uw, lane);
@@ -1045,10 +1133,41 @@ sb_fused_small_kernel(const SbLaunch* __restrict__ Lp) {
       pf = fwd0 || fwd1;
       oq = oqn;
     };
-    for (int i = 0; i < nint; i += 2) {
-      ostep(i, ocur, cd, cdw, onxt, nd, ndw);
-      if (i + 1 >= nint) break;
-      ostep(i + 1, onxt, nd, ndw, ocur, cd, cdw);
+    if constexpr (!SPLIT) {
+      for (int i = 0; i < nint; i += 2) {
+        ostep(i, ocur, cd, cdw, onxt, nd, ndw);
+        if (i + 1 >= nint) break;
+        ostep(i + 1, onxt, nd, ndw, ocur, cd, cdw);
+      }
+    } else {
+      // records lo up to hi; the first one reads its own outside vector staged
+      auto walk_out = [&](int lo, int hi) __attribute__((always_inline)) {
+        if (hi < lo) return;
+        hi_out = hi;
+        oq = in_rec(sched, lo);
+        fetch_children<T, AP>(ocur, oq, -2, ts, Ureg, Uw, lane);
+        fetch_vec<T, AP>(cd, cdw, lo, Dreg, Dw, lane);
+        pf = false;
+        for (int i = lo; i <= hi; i += 2) {
+          ostep(i, ocur, cd, cdw, onxt, nd, ndw);
+          if (i + 1 > hi) break;
+          ostep(i + 1, onxt, nd, ndw, ocur, cd, cdw);
+        }
+      };
+      if (role == 0) walk_out(0, s0 - 1);
+      __syncthreads();               // the subtree's outside vector and the root's scale are stored
+      if (role == 1) {
+        e0 = split_e0[lane];
+        inv_s0 = split_inv[lane];
+        walk_out(s0, s1 - 1);
+      } else {
+        walk_out(s1, nint - 1);
+      }
+      // the second wave's accumulators join the first wave's
+      if (role == 1) acc.store_lds(split_acc, lane);
+      __syncthreads();
+      if (role == 1) continue;
+      if (s1 > s0) acc.add_lds(split_acc, lane);
     }
 
     // ---------------- epilogue: natural-basis counts ----------------
@@ -1200,38 +1319,58 @@ __global__ void __launch_bounds__(256) sb_grad_reduce_kernel(const SbLaunch* __r
 }
 
 // Explicit instantiations and host-visible launch helpers.
-#define SB_INST(T, AP, F81, GRAD, IRR, ROWS) \
-  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD, IRR, ROWS>(const SbLaunch*);
-#define SB_INST_T(T, AP)                       \
-  SB_INST(T, AP, false, false, false, false)   \
-  SB_INST(T, AP, true, false, false, false)    \
-  SB_INST(T, AP, false, true, false, false)    \
-  SB_INST(T, AP, false, false, true, false)    \
-  SB_INST(T, AP, false, true, true, false)     \
-  SB_INST(T, AP, false, false, false, true)    \
-  SB_INST(T, AP, false, false, true, true)
+#define SB_INST(T, AP, F81, GRAD, IRR, ROWS, SPLIT) \
+  template __global__ void sb_fused_small_kernel<T, AP, F81, GRAD, IRR, ROWS, SPLIT>(const SbLaunch*);
+#define SB_INST_T(T, AP)                              \
+  SB_INST(T, AP, false, false, false, false, false)   \
+  SB_INST(T, AP, true, false, false, false, false)    \
+  SB_INST(T, AP, false, true, false, false, false)    \
+  SB_INST(T, AP, false, false, true, false, false)    \
+  SB_INST(T, AP, false, true, true, false, false)     \
+  SB_INST(T, AP, false, false, false, true, false)    \
+  SB_INST(T, AP, false, false, true, true, false)
+// split walks: the single-model reversible and F81 kinds at four states
+#define SB_INST_SPLIT(T)                              \
+  SB_INST(T, 4, false, false, false, false, true)     \
+  SB_INST(T, 4, true, false, false, false, true)      \
+  SB_INST(T, 4, false, true, false, false, true)
 SB_INST_T(float, 4)
 SB_INST_T(float, 8)
 SB_INST_T(double, 4)
 SB_INST_T(double, 8)
+SB_INST_SPLIT(float)
+SB_INST_SPLIT(double)
 template __global__ void sb_branch_params_kernel<float>(const SbLaunch*);
 template __global__ void sb_branch_params_kernel<double>(const SbLaunch*);
 template __global__ void sb_branch_params_irrev_kernel<float>(const SbLaunch*);
 template __global__ void sb_branch_params_irrev_kernel<double>(const SbLaunch*);
 
 static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                         bool rows))(const SbLaunch*) {
+                         bool rows, bool split))(const SbLaunch*) {
   void (*k)(const SbLaunch*) = nullptr;
-  if ((irr && f81) || (rows && (f81 || grad))) return nullptr;
-#define SB_PICK(T, APV)                                                                   \
-  if (AP == APV) {                                                                        \
-    if (rows) k = irr ? sb_fused_small_kernel<T, APV, false, false, true, true>           \
-                      : sb_fused_small_kernel<T, APV, false, false, false, true>;         \
-    else if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true, false>      \
-                           : sb_fused_small_kernel<T, APV, false, false, true, false>;    \
-    else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false, false>;          \
-    else if (f81) k = sb_fused_small_kernel<T, APV, true, false, false, false>;           \
-    else k = sb_fused_small_kernel<T, APV, false, false, false, false>;                   \
+  if ((irr && f81) || (rows && (f81 || grad)) || (f81 && grad)) return nullptr;
+  if (split) {
+    // two waves per tile: the single-model reversible and F81 kinds, four states
+    if (AP != 4 || irr || rows) return nullptr;
+    if (dtype == 0)
+      k = grad ? sb_fused_small_kernel<float, 4, false, true, false, false, true>
+                : f81 ? sb_fused_small_kernel<float, 4, true, false, false, false, true>
+                      : sb_fused_small_kernel<float, 4, false, false, false, false, true>;
+    else
+      k = grad ? sb_fused_small_kernel<double, 4, false, true, false, false, true>
+                : f81 ? sb_fused_small_kernel<double, 4, true, false, false, false, true>
+                      : sb_fused_small_kernel<double, 4, false, false, false, false, true>;
+    return k;
+  }
+#define SB_PICK(T, APV)                                                                          \
+  if (AP == APV) {                                                                               \
+    if (rows) k = irr ? sb_fused_small_kernel<T, APV, false, false, true, true, false>           \
+                      : sb_fused_small_kernel<T, APV, false, false, false, true, false>;         \
+    else if (irr) k = grad ? sb_fused_small_kernel<T, APV, false, true, true, false, false>      \
+                           : sb_fused_small_kernel<T, APV, false, false, true, false, false>;    \
+    else if (grad) k = sb_fused_small_kernel<T, APV, false, true, false, false, false>;          \
+    else if (f81) k = sb_fused_small_kernel<T, APV, true, false, false, false, false>;           \
+    else k = sb_fused_small_kernel<T, APV, false, false, false, false, false>;                   \
   }
   if (dtype == 0) { SB_PICK(float, 4) SB_PICK(float, 8) }
   else { SB_PICK(double, 4) SB_PICK(double, 8) }
@@ -1242,8 +1381,8 @@ static void (*pick_small(int dtype, int AP, bool f81, bool grad, bool irr,
 // Resident workgroups per CU for a block of `threads` threads with `lds`
 // bytes of token stage (0 on error).
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        bool rows, int threads, int lds) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
+                                        bool rows, bool split, int threads, int lds) {
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows, split);
   if (!k) return 0;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, size_t(lds)) != hipSuccess) return 0;
@@ -1251,9 +1390,9 @@ extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, 
 }
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            bool rows, dim3 grid, dim3 block, int lds,
+                                            bool rows, bool split, dim3 grid, dim3 block, int lds,
                                             const SbLaunch* L, hipStream_t s) {
-  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows);
+  void (*k)(const SbLaunch*) = pick_small(dtype, AP, f81, grad, irr, rows, split);
   if (!k) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, grid, block, size_t(lds), s, L);
   return hipGetLastError();

@@ -25,7 +25,7 @@
 #include "sb_plan.h"
 
 extern "C" hipError_t sb_launch_fused_small(int dtype, int AP, bool f81, bool grad, bool irr,
-                                            bool rows, dim3 grid, dim3 block, int lds,
+                                            bool rows, bool split, dim3 grid, dim3 block, int lds,
                                             const SbLaunch* L, hipStream_t s);
 extern "C" hipError_t sb_launch_gather_cols(const void* src, void* dst, int esz, int rows, int C,
                                             const int* const* cols, hipStream_t s);
@@ -41,7 +41,7 @@ extern "C" hipError_t sb_launch_large(int dtype, int AP, bool f81, bool grad, in
 extern "C" long long sb_large_wave_stride(int dtype, int AP, int nint_max, bool f81, int out);
 extern "C" int sb_large_tile_cols(int dtype, int AP, bool f81, int out);
 extern "C" int sb_fused_small_occupancy(int dtype, int AP, bool f81, bool grad, bool irr,
-                                        bool rows, int threads, int lds);
+                                        bool rows, bool split, int threads, int lds);
 extern "C" int sb_large_occupancy(int dtype, int AP, bool f81, bool grad, int out);
 extern "C" int sb_f81w_tile_cols();
 extern "C" int sb_f81w_waves_per_group();
@@ -191,10 +191,11 @@ struct sb_context {
 };
 
 struct GraphKey {
-  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds;
+  int dtype, AP, f81, grad, large, nwaves, wpg, irr, rows, lds, split;
   bool operator==(const GraphKey& o) const {
     return dtype == o.dtype && AP == o.AP && f81 == o.f81 && grad == o.grad && large == o.large &&
-           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds;
+           nwaves == o.nwaves && wpg == o.wpg && irr == o.irr && rows == o.rows && lds == o.lds &&
+           split == o.split;
   }
 };
 
@@ -236,6 +237,8 @@ struct sb_batch {
   int waves_per_group = 4;
   int waves_per_cu = 0;             // 0: as many as the kernel's occupancy allows
   int gen2_round = 0;               // two-pass path: tiles per round (0: as the stage budget allows)
+  int split = 1;                    // split walks: 0 never, 1 wherever a tree names a subtree
+  int last_split = 0;               // the last submission walked its tiles with two waves
   // fixed buffers (the no-allocation launcher): never grown, never freed here
   bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
@@ -302,6 +305,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 struct TreePlan {
   int nint = 0;
   std::vector<int> recs;  // nint * 8
+  bool split = false;     // a subtree for a second wave is named (root record: r[6], r[7])
 };
 
 int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
@@ -347,6 +351,33 @@ int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
     r[2] = second[n];
     r[4] = slot[first[n]];
     r[5] = slot[second[n]];
+  }
+  // Split walks (small-alphabet kernels, two waves per tile): the second wave
+  // walks the subtree of records [s0, s1) while the first walks the rest, in
+  // s0 + max(nint - s1, s1 - s0) records of latency.  The subtree chosen
+  // minimises that; a tree that gains less than 15% keeps one wave
+  // (s0 = s1 = 0; random 64-leaf trees gain 16-49%, 36% at the median, a
+  // caterpillar nothing).  The one-wave kernels ignore it.
+  if (nint >= 8) {
+    std::vector<int> sz(nint, 1);
+    for (int i = nint - 1; i >= 0; --i) {
+      const int* r = &tp.recs[size_t(i) * 8];
+      if (r[4] >= 0) sz[i] += sz[r[4]];
+      if (r[5] >= 0) sz[i] += sz[r[5]];
+    }
+    int best = nint, bs = 0;
+    for (int v = 1; v < nint; ++v) {
+      const int cost = v + std::max(nint - v - sz[v], sz[v]);
+      if (cost < best) {
+        best = cost;
+        bs = v;
+      }
+    }
+    if (20 * best <= 17 * nint) {
+      tp.recs[6] = bs;
+      tp.recs[7] = bs + sz[bs];
+      tp.split = true;
+    }
   }
   return SB_OK;
 }
@@ -584,8 +615,8 @@ int enqueue(sb_batch* b, const GraphKey& key, dim3 grid, dim3 block, int nwaves,
                            key.large == 5 ? 1 : key.large == 6 ? 2 : key.large == 7 ? 3 : 0, nwaves,
                            b->header, s));
   else
-    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0, grid,
-                                 block, key.lds, b->header, s));
+    SB_HIP(sb_launch_fused_small(key.dtype, key.AP, key.f81, key.grad, key.irr, key.rows != 0,
+                                 key.split != 0, grid, block, key.lds, b->header, s));
   if (b->timing) {
     SB_HIP(hipEventRecord(ev.second, s));
     b->tev.push_back(ev);
@@ -806,16 +837,27 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
   rc = ensure(b, b->bt, std::max<size_t>(1, njobs) * bts * esz);
   if (rc) return rc;
 
+  // Split walks: the small-alphabet kinds that can split (single model,
+  // reversible or F81, four states) walk each tile with two waves when a tree
+  // of the launch names a subtree for the second wave (tiles of the others
+  // leave it idle).  Up to half the walk's latency, and twice the waves to
+  // hide the latency of each (3,125 tiles of the 127-node workload: 0.243 ms
+  // against 0.280 ms).  Whether a tile splits depends on its tree alone, so
+  // a block's results do not depend on the rest of the batch.
+  bool split = false;
+  if (!large && !post && !irr && !rowsk && AP == 4 && b->split > 0)
+    for (int i = 0; i < nb; ++i) split = split || plans[i].split;
   // persistent grid: resident waves per CU from the kernel's occupancy unless
   // the launcher was given an explicit value; never more waves than tiles
-  const int wpg = (large || post) ? 1 : b->waves_per_group;
+  b->last_split = split ? 1 : 0;
+  const int wpg = (large || post) ? 1 : split ? 2 : b->waves_per_group;
   const int lds_tok = 0;   // (no dynamic LDS in the small-alphabet kernels)
   int wpc = b->waves_per_cu;
   if (wpc <= 0) {
     const int occ = post    ? sb_post_occupancy(dtype, AP, f81, irr)
                     : f81w  ? sb_f81w_occupancy(AP)
                     : large ? sb_large_occupancy(dtype, AP, f81, grad, lpost ? 1 : lbc ? 2 : lirr ? 3 : 0)
-                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, 64 * wpg, lds_tok);
+                            : sb_fused_small_occupancy(dtype, AP, f81, grad, irr, rowsk, split, 64 * wpg, lds_tok);
     wpc = occ > 0 ? occ * wpg : 8;
   }
   int nwaves = ctx->num_cus * wpc;
@@ -827,11 +869,11 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
     // stack four walking waves on some SIMDs while others hold two (3,125
     // tiles of the 127-node workload: 0.275 ms at 13 waves per CU, 0.289 ms
     // at 16)
-    const long long wpt = f81w ? sb_f81w_waves_per_group() : 1;   // waves per tile
+    const long long wpt = f81w ? sb_f81w_waves_per_group() : split ? 2 : 1;   // waves per tile
     long long want = 64;
     while (want < (long long)ntiles * wpt) want *= 2;
     if (!large)
-      want = std::min(want, ((long long)ntiles + ctx->num_cus - 1) / ctx->num_cus * ctx->num_cus);
+      want = std::min(want, ((long long)ntiles * wpt + ctx->num_cus - 1) / ctx->num_cus * ctx->num_cus);
     if (want < nwaves) nwaves = int(want);
   }
   nwaves = (nwaves + wpg - 1) / wpg * wpg;
@@ -1005,7 +1047,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
 
   GraphKey key{dtype, AP, f81 ? 1 : 0, grad ? 1 : 0,
                gen2b ? 8 : lpost ? 5 : lbc ? 6 : lirr ? 7 : post ? 4 : gen2 ? 3 : f81w ? 2 : large ? 1 : 0,
-               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok};
+               nwaves, wpg, irr ? 1 : 0, rowsk ? 1 : 0, lds_tok, split ? 1 : 0};
   const dim3 block(64 * wpg);
   const dim3 grid(nwaves / wpg);
   const int nwg_params = 1024;
@@ -2025,8 +2067,24 @@ int sb_batch_set_tune(sb_batch* b, int knob, int value) {
       if (value < 0) return fail(SB_ERR_ARG, "gen2 round must be >= 0 tiles");
       b->gen2_round = value;
       return SB_OK;
+    case SB_TUNE_SPLIT:
+      if (value < 0 || value > 1) return fail(SB_ERR_ARG, "split must be 0 or 1");
+      b->split = value;
+      return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
+}
+
+int sb_context_set_tune(sb_context* ctx, int knob, int value) {
+  if (!ctx) return fail(SB_ERR_ARG, "null context");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  SB_HIP(hipSetDevice(ctx->device));
+  if (!ctx->internal) {
+    // (its stream is set by the first single call, which waits on nothing yet)
+    int rc = batch_create(ctx, nullptr, &ctx->internal);
+    if (rc) return rc;
+  }
+  return sb_batch_set_tune(ctx->internal, knob, value);
 }
 
 int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
@@ -2035,6 +2093,8 @@ int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
     case SB_TUNE_WAVES_PER_GROUP: *value = b->waves_per_group; return SB_OK;
     case SB_TUNE_WAVES_PER_CU: *value = b->waves_per_cu; return SB_OK;
     case SB_TUNE_GEN2_ROUND: *value = b->gen2_round; return SB_OK;
+    case SB_TUNE_SPLIT: *value = b->split; return SB_OK;
+    case SB_TUNE_LAST_SPLIT: *value = b->last_split; return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
 }
