@@ -262,12 +262,13 @@ int sb_batch_set_tiling(sb_batch* b, int waves_per_group, int waves_per_cu);
  *                            ignored), 0 never
  *   SB_TUNE_LAST_SPLIT       read only: 1 when the launcher's last submission
  *                            walked its tiles with two waves
+ *   SB_TUNE_SPLIT_MIN        internal nodes a tree needs to split (>= 8)
  * Results do not depend on any knob (bitwise), except that a split walk sums
  * the subtree's branch terms apart: its Counts (and rate gradients) may differ
  * from the one-wave walk's in the last bits; log-likelihoods, root
  * posteriors and branch-length gradients do not. */
 enum { SB_TUNE_WAVES_PER_GROUP = 0, SB_TUNE_WAVES_PER_CU = 1, SB_TUNE_GEN2_ROUND = 3, SB_TUNE_SPLIT = 4,
-       SB_TUNE_LAST_SPLIT = 5 };
+       SB_TUNE_LAST_SPLIT = 5, SB_TUNE_SPLIT_MIN = 6 };
 int sb_batch_set_tune(sb_batch* b, int knob, int value);
 int sb_batch_get_tune(sb_batch* b, int knob, int* value);
 /* The same knobs on the launcher behind the single-call entry points

@@ -16,7 +16,7 @@ SB_F32, SB_F64 = 0, 1
 SB_MASK_NONE, SB_MASK_AUTO, SB_MASK_EXPLICIT = 0, 1, 2
 SB_WANT_LOGLIKE, SB_WANT_ROOT, SB_WANT_COUNTS = 1, 2, 4
 SB_TUNE_WAVES_PER_GROUP, SB_TUNE_WAVES_PER_CU, SB_TUNE_GEN2_ROUND, SB_TUNE_SPLIT = 0, 1, 3, 4
-SB_TUNE_LAST_SPLIT = 5
+SB_TUNE_LAST_SPLIT, SB_TUNE_SPLIT_MIN = 5, 6
 
 # Every exported symbol of include/subby_amd.h.
 EXPORTS = [

@@ -239,6 +239,7 @@ struct sb_batch {
   int gen2_round = 0;               // two-pass path: tiles per round (0: as the stage budget allows)
   int split = 1;                    // split walks: 0 never, 1 wherever a tree names a subtree
   int last_split = 0;               // the last submission walked its tiles with two waves
+  int split_min = 8;                // trees with fewer internal nodes walk with one wave
   // fixed buffers (the no-allocation launcher): never grown, never freed here
   bool fixed = false;
   int g2_ntiles = 0, g2_round = 0;  // two-pass general path: tiles, tiles per round
@@ -308,7 +309,7 @@ struct TreePlan {
   bool split = false;     // a subtree for a second wave is named (root record: r[6], r[7])
 };
 
-int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
+int plan_tree(const int32_t* parent, int R, TreePlan& tp, int split_min = 8) {
   if (R < 1) return fail(SB_ERR_TREE, "tree has %d nodes", R);
   if (parent[0] != -1) return fail(SB_ERR_TREE, "parentIndex[0] must be -1, got %d", parent[0]);
   std::vector<int> first(R, -1), second(R, -1);
@@ -358,7 +359,7 @@ int plan_tree(const int32_t* parent, int R, TreePlan& tp) {
   // minimises that; a tree that gains less than 15% keeps one wave
   // (s0 = s1 = 0; random 64-leaf trees gain 16-49%, 36% at the median, a
   // caterpillar nothing).  The one-wave kernels ignore it.
-  if (nint >= 8) {
+  if (nint >= std::max(split_min, 8)) {
     std::vector<int> sz(nint, 1);
     for (int i = nint - 1; i >= 0; --i) {
       const int* r = &tp.recs[size_t(i) * 8];
@@ -697,7 +698,7 @@ int submit(sb_batch* b, const std::vector<BlockIn>& in_, int dtype, int use_grap
       return fail(SB_ERR_ARG, "block %d: explicit mask without mask buffer", i);
     if (in[i].flags & SB_F_GRAD) grad = true;
     if (k.f81_fast) f81 = true;
-    int rc = plan_tree(k.parent_index, k.R, plans[i]);
+    int rc = plan_tree(k.parent_index, k.R, plans[i], b->split_min);
     if (rc != SB_OK) {
       std::string msg = g_err;
       return fail(rc, "block %d: %s", i, msg.c_str());
@@ -2071,6 +2072,10 @@ int sb_batch_set_tune(sb_batch* b, int knob, int value) {
       if (value < 0 || value > 1) return fail(SB_ERR_ARG, "split must be 0 or 1");
       b->split = value;
       return SB_OK;
+    case SB_TUNE_SPLIT_MIN:
+      if (value < 8) return fail(SB_ERR_ARG, "split_min must be >= 8 internal nodes");
+      b->split_min = value;
+      return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
 }
@@ -2095,6 +2100,7 @@ int sb_batch_get_tune(sb_batch* b, int knob, int* value) {
     case SB_TUNE_GEN2_ROUND: *value = b->gen2_round; return SB_OK;
     case SB_TUNE_SPLIT: *value = b->split; return SB_OK;
     case SB_TUNE_LAST_SPLIT: *value = b->last_split; return SB_OK;
+    case SB_TUNE_SPLIT_MIN: *value = b->split_min; return SB_OK;
     default: return fail(SB_ERR_ARG, "unknown tuning knob %d", knob);
   }
 }
