@@ -122,6 +122,14 @@ def test_split_mixed_batch_and_graph(sb, eng):
         for x, y in zip(two, g):
             for a, b in zip(x, y):
                 assert torch.equal(a, b), rep
+    # no tree here has 64 internal nodes: none splits, the one-wave results exactly
+    from subby_amd import _lib
+    L.set_tune(_lib.SB_TUNE_SPLIT_MIN, 64)
+    s64, same = _run(sb, L, blocks, 'fp32', 1)
+    assert s64 == 0 and L.get_tune(_lib.SB_TUNE_SPLIT_MIN) == 64
+    for x, y in zip(one, same):
+        for a, b in zip(x, y):
+            assert torch.equal(a, b)
     b = blocks[4]
     ref_cnt, ref_ll, _ = O.counts_loglike_root(b.alignment, otree(b.tree), omodel(b.model))
     assert err_counts(two[4][0], ref_cnt) <= TOL32['counts'] and err_ll(two[4][1], ref_ll) <= TOL32['ll']
