@@ -618,8 +618,11 @@ def other_workload(args, log):
                             "frac": b_alg_grad(R, 4, C) / wall / 1e9 / HBM_PEAK_GBS,
                             "alg_bytes_per_step": b_alg_grad(R, 4, C),
                             "note": "whole step (forward + adjoint kernel, per-tile reduction, host glue) "
-                                    "against the fused pass's algorithmic bytes; 782 column tiles "
-                                    "fill under a fifth of the wave slots"},
+                                    "against the fused pass's algorithmic bytes; 782 column tiles, "
+                                    "two waves each (split walks), fill under two fifths of the "
+                                    "wave slots",
+                            "kernel": "sb_fused_small_kernel<float,4,false,true,false,false,true> + "
+                                      "sb_grad_reduce_kernel"},
                "parity": par}
     print(json.dumps(out), flush=True)
 
