@@ -310,7 +310,7 @@ def other_workload(args, log):
             units = C * (R - 1) * K
             metric = ("MixturePosterior column*branches/sec (FP32, HKY85 x 4 gamma categories, R=127; "
                       "units = columns x branches x categories)")
-            kname = "sb_fused_small_kernel<float,4,false,false,false,false> (LogLike only) + sb_mixture_softmax_kernel"
+            kname = "sb_fused_small_kernel<float,4,false,false,false,false,true> (LogLike only, split walks) + sb_mixture_softmax_kernel"
         else:
             rows = [sb.hky85_diag(float(rng.uniform(1, 5)), rng.dirichlet(np.full(4, 5.0))) for _ in range(R)]
             fn = lambda: sb.Counts(aln, t, [rows], engine=eng)  # noqa: E731
