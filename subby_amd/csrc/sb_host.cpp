@@ -2085,9 +2085,11 @@ int sb_context_set_tune(sb_context* ctx, int knob, int value) {
   std::lock_guard<std::mutex> lock(ctx->mu);
   SB_HIP(hipSetDevice(ctx->device));
   if (!ctx->internal) {
-    // (its stream is set by the first single call, which waits on nothing yet)
+    // (its stream is set by the first single call, which then waits on this
+    // recorded, already complete event)
     int rc = batch_create(ctx, nullptr, &ctx->internal);
     if (rc) return rc;
+    SB_HIP(hipEventRecord(ctx->internal->done, nullptr));
   }
   return sb_batch_set_tune(ctx->internal, knob, value);
 }

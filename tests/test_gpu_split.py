@@ -162,3 +162,11 @@ def test_split_gradient(sb, eng):
         errk = abs(float(r[0]) - refk) / max(1.0, abs(refk))
         say('   grad err', prec, f'{err:.2e}', 'log kappa', f'{errk:.2e}')
         assert err <= tol and errk <= (1e-9 if prec == 'fp64' else 1e-4)
+    # a fresh context tuned before its first call, then called on a side stream
+    e2 = sb.Engine(0)
+    _lib.check(e2.lib.sb_context_set_tune(e2.ctx, _lib.SB_TUNE_SPLIT, 0))
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        d2, _, ll2 = sb.loglike_grad(aln, t, m, g=g, rate_weights=W, precision='fp32', engine=e2)
+    st.synchronize()
+    assert torch.equal(d2, d) and torch.equal(ll2, ll)
