@@ -50,7 +50,12 @@ def family(blocks, precision='fp32'):
     return 'large'
 
 
-def candidates(fam):
+def candidates(fam, split=False):
+    """Launch shapes to time.  ``split``: the batch's tiles are walked by two
+    waves each (split walks), whose workgroups always hold two waves, so only
+    the resident waves per CU are searched."""
+    if fam == 'small' and split:
+        return [dict(waves_per_cu=c) for c in WAVES_PER_CU]
     if fam == 'small':
         return [dict(waves_per_group=g, waves_per_cu=c)
                 for g, c in itertools.product(WAVES_PER_GROUP, WAVES_PER_CU)]
@@ -88,7 +93,11 @@ def tune(launcher, blocks, precision: str = 'fp32', reps: int = 5, cache: str | 
             apply(launcher, db[key]['best'])
             return dict(db[key], cached=True, family=fam)
     prep = launcher.prepare(blocks, precision)
-    cands = cands or candidates(fam)
+    split = False
+    if fam == 'small':
+        launcher.launch(prep, use_graph=False)
+        split = launcher.get_tune(_lib.SB_TUNE_LAST_SPLIT) == 1
+    cands = cands or candidates(fam, split)
     table = []
     launcher.set_timing(True)
     try:
@@ -107,7 +116,7 @@ def tune(launcher, blocks, precision: str = 'fp32', reps: int = 5, cache: str | 
     best_cand = {k: v for k, v in best.items() if k != 'ms'}
     apply(launcher, best_cand)
     res = {'best': best_cand, 'ms': best['ms'], 'worst_ms': worst['ms'], 'table': table, 'key': key,
-           'family': fam}
+           'family': fam, 'split': split}
     if cache:
         db = {}
         if os.path.exists(cache):

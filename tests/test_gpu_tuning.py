@@ -56,7 +56,8 @@ def test_every_candidate_gives_the_same_bits(eng, A, C, leaves):
             for a, b in zip(ref, got):
                 assert torch.equal(a, b), cand
     res = tune(L, [Block(aln_d, t, m)], reps=1)
-    assert res['family'] == fam and len(res['table']) == len(candidates(fam))
+    assert res['family'] == fam and len(res['table']) == len(candidates(fam, res['split']))
+    assert res['split'] == (fam == 'small')   # (the 64-leaf tree gives a second wave a subtree)
     assert res['ms'] == min(r['ms'] for r in res['table'])
     say('   best', res['best'], f"{res['ms']:.3f} ms, slowest {res['worst_ms']:.3f} ms")
     blk = Block(aln_d, t, m)
