@@ -226,6 +226,45 @@ def other_workload(args, log):
         if tuned is not None:
             out["launch_shape"] = {"family": tuned['family'], "best": tuned['best'], "best_ms": tuned['ms'],
                                    "slowest_ms": tuned['worst_ms'], "autotune_table": tuned['table']}
+    elif w == 'latency-hky85':
+        # one call at a time (submit, then wait), the headline's tree and model, a
+        # block of 64, 6,400 and 64,000 columns, split walks on and off
+        from subby_amd import _lib
+        rng = np.random.default_rng(20260)
+        pi = rng.dirichlet(np.full(4, 5.0))
+        m = sb.hky85_diag(2.0, pi)
+        t = sb.synthetic.random_tree(64, rng, 0.05)
+        R = len(t.parentIndex)
+        rows = []
+        for C in (64, 6400, 64000):
+            aln = torch.from_numpy(sb.synthetic.simulate(t, m, C, rng, gap_frac=0.03, ambig_frac=0.01)).to(dev)
+            for split in (1, 0):
+                L = Launcher(eng)
+                L.set_tune(_lib.SB_TUNE_SPLIT, split)
+                blk = [Block(aln, t, m)]
+                for _ in range(5):
+                    L.submit(blk)
+                    L.sync()
+                L.set_timing(True)
+                L.kernel_ms()
+                lat = []
+                for _ in range(max(10, args.steps)):
+                    t0 = time.perf_counter()
+                    L.submit(blk)
+                    L.sync()
+                    lat.append(time.perf_counter() - t0)
+                kms, nl = L.kernel_ms()
+                rows.append({"C": C, "tiles": (C + 63) // 64, "split": L.get_tune(_lib.SB_TUNE_LAST_SPLIT),
+                             "kernel_ms": kms / nl, "call_p50_ms": 1e3 * float(np.median(lat))})
+                L.set_timing(False)
+                L.close()
+        first = rows[0]
+        out = {"metric": "call latency, one 64-column block (FP32 Counts+logLike+root, HKY85 R=127)",
+               "value": first["call_p50_ms"], "unit": "ms", "n_gpus": 1, "steps": max(10, args.steps),
+               "ms_per_step": first["call_p50_ms"], "higher_is_better": False, "dtype": "f32",
+               "config": {"workload": w, "A": 4, "R": R, "C": [64, 6400, 64000]},
+               "kernel": "sb_fused_small_kernel<float,4,false,false,false,false,true|false>",
+               "calls": rows}
     elif w == 'irrev':
         # irreversible A=4 model (complex eigensystem) on a block of the headline's shape
         rng = np.random.default_rng(6000)
@@ -644,12 +683,13 @@ def main():
                     help='served blocks timed through the plain path')
     ap.add_argument('--workload', default='hky85',
                     choices=['hky85', 'f81-a64', 'general-a64', 'grad-hky85', 'branch-hky85', 'post-a20',
-                             'irrev', 'mixture', 'rows', 'maf'],
+                             'irrev', 'mixture', 'rows', 'maf', 'latency-hky85'],
                     help='hky85 (default, the headline line) or one of the other configurations: '
                          'f81-a64 F81 A=64 R=199 C=1e6; general-a64 general A=64 R=199 C=2e4; '
                          'grad-hky85 HKY85 gradient R=127 C=5e4; branch-hky85 per-branch outputs (BranchCounts, '
                          'node and branch posteriors) HKY85 R=127 C=2e5; post-a20 node and branch '
-                         'posteriors, general A=20 R=199 C=2e4')
+                         'posteriors, general A=20 R=199 C=2e4; latency-hky85 one call at a time, '
+                         'split walks on and off')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='first run a 1 GiB device copy (reference kernel for HBM byte counters)')
     ap.add_argument('--graph', action='store_true',
