@@ -805,6 +805,9 @@ def main():
                    "parallelism": "single GPU", "launch": "hipGraph replay" if args.graph else "direct"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+                     # the counters' bytes (FETCH_SIZE + WRITE_SIZE) over this run's launch time
+                     "counter_frac": (profiled_traffic() / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                      if profiled_traffic() else None),
                      "kernel": "sb_fused_small_kernel<float,4,false,false,false,false,true> (two waves per tile)",
                      "kernel_ms": kavg,
                      "alg_bytes_per_launch": balg},
